@@ -1,0 +1,213 @@
+/*
+ * xtracer_amd: C ABI of the MI355X ray-traversal and integrator path.
+ *
+ * It replaces, below the reference's IIntegrator::render()
+ * (src/xtcore/integrator.cc:29-53), the OpenMP tile loop, the per-sample
+ * Integrator::render_tile() of the ao / depth / stencil / pathtracer
+ * integrators and every Scene::intersection() query they make
+ * (src/xtcore/scene.cc:191-216). Scene loading (io::scn::load,
+ * src/xtcore/parseutil.cc:697-822) and the octree build (src/xtcore/octree.tml,
+ * src/xtcore/mesh.cc:30-113) stay host-side and are exposed here too.
+ *
+ * Conventions: every call returns XT_OK (0) or an XT_E_* code, like the
+ * reference loader's int returns; no C++ exception crosses this boundary.
+ * Buffers are caller-owned; device memory behind an xt_dev is owned by it.
+ * One xt_dev is driven by one host thread at a time.
+ */
+#ifndef XTRACER_AMD_H
+#define XTRACER_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XT_OK 0
+#define XT_E_INVALID 1   /* bad argument */
+#define XT_E_IO 2        /* scene file missing or unparsable (loader codes 1 / 2) */
+#define XT_E_SCENE 3     /* scene content the GPU path does not evaluate */
+#define XT_E_DEVICE 4    /* HIP runtime error */
+#define XT_E_BUDGET 5    /* job not admitted: HBM budget */
+#define XT_E_NOTFOUND 6  /* unknown scene id, ticket or camera */
+#define XT_E_STATE 7     /* call not valid in the current state */
+
+#define XT_ACCEL_KDTREE 0
+#define XT_ACCEL_OCTREE 1
+
+/* integrators, src/xtcore/integrator/{stencil,depth,ao,pathtracer} */
+#define XT_INTEGRATOR_STENCIL 0
+#define XT_INTEGRATOR_DEPTH 1
+#define XT_INTEGRATOR_AO 2
+#define XT_INTEGRATOR_PATHTRACER 3
+
+/* antialiasing::SAMPLE_DISTRIBUTION (src/xtcore/aa.h) */
+#define XT_AA_GRID 0
+#define XT_AA_RANDOM 1
+
+/* per-pixel accumulation */
+#define XT_ACCUM_DEFAULT 0   /* REFERENCE for stencil/depth, FP64 for ao/pathtracer */
+#define XT_ACCUM_REFERENCE 1 /* the reference's float read-modify-write per sample */
+#define XT_ACCUM_FP64 2      /* compensated (Neumaier) FP64 sum, rounded once */
+
+/* hit primitive ids returned by xt_trace */
+#define XT_HIT_NONE 0xffffffffu
+#define XT_HIT_PLANE_BASE 0xffff0000u /* XT_HIT_PLANE_BASE + k: plane k */
+
+typedef struct xt_scene xt_scene; /* host scene: loaded, flattened, trees built */
+typedef struct xt_dev xt_dev;     /* one HIP device with resident scenes */
+
+/* ---------------------------------------------------------------- host side */
+
+/* io::scn::load (parseutil.cc:697). mods: "group.group.prop:value" strings.
+ * Returns XT_OK, XT_E_IO (parse error / missing file) or XT_E_SCENE. */
+int xt_scene_load(const char *scn_path, const char *const *mods, int n_mods, xt_scene **out);
+
+typedef struct xt_build_params {
+    int kd_max_depth;     /* 0: 8 + 1.3 log2(N) */
+    int kd_leaf_prims;    /* 0: 4 */
+    float kd_cost_intersect; /* 0: 12 */
+    int threads;          /* 0: all cores */
+} xt_build_params;
+
+/* Flattens the scene (objects in scan order, mesh triangles in octree
+ * pre-order) and builds the octree (reference rules) and the SAH kd-tree. */
+int xt_scene_build(xt_scene *s, const xt_build_params *p);
+void xt_scene_free(xt_scene *s);
+const char *xt_scene_error(const xt_scene *s);
+
+typedef struct xt_scene_info {
+    uint32_t n_objects, n_prims, n_planes, n_materials, n_cameras;
+    uint32_t n_kd_nodes, n_kd_refs, kd_depth, n_oct_nodes, n_oct_refs;
+    uint32_t env_type;
+    uint32_t any_vertex_normals;
+    double build_seconds_octree, build_seconds_kd;
+} xt_scene_info;
+int xt_scene_get_info(const xt_scene *s, xt_scene_info *info);
+
+typedef struct xt_object_info {
+    char name[64];
+    uint64_t handle;      /* first-seen name order (string-pool order) */
+    uint32_t geom_type;   /* 0 plane, 1 sphere, 2 triangle, 3 mesh, ~0 missing */
+    uint32_t material;    /* index, ~0 if missing */
+    uint32_t first_prim, prim_count;
+    int32_t plane;
+} xt_object_info;
+int xt_scene_get_object(const xt_scene *s, uint32_t index, xt_object_info *info);
+
+/* Copies of the flattened arrays (any pointer may be NULL):
+ * prims: n_prims * 12 floats (XtPrim), prims64: n_prims * 9 doubles,
+ * prim_local: index of each prim inside its geometry (OBJ face order),
+ * planes: n_planes * 20 words (XtPlane), materials: n_materials * 12 words. */
+int xt_scene_get_arrays(const xt_scene *s, void *prims, double *prims64, uint32_t *prim_local, void *planes,
+                        void *materials, float env[7]);
+
+/* Geometry parameters of object `index` as loaded (FP64): plane normal +
+ * distance, sphere centre + radius, triangle v0 v1 v2 (9 doubles). */
+int xt_scene_get_object_params(const xt_scene *s, uint32_t index, double out[9]);
+/* Per-vertex normals of the soup, 9 floats per prim (zeros where absent);
+ * XT_E_NOTFOUND when no triangle carries vertex normals. */
+int xt_scene_get_vnormals(const xt_scene *s, float *out);
+
+/* Per-mesh reference octree statistics (node count, deepest level). */
+int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint64_t *nodes, uint64_t *max_level,
+                         uint64_t *triangles);
+
+/* Pinhole / thin-lens camera, evaluated on the host in FP64 exactly as
+ * Perspective::get_primary_ray (src/xtcore/camera/perspective.cc:26-98). */
+typedef struct xt_camera {
+    double position[3];
+    double basis[9];      /* row-major m_transform 3x3 */
+    double dir_z;         /* 1 / tan(fov * RADIAN / 2) */
+    double aperture, flength;
+    uint32_t thin_lens;   /* flength > 0 */
+    uint32_t pad;
+} xt_camera;
+/* camera_name NULL or "": the first camera in name order (workspace.cc:33-38) */
+int xt_scene_camera(const xt_scene *s, const char *camera_name, xt_camera *out);
+/* A camera from an explicit pose (serving requests with their own views). */
+int xt_camera_make(const double position[3], const double target[3], const double up[3], float fov_deg,
+                   float aperture, float flength, xt_camera *out);
+
+/* ----------------------------------------------------------------- device */
+
+int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes);
+void xt_dev_destroy(xt_dev *d);
+const char *xt_last_error(const xt_dev *d);
+
+/* Makes a built scene resident in HBM. The scene may be freed afterwards. */
+int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id);
+int xt_scene_release(xt_dev *d, uint32_t scene_id);
+
+/* Closest-hit queries: the surface of Scene::intersection. Rays are FP32 SoA
+ * (origin, unit direction); hits receive t, the primitive id (soup index,
+ * XT_HIT_PLANE_BASE + plane, or XT_HIT_NONE), the barycentrics (u, v) of a
+ * triangle hit, and flags (bit 0: re-tested in FP64). Host pointers. */
+typedef struct xt_rays {
+    const float *ox, *oy, *oz, *dx, *dy, *dz;
+} xt_rays;
+typedef struct xt_hits {
+    float *t;
+    uint32_t *prim;
+    float *u, *v;
+    uint32_t *flags;
+} xt_hits;
+int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *rays, xt_hits *hits, uint64_t n);
+
+/* Same on device-resident buffers: ray0[i] = (ox, oy, oz, origin) and
+ * ray1[i] = (dx, dy, dz, 0) as float4, where `origin` holds the bits of the
+ * primitive id the ray leaves from (XT_HIT_NONE for a free ray); hit[i] =
+ * (t, prim bits, u, v) as float4.
+ * Runs on the device's stream; *kernel_ms (optional) gets the trace kernel
+ * time measured with HIP events on that stream. */
+int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const void *ray0, const void *ray1, void *hit,
+                    uint32_t *flags, uint64_t n, float *kernel_ms);
+
+typedef struct xt_job {
+    uint32_t scene_id;
+    uint32_t integrator;   /* XT_INTEGRATOR_* */
+    uint32_t width, height;
+    uint32_t samples;      /* params_t::samples */
+    uint32_t aa;           /* params_t::aa (level) */
+    uint32_t aa_mode;      /* XT_AA_GRID / XT_AA_RANDOM */
+    uint32_t rdepth;       /* params_t::rdepth */
+    uint32_t accel;        /* XT_ACCEL_* */
+    uint32_t accum;        /* XT_ACCUM_* */
+    uint32_t seed;
+    uint32_t job_key;      /* distinguishes jobs in the random streams */
+    uint32_t samples_per_pass; /* 0: auto (wavefront size) */
+    uint32_t pad;
+    xt_camera camera;
+} xt_job;
+
+typedef struct xt_job_stats {
+    uint64_t queries;      /* executed closest-hit queries (Scene::intersection calls) */
+    uint64_t retests;      /* rays re-tested in FP64 */
+    uint64_t samples;      /* camera samples */
+    double ms_total;       /* device time of the render, HIP events */
+    double ms_trace;       /* sum of trace kernel times */
+    uint32_t trace_launches;
+    uint32_t passes;
+    double latency_ms;     /* serving: completion - submission */
+    double queue_ms;       /* serving: admission - submission */
+} xt_job_stats;
+
+/* Plain one-job-per-launch render. rgba_out: width*height*4 floats, host. */
+int xt_render(xt_dev *d, const xt_job *job, float *rgba_out, xt_job_stats *stats);
+/* Same with a device-resident output buffer. */
+int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_stats *stats);
+
+/* Serving: continuous batching of jobs on the device. xt_submit admits the
+ * job when its buffers fit the HBM budget (else XT_E_BUDGET); xt_poll reports
+ * completion and copies the image out once done. xt_serve_step advances the
+ * device work by one bounce round; xt_drain runs until every job is done. */
+int xt_submit(xt_dev *d, const xt_job *job, uint64_t *ticket);
+int xt_poll(xt_dev *d, uint64_t ticket, int *done, float *rgba_out, xt_job_stats *stats);
+int xt_serve_step(xt_dev *d, int *active_jobs);
+int xt_drain(xt_dev *d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XTRACER_AMD_H */

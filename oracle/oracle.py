@@ -1,0 +1,223 @@
+"""TEST INFRASTRUCTURE ONLY: Python access to the oracle (oracle/liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libxtref.so")
+
+_lib = None
+_ref = None
+
+
+class SceneDesc(ctypes.Structure):
+    P = ctypes.c_void_p
+    _fields_ = [("n_objects", ctypes.c_uint32), ("obj_type", P), ("obj_material", P), ("obj_param", P),
+                ("mesh_first", P), ("mesh_count", P), ("mesh_v", P), ("mesh_n", P), ("n_materials", ctypes.c_uint32),
+                ("mat_type", P), ("mat_param", P), ("env_type", ctypes.c_uint32), ("env", P)]
+
+
+class OrJob(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("integrator", "width", "height", "samples", "aa", "aa_mode",
+                                                "rdepth", "seed", "job_key", "tile_size", "tile_stride")] + [
+        ("threads", ctypes.c_int), ("cam", ctypes.c_double * 9), ("fov", ctypes.c_float),
+        ("aperture", ctypes.c_float), ("flength", ctypes.c_float)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise ImportError(f"{LIB} missing: build it with `make -C oracle`")
+        L = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        L.or_build.restype = P
+        L.or_build.argtypes = [ctypes.POINTER(SceneDesc)]
+        L.or_free.argtypes = [P]
+        L.or_trace.argtypes = [P, ctypes.c_uint64, P, P, P, P, P, P, ctypes.c_int]
+        L.or_render.restype = ctypes.c_uint64
+        L.or_render.argtypes = [P, ctypes.POINTER(OrJob), P, P, P]
+        L.or_mesh_octree.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleScene:
+    """Oracle scene built from a product Scene's export: objects in scan order,
+    mesh triangles put back into OBJ face order with prim_local."""
+
+    def __init__(self, scene):
+        objs = scene.objects()
+        arr = scene.arrays()
+        vn = scene.vnormals()
+        n = len(objs)
+        self.obj_type = np.array([o["geom_type"] for o in objs], np.uint32)
+        self.obj_material = np.array([o["material"] for o in objs], np.uint32)
+        self.obj_param = np.zeros((n, 9), np.float64)
+        self.mesh_first = np.zeros(n, np.uint32)
+        self.mesh_count = np.zeros(n, np.uint32)
+        tris, norms = [], []
+        base = 0
+        # soup index -> (object, OBJ-order index) map, for comparing hits
+        self.soup_to_local = arr["prim_local"].astype(np.int64)
+        for k, o in enumerate(objs):
+            self.obj_param[k] = scene.object_params(k)
+            if o["geom_type"] == 3:
+                lo, cnt = o["first_prim"], o["prim_count"]
+                order = np.argsort(arr["prim_local"][lo:lo + cnt], kind="stable")
+                tris.append(arr["prims64"][lo:lo + cnt][order])
+                if vn is not None:
+                    norms.append(vn[lo:lo + cnt][order])
+                self.mesh_first[k] = base
+                self.mesh_count[k] = cnt
+                base += cnt
+        self.mesh_v = np.ascontiguousarray(np.concatenate(tris) if tris else np.zeros((1, 9)), np.float64)
+        self.mesh_n = (np.ascontiguousarray(np.concatenate(norms), np.float32) if (vn is not None and norms) else None)
+        mats = arr["materials"]
+        self.mat_type = np.ascontiguousarray(mats["type"], np.uint32)
+        mp = np.zeros((max(1, len(mats)), 11), np.float32)
+        if len(mats):
+            mp[:, 0:3] = mats["kd"]
+            mp[:, 3:6] = mats["ks"]
+            mp[:, 6:9] = mats["ke"]
+            mp[:, 9] = mats["reflectance"]
+            mp[:, 10] = mats["exponent"]
+        self.mat_param = mp
+        self.env = np.ascontiguousarray(arr["env"][1:7], np.float32)
+        d = SceneDesc(n, _p(self.obj_type), _p(self.obj_material), _p(self.obj_param), _p(self.mesh_first),
+                      _p(self.mesh_count), _p(self.mesh_v), _p(self.mesh_n), len(mats), _p(self.mat_type),
+                      _p(self.mat_param), int(arr["env"][0]), _p(self.env))
+        self._h = lib().or_build(ctypes.byref(d))
+        self.objects = objs
+
+    def trace(self, origins, directions, threads=0):
+        n = len(origins)
+        rays = np.ascontiguousarray(np.concatenate([origins, directions], axis=1), np.float64)
+        t = np.zeros(n, np.float64)
+        obj = np.zeros(n, np.int32)
+        prim = np.zeros(n, np.int64)
+        point = np.zeros((n, 3), np.float64)
+        normal = np.zeros((n, 3), np.float64)
+        lib().or_trace(self._h, n, _p(rays), _p(t), _p(obj), _p(prim), _p(point), _p(normal), threads)
+        return t, obj, prim, point, normal
+
+    def render(self, integrator, width, height, cam, samples=1, aa=1, aa_mode=0, rdepth=100, seed=1, job_key=0,
+               threads=0, tile_stride=1, stats=False):
+        """cam: dict(position, target, up, fov, aperture, flength)."""
+        j = OrJob(integrator, width, height, samples, aa, aa_mode, rdepth, seed, job_key, 32, tile_stride, threads,
+                  (ctypes.c_double * 9)(*cam["position"], *cam["target"], *cam["up"]), cam["fov"],
+                  cam.get("aperture", 0.0), cam.get("flength", 0.0))
+        img = np.zeros((height, width, 4), np.float32)
+        mean = np.zeros((height, width, 3), np.float64) if stats else None
+        m2 = np.zeros((height, width, 3), np.float64) if stats else None
+        nq = lib().or_render(self._h, ctypes.byref(j), _p(img), _p(mean), _p(m2))
+        if stats:
+            return img, nq, mean, m2
+        return img, nq
+
+    def mesh_preorder(self, obj):
+        n = ctypes.c_uint64()
+        cnt = int(self.mesh_count[obj])
+        out = np.zeros(cnt, np.uint64)
+        if lib().or_mesh_octree(self._h, obj, ctypes.byref(n), _p(out)):
+            raise ValueError("not a mesh object")
+        return n.value, out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().or_free(self._h)
+        except Exception:
+            pass
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    """The real reference (oracle/_ref/libxtref.so) built from /root/reference."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise ImportError(f"{REF_LIB} missing: build it with `make -C oracle/ref`")
+        L = ctypes.CDLL(REF_LIB)
+        P = ctypes.c_void_p
+        L.xr_load.restype = P
+        L.xr_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+        L.xr_free.argtypes = [P]
+        L.xr_object_count.restype = ctypes.c_uint32
+        L.xr_object_count.argtypes = [P]
+        L.xr_object.argtypes = [P, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), P]
+        L.xr_trace.argtypes = [P, ctypes.c_uint64, P, P, P, P, P]
+        L.xr_octree_trace.argtypes = [ctypes.c_uint64, P, ctypes.c_uint64, P, P, P, P]
+        L.xr_render.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_uint, P]
+        _ref = L
+    return _ref
+
+
+class RefScene:
+    """A scene loaded by the real reference loader (io::scn::load)."""
+
+    def __init__(self, path):
+        st = ctypes.c_int()
+        self._h = ref().xr_load(str(path).encode(), ctypes.byref(st))
+        if not self._h:
+            raise RuntimeError(f"reference loader returned {st.value} for {path}")
+
+    def objects(self):
+        out = []
+        for i in range(ref().xr_object_count(self._h)):
+            name = ctypes.create_string_buffer(128)
+            kind = ctypes.c_int()
+            par = np.zeros(9, np.float64)
+            ref().xr_object(self._h, i, name, 128, ctypes.byref(kind), _p(par))
+            out.append({"name": name.value.decode(), "kind": kind.value, "param": par})
+        return out
+
+    def trace(self, origins, directions):
+        n = len(origins)
+        rays = np.ascontiguousarray(np.concatenate([origins, directions], axis=1), np.float64)
+        t = np.zeros(n, np.float64)
+        obj = np.zeros(n, np.int32)
+        point = np.zeros((n, 3), np.float64)
+        normal = np.zeros((n, 3), np.float64)
+        ref().xr_trace(self._h, n, _p(rays), _p(t), _p(obj), _p(point), _p(normal))
+        return t, obj, point, normal
+
+    def render(self, integrator, width, height, samples=1, aa=1, rdepth=100, threads=1, seed=1):
+        img = np.zeros((height, width, 4), np.float32)
+        rc = ref().xr_render(self._h, integrator, width, height, samples, aa, rdepth, threads, seed, _p(img))
+        if rc:
+            raise RuntimeError("xr_render failed")
+        return img
+
+    def __del__(self):
+        try:
+            if self._h:
+                ref().xr_free(self._h)
+        except Exception:
+            pass
+
+
+def ref_octree_trace(tris, origins, directions):
+    """Octree<Triangle>::intersection of the real reference over `tris`
+    (n x 9, OBJ face order); returns (triangle index or -1, t)."""
+    n = len(origins)
+    rays = np.ascontiguousarray(np.concatenate([origins, directions], axis=1), np.float64)
+    tris = np.ascontiguousarray(tris, np.float64)
+    idx = np.zeros(n, np.int64)
+    t = np.zeros(n, np.float64)
+    ref().xr_octree_trace(len(tris), _p(tris), n, _p(rays), _p(idx), _p(t), None)
+    return idx, t

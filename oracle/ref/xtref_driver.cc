@@ -1,0 +1,189 @@
+// TEST INFRASTRUCTURE ONLY. A C-callable driver around the REAL reference
+// xtcore, compiled from /root/reference by oracle/ref/Makefile into
+// oracle/_ref/libxtref.so. Tests use it to pin the oracle and the loader;
+// nothing in the product links it. Modelled on the GUI's render driver
+// (src/frontend/gui/workspace.cc:55-81), since the reference CLI cannot
+// render (its integrator selection is commented out).
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include <xtcore/context.h>
+#include <xtcore/integrator.h>
+#include <xtcore/math/plane.h>
+#include <xtcore/math/sphere.h>
+#include <xtcore/math/triangle.h>
+#include <xtcore/mesh.h>
+#include <xtcore/octree.h>
+#include <xtcore/parseutil.h>
+#include <xtcore/strpool.h>
+#include <xtcore/log.h>
+
+using namespace xtcore;
+
+namespace {
+bool g_pool = false;
+}
+
+extern "C" {
+
+void *xr_load(const char *path, int *status) {
+    if (!g_pool) {
+        xtcore::pool::str::init();
+        xtcore::Log::handle().echo(false);
+        g_pool = true;
+    }
+    render::context_t *ctx = new render::context_t;
+    int rc = io::scn::load(&ctx->scene, path, nullptr);
+    if (status) *status = rc;
+    if (rc) {
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+void xr_free(void *h) { delete (render::context_t *)h; }
+
+uint32_t xr_object_count(void *h) { return (uint32_t)((render::context_t *)h)->scene.m_objects.size(); }
+
+// Object i in scan order: name, geometry kind (0 plane, 1 sphere, 2
+// triangle, 3 mesh, -1 missing) and its parameters.
+int xr_object(void *h, uint32_t i, char *name, int name_len, int *kind, double *param) {
+    Scene &s = ((render::context_t *)h)->scene;
+    if (i >= s.m_objects.size()) return 1;
+    auto it = s.m_objects.begin();
+    std::advance(it, i);
+    std::strncpy(name, pool::str::get(it->first), name_len - 1);
+    name[name_len - 1] = 0;
+    auto g = s.m_surface.find(it->second->surface);
+    *kind = -1;
+    if (g == s.m_surface.end() || !g->second) return 0;
+    asset::ISurface *surf = g->second;
+    if (auto *p = dynamic_cast<surface::Plane *>(surf)) {
+        *kind = 0;
+        param[0] = p->normal.x;
+        param[1] = p->normal.y;
+        param[2] = p->normal.z;
+        param[3] = p->offset;
+    } else if (auto *sp = dynamic_cast<surface::Sphere *>(surf)) {
+        *kind = 1;
+        param[0] = sp->origin.x;
+        param[1] = sp->origin.y;
+        param[2] = sp->origin.z;
+        param[3] = sp->radius;
+    } else if (auto *t = dynamic_cast<surface::Triangle *>(surf)) {
+        *kind = 2;
+        for (int k = 0; k < 3; ++k) {
+            param[3 * k] = t->v[k].x;
+            param[3 * k + 1] = t->v[k].y;
+            param[3 * k + 2] = t->v[k].z;
+        }
+    } else if (dynamic_cast<surface::Mesh *>(surf)) {
+        *kind = 3;
+    }
+    return 0;
+}
+
+// Scene::intersection for each ray (origin and direction assigned as given,
+// bypassing the normalising Ray constructor). obj: scan index or -1.
+void xr_trace(void *h, uint64_t n, const double *rays, double *t, int32_t *obj, double *point, double *normal) {
+    Scene &s = ((render::context_t *)h)->scene;
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r;
+        r.origin = nmath::Vector3f(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.direction = nmath::Vector3f(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        hit_record_t hr;
+        bool hit = s.intersection(r, hr);
+        t[i] = hit ? hr.t : INFINITY;
+        obj[i] = -1;
+        if (hit) {
+            int k = 0;
+            for (auto it = s.m_objects.begin(); it != s.m_objects.end(); ++it, ++k)
+                if (it->first == hr.id_object) obj[i] = k;
+        }
+        if (point) {
+            point[3 * i] = hr.point.x;
+            point[3 * i + 1] = hr.point.y;
+            point[3 * i + 2] = hr.point.z;
+        }
+        if (normal) {
+            normal[3 * i] = hr.normal.x;
+            normal[3 * i + 1] = hr.normal.y;
+            normal[3 * i + 2] = hr.normal.z;
+        }
+    }
+}
+
+// Builds Octree<Triangle> exactly as Mesh::build_octree does (mesh.cc:70-113)
+// from n_tris triangles (9 doubles each) and returns, per ray, the index of
+// the triangle Octree::intersection picks (-1 on a miss) and its t. The index
+// rides in the triangle's unused texcoord slot.
+void xr_octree_trace(uint64_t n_tris, const double *tris, uint64_t n, const double *rays, int64_t *index, double *t,
+                     uint64_t *node_count) {
+    Octree<surface::Triangle> oct;
+    for (uint64_t i = 0; i < n_tris; ++i) {
+        surface::Triangle p;
+        for (int k = 0; k < 3; ++k)
+            p.v[k] = nmath::Vector3f(tris[9 * i + 3 * k], tris[9 * i + 3 * k + 1], tris[9 * i + 3 * k + 2]);
+        p.tc[0] = nmath::Vector2f((double)i, 0.0);
+        p.calc_aabb();
+        oct.add(p.aabb, p);
+    }
+    oct.max_items_per_node(LIMITS_MAX_ITEMS_PER_NODE);
+    oct.max_depth(LIMITS_MAX_DEPTH);
+    oct.build();
+    (void)node_count;
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r;
+        r.origin = nmath::Vector3f(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.direction = nmath::Vector3f(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        hit_record_t hr;
+        OctreeItem<surface::Triangle> *it = oct.intersection(r, &hr);
+        index[i] = it ? (int64_t)it->data.tc[0].x : -1;
+        t[i] = it ? hr.t : INFINITY;
+    }
+}
+
+// Renders with a reference integrator (0 stencil, 1 depth, 2 ao,
+// 3 pathtracer) and assembles the framebuffer into rgba (W*H*4 floats).
+int xr_render(void *h, int integrator, int width, int height, int samples, int aa, int rdepth, int threads,
+              unsigned seed, float *rgba) {
+    render::context_t *ctx = (render::context_t *)h;
+    render::IIntegrator *in = nullptr;
+    switch (integrator) {
+        case 0: in = new integrator::stencil::Integrator; break;
+        case 1: in = new integrator::depth::Integrator; break;
+        case 2: in = new integrator::ao::Integrator; break;
+        case 3: in = new integrator::pathtracer::Integrator; break;
+        default: return 1;
+    }
+    render::params_t p;
+    p.width = width;
+    p.height = height;
+    p.samples = samples;
+    p.aa = aa;
+    p.rdepth = rdepth;
+    p.threads = threads;
+    p.camera = ctx->scene.m_cameras.empty() ? 0 : ctx->scene.m_cameras.begin()->first;
+    ctx->params = p;
+    ctx->init();
+    in->setup(*ctx);
+    srand(seed);
+    in->render();
+    nimg::Pixmap fb;
+    render::assemble(fb, *ctx);
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const nimg::ColorRGBAf &c = fb.pixel_ro(x, y);
+            float *o = rgba + 4 * ((size_t)y * width + x);
+            o[0] = c.r();
+            o[1] = c.g();
+            o[2] = c.b();
+            o[3] = c.a();
+        }
+    delete in;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,774 @@
+// TEST INFRASTRUCTURE ONLY: the oracle. Nothing in the xtracer_amd product
+// links, imports or calls this file; tests, the smoke check and the CPU
+// baseline leg of bench.py use it as the checker.
+//
+// FP64 CPU restatement of the reference's ray-query and integrator path,
+// written from the reference sources (cited per function) and compiled with
+// -ffp-contract=off so that its roundings are the reference's:
+//   primitives          src/xtcore/math/{triangle,sphere,plane,aabb}.cc
+//   octree build/walk   src/xtcore/octree.tml:97-249, mesh.cc:30-113
+//   Scene::intersection src/xtcore/scene.cc:191-216
+//   camera              src/xtcore/camera/perspective.cc:26-98
+//   anti-aliasing       src/xtcore/aa.cc:7-77
+//   integrators         src/xtcore/integrator/{stencil,depth,ao,pathtracer}
+//   materials/sampling  src/xtcore/material/*.cc, lib/nmath/sample.inl
+//   colour clamps       lib/nimg/color.inl, lib/nimg/color.cc
+// Random numbers use the same counter-based streams as the device path
+// (xtracer_amd/csrc/device/xt_rng.h restated below), because the reference's
+// own glibc rand() stream is not reproducible.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <omp.h>
+
+namespace orc {
+
+struct V {
+    double x = 0, y = 0, z = 0;
+};
+static inline V mk(double x, double y, double z) { return V{x, y, z}; }
+static inline V operator+(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V operator-(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V operator-(V a) { return mk(-a.x, -a.y, -a.z); }
+static inline V operator*(V a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline V operator*(double s, V a) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline V operator/(V a, double s) { return mk(a.x / s, a.y / s, a.z / s); }
+static inline V operator+(V a, double s) { return mk(a.x + s, a.y + s, a.z + s); }
+static inline V mul(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline double len(V a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline V nrm(V a) {
+    double l = len(a);
+    return l != 0 ? mk(a.x / l, a.y / l, a.z / l) : a;
+}
+static inline double comp(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// ---------------------------------------------------------------- RNG
+static inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+static inline uint32_t path_key(uint32_t seed, uint32_t job, uint32_t pixel, uint32_t sample) {
+    uint32_t h = mix32(seed + 0x9e3779b9u);
+    h = mix32(h ^ job);
+    h = mix32(h ^ pixel);
+    return mix32(h ^ (sample * 0x85ebca6bu + 0x27d4eb2fu));
+}
+static inline uint32_t stream_of(uint32_t key, uint32_t q) { return mix32(key ^ (q * 0xc2b2ae35u + 0x165667b1u)); }
+static inline uint32_t draw(uint32_t st, uint32_t i) { return mix32(st + i * 0x9e3779b9u + 0x7f4a7c15u); }
+static inline double unit(uint32_t x) { return (double)x * (1.0 / 4294967295.0); }
+struct Draws {
+    uint32_t st, i;
+    double next() { return unit(draw(st, i++)); }
+};
+
+// ---------------------------------------------------------------- scene
+struct Tri {
+    V v[3];
+    V n[3];
+};
+struct Box {
+    V lo, hi;
+};
+struct ONode {
+    Box box;
+    std::vector<uint32_t> items;
+    int child[8];
+};
+struct Mesh {
+    std::vector<Tri> tris;
+    std::vector<Box> boxes;
+    std::vector<ONode> nodes;
+};
+struct Obj {
+    uint32_t type;  // 0 plane, 1 sphere, 2 triangle, 3 mesh, other: absent
+    uint32_t mat;
+    V pn;
+    double poff;
+    V sc;
+    double sr;
+    Tri tri;
+    int mesh;
+};
+struct Mat {
+    uint32_t type;  // 0 lambert, 1 phong, 2 blinn-phong, 3 emissive
+    float kd[3], ks[3], ke[3];
+    float refl, expo;
+};
+struct Scene {
+    std::vector<Obj> objs;
+    std::vector<Mesh> meshes;
+    std::vector<Mat> mats;
+    uint32_t env_type;
+    float env_a[3], env_b[3];
+};
+
+// AABB3::intersection (aabb.cc:32-71): float slabs on double data, early
+// accept when the origin is strictly inside, no entry t.
+static bool box_hit(const Box &b, V o, V d) {
+    if (o.x > b.lo.x && o.y > b.lo.y && o.z > b.lo.z && o.x < b.hi.x && o.y < b.hi.y && o.z < b.hi.z) return true;
+    const V bb[2] = {b.lo, b.hi};
+    static const float t0 = 0.0;
+    int xs = (int)(d.x < 0.0), ys = (int)(d.y < 0.0);
+    float ix = 1.0 / d.x, iy = 1.0 / d.y;
+    float tmin = (bb[xs].x - o.x) * ix;
+    float tmax = (bb[1 - xs].x - o.x) * ix;
+    float tymin = (bb[ys].y - o.y) * iy;
+    float tymax = (bb[1 - ys].y - o.y) * iy;
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    int zs = (int)(d.z < 0.0);
+    float iz = 1.0 / d.z;
+    float tzmin = (bb[zs].z - o.z) * iz;
+    float tzmax = (bb[1 - zs].z - o.z) * iz;
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return tmax > t0;
+}
+
+struct Hit {
+    double t = INFINITY;
+    V point, normal;
+    int obj = -1;
+    long prim = -1;  // triangle index inside its mesh (OBJ face order)
+};
+
+// Triangle::intersection / calc_barycentric (triangle.cc:82-184).
+static bool tri_hit(const Tri &tr, V o, V d, Hit *h) {
+    V normal = nrm(cross(tr.v[1] - tr.v[0], tr.v[2] - tr.v[0]));
+    double ndd = dot(normal, d);
+    if (std::fabs(ndd) < 1e-8) return false;
+    V vo = o - tr.v[0];
+    double t = -dot(normal, vo) / ndd;
+    if (t < 1e-8) return false;
+    V pos = o + d * t;
+    V v1 = tr.v[1] - tr.v[0], v2 = tr.v[2] - tr.v[0];
+    V x = cross(v1, v2);
+    V n2 = nrm(x);
+    double area = std::fabs(dot(x, n2)) * 0.5;
+    double b0 = 0, b1 = 0, b2 = 0;
+    if (!(area < 1e-8)) {
+        V p0 = tr.v[0] - pos, p1 = tr.v[1] - pos, p2 = tr.v[2] - pos;
+        b0 = std::fabs(dot(cross(p1, p2), n2)) * 0.5 / area;
+        b1 = std::fabs(dot(cross(p2, p0), n2)) * 0.5 / area;
+        b2 = std::fabs(dot(cross(p0, p1), n2)) * 0.5 / area;
+    }
+    double s = b0 + b1 + b2;
+    if (s < 1.0 - 1e-8 || s > 1.0 + 1e-8) return false;
+    if (h) {
+        h->t = t;
+        h->point = pos;
+        V pn = tr.n[0] * b0 + tr.n[1] * b1 + tr.n[2] * b2;
+        h->normal = len(pn) ? pn : normal;
+    }
+    return true;
+}
+
+// Sphere::intersection (sphere.cc:35-61).
+static bool sph_hit(V c, double r, V o, V d, Hit *h) {
+    double b = 2 * dot(o - c, d);
+    double cq = dot(c, c) + dot(o, o) + 2 * dot(-c, o) - r * r;
+    double disc = (b * b - 4 * cq);
+    if (disc > 0.0) {
+        double s = std::sqrt(disc);
+        double t1 = (-b + s) / 2.0, t2 = (-b - s) / 2.0;
+        double t = t1 < t2 ? t1 : t2;
+        if (t > 1e-8 && h) {
+            h->t = t;
+            h->point = o + d * t;
+            h->normal = (h->point - c) / r * (t1 * t2 > 0. ? 1. : -1);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Plane::intersection (plane.cc:19-61).
+static bool plane_hit(V n, double off, V o, V d, Hit *h) {
+    double ndd = dot(n, d);
+    if (std::fabs(ndd) < 1e-8) return false;
+    V v = mk(std::fabs(n.x), std::fabs(n.y), std::fabs(n.z)) * off;
+    V vo = v - o;
+    double t = dot(vo, n) / ndd;
+    if (t < 1e-8) return false;
+    if (h) {
+        h->t = t;
+        h->point = o + d * t;
+        h->normal = n;
+    }
+    return true;
+}
+
+// Octree<T>::build / subdivide (octree.tml:119-198).
+static Box mkbox(V a, V b) {
+    Box r;
+    r.lo = mk(a.x <= b.x ? a.x : b.x, a.y <= b.y ? a.y : b.y, a.z <= b.z ? a.z : b.z);
+    r.hi = mk(a.x >= b.x ? a.x : b.x, a.y >= b.y ? a.y : b.y, a.z >= b.z ? a.z : b.z);
+    return r;
+}
+static bool contains(const Box &b, V p) {
+    return (p.x >= b.lo.x) && (p.y >= b.lo.y) && (p.z >= b.lo.z) && (p.x <= b.hi.x) && (p.y <= b.hi.y) &&
+           (p.z <= b.hi.z);
+}
+static void subdivide(Mesh &m, int ni, size_t level, double root_zmax) {
+    if (m.nodes[ni].items.size() <= 10 || level >= 10) return;
+    Box nb = m.nodes[ni].box;
+    V c = (nb.lo + nb.hi) / 2.0;
+    Box cb[8] = {mkbox(nb.lo, c),
+                 mkbox(mk(c.x, nb.lo.y, nb.lo.z), mk(nb.hi.x, c.y, c.z)),
+                 mkbox(mk(c.x, c.y, nb.lo.z), mk(nb.hi.x, nb.hi.y, c.z)),
+                 mkbox(mk(nb.lo.x, c.y, nb.lo.z), mk(c.x, nb.hi.y, c.z)),
+                 mkbox(mk(nb.lo.x, nb.lo.y, c.z), mk(c.x, c.y, nb.hi.z)),
+                 mkbox(mk(c.x, nb.lo.y, c.z), mk(nb.hi.x, c.y, nb.hi.z)),
+                 mkbox(c, nb.hi),
+                 mkbox(mk(nb.lo.x, c.y, c.z), mk(c.x, nb.hi.y, root_zmax))};
+    for (int i = 0; i < 8; ++i) {
+        ONode ch;
+        ch.box = cb[i];
+        for (int k = 0; k < 8; ++k) ch.child[k] = -1;
+        std::vector<uint32_t> keep;
+        for (uint32_t it : m.nodes[ni].items) {
+            const Tri &t = m.tris[it];
+            if (contains(ch.box, t.v[0]) && contains(ch.box, t.v[1]) && contains(ch.box, t.v[2])) ch.items.push_back(it);
+            else keep.push_back(it);
+        }
+        m.nodes[ni].items.swap(keep);
+        int id = (int)m.nodes.size();
+        m.nodes.push_back(ch);
+        m.nodes[ni].child[i] = id;
+    }
+    for (int i = 0; i < 8; ++i) subdivide(m, m.nodes[ni].child[i], level + 1, root_zmax);
+}
+static void build_mesh(Mesh &m) {
+    Box root;  // (0,0,0) box augmented by every triangle box (Octree::clear/add)
+    for (const Tri &t : m.tris) {
+        Box b;
+        b.lo = mk(INFINITY, INFINITY, INFINITY);
+        b.hi = mk(-INFINITY, -INFINITY, -INFINITY);
+        for (int k = 0; k < 3; ++k) {
+            V p = t.v[k];
+            if (p.x < b.lo.x) b.lo.x = p.x;
+            if (p.y < b.lo.y) b.lo.y = p.y;
+            if (p.z < b.lo.z) b.lo.z = p.z;
+            if (p.x > b.hi.x) b.hi.x = p.x;
+            if (p.y > b.hi.y) b.hi.y = p.y;
+            if (p.z > b.hi.z) b.hi.z = p.z;
+        }
+        if (b.hi.x > root.hi.x) root.hi.x = b.hi.x;
+        if (b.lo.x < root.lo.x) root.lo.x = b.lo.x;
+        if (b.hi.y > root.hi.y) root.hi.y = b.hi.y;
+        if (b.lo.y < root.lo.y) root.lo.y = b.lo.y;
+        if (b.hi.z > root.hi.z) root.hi.z = b.hi.z;
+        if (b.lo.z < root.lo.z) root.lo.z = b.lo.z;
+    }
+    m.nodes.clear();
+    ONode r;
+    r.box = root;
+    for (int k = 0; k < 8; ++k) r.child[k] = -1;
+    for (size_t i = 0; i < m.tris.size(); ++i) r.items.push_back((uint32_t)i);
+    m.nodes.push_back(r);
+    if (m.nodes[0].items.size() > 10) subdivide(m, 0, 0, root.hi.z);
+}
+
+// Octree::r_intersection (octree.tml:206-249).
+static long oct_walk(const Mesh &m, int ni, V o, V d, Hit *out) {
+    if (ni < 0 || !box_hit(m.nodes[ni].box, o, d)) return -1;
+    Hit best;
+    best.t = FLT_MAX;
+    long closest = -1;
+    for (uint32_t it : m.nodes[ni].items) {
+        Hit h;
+        if (tri_hit(m.tris[it], o, d, &h) && h.t < best.t) {
+            best = h;
+            closest = it;
+        }
+    }
+    for (int i = 0; i < 8; ++i) {
+        Hit h;
+        long c = oct_walk(m, m.nodes[ni].child[i], o, d, &h);
+        if (c >= 0 && h.t < best.t) {
+            best = h;
+            closest = c;
+        }
+    }
+    if (closest >= 0 && out) *out = best;
+    return closest;
+}
+
+// Scene::intersection (scene.cc:191-216).
+static bool scene_hit(const Scene &s, V o, V d, Hit &res) {
+    res = Hit();
+    bool any = false;
+    for (size_t i = 0; i < s.objs.size(); ++i) {
+        const Obj &ob = s.objs[i];
+        Hit h;
+        bool ok = false;
+        long prim = -1;
+        switch (ob.type) {
+            case 0: ok = plane_hit(ob.pn, ob.poff, o, d, &h); break;
+            case 1: ok = sph_hit(ob.sc, ob.sr, o, d, &h); break;
+            case 2: ok = tri_hit(ob.tri, o, d, &h); break;
+            case 3:
+                prim = oct_walk(s.meshes[ob.mesh], 0, o, d, &h);
+                ok = prim >= 0;
+                break;
+            default: break;
+        }
+        if (ok) {
+            any = true;
+            if (res.t > h.t) {
+                res = h;
+                res.obj = (int)i;
+                res.prim = ob.type == 3 ? prim : 0;
+            }
+        }
+    }
+    return any;
+}
+
+// ---------------------------------------------------------------- sampling
+// lib/nmath/sample.inl:28-59
+static V hemisphere(V normal, V direction, Draws &r) {
+    double u = r.next(), v = r.next();
+    float phi = std::acos(std::sqrt(u));
+    float theta = 2.0 * M_PI * v;
+    V dd = mk(std::cos(theta) * std::sin(phi), std::cos(phi), std::sin(theta) * std::sin(phi));
+    V n = nrm(normal);
+    V xv = nrm(cross(n + 1e-8, n));
+    V yv = n;
+    V zv = nrm(cross(xv, yv));
+    // columns x, y, z; Vector3f::transformed(Matrix3x3f)
+    V w = mk(xv.x * dd.x + yv.x * dd.y + zv.x * dd.z, xv.y * dd.x + yv.y * dd.y + zv.y * dd.z,
+             xv.z * dd.x + yv.z * dd.y + zv.z * dd.z);
+    V dirc = nrm(direction);
+    float ndotl = dot(n, dirc);
+    return nrm(w * (ndotl > 0 ? ndotl : 0));
+}
+// lib/nmath/sample.inl:61-90
+static V lobe(V normal, V direction, double e, Draws &r) {
+    V c0 = mk(1, 0, 0), c1 = mk(0, 1, 0), c2 = mk(0, 0, 1);
+    V i = nrm(direction), n = nrm(normal);
+    V refl = (2 * dot(i, n) * n) - i;
+    if (1.0f - dot(direction, normal) > 1e-8) {
+        V iv = nrm(cross(direction, refl));
+        V kv = cross(refl, iv);
+        c0 = iv;
+        c1 = refl;
+        c2 = kv;
+    }
+    double u = r.next(), v = r.next();
+    double theta = 2.0 * M_PI * u;
+    double phi = std::acos(std::pow(v, 1.0 / (e + 1)));
+    V l = mk(std::cos(theta) * std::sin(phi), std::cos(phi), std::sin(theta) * std::sin(phi));
+    V vc = mk(c0.x * l.x + c1.x * l.y + c2.x * l.z, c0.y * l.x + c1.y * l.y + c2.y * l.z,
+              c0.z * l.x + c1.z * l.y + c2.z * l.z);
+    double vdotr = dot(refl, vc);
+    if (vdotr < 0.0) vdotr = 0.0;
+    return nrm(vc * std::pow(vdotr, e));
+}
+// lib/nmath/sample.inl:92-107
+static V diffuse(V normal, Draws &r) {
+    V p;
+    for (int k = 0; k < 64; ++k) {
+        double a = r.next(), b = r.next(), c = r.next();
+        p = 2.0 * mk(a, b, c) - mk(1, 1, 1);
+        float l = len(mul(p, p));
+        l *= l;
+        if (!(l >= 1.0)) break;
+    }
+    p = p + normal;
+    return nrm(p);
+}
+
+// ColorRGBf with the setters' clamp at 0 (lib/nimg/color.inl).
+struct C3 {
+    float r = 0, g = 0, b = 0;
+};
+static inline float cl(float x) { return x < 0.f ? 0.f : x; }
+static inline C3 cmul(C3 a, C3 b) { return C3{cl(a.r * b.r), cl(a.g * b.g), cl(a.b * b.b)}; }
+static inline C3 cscale(C3 a, float s) { return C3{cl(a.r * s), cl(a.g * s), cl(a.b * s)}; }
+static inline C3 cadd(C3 a, C3 b) { return C3{cl(a.r + b.r), cl(a.g + b.g), cl(a.b + b.b)}; }
+
+static C3 environment(const Scene &s, V dir) {
+    if (s.env_type == 1) {  // sampler_gradient.cc:11-17
+        V dn = nrm(dir);
+        float t = 0.5f * dn.y + 1.0;
+        C3 a{s.env_a[0], s.env_a[1], s.env_a[2]}, b{s.env_b[0], s.env_b[1], s.env_b[2]};
+        return cadd(cscale(a, (float)(1.0 - t)), cscale(b, t));
+    }
+    if (s.env_type == 2) return C3{s.env_a[0], s.env_a[1], s.env_a[2]};
+    return C3{};
+}
+
+// ---------------------------------------------------------------- camera
+struct Cam {
+    V pos, target, up;
+    float fov, aperture, flength;
+    V rx, ry, rz;
+};
+static void cam_setup(Cam &c) {  // perspective.cc:26-36
+    c.rz = nrm(c.target - c.pos);
+    c.rx = nrm(cross(c.up, c.rz));
+    c.ry = nrm(cross(c.rx, c.rz));
+}
+// perspective.cc:38-98
+static void primary(const Cam &c, float x, float y, float w, float h, uint32_t key, V &o, V &d) {
+    double aspect = (double)w / (double)h;
+    o = c.pos;
+    V dir = mk((2.0 * (double)x / (double)w) - 1.0, ((2.0 * (double)y / (double)h) - 1.0) / aspect,
+               1.0 / std::tan(c.fov * 0.017453292519943 / 2.0));
+    V org = o;
+    if (c.flength > 0) {
+        V fo = dir;
+        double half = c.aperture / 2.f;
+        uint32_t st = stream_of(key, 0);
+        fo.x += unit(draw(st, 0)) * ((half - -half)) + -half;
+        fo.y += unit(draw(st, 1)) * ((half - -half)) + -half;
+        V fp = dir + (double)c.flength * nrm(dir);
+        dir = fp - fo;
+        org = fo;
+    }
+    auto xf = [&](V v) {
+        return mk(c.rx.x * v.x + c.ry.x * v.y + c.rz.x * v.z + 0.0, c.rx.y * v.x + c.ry.y * v.y + c.rz.y * v.z + 0.0,
+                  c.rx.z * v.x + c.ry.z * v.y + c.rz.z * v.z + 0.0);
+    };
+    d = nrm(xf(dir));
+    if (c.flength > 0) o = xf(org) + c.pos;
+}
+
+struct Job {
+    uint32_t integrator, width, height, samples, aa, aa_mode, rdepth, seed, job_key;
+};
+
+struct Ctx {
+    const Scene *s;
+    const Cam *cam;
+    Job job;
+    uint64_t *queries;
+};
+
+// pathtracer::Integrator::eval (pathtracer/integrator.cc:21-47); q is the
+// 1-based index of the query this call makes.
+static C3 eval(const Ctx &cx, size_t depth, C3 in_i, V o, V d, uint32_t key, uint32_t q, uint64_t &nq) {
+    if (depth == 0) return C3{};
+    Hit h;
+    ++nq;
+    bool hit = scene_hit(*cx.s, o, d, h);
+    if (!hit) return environment(*cx.s, d);
+    const Obj &ob = cx.s->objs[h.obj];
+    const Mat &m = cx.s->mats[ob.mat];
+    Draws r{stream_of(key, q), 0};
+    C3 f;
+    V no, nd;
+    bool cont = true;
+    if (m.type == 3) {
+        f = C3{m.ke[0], m.ke[1], m.ke[2]};
+        cont = false;
+    } else if (m.type == 0) {  // lambert.cc:30-40
+        no = h.point + h.normal * 1e-8;
+        nd = nrm(diffuse(h.normal, r));
+        f = cscale(C3{m.kd[0], m.kd[1], m.kd[2]}, (float)dot(h.normal, nd));
+    } else {  // phong.cc:35-55, blinnphong.cc:34-61
+        no = h.point + h.normal * 1e-8;
+        double s = m.refl;
+        double k = r.next();
+        if (k > s) {
+            f = C3{m.kd[0], m.kd[1], m.kd[2]};
+            nd = diffuse(h.normal, r);
+        } else {
+            f = C3{m.ks[0], m.ks[1], m.ks[2]};
+            nd = lobe(h.normal, -d, m.expo, r);
+            if (m.type == 2) nd = nrm(nd);
+        }
+    }
+    C3 value = cmul(in_i, f);
+    if (cont) value = cmul(value, eval(cx, depth - 1, f, no, nd, key, q + 1, nq));
+    return value;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// Scene description handed in by the tests (objects in scan order).
+struct or_scene_desc {
+    uint32_t n_objects;
+    const uint32_t *obj_type;
+    const uint32_t *obj_material;
+    const double *obj_param;     // 9 doubles per object: plane n, off | sphere c, r | triangle v0..v2
+    const uint32_t *mesh_first;  // per object: first triangle in mesh arrays (meshes only)
+    const uint32_t *mesh_count;
+    const double *mesh_v;        // 9 doubles per triangle, OBJ face order
+    const float *mesh_n;         // 9 floats per triangle or NULL
+    uint32_t n_materials;
+    const uint32_t *mat_type;
+    const float *mat_param;      // 11 floats: kd[3], ks[3], ke[3], reflectance, exponent
+    uint32_t env_type;
+    const float *env;            // a[3], b[3]
+};
+
+void *or_build(const or_scene_desc *in) {
+    Scene *s = new Scene;
+    for (uint32_t i = 0; i < in->n_objects; ++i) {
+        Obj o{};
+        o.type = in->obj_type[i];
+        o.mat = in->obj_material[i];
+        const double *p = in->obj_param + 9 * (size_t)i;
+        o.mesh = -1;
+        if (o.type == 0) {
+            o.pn = mk(p[0], p[1], p[2]);
+            o.poff = p[3];
+        } else if (o.type == 1) {
+            o.sc = mk(p[0], p[1], p[2]);
+            o.sr = p[3];
+        } else if (o.type == 2) {
+            for (int k = 0; k < 3; ++k) o.tri.v[k] = mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+        } else if (o.type == 3) {
+            Mesh m;
+            for (uint32_t t = 0; t < in->mesh_count[i]; ++t) {
+                size_t g = (size_t)in->mesh_first[i] + t;
+                Tri tr;
+                for (int k = 0; k < 3; ++k) {
+                    tr.v[k] = mk(in->mesh_v[9 * g + 3 * k], in->mesh_v[9 * g + 3 * k + 1], in->mesh_v[9 * g + 3 * k + 2]);
+                    if (in->mesh_n)
+                        tr.n[k] = mk(in->mesh_n[9 * g + 3 * k], in->mesh_n[9 * g + 3 * k + 1], in->mesh_n[9 * g + 3 * k + 2]);
+                }
+                m.tris.push_back(tr);
+            }
+            build_mesh(m);
+            o.mesh = (int)s->meshes.size();
+            s->meshes.push_back(std::move(m));
+        }
+        s->objs.push_back(o);
+    }
+    for (uint32_t i = 0; i < in->n_materials; ++i) {
+        Mat m{};
+        m.type = in->mat_type[i];
+        const float *p = in->mat_param + 11 * (size_t)i;
+        for (int k = 0; k < 3; ++k) {
+            m.kd[k] = p[k];
+            m.ks[k] = p[3 + k];
+            m.ke[k] = p[6 + k];
+        }
+        m.refl = p[9];
+        m.expo = p[10];
+        s->mats.push_back(m);
+    }
+    s->env_type = in->env_type;
+    for (int k = 0; k < 3; ++k) {
+        s->env_a[k] = in->env ? in->env[k] : 0.f;
+        s->env_b[k] = in->env ? in->env[3 + k] : 0.f;
+    }
+    return s;
+}
+
+void or_free(void *h) { delete (Scene *)h; }
+
+// Mesh octree statistics for object `obj` (must be a mesh).
+int or_mesh_octree(void *h, uint32_t obj, uint64_t *nodes, uint64_t *preorder) {
+    Scene *s = (Scene *)h;
+    if (obj >= s->objs.size() || s->objs[obj].type != 3) return 1;
+    const Mesh &m = s->meshes[s->objs[obj].mesh];
+    if (nodes) *nodes = m.nodes.size();
+    if (preorder) {
+        size_t k = 0;
+        std::vector<int> st{0};
+        while (!st.empty()) {
+            int n = st.back();
+            st.pop_back();
+            for (uint32_t it : m.nodes[n].items) preorder[k++] = it;
+            for (int c = 7; c >= 0; --c)
+                if (m.nodes[n].child[c] >= 0) st.push_back(m.nodes[n].child[c]);
+        }
+    }
+    return 0;
+}
+
+// Closest hit per ray (double SoA in: o xyz, d xyz). Outputs t (INF on a
+// miss), object index (-1), mesh-local triangle (-1 unless a mesh), point and
+// normal (3 doubles each, may be NULL).
+void or_trace(void *h, uint64_t n, const double *rays, double *t, int32_t *obj, int64_t *prim, double *point,
+              double *normal, int threads) {
+    Scene *s = (Scene *)h;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const double *r = rays + 6 * (size_t)i;
+        Hit hh;
+        bool ok = scene_hit(*s, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), hh);
+        t[i] = ok ? hh.t : INFINITY;
+        obj[i] = ok ? hh.obj : -1;
+        prim[i] = ok ? hh.prim : -1;
+        if (point) {
+            point[3 * i] = hh.point.x;
+            point[3 * i + 1] = hh.point.y;
+            point[3 * i + 2] = hh.point.z;
+        }
+        if (normal) {
+            normal[3 * i] = hh.normal.x;
+            normal[3 * i + 1] = hh.normal.y;
+            normal[3 * i + 2] = hh.normal.z;
+        }
+    }
+}
+
+struct or_job {
+    uint32_t integrator;  // 0 stencil, 1 depth, 2 ao, 3 pathtracer
+    uint32_t width, height, samples, aa, aa_mode, rdepth, seed, job_key;
+    uint32_t tile_size;   // 32 (context.cc defaults)
+    uint32_t tile_stride; // render every k-th tile in row-major order (1: all)
+    int threads;
+    double cam[9];        // position, target, up
+    float fov, aperture, flength;
+};
+
+// Renders with the reference's per-sample float read-modify-write (tile.cc,
+// color.cc clamps) into rgba (W*H*4, zero-initialised by the caller). When
+// mean/m2 are given (W*H*3 doubles each) the per-pixel sample mean and sum of
+// squared deviations (Welford, FP64) of the per-sample values are returned
+// too. Returns the number of executed scene queries.
+uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double *m2) {
+    Scene *s = (Scene *)h;
+    Cam cam;
+    cam.pos = mk(jb->cam[0], jb->cam[1], jb->cam[2]);
+    cam.target = mk(jb->cam[3], jb->cam[4], jb->cam[5]);
+    cam.up = mk(jb->cam[6], jb->cam[7], jb->cam[8]);
+    cam.fov = jb->fov;
+    cam.aperture = jb->aperture;
+    cam.flength = jb->flength;
+    cam_setup(cam);
+    const uint32_t W = jb->width, H = jb->height, T = jb->tile_size ? jb->tile_size : 32;
+    const uint32_t tx = W / T + (W % T ? 1 : 0), ty = H / T + (H % T ? 1 : 0);
+    const uint32_t ntiles = tx * ty, stride = jb->tile_stride ? jb->tile_stride : 1;
+    const uint32_t L = jb->aa;
+    uint32_t n_aa;
+    float aw;
+    if (jb->aa_mode == 0) {
+        n_aa = L == 0 ? 0 : L * L;
+        aw = L <= 1 ? 1.f : 1.f / (L * L);
+    } else {
+        n_aa = L;
+        aw = L ? 1.f / L : 1.f;
+    }
+    const double weight = (double)aw * (jb->samples ? 1. / jb->samples : 1.);
+    Ctx cx{s, &cam, {jb->integrator, W, H, jb->samples, L, jb->aa_mode, jb->rdepth, jb->seed, jb->job_key}, nullptr};
+    uint64_t total_q = 0;
+    if (jb->threads > 0) omp_set_num_threads(jb->threads);
+#pragma omp parallel for schedule(dynamic) reduction(+ : total_q)
+    for (long long ti = 0; ti < (long long)ntiles; ti += stride) {
+        const uint32_t x0 = (uint32_t)(ti % tx) * T, y0 = (uint32_t)(ti / tx) * T;
+        const uint32_t x1 = std::min(x0 + T, W), y1 = std::min(y0 + T, H);
+        uint64_t nq = 0;
+        for (uint32_t x = x0; x < x1; ++x) {  // produce(): x outer, y inner (aa.cc:55-56)
+            for (uint32_t y = y0; y < y1; ++y) {
+                const uint32_t pix = y * W + x;
+                float *px = rgba + 4 * (size_t)pix;
+                double mu[3] = {0, 0, 0}, q2[3] = {0, 0, 0};
+                uint32_t cnt = 0;
+                for (uint32_t a = 0; a < n_aa; ++a) {
+                    float cxp, cyp;
+                    if (jb->aa_mode == 1) {
+                        uint32_t k2 = path_key(jb->seed, jb->job_key, pix, 0xfff00000u + a);
+                        uint32_t st = stream_of(k2, 0);
+                        float rx = (float)unit(draw(st, 0)), ry = (float)unit(draw(st, 1));
+                        cxp = (float)((double)x + (double)rx);
+                        cyp = (float)((double)y + (double)ry);
+                    } else if (L <= 1) {
+                        cxp = (float)((double)x + 0.5);
+                        cyp = (float)((double)y + 0.5);
+                    } else {
+                        float sub = 1.f / L, off = sub * 0.5f;
+                        cxp = (float)((double)x + (double)(a % L) * (double)sub + (double)off);
+                        cyp = (float)((double)y + (double)(a / L) * (double)sub + (double)off);
+                    }
+                    for (uint32_t k = 0; k < jb->samples; ++k) {
+                        const uint32_t sidx = a * jb->samples + k;
+                        const uint32_t key = path_key(jb->seed, jb->job_key, pix, sidx);
+                        V o, d;
+                        primary(cam, cxp, cyp, (float)W, (float)H, key, o, d);
+                        double val[3] = {0, 0, 0};
+                        if (jb->integrator == 0) {  // stencil/integrator.cc:9-34
+                            Hit hh;
+                            ++nq;
+                            if (scene_hit(*s, o, d, hh)) {
+                                float na = (float)((double)px[3] + weight);
+                                px[0] = px[1] = px[2] = 1.f;
+                                px[3] = na < 0.f ? 0.f : (na > 1.f ? 1.f : na);
+                                val[0] = val[1] = val[2] = 1;
+                            }
+                        } else if (jb->integrator == 1) {  // depth/integrator.cc:11-55
+                            Hit hh;
+                            ++nq;
+                            if (scene_hit(*s, o, d, hh)) {
+                                float dep = 1. / std::log(len(o - hh.point));
+                                float r = (float)((double)px[0] + (double)dep * weight);
+                                r = r < 0.f ? 0.f : r;
+                                px[0] = px[1] = px[2] = r;
+                                val[0] = val[1] = val[2] = dep;
+                            }
+                            px[3] = 1.f;
+                        } else if (jb->integrator == 2) {  // ao/integrator.cc:12-53
+                            Hit hh;
+                            float w = 1.0f;
+                            ++nq;
+                            if (scene_hit(*s, o, d, hh)) {
+                                Draws r{stream_of(key, 1), 0};
+                                V ad = hemisphere(hh.normal, hh.normal, r);
+                                V ao = hh.point + hh.normal * 1e-8;
+                                Hit h2;
+                                ++nq;
+                                if (scene_hit(*s, ao, ad, h2)) {
+                                    float dist = len(h2.point - ao);
+                                    w = std::min(dist, 100.0f) / 100.0f;
+                                }
+                            }
+                            for (int c = 0; c < 3; ++c) {
+                                float v = (float)((double)px[c] + (double)w * weight);
+                                px[c] = v < 0.f ? 0.f : v;
+                                val[c] = w;
+                            }
+                            px[3] = 1.f;
+                        } else {  // pathtracer/integrator.cc:51-74
+                            C3 L3 = eval(cx, jb->rdepth, C3{1, 1, 1}, o, d, key, 1, nq);
+                            C3 add = cscale(L3, (float)weight);
+                            px[0] = cl(px[0] + add.r);
+                            px[1] = cl(px[1] + add.g);
+                            px[2] = cl(px[2] + add.b);
+                            px[3] = 1.f;
+                            val[0] = L3.r;
+                            val[1] = L3.g;
+                            val[2] = L3.b;
+                        }
+                        ++cnt;
+                        for (int c = 0; c < 3; ++c) {
+                            double dlt = val[c] - mu[c];
+                            mu[c] += dlt / cnt;
+                            q2[c] += dlt * (val[c] - mu[c]);
+                        }
+                    }
+                }
+                if (mean)
+                    for (int c = 0; c < 3; ++c) mean[3 * (size_t)pix + c] = mu[c];
+                if (m2)
+                    for (int c = 0; c < 3; ++c) m2[3 * (size_t)pix + c] = q2[c];
+            }
+        }
+        total_q += nq;
+    }
+    return total_q;
+}
+
+}  // extern "C"

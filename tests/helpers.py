@@ -1,0 +1,56 @@
+"""Shared helpers for the parity tests."""
+import numpy as np
+
+import xtracer_amd as xa
+from xtracer_amd import scenes
+
+
+def soup_to_object_local(scene_arrays, prim):
+    """Maps device hit ids to (object index, index inside the geometry)."""
+    prims = scene_arrays["prims"]
+    planes = scene_arrays["planes"]
+    local = scene_arrays["prim_local"]
+    prim = np.asarray(prim, np.uint64)
+    obj = np.full(len(prim), -1, np.int64)
+    loc = np.full(len(prim), -1, np.int64)
+    soup = prim < len(prims)
+    obj[soup] = prims["object"][prim[soup].astype(np.int64)]
+    loc[soup] = local[prim[soup].astype(np.int64)]
+    pl = (prim >= xa.HIT_PLANE_BASE) & (prim != xa.HIT_NONE)
+    if pl.any():
+        obj[pl] = planes["object"][(prim[pl] - xa.HIT_PLANE_BASE).astype(np.int64)]
+        loc[pl] = 0
+    return obj, loc
+
+
+def camera_rays(cam, width, height):
+    """Pinhole primary rays (pixel centres) in FP64, as Perspective::get_primary_ray."""
+    pos = np.array(cam.position)
+    m = np.array(cam.basis).reshape(3, 3)
+    ys, xs = np.mgrid[0:height, 0:width]
+    x = (xs + 0.5).astype(np.float32).astype(np.float64).ravel()
+    y = (ys + 0.5).astype(np.float32).astype(np.float64).ravel()
+    w, h = float(np.float32(width)), float(np.float32(height))
+    aspect = w / h
+    d = np.stack([(2.0 * x / w) - 1.0, ((2.0 * y / h) - 1.0) / aspect, np.full(x.shape, cam.dir_z)], 1)
+    d = d @ m.T
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    o = np.broadcast_to(pos, d.shape).copy()
+    return o, d
+
+
+def mixed_rays(scene_path, n, seed=0, box=(-3, 6)):
+    """Random rays: half from random points in a box in random directions, half
+    camera-like rays through the scene, as FP32 (origins, unit directions)."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(box[0], box[1], (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    o = o.astype(np.float32)
+    d = d.astype(np.float32)
+    d = (d / np.linalg.norm(d.astype(np.float64), axis=1)[:, None]).astype(np.float32)
+    return o, d
+
+
+def scene_path(kind, **kw):
+    return scenes.SCENES[kind][0](**kw)

@@ -1,0 +1,369 @@
+"""ctypes binding of libxtracer_amd.so (the C ABI declared in include/xtracer_amd.h).
+
+The library is built in-tree by ``make -C xtracer_amd`` (or ``__graft_entry__.build()``).
+Nothing here falls back to a CPU path: if the library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxtracer_amd.so")
+
+XT_OK = 0
+ERRORS = {1: "XT_E_INVALID", 2: "XT_E_IO", 3: "XT_E_SCENE", 4: "XT_E_DEVICE", 5: "XT_E_BUDGET",
+          6: "XT_E_NOTFOUND", 7: "XT_E_STATE"}
+
+ACCEL_KDTREE, ACCEL_OCTREE = 0, 1
+STENCIL, DEPTH, AO, PATHTRACER = 0, 1, 2, 3
+INTEGRATORS = {"stencil": STENCIL, "depth": DEPTH, "ao": AO, "pathtracer": PATHTRACER}
+AA_GRID, AA_RANDOM = 0, 1
+ACCUM_DEFAULT, ACCUM_REFERENCE, ACCUM_FP64 = 0, 1, 2
+HIT_NONE = 0xFFFFFFFF
+HIT_PLANE_BASE = 0xFFFF0000
+
+# Exported symbols, in the order include/xtracer_amd.h declares them.
+SYMBOLS = [
+    "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
+    "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals",
+    "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
+    "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_scene_upload", "xt_scene_release", "xt_trace",
+    "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
+]
+
+
+class XtError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+
+
+class BuildParams(ctypes.Structure):
+    _fields_ = [("kd_max_depth", ctypes.c_int), ("kd_leaf_prims", ctypes.c_int),
+                ("kd_cost_intersect", ctypes.c_float), ("threads", ctypes.c_int)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "n_objects", "n_prims", "n_planes", "n_materials", "n_cameras", "n_kd_nodes", "n_kd_refs", "kd_depth",
+        "n_oct_nodes", "n_oct_refs", "env_type", "any_vertex_normals")] + [
+        ("build_seconds_octree", ctypes.c_double), ("build_seconds_kd", ctypes.c_double)]
+
+
+class ObjectInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("handle", ctypes.c_uint64), ("geom_type", ctypes.c_uint32),
+                ("material", ctypes.c_uint32), ("first_prim", ctypes.c_uint32), ("prim_count", ctypes.c_uint32),
+                ("plane", ctypes.c_int32)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_double * 3), ("basis", ctypes.c_double * 9), ("dir_z", ctypes.c_double),
+                ("aperture", ctypes.c_double), ("flength", ctypes.c_double), ("thin_lens", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "scene_id", "integrator", "width", "height", "samples", "aa", "aa_mode", "rdepth", "accel", "accum",
+        "seed", "job_key", "samples_per_pass", "pad")] + [("camera", Camera)]
+
+
+class JobStats(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_uint64), ("retests", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+                ("ms_total", ctypes.c_double), ("ms_trace", ctypes.c_double), ("trace_launches", ctypes.c_uint32),
+                ("passes", ctypes.c_uint32), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Rays(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_float)) for n in ("ox", "oy", "oz", "dx", "dy", "dz")]
+
+
+class Hits(ctypes.Structure):
+    _fields_ = [("t", ctypes.POINTER(ctypes.c_float)), ("prim", ctypes.POINTER(ctypes.c_uint32)),
+                ("u", ctypes.POINTER(ctypes.c_float)), ("v", ctypes.POINTER(ctypes.c_float)),
+                ("flags", ctypes.POINTER(ctypes.c_uint32))]
+
+
+# numpy views of the flattened arrays (xtracer_amd/csrc/xt_layout.h)
+PRIM_DTYPE = np.dtype([("p0", "<f4", 3), ("tf", "<u4"), ("p1", "<f4", 3), ("object", "<u4"), ("p2", "<f4", 3),
+                       ("aux", "<f4")])
+PLANE_DTYPE = np.dtype([("normal", "<f4", 3), ("object", "<u4"), ("point", "<f4", 3), ("pad", "<u4"),
+                        ("normal64", "<f8", 3), ("point64", "<f8", 3)])
+MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("kd", "<f4", 3), ("ks", "<f4", 3), ("ke", "<f4", 3),
+                           ("reflectance", "<f4"), ("exponent", "<f4")])
+
+_lib = None
+
+
+def lib():
+    """Loads libxtracer_amd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C xtracer_amd` "
+                          "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    P, U32, U64, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "xt_scene_load": (I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), I, ctypes.POINTER(P)]),
+        "xt_scene_build": (I, [P, ctypes.POINTER(BuildParams)]),
+        "xt_scene_free": (None, [P]),
+        "xt_scene_error": (ctypes.c_char_p, [P]),
+        "xt_scene_get_info": (I, [P, ctypes.POINTER(SceneInfo)]),
+        "xt_scene_get_object": (I, [P, U32, ctypes.POINTER(ObjectInfo)]),
+        "xt_scene_get_arrays": (I, [P, P, P, P, P, P, P]),
+        "xt_scene_get_object_params": (I, [P, U32, P]),
+        "xt_scene_get_vnormals": (I, [P, P]),
+        "xt_scene_mesh_octree": (I, [P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64),
+                                     ctypes.POINTER(U64)]),
+        "xt_scene_camera": (I, [P, ctypes.c_char_p, ctypes.POINTER(Camera)]),
+        "xt_camera_make": (I, [ctypes.POINTER(ctypes.c_double)] * 3 + [ctypes.c_float] * 3 +
+                           [ctypes.POINTER(Camera)]),
+        "xt_dev_create": (I, [ctypes.POINTER(P), I, U64]),
+        "xt_dev_destroy": (None, [P]),
+        "xt_last_error": (ctypes.c_char_p, [P]),
+        "xt_scene_upload": (I, [P, P, ctypes.POINTER(U32)]),
+        "xt_scene_release": (I, [P, U32]),
+        "xt_trace": (I, [P, U32, I, ctypes.POINTER(Rays), ctypes.POINTER(Hits), U64]),
+        "xt_trace_device": (I, [P, U32, I, P, P, P, P, U64, ctypes.POINTER(ctypes.c_float)]),
+        "xt_render": (I, [P, ctypes.POINTER(Job), P, ctypes.POINTER(JobStats)]),
+        "xt_render_device": (I, [P, ctypes.POINTER(Job), P, ctypes.POINTER(JobStats)]),
+        "xt_submit": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(U64)]),
+        "xt_poll": (I, [P, U64, ctypes.POINTER(I), P, ctypes.POINTER(JobStats)]),
+        "xt_serve_step": (I, [P, ctypes.POINTER(I)]),
+        "xt_drain": (I, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Scene:
+    """Host scene: io::scn::load + flattening + octree/kd-tree builds (no GPU needed)."""
+
+    def __init__(self, path, mods=(), build=True, **build_params):
+        L = lib()
+        self._h = ctypes.c_void_p()
+        arr = (ctypes.c_char_p * max(1, len(mods)))(*[m.encode() for m in mods])
+        rc = L.xt_scene_load(str(path).encode(), arr, len(mods), ctypes.byref(self._h))
+        if rc != XT_OK:
+            msg = L.xt_scene_error(self._h).decode() if self._h else ""
+            if self._h:
+                L.xt_scene_free(self._h)
+                self._h = None
+            raise XtError(rc, msg)
+        if build:
+            self.build(**build_params)
+
+    def build(self, kd_max_depth=0, kd_leaf_prims=0, kd_cost_intersect=0.0, threads=0):
+        p = BuildParams(kd_max_depth, kd_leaf_prims, kd_cost_intersect, threads)
+        rc = lib().xt_scene_build(self._h, ctypes.byref(p))
+        if rc != XT_OK:
+            raise XtError(rc, lib().xt_scene_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        i = SceneInfo()
+        rc = lib().xt_scene_get_info(self._h, ctypes.byref(i))
+        if rc:
+            raise XtError(rc)
+        return i
+
+    def objects(self):
+        out = []
+        for k in range(self.info().n_objects):
+            o = ObjectInfo()
+            rc = lib().xt_scene_get_object(self._h, k, ctypes.byref(o))
+            if rc:
+                raise XtError(rc)
+            out.append({"name": o.name.decode(), "handle": o.handle, "geom_type": o.geom_type,
+                        "material": o.material, "first_prim": o.first_prim, "prim_count": o.prim_count,
+                        "plane": o.plane})
+        return out
+
+    def arrays(self):
+        i = self.info()
+        prims = np.zeros(i.n_prims, PRIM_DTYPE)
+        prims64 = np.zeros((i.n_prims, 9), np.float64)
+        local = np.zeros(i.n_prims, np.uint32)
+        planes = np.zeros(i.n_planes, PLANE_DTYPE)
+        mats = np.zeros(i.n_materials, MATERIAL_DTYPE)
+        env = np.zeros(7, np.float32)
+        rc = lib().xt_scene_get_arrays(self._h, _ptr(prims), _ptr(prims64), _ptr(local), _ptr(planes), _ptr(mats),
+                                       _ptr(env))
+        if rc:
+            raise XtError(rc)
+        return {"prims": prims, "prims64": prims64, "prim_local": local, "planes": planes, "materials": mats,
+                "env": env}
+
+    def object_params(self, index):
+        out = np.zeros(9, np.float64)
+        rc = lib().xt_scene_get_object_params(self._h, index, _ptr(out))
+        if rc:
+            raise XtError(rc)
+        return out
+
+    def vnormals(self):
+        out = np.zeros((self.info().n_prims, 9), np.float32)
+        rc = lib().xt_scene_get_vnormals(self._h, _ptr(out))
+        if rc == 6:
+            return None
+        if rc:
+            raise XtError(rc)
+        return out
+
+    def mesh_octrees(self):
+        res = []
+        k = 0
+        while True:
+            g, n, lv, nt = (ctypes.c_uint64() for _ in range(4))
+            rc = lib().xt_scene_mesh_octree(self._h, k, ctypes.byref(g), ctypes.byref(n), ctypes.byref(lv),
+                                            ctypes.byref(nt))
+            if rc:
+                return res
+            res.append({"geometry": g.value, "nodes": n.value, "max_level": lv.value, "triangles": nt.value})
+            k += 1
+
+    def camera(self, name=None):
+        c = Camera()
+        rc = lib().xt_scene_camera(self._h, (name or "").encode(), ctypes.byref(c))
+        if rc:
+            raise XtError(rc, f"camera {name!r}")
+        return c
+
+    def close(self):
+        if self._h:
+            lib().xt_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_camera(position, target, up, fov, aperture=0.0, flength=0.0):
+    c = Camera()
+    d3 = ctypes.c_double * 3
+    rc = lib().xt_camera_make(d3(*position), d3(*target), d3(*up), fov, aperture, flength, ctypes.byref(c))
+    if rc:
+        raise XtError(rc)
+    return c
+
+
+class Device:
+    """One HIP device with resident scenes (xt_dev)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        L = lib()
+        self._d = ctypes.c_void_p()
+        rc = L.xt_dev_create(ctypes.byref(self._d), device, hbm_budget)
+        if rc != XT_OK:
+            raise XtError(rc, "xt_dev_create")
+
+    def _check(self, rc):
+        if rc != XT_OK:
+            raise XtError(rc, lib().xt_last_error(self._d).decode())
+
+    @property
+    def handle(self):
+        return self._d
+
+    def upload(self, scene):
+        sid = ctypes.c_uint32()
+        self._check(lib().xt_scene_upload(self._d, scene.handle, ctypes.byref(sid)))
+        return sid.value
+
+    def release(self, scene_id):
+        self._check(lib().xt_scene_release(self._d, scene_id))
+
+    def trace(self, scene_id, origins, directions, accel=ACCEL_KDTREE):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        n = o.shape[0]
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        fp = ctypes.POINTER(ctypes.c_float)
+        rays = Rays(*[c.ctypes.data_as(fp) for c in cols])
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.uint32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        flags = np.zeros(n, np.uint32)
+        hits = Hits(t.ctypes.data_as(fp), prim.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                    u.ctypes.data_as(fp), v.ctypes.data_as(fp), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        self._check(lib().xt_trace(self._d, scene_id, accel, ctypes.byref(rays), ctypes.byref(hits), n))
+        return t, prim, u, v, flags
+
+    def render(self, job):
+        img = np.zeros((job.height, job.width, 4), np.float32)
+        st = JobStats()
+        self._check(lib().xt_render(self._d, ctypes.byref(job), _ptr(img), ctypes.byref(st)))
+        return img, st
+
+    def render_device(self, job, out_ptr):
+        st = JobStats()
+        self._check(lib().xt_render_device(self._d, ctypes.byref(job), ctypes.c_void_p(out_ptr), ctypes.byref(st)))
+        return st
+
+    def trace_device(self, scene_id, ray0_ptr, ray1_ptr, hit_ptr, n, accel=ACCEL_KDTREE, flags_ptr=None):
+        ms = ctypes.c_float()
+        self._check(lib().xt_trace_device(self._d, scene_id, accel, ctypes.c_void_p(ray0_ptr),
+                                          ctypes.c_void_p(ray1_ptr), ctypes.c_void_p(hit_ptr),
+                                          ctypes.c_void_p(flags_ptr) if flags_ptr else None, n, ctypes.byref(ms)))
+        return ms.value
+
+    def submit(self, job):
+        t = ctypes.c_uint64()
+        self._check(lib().xt_submit(self._d, ctypes.byref(job), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket, width, height):
+        done = ctypes.c_int()
+        img = np.zeros((height, width, 4), np.float32)
+        st = JobStats()
+        self._check(lib().xt_poll(self._d, ticket, ctypes.byref(done), _ptr(img), ctypes.byref(st)))
+        return bool(done.value), img, st
+
+    def serve_step(self):
+        n = ctypes.c_int()
+        self._check(lib().xt_serve_step(self._d, ctypes.byref(n)))
+        return n.value
+
+    def drain(self):
+        self._check(lib().xt_drain(self._d))
+
+    def close(self):
+        if self._d:
+            lib().xt_dev_destroy(self._d)
+            self._d = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_job(scene_id, integrator, width, height, camera, samples=1, aa=1, aa_mode=AA_GRID, rdepth=100,
+             accel=ACCEL_KDTREE, accum=ACCUM_DEFAULT, seed=1, job_key=0, samples_per_pass=0):
+    """An xt_job; defaults follow params_t (src/xtcore/context.cc:3-27)."""
+    if isinstance(integrator, str):
+        integrator = INTEGRATORS[integrator]
+    return Job(scene_id, integrator, width, height, samples, aa, aa_mode, rdepth, accel, accum, seed, job_key,
+               samples_per_pass, 0, camera)

@@ -1,0 +1,203 @@
+// Host half of the C ABI: scene loading, flattening and tree builds, and the
+// FP64 camera set-up. Needs no GPU.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "../../include/xtracer_amd.h"
+#include "capi_internal.h"
+
+using namespace xt;
+
+extern "C" {
+
+int xt_scene_load(const char *scn_path, const char *const *mods, int n_mods, xt_scene **out) {
+    if (!scn_path || !out || n_mods < 0) return XT_E_INVALID;
+    *out = nullptr;
+    xt_scene *s = new (std::nothrow) xt_scene;
+    if (!s) return XT_E_INVALID;
+    std::vector<std::string> m;
+    for (int i = 0; i < n_mods; ++i)
+        if (mods && mods[i]) m.push_back(mods[i]);
+    int rc;
+    try {
+        rc = load_scn(s->scene, scn_path, m);
+    } catch (const std::exception &e) {
+        s->scene.error = e.what();
+        rc = LOAD_FAIL;
+    }
+    s->error = s->scene.error;
+    *out = s;  // returned even on failure so that xt_scene_error can be read
+    if (rc == LOAD_PARSE) return XT_E_IO;
+    if (rc != LOAD_OK) return s->error.find("failed to load mesh") != std::string::npos ? XT_E_IO : XT_E_SCENE;
+    return XT_OK;
+}
+
+int xt_scene_build(xt_scene *s, const xt_build_params *p) {
+    if (!s) return XT_E_INVALID;
+    KdParams kp;
+    if (p) {
+        if (p->kd_max_depth > 0) kp.max_depth = p->kd_max_depth;
+        if (p->kd_leaf_prims > 0) kp.leaf_prims = p->kd_leaf_prims;
+        if (p->kd_cost_intersect > 0.f) kp.cost_intersect = p->kd_cost_intersect;
+        if (p->threads > 0) kp.threads = p->threads;
+    }
+    try {
+        std::string err;
+        if (flatten_scene(s->scene, s->flat, kp, &err)) {
+            s->error = err;
+            return XT_E_SCENE;
+        }
+    } catch (const std::exception &e) {
+        s->error = e.what();
+        return XT_E_SCENE;
+    }
+    s->built = true;
+    return XT_OK;
+}
+
+void xt_scene_free(xt_scene *s) { delete s; }
+
+const char *xt_scene_error(const xt_scene *s) { return s ? s->error.c_str() : "null scene"; }
+
+int xt_scene_get_info(const xt_scene *s, xt_scene_info *info) {
+    if (!s || !info) return XT_E_INVALID;
+    std::memset(info, 0, sizeof *info);
+    const FlatScene &f = s->flat;
+    info->n_objects = (uint32_t)(s->built ? f.objects.size() : s->scene.objects.size());
+    info->n_prims = (uint32_t)f.prims.size();
+    info->n_planes = (uint32_t)f.planes.size();
+    info->n_materials = (uint32_t)(s->built ? f.materials.size() : s->scene.materials.size());
+    info->n_cameras = (uint32_t)s->scene.cameras.size();
+    info->n_kd_nodes = (uint32_t)f.kd.nodes.size();
+    info->n_kd_refs = (uint32_t)f.kd.refs.size();
+    info->kd_depth = (uint32_t)f.kd.depth;
+    info->n_oct_nodes = (uint32_t)f.oct_nodes.size();
+    info->n_oct_refs = (uint32_t)f.oct_refs.size();
+    info->env_type = s->scene.env.type;
+    info->any_vertex_normals = f.vnormals.empty() ? 0u : 1u;
+    info->build_seconds_octree = f.build_seconds_octree;
+    info->build_seconds_kd = f.build_seconds_kd;
+    return XT_OK;
+}
+
+int xt_scene_get_object(const xt_scene *s, uint32_t index, xt_object_info *info) {
+    if (!s || !info || !s->built) return XT_E_INVALID;
+    if (index >= s->flat.objects.size()) return XT_E_NOTFOUND;
+    const FlatObject &o = s->flat.objects[index];
+    std::memset(info, 0, sizeof *info);
+    std::strncpy(info->name, o.name.c_str(), sizeof(info->name) - 1);
+    info->handle = o.handle;
+    info->geom_type = o.geom_type;
+    info->material = o.material;
+    info->first_prim = o.first_prim;
+    info->prim_count = o.prim_count;
+    info->plane = o.plane;
+    return XT_OK;
+}
+
+int xt_scene_get_arrays(const xt_scene *s, void *prims, double *prims64, uint32_t *prim_local, void *planes,
+                        void *materials, float env[7]) {
+    if (!s || !s->built) return XT_E_INVALID;
+    const FlatScene &f = s->flat;
+    if (prims && !f.prims.empty()) std::memcpy(prims, f.prims.data(), f.prims.size() * sizeof(XtPrim));
+    if (prims64 && !f.prims64.empty()) std::memcpy(prims64, f.prims64.data(), f.prims64.size() * sizeof(XtPrim64));
+    if (prim_local && !f.prim_local.empty())
+        std::memcpy(prim_local, f.prim_local.data(), f.prim_local.size() * sizeof(uint32_t));
+    if (planes && !f.planes.empty()) std::memcpy(planes, f.planes.data(), f.planes.size() * sizeof(XtPlane));
+    if (materials && !f.materials.empty())
+        std::memcpy(materials, f.materials.data(), f.materials.size() * sizeof(XtMaterial));
+    if (env) {
+        env[0] = (float)f.env.type;
+        for (int i = 0; i < 3; ++i) {
+            env[1 + i] = f.env.a[i];
+            env[4 + i] = f.env.b[i];
+        }
+    }
+    return XT_OK;
+}
+
+int xt_scene_get_object_params(const xt_scene *s, uint32_t index, double out[9]) {
+    if (!s || !s->built || !out) return XT_E_INVALID;
+    if (index >= s->flat.objects.size()) return XT_E_NOTFOUND;
+    for (int k = 0; k < 9; ++k) out[k] = 0.0;
+    const auto &obj = s->scene.objects.at(s->flat.objects[index].handle);
+    auto g = s->scene.geometries.find(obj.geometry);
+    if (g == s->scene.geometries.end()) return XT_OK;
+    const Geometry &geo = g->second;
+    if (geo.type == GEOM_PLANE) {
+        out[0] = geo.normal.x;
+        out[1] = geo.normal.y;
+        out[2] = geo.normal.z;
+        out[3] = geo.offset;
+    } else if (geo.type == GEOM_SPHERE) {
+        out[0] = geo.center.x;
+        out[1] = geo.center.y;
+        out[2] = geo.center.z;
+        out[3] = geo.radius;
+    } else if (geo.type == GEOM_TRIANGLE) {
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) out[3 * k + a] = geo.v[k][a];
+    }
+    return XT_OK;
+}
+
+int xt_scene_get_vnormals(const xt_scene *s, float *out) {
+    if (!s || !s->built || !out) return XT_E_INVALID;
+    if (s->flat.vnormals.empty()) return XT_E_NOTFOUND;
+    std::memcpy(out, s->flat.vnormals.data(), s->flat.vnormals.size() * sizeof(float));
+    return XT_OK;
+}
+
+int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint64_t *nodes, uint64_t *max_level,
+                         uint64_t *triangles) {
+    if (!s || !s->built) return XT_E_INVALID;
+    if (k >= s->flat.mesh_octrees.size()) return XT_E_NOTFOUND;
+    const auto &m = s->flat.mesh_octrees[k];
+    if (geometry) *geometry = m.geometry;
+    if (nodes) *nodes = m.nodes;
+    if (max_level) *max_level = m.max_level;
+    if (triangles) *triangles = m.triangles;
+    return XT_OK;
+}
+
+// Perspective::calculate_transform (perspective.cc:26-36) and the projection
+// constant 1 / tan(fov * RADIAN / 2) (perspective.cc:53), in FP64.
+int xt_camera_make(const double position[3], const double target[3], const double up[3], float fov_deg,
+                   float aperture, float flength, xt_camera *out) {
+    if (!position || !target || !up || !out) return XT_E_INVALID;
+    std::memset(out, 0, sizeof *out);
+    const dvec3 pos(position[0], position[1], position[2]);
+    const dvec3 tg(target[0], target[1], target[2]);
+    const dvec3 u(up[0], up[1], up[2]);
+    const dvec3 rz = normalized(tg - pos);
+    const dvec3 rx = normalized(cross(u, rz));
+    const dvec3 ry = normalized(cross(rx, rz));
+    const double m[9] = {rx.x, ry.x, rz.x, rx.y, ry.y, rz.y, rx.z, ry.z, rz.z};
+    std::memcpy(out->basis, m, sizeof m);
+    for (int i = 0; i < 3; ++i) out->position[i] = position[i];
+    const double RADIAN = 0.017453292519943;  // lib/nmath/precision.h
+    out->dir_z = 1.0 / std::tan((double)fov_deg * RADIAN / 2.0);
+    out->aperture = aperture;
+    out->flength = flength;
+    out->thin_lens = flength > 0.f ? 1u : 0u;
+    return XT_OK;
+}
+
+int xt_scene_camera(const xt_scene *s, const char *camera_name, xt_camera *out) {
+    if (!s || !out) return XT_E_INVALID;
+    const Camera *cam = nullptr;
+    if (!camera_name || !*camera_name) {
+        if (!s->scene.cameras.empty()) cam = &s->scene.cameras.begin()->second;
+    } else {
+        for (auto &kv : s->scene.cameras)
+            if (kv.second.name == camera_name) cam = &kv.second;
+    }
+    if (!cam) return XT_E_NOTFOUND;
+    const double p[3] = {cam->position.x, cam->position.y, cam->position.z};
+    const double t[3] = {cam->target.x, cam->target.y, cam->target.z};
+    const double u[3] = {cam->up.x, cam->up.y, cam->up.z};
+    return xt_camera_make(p, t, u, cam->fov, cam->aperture, cam->flength, out);
+}
+
+}  // extern "C"

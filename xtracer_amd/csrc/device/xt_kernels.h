@@ -1,0 +1,661 @@
+// Wavefront kernels. One "pass" renders `spp_pass` samples of every pixel:
+//   k_camera  -> queue of camera rays (compacted), path state, zeroed radiance
+//   repeat per query index q = 1..rdepth:
+//     k_trace   FP32 closest hit per queued ray, rays needing the FP64 re-test
+//               appended to a list
+//     k_retest  FP64 re-trace of the listed rays with the reference predicates
+//     k_shade_* integrator rule; surviving paths appended (ballot + prefix
+//               sum, one atomic per wave) to the next queue
+//   k_accumulate  per-pixel sum over the pass's samples in sample order
+// Every kernel reads its work count from device memory and loops over it, so
+// the launch sequence is fixed and can be captured in a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../../include/xtracer_amd.h"
+#include "xt_sample.h"
+#include "xt_trace.h"
+
+#define XT_BLOCK 256
+#define XT_OCT_BLOCK 64
+
+// Indices into the device counter block.
+enum {
+    CNT_Q0 = 0,      // ray count of queue 0
+    CNT_Q1 = 1,      // ray count of queue 1
+    CNT_RETEST = 2,  // rays listed for the FP64 re-test
+    CNT_WORDS = 8
+};
+struct Counters {
+    uint32_t n[CNT_WORDS];
+    unsigned long long queries, retests, steps_exceeded, pad;
+};
+
+struct Queue {
+    float4 *r0;  // origin xyz, origin primitive bits
+    float4 *r1;  // direction xyz, analytic self-hit t
+    uint32_t *path;
+};
+
+struct JobDev {
+    uint32_t W, H, tiles_x, NT;  // NT: pixel slots in 8x8 tile order (>= W*H)
+    uint32_t spp, aa, aa_mode, n_aa, total;  // total = n_aa * spp samples per pixel
+    uint32_t rdepth, integrator, seed, job_key, accum;
+    float wf, hf, aa_sub, aa_off;
+    double weight;  // per-sample weight: float(aa weight) * (1. / spp)
+    float weight_f;
+    xt_camera cam;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append: returns this lane's slot (only valid if `want`).
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+    const unsigned long long m = __ballot(want);
+    const uint32_t n = (uint32_t)__popcll(m);
+    const uint32_t lane = lane_id();
+    const uint32_t leader = m ? (uint32_t)__ffsll((long long)m) - 1 : 0;
+    uint32_t base = 0;
+    if (lane == leader && n) base = atomicAdd(counter, n);
+    base = __shfl(base, (int)leader);
+    const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
+    return base + (uint32_t)__popcll(below);
+}
+
+// Slot -> pixel in 8x8 tile order, so that a wave holds one 8x8 block.
+__device__ __forceinline__ bool slot_pixel(const JobDev &j, uint32_t q, uint32_t &x, uint32_t &y) {
+    const uint32_t tile = q >> 6, within = q & 63u;
+    x = (tile % j.tiles_x) * 8u + (within & 7u);
+    y = (tile / j.tiles_x) * 8u + (within >> 3);
+    return x < j.W && y < j.H;
+}
+
+static __global__ void k_reset(Counters *c, uint32_t mask) {
+    const uint32_t i = threadIdx.x;
+    if (i < CNT_WORDS && (mask >> i) & 1u) c->n[i] = 0;
+    if (i == 0 && (mask >> 31) & 1u) {
+        c->queries = 0;
+        c->retests = 0;
+        c->steps_exceeded = 0;
+    }
+}
+
+static __global__ void k_set_count(Counters *c, uint32_t which, uint32_t value) { c->n[which] = value; }
+
+static __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// --------------------------------------------------------------- camera
+
+#pragma clang fp contract(off)
+// Perspective::get_primary_ray (src/xtcore/camera/perspective.cc:38-98) in
+// FP64; the lens draws come from query stream 0 of the path.
+__device__ __forceinline__ void camera_ray(const JobDev &j, float x, float y, uint32_t key, double o[3],
+                                           double d[3]) {
+    const xt_camera &c = j.cam;
+    const double aspect = (double)j.wf / (double)j.hf;
+    double dir[3] = {(2.0 * (double)x / (double)j.wf) - 1.0, ((2.0 * (double)y / (double)j.hf) - 1.0) / aspect,
+                     c.dir_z};
+    double org[3] = {c.position[0], c.position[1], c.position[2]};
+    if (c.thin_lens) {
+        const double half = (double)((float)c.aperture / 2.f);
+        const uint32_t st = xt_stream(key, 0);
+        double fo[3] = {dir[0], dir[1], dir[2]};
+        fo[0] += xt_unit64(xt_draw(st, 0)) * ((half - -half)) + -half;
+        fo[1] += xt_unit64(xt_draw(st, 1)) * ((half - -half)) + -half;
+        const double l = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        double nd[3] = {dir[0], dir[1], dir[2]};
+        if (l != 0.0) {
+            nd[0] = dir[0] / l;
+            nd[1] = dir[1] / l;
+            nd[2] = dir[2] / l;
+        }
+        const double fl = (double)(float)c.flength;
+        const double fp[3] = {dir[0] + fl * nd[0], dir[1] + fl * nd[1], dir[2] + fl * nd[2]};
+        dir[0] = fp[0] - fo[0];
+        dir[1] = fp[1] - fo[1];
+        dir[2] = fp[2] - fo[2];
+        org[0] = fo[0];
+        org[1] = fo[1];
+        org[2] = fo[2];
+    }
+    const double *m = c.basis;
+    double t0 = m[0] * dir[0] + m[1] * dir[1] + m[2] * dir[2] + 0.0;
+    double t1 = m[3] * dir[0] + m[4] * dir[1] + m[5] * dir[2] + 0.0;
+    double t2 = m[6] * dir[0] + m[7] * dir[1] + m[8] * dir[2] + 0.0;
+    const double l = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    if (l != 0.0) {
+        t0 /= l;
+        t1 /= l;
+        t2 /= l;
+    }
+    d[0] = t0;
+    d[1] = t1;
+    d[2] = t2;
+    if (c.thin_lens) {
+        const double a0 = m[0] * org[0] + m[1] * org[1] + m[2] * org[2] + 0.0;
+        const double a1 = m[3] * org[0] + m[4] * org[1] + m[5] * org[2] + 0.0;
+        const double a2 = m[6] * org[0] + m[7] * org[1] + m[8] * org[2] + 0.0;
+        o[0] = a0 + c.position[0];
+        o[1] = a1 + c.position[1];
+        o[2] = a2 + c.position[2];
+    } else {
+        o[0] = org[0];
+        o[1] = org[1];
+        o[2] = org[2];
+    }
+}
+
+// Sample coordinates of anti-aliasing sub-sample `a` of pixel (px, py)
+// (src/xtcore/aa.cc:7-48).
+__device__ __forceinline__ void aa_coords(const JobDev &j, uint32_t px, uint32_t py, uint32_t a, float &cx,
+                                          float &cy) {
+    if (j.aa_mode == XT_AA_RANDOM) {
+        const uint32_t key = xt_path_key(j.seed, j.job_key, py * j.W + px, 0xfff00000u + a);
+        const uint32_t st = xt_stream(key, 0);
+        const float rx = (float)xt_unit64(xt_draw(st, 0));
+        const float ry = (float)xt_unit64(xt_draw(st, 1));
+        cx = (float)((double)px + (double)rx);
+        cy = (float)((double)py + (double)ry);
+    } else if (j.aa <= 1) {
+        cx = (float)((double)px + 0.5);
+        cy = (float)((double)py + 0.5);
+    } else {
+        const uint32_t ix = a % j.aa, iy = a / j.aa;
+        cx = (float)((double)px + (double)ix * (double)j.aa_sub + (double)j.aa_off);
+        cy = (float)((double)py + (double)iy * (double)j.aa_sub + (double)j.aa_off);
+    }
+}
+#pragma clang fp contract(on)
+
+// Slots [0, spp_pass * NT): slot i renders sample pass_base + i / NT of the
+// pixel at tile position i % NT.
+static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
+                                                     Queue out, uint32_t *out_count, float4 *ps0, float4 *ps1,
+                                                     float4 *rad, double *ray64) {
+    const uint32_t n = spp_pass * j.NT;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool live = false;
+        uint32_t x = 0, y = 0, s = 0;
+        if (i < n) {
+            const uint32_t sl = i / j.NT, q = i - sl * j.NT;
+            s = pass_base + sl;
+            live = slot_pixel(j, q, x, y) && s < j.total;
+            rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        double o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
+        uint32_t key = 0;
+        if (live) {
+            const uint32_t pixel = y * j.W + x;
+            key = xt_path_key(j.seed, j.job_key, pixel, s);
+            float cx, cy;
+            aa_coords(j, x, y, s / j.spp, cx, cy);
+            camera_ray(j, cx, cy, key, o, d);
+            ps0[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
+            ps1[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(1u));
+            if (ray64) {
+                double *r = ray64 + 6 * (size_t)i;
+                r[0] = o[0];
+                r[1] = o[1];
+                r[2] = o[2];
+                r[3] = d[0];
+                r[4] = d[1];
+                r[5] = d[2];
+            }
+        }
+        const uint32_t slot = wave_append(live, out_count);
+        if (live) {
+            out.r0[slot] = make_float4((float)o[0], (float)o[1], (float)o[2], __uint_as_float(XT_NONE));
+            out.r1[slot] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
+            out.path[slot] = i;
+        }
+    }
+}
+
+// --------------------------------------------------------------- trace
+
+__device__ __forceinline__ float tsmall_of(f3 o) {
+    return XT_TSMALL * (1.f + fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z))));
+}
+
+__device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Best &b) {
+    b.t = INFINITY;
+    b.prim = XT_NONE;
+    b.obj = XT_NONE;
+    b.u = b.v = 0.f;
+    b.mark_t = INFINITY;
+    if (r.self_t > 0.f && r.origin != XT_NONE) {
+        b.t = r.self_t;
+        b.prim = r.origin;
+        b.obj = r.origin >= XT_PLANE_BASE ? S.planes[r.origin - XT_PLANE_BASE].object
+                                          : __float_as_uint(S.prims[3 * r.origin + 1].w);
+    }
+}
+
+template <int ACCEL>
+static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK) void k_trace(
+    const SceneView S, const Queue in, const uint32_t *in_count, float4 *hit, uint32_t *retest_list,
+    Counters *cnt) {
+    constexpr int BS = ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK;
+    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? XT_KD_STACK * XT_BLOCK : 1];
+    __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * XT_OCT_BLOCK : 1];
+    const uint32_t n = *in_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->queries, (unsigned long long)n);
+    const uint32_t stride = gridDim.x * BS;
+    for (uint32_t base = blockIdx.x * BS; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool mark = false;
+        if (i < n) {
+            const float4 a = in.r0[i], b = in.r1[i];
+            RayF r;
+            r.o = mk(a.x, a.y, a.z);
+            r.d = mk(b.x, b.y, b.z);
+            r.origin = __float_as_uint(a.w);
+            r.self_t = b.w;
+            Tester32 tst{S, r, tsmall_of(r.o), {}};
+            seed_best(S, r, tst.best);
+            for (uint32_t k = 0; k < S.n_planes; ++k)
+                if (XT_PLANE_BASE + k != r.origin) plane32(r.o, r.d, tst.tsmall, S.planes[k], XT_PLANE_BASE + k, tst.best);
+            bool ok;
+            if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd(S, tst, r.o, r.d, kstk + threadIdx.x, BS, 1e-6f);
+            else ok = walk_oct(S, tst, r.o, r.d, ostk + threadIdx.x, BS, 1e-6f);
+            const Best &bb = tst.best;
+            mark = !ok || bb.mark_t <= bb.t * (1.f + XT_TIE);
+            hit[i] = make_float4(bb.t, __uint_as_float(bb.prim), bb.u, bb.v);
+        }
+        const uint32_t slot = wave_append(mark, &cnt->n[CNT_RETEST]);
+        if (mark) retest_list[slot] = i;
+    }
+}
+
+// FP64 re-test of listed rays. ray64 (optional) holds the FP64 rays by path
+// id; otherwise the FP32 ray is widened.
+template <int ACCEL>
+static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK) void k_retest(
+    const SceneView S, const Queue in, float4 *hit, const uint32_t *list, Counters *cnt, const double *ray64,
+    uint32_t *flags) {
+    constexpr int BS = ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK;
+    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? XT_KD_STACK * XT_BLOCK : 1];
+    __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * XT_OCT_BLOCK : 1];
+    const uint32_t n = cnt->n[CNT_RETEST];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->retests, (unsigned long long)n);
+    const uint32_t stride = gridDim.x * BS;
+    for (uint32_t base = blockIdx.x * BS; base < n; base += stride) {
+        const uint32_t k = base + threadIdx.x;
+        if (k >= n) continue;
+        const uint32_t i = list[k];
+        const float4 a = in.r0[i], b = in.r1[i];
+        RayF r;
+        r.o = mk(a.x, a.y, a.z);
+        r.d = mk(b.x, b.y, b.z);
+        r.origin = __float_as_uint(a.w);
+        r.self_t = b.w;
+        Tester64 tst{S, r, {r.o.x, r.o.y, r.o.z}, {r.d.x, r.d.y, r.d.z}, INFINITY, XT_NONE, XT_NONE, 0, 0, 0};
+        if (ray64) {
+            const double *q = ray64 + 6 * (size_t)in.path[i];
+            tst.o = {q[0], q[1], q[2]};
+            tst.d = {q[3], q[4], q[5]};
+        }
+        if (r.self_t > 0.f && r.origin != XT_NONE) {
+            tst.bt = r.self_t;
+            tst.prim = r.origin;
+            tst.obj = r.origin >= XT_PLANE_BASE ? S.planes[r.origin - XT_PLANE_BASE].object
+                                                : __float_as_uint(S.prims[3 * r.origin + 1].w);
+        }
+        for (uint32_t p = 0; p < S.n_planes; ++p) {
+            if (XT_PLANE_BASE + p == r.origin) continue;
+            const double t = plane64(tst.o, tst.d, S.planes[p]);
+            if (t > 0.0) tst.offer64(t, XT_PLANE_BASE + p, S.planes[p].object, nullptr);
+        }
+        const f3 o32 = mk((float)tst.o.x, (float)tst.o.y, (float)tst.o.z);
+        const f3 d32 = mk((float)tst.d.x, (float)tst.d.y, (float)tst.d.z);
+        bool ok;
+        if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd(S, tst, o32, d32, kstk + threadIdx.x, BS, 1e-4f);
+        else ok = walk_oct(S, tst, o32, d32, ostk + threadIdx.x, BS, 1e-4f);
+        if (!ok) atomicAdd(&cnt->steps_exceeded, 1ull);
+        const bool is_tri = tst.prim < S.n_prims;
+        hit[i] = make_float4(tst.prim == XT_NONE ? INFINITY : (float)tst.bt, __uint_as_float(tst.prim),
+                             is_tri ? (float)tst.b1 : 0.f, is_tri ? (float)tst.b2 : 0.f);
+        if (flags) flags[i] |= 1u;
+    }
+}
+
+// --------------------------------------------------------------- shading
+
+// Surface data of a hit: point, shading normal N (unnormalised, as the
+// reference hands it on) and a geometric normal g of any length.
+__device__ __forceinline__ void surface(const SceneView &S, uint32_t prim, f3 o, f3 d, float t, float u, float v,
+                                        f3 &p, f3 &N, f3 &g, uint32_t &obj) {
+    p = add(o, scl(d, t));
+    if (prim >= XT_PLANE_BASE) {
+        const XtPlane &pl = S.planes[prim - XT_PLANE_BASE];
+        N = mk(pl.normal[0], pl.normal[1], pl.normal[2]);
+        g = N;
+        obj = pl.object;
+        return;
+    }
+    const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1];
+    obj = __float_as_uint(b.w);
+    const uint32_t tf = __float_as_uint(a.w);
+    if (XT_PRIM_TYPE(tf) == XT_PRIM_SPHERE) {
+        N = scl(sub(p, mk(a.x, a.y, a.z)), b.z);  // (p - c) / r
+        g = N;
+        return;
+    }
+    const float4 c = S.prims[3 * prim + 2];
+    const f3 v0 = mk(a.x, a.y, a.z);
+    g = cross3(sub(mk(b.x, b.y, b.z), v0), sub(mk(c.x, c.y, c.z), v0));
+    N = norm3(g);
+    if ((XT_PRIM_FLAGS(tf) & XT_PF_VNORMALS) && S.vnormals) {
+        const float *vn = S.vnormals + 9 * (size_t)prim;
+        const float w = 1.f - u - v;
+        const f3 pn = mk(vn[0] * w + vn[3] * u + vn[6] * v, vn[1] * w + vn[4] * u + vn[7] * v,
+                         vn[2] * w + vn[5] * u + vn[8] * v);
+        if (dot3(pn, pn) != 0.f) N = pn;
+    }
+}
+
+// t of the reference's immediate re-hit of the surface a ray leaves from:
+// the origin is p + N * 1e-8, so a direction into the surface meets it again
+// at t = -1e-8 (g.N) / (g.d) when that is >= 1e-8 (lambert.cc:35,
+// ao/integrator.cc:38, and the t < EPSILON tests of the primitives).
+__device__ __forceinline__ float self_hit_t(f3 g, f3 N, f3 d) {
+    const float gd = dot3(g, d);
+    if (gd == 0.f) return 0.f;
+    const float r = -dot3(g, N) / gd;
+    return r >= 1.f ? 1e-8f * r : 0.f;
+}
+
+__device__ __forceinline__ void push_ray(Queue &out, uint32_t slot, f3 o, f3 d, uint32_t origin, float self_t,
+                                         uint32_t path) {
+    out.r0[slot] = make_float4(o.x, o.y, o.z, __uint_as_float(origin));
+    out.r1[slot] = make_float4(d.x, d.y, d.z, self_t);
+    out.path[slot] = path;
+}
+
+// Pathtracer (src/xtcore/integrator/pathtracer/integrator.cc:21-47). Path
+// state: T (product so far) and F (factor of the last hit); L = T * X where
+// X is the value of the current query: miss -> env, emissive -> F * Le,
+// continue -> F * f * X'. Materials: lambert.cc:30-40, phong.cc:35-55,
+// blinnphong.cc:34-61, emissive.cc:20-27.
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const JobDev j, const Queue in,
+                                                       const uint32_t *in_count, const float4 *hit, Queue out,
+                                                       uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false;
+        f3 no = mk(0, 0, 0), nd = mk(0, 0, 1);
+        uint32_t nprim = XT_NONE, path = 0;
+        float nself = 0.f;
+        if (i < n) {
+            path = in.path[i];
+            const float4 h = hit[i];
+            const float4 a = in.r0[i], b = in.r1[i];
+            const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+            const float4 s0 = ps0[path], s1 = ps1[path];
+            const f3 T = mk(s0.x, s0.y, s0.z), F = mk(s1.x, s1.y, s1.z);
+            const uint32_t key = __float_as_uint(s0.w), q = __float_as_uint(s1.w);
+            const uint32_t prim = __float_as_uint(h.y);
+            if (prim == XT_NONE) {
+                const f3 L = cmul(T, env_sample(S.env, d));
+                rad[path] = make_float4(L.x, L.y, L.z, 0.f);
+            } else {
+                f3 p, N, g;
+                uint32_t obj;
+                surface(S, prim, o, d, h.x, h.z, h.w, p, N, g, obj);
+                const uint32_t mi = S.obj_material[obj];
+                const XtMaterial m = S.materials[mi == XT_NONE ? 0 : mi];
+                if (m.type == XT_MAT_EMISSIVE) {
+                    const f3 L = cmul(cmul(T, F), mk(m.ke[0], m.ke[1], m.ke[2]));
+                    rad[path] = make_float4(L.x, L.y, L.z, 0.f);
+                } else if (q < j.rdepth) {
+                    Draws r{xt_stream(key, q), 0};
+                    f3 f, dir;
+                    if (m.type == XT_MAT_LAMBERT) {
+                        dir = norm3(smp_diffuse(N, r));
+                        f = cscale(mk(m.kd[0], m.kd[1], m.kd[2]), dot3(N, dir));
+                    } else {
+                        const float k = r.next();
+                        if (k > m.reflectance) {
+                            f = mk(m.kd[0], m.kd[1], m.kd[2]);
+                            dir = smp_diffuse(N, r);
+                        } else {
+                            f = mk(m.ks[0], m.ks[1], m.ks[2]);
+                            dir = smp_lobe(N, scl(d, -1.f), m.exponent, r);
+                            if (m.type == XT_MAT_BLINNPHONG) dir = norm3(dir);
+                        }
+                        f = cscale(f, 1.f);
+                    }
+                    const f3 T2 = cmul(cmul(T, F), f);
+                    ps0[path] = make_float4(T2.x, T2.y, T2.z, s0.w);
+                    ps1[path] = make_float4(f.x, f.y, f.z, __uint_as_float(q + 1));
+                    cont = true;
+                    no = p;
+                    nd = dir;
+                    nprim = prim;
+                    nself = self_hit_t(g, N, dir);
+                }
+                // q == rdepth with a continuing material: eval(0) == 0
+            }
+        }
+        const uint32_t slot = wave_append(cont, out_count);
+        if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
+    }
+}
+
+// AO, first query (src/xtcore/integrator/ao/integrator.cc:12-53): a hit
+// spawns one hemisphere ray; a miss scores w = 1.
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao1(const SceneView S, const JobDev j, const Queue in,
+                                                        const uint32_t *in_count, const float4 *hit, Queue out,
+                                                        uint32_t *out_count, const float4 *ps0, float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false;
+        f3 no = mk(0, 0, 0), nd = mk(0, 0, 1);
+        uint32_t nprim = XT_NONE, path = 0;
+        float nself = 0.f;
+        if (i < n) {
+            path = in.path[i];
+            const float4 h = hit[i];
+            const uint32_t prim = __float_as_uint(h.y);
+            if (prim == XT_NONE) {
+                rad[path] = make_float4(1.f, 1.f, 1.f, 0.f);
+            } else {
+                const float4 a = in.r0[i], b = in.r1[i];
+                const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+                f3 p, N, g;
+                uint32_t obj;
+                surface(S, prim, o, d, h.x, h.z, h.w, p, N, g, obj);
+                Draws r{xt_stream(__float_as_uint(ps0[path].w), 1), 0};
+                const f3 dir = smp_hemisphere(N, N, r);
+                cont = true;
+                no = p;
+                nd = dir;
+                nprim = prim;
+                nself = self_hit_t(g, N, dir);
+            }
+        }
+        const uint32_t slot = wave_append(cont, out_count);
+        if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
+    }
+}
+
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao2(const JobDev j, const Queue in, const uint32_t *in_count,
+                                                        const float4 *hit, float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = in.path[i];
+        const float4 h = hit[i];
+        float w = 1.f;
+        if (__float_as_uint(h.y) != XT_NONE) {
+            const float4 b = in.r1[i];
+            const float dist = h.x * sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
+            w = fminf(dist, 100.0f) / 100.0f;
+        }
+        rad[path] = make_float4(w, w, w, 0.f);
+    }
+}
+
+// Depth (src/xtcore/integrator/depth/integrator.cc:33-39): 1 / log of the
+// FP64 hit distance, recomputed from the FP64 camera ray with the
+// reference's predicate for the primitive found, rounded to float.
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_depth(const SceneView S, const Queue in,
+                                                          const uint32_t *in_count, const float4 *hit,
+                                                          const double *ray64, float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = in.path[i];
+        const float4 h = hit[i];
+        const uint32_t prim = __float_as_uint(h.y);
+        if (prim == XT_NONE) {
+            rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        const double *q = ray64 + 6 * (size_t)path;
+        const d3 o = {q[0], q[1], q[2]}, d = {q[3], q[4], q[5]};
+        double t;
+        if (prim >= XT_PLANE_BASE) {
+            t = plane64(o, d, S.planes[prim - XT_PLANE_BASE]);
+        } else if (XT_PRIM_TYPE(__float_as_uint(S.prims[3 * prim].w)) == XT_PRIM_TRIANGLE) {
+            double bc[3];
+            t = tri64(o, d, S.prims64[prim], bc);
+        } else {
+            t = sph64(o, d, S.prims64[prim]);
+        }
+        if (!(t > 0.0)) t = (double)h.x;
+        const d3 pt = dadd(o, dscl(d, t));
+        const double dist = dlen(dsub(o, pt));
+        const float depth = (float)(1. / log(dist));
+        rad[path] = make_float4(depth, 1.f, 0.f, 0.f);  // y = 1 marks a hit
+    }
+}
+
+// Stencil (src/xtcore/integrator/stencil/integrator.cc:28-30).
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_stencil(const Queue in, const uint32_t *in_count,
+                                                            const float4 *hit, float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = in.path[i];
+        const bool h = __float_as_uint(hit[i].y) != XT_NONE;
+        rad[path] = make_float4(h ? 1.f : 0.f, h ? 1.f : 0.f, 0.f, 0.f);
+    }
+}
+
+// --------------------------------------------------------------- accumulation
+
+// Per pixel, samples of this pass in sample order. REFERENCE reproduces the
+// float read-modify-write of each integrator (tile_t::read/write,
+// lib/nimg/color.cc:35-40 clamps); FP64 keeps a Neumaier sum per channel.
+static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
+                                                         const float4 *rad, float4 *accf, double2 *acc64) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < j.NT; q += stride) {
+        uint32_t x, y;
+        if (!slot_pixel(j, q, x, y)) continue;
+        const uint32_t pix = y * j.W + x;
+        float4 a = accf[pix];
+        double2 s0 = acc64[3 * (size_t)pix], s1 = acc64[3 * (size_t)pix + 1], s2 = acc64[3 * (size_t)pix + 2];
+        for (uint32_t sl = 0; sl < spp_pass; ++sl) {
+            if (pass_base + sl >= j.total) break;
+            const float4 v = rad[sl * j.NT + q];
+            if (j.accum == XT_ACCUM_REFERENCE) {
+                if (j.integrator == XT_INTEGRATOR_STENCIL) {
+                    if (v.x != 0.f) {
+                        const float na = (float)((double)a.w + j.weight);
+                        a = make_float4(1.f, 1.f, 1.f, fminf(fmaxf(na, 0.f), 1.f));
+                    }
+                } else if (j.integrator == XT_INTEGRATOR_DEPTH) {
+                    if (v.y != 0.f) {
+                        const float r = fmaxf((float)((double)a.x + (double)v.x * j.weight), 0.f);
+                        a = make_float4(r, r, r, 1.f);
+                    } else {
+                        a.w = 1.f;
+                    }
+                } else if (j.integrator == XT_INTEGRATOR_AO) {
+                    a.x = fmaxf((float)((double)a.x + (double)v.x * j.weight), 0.f);
+                    a.y = fmaxf((float)((double)a.y + (double)v.y * j.weight), 0.f);
+                    a.z = fmaxf((float)((double)a.z + (double)v.z * j.weight), 0.f);
+                    a.w = 1.f;
+                } else {
+                    a.x = fmaxf(a.x + fmaxf(v.x * j.weight_f, 0.f), 0.f);
+                    a.y = fmaxf(a.y + fmaxf(v.y * j.weight_f, 0.f), 0.f);
+                    a.z = fmaxf(a.z + fmaxf(v.z * j.weight_f, 0.f), 0.f);
+                    a.w = 1.f;
+                }
+            } else {
+                const double w = j.weight;
+                double xs[3] = {(double)v.x * w, (double)v.y * w, (double)v.z * w};
+                if (j.integrator == XT_INTEGRATOR_STENCIL) xs[1] = xs[2] = xs[0];
+                if (j.integrator == XT_INTEGRATOR_DEPTH && v.y == 0.f) xs[0] = xs[1] = xs[2] = 0.0;
+                if (j.integrator == XT_INTEGRATOR_DEPTH) xs[1] = xs[2] = xs[0];
+                double2 *acc[3] = {&s0, &s1, &s2};
+                for (int c = 0; c < 3; ++c) {
+                    const double sum = acc[c]->x, t = sum + xs[c];
+                    if (fabs(sum) >= fabs(xs[c])) acc[c]->y += (sum - t) + xs[c];
+                    else acc[c]->y += (xs[c] - t) + sum;
+                    acc[c]->x = t;
+                }
+            }
+        }
+        accf[pix] = a;
+        acc64[3 * (size_t)pix] = s0;
+        acc64[3 * (size_t)pix + 1] = s1;
+        acc64[3 * (size_t)pix + 2] = s2;
+    }
+}
+
+static __global__ __launch_bounds__(XT_BLOCK) void k_resolve(const JobDev j, const float4 *accf, const double2 *acc64,
+                                                      float4 *out) {
+    const uint32_t n = j.W * j.H;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        if (j.accum == XT_ACCUM_REFERENCE) {
+            out[p] = accf[p];
+            continue;
+        }
+        float c[3];
+        for (int k = 0; k < 3; ++k) {
+            const double2 s = acc64[3 * (size_t)p + k];
+            c[k] = fmaxf((float)(s.x + s.y), 0.f);
+        }
+        if (j.integrator == XT_INTEGRATOR_STENCIL) {
+            const float a = fminf(c[0], 1.f);
+            const float rgb = c[0] > 0.f ? 1.f : 0.f;
+            out[p] = make_float4(rgb, rgb, rgb, a);
+        } else {
+            out[p] = make_float4(c[0], c[1], c[2], 1.f);
+        }
+    }
+}
+
+// Generic-query helpers (xt_trace): pack SoA rays into the queue layout and
+// unpack hits.
+static __global__ void k_pack_rays(const float *ox, const float *oy, const float *oz, const float *dx, const float *dy,
+                            const float *dz, Queue q, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        q.r0[i] = make_float4(ox[i], oy[i], oz[i], __uint_as_float(XT_NONE));
+        q.r1[i] = make_float4(dx[i], dy[i], dz[i], 0.f);
+        q.path[i] = i;
+    }
+}
+
+static __global__ void k_unpack_hits(const float4 *hit, float *t, uint32_t *prim, float *u, float *v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 h = hit[i];
+        t[i] = h.x;
+        prim[i] = __float_as_uint(h.y);
+        u[i] = h.z;
+        v[i] = h.w;
+    }
+}

@@ -1,0 +1,39 @@
+// SAH kd-tree over the primitive soup. The reference has no kd-tree (its
+// README lists one, src/xtcore/bvh.cc is an empty stub), so the structure is
+// this project's own; only its closest-hit results are checked against the
+// reference. Binned SAH with an empty-space bonus; straddling primitives are
+// referenced from both children.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../xt_layout.h"
+
+namespace xt {
+
+struct Box32 {
+    float lo[3], hi[3];
+};
+
+struct KdParams {
+    int max_depth = 0;         // 0: 8 + 1.3 * log2(N)
+    int leaf_prims = 4;        // stop splitting at or below this many refs
+    float cost_traverse = 1.f;
+    float cost_intersect = 12.f;
+    float empty_bonus = 0.2f;  // cost reduction when one side is empty
+    int bins = 64;
+    int threads = 0;           // 0: hardware concurrency
+};
+
+struct KdTree {
+    std::vector<XtKdNode> nodes;  // nodes[0] is the root
+    std::vector<uint32_t> refs;   // leaf primitive references (soup indices)
+    Box32 bounds;                 // root box
+    int depth = 0;                // deepest leaf level
+    size_t leaves = 0;
+};
+
+void kdtree_build(const std::vector<Box32> &prim_boxes, const KdParams &p, KdTree &out);
+
+}  // namespace xt

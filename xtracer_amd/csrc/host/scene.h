@@ -1,0 +1,101 @@
+// Host-side scene model and .scn loader.
+//
+// The loader mirrors xtcore::io::scn::load (src/xtcore/parseutil.cc:697-822):
+// sections are visited camera -> geometry -> material -> texture -> object,
+// each in alphabetical group order, and every asset name is registered in a
+// first-seen order that stands in for the string-pool handles the reference
+// keys its asset maps with (src/xtcore/strpool.cc:27-34). Objects are kept
+// sorted by that order, which is the order Scene::intersection scans them in
+// (src/xtcore/scene.cc:191-216) and therefore the tie-break order of hits.
+#pragma once
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "xt_vec.h"
+
+namespace xt {
+
+enum LoadStatus { LOAD_OK = 0, LOAD_FAIL = 1, LOAD_PARSE = 2 };
+
+struct Camera {
+    std::string name;
+    uint64_t handle = 0;
+    dvec3 position, target{0, 0, 1}, up{0, 1, 0};
+    float fov = 0.f, aperture = 0.f, flength = 0.f;  // float members, perspective.h:22-26
+};
+
+enum GeomType : uint32_t { GEOM_PLANE = 0, GEOM_SPHERE = 1, GEOM_TRIANGLE = 2, GEOM_MESH = 3 };
+
+// Triangle soup of a mesh after modifiers; coordinates are float, as in
+// nmesh::attrib_t (lib/nmesh/structs.h:56-60).
+struct MeshData {
+    std::vector<float> v, n, uv;      // flat xyz / xyz / uv arrays
+    std::vector<int32_t> iv, in, it;  // 3 per triangle, -1 when absent
+    size_t tri_count() const { return iv.size() / 3; }
+};
+
+struct Geometry {
+    std::string name;
+    uint64_t handle = 0;
+    GeomType type = GEOM_PLANE;
+    dvec3 normal;          // plane
+    double offset = 0;     // plane "distance"
+    dvec3 center;          // sphere / point
+    double radius = 0;     // sphere (0 for a point)
+    dvec3 v[3];            // triangle
+    double tc[3][2] = {{0, 0}, {1, 0}, {0, 0}};  // triangle texcoords (triangle.cc:12-17 defaults)
+    MeshData mesh;         // mesh
+    double u_scale = 0, v_scale = 0;
+};
+
+enum MatType : uint32_t { MAT_LAMBERT = 0, MAT_PHONG = 1, MAT_BLINNPHONG = 2, MAT_EMISSIVE = 3, MAT_DIELECTRIC = 4 };
+
+// Constant-colour material parameters. A sampler that is absent reads as
+// (0,0,0) and a scalar as 0 (src/xtcore/material.cc:58-70).
+struct Material {
+    std::string name;
+    uint64_t handle = 0;
+    MatType type = MAT_LAMBERT;
+    float diffuse[3] = {0, 0, 0}, specular[3] = {0, 0, 0}, emissive[3] = {0, 0, 0};
+    float reflectance = 0, exponent = 0, ior = 0;
+    bool is_emissive = false;           // an "emissive" sampler or scalar exists
+    std::vector<std::string> unsupported;  // samplers whose type the GPU path does not evaluate
+};
+
+struct Object {
+    std::string name;
+    uint64_t handle = 0;
+    uint64_t geometry = 0, material = 0;  // name handles
+};
+
+enum EnvType : uint32_t { ENV_NONE = 0, ENV_GRADIENT = 1, ENV_COLOR = 2, ENV_UNSUPPORTED = 3 };
+struct Environment {
+    EnvType type = ENV_NONE;
+    float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+};
+
+struct Scene {
+    std::string source, title, description, version;
+    Environment env;
+    std::map<uint64_t, Camera> cameras;
+    std::map<uint64_t, Geometry> geometries;
+    std::map<uint64_t, Material> materials;
+    std::map<uint64_t, Object> objects;  // iteration order == scan order
+    std::map<std::string, uint64_t> names;
+    std::string error;
+
+    uint64_t intern(const std::string &name);  // first-seen order handle
+    const std::string *name_of(uint64_t handle) const;
+};
+
+// Loads a .scn file. `mods` are "group.group.prop:value" overrides applied
+// after parsing (parseutil.cc:715-738). Returns a LoadStatus.
+int load_scn(Scene &scene, const std::string &path, const std::vector<std::string> &mods);
+
+// Minimal Wavefront OBJ reader (positions, normals, texcoords, fan-triangulated
+// faces, 1-based and negative indices). Returns 0 on success.
+int read_obj(const std::string &path, MeshData &out, std::string *err);
+
+}  // namespace xt

@@ -1,0 +1,76 @@
+// Internal state behind an xt_dev handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device/xt_kernels.h"
+
+#define XT_TRACE_EVENTS 512
+
+struct DevScene {
+    SceneView view;
+    std::vector<void *> allocs;
+    uint64_t bytes = 0;
+    bool pt_supported = true;
+    bool missing_material = false;
+};
+
+struct WaveQueue {
+    float4 *r0 = nullptr, *r1 = nullptr;
+    uint32_t *path = nullptr;
+};
+
+struct WaveBuffers {
+    size_t cap = 0;
+    WaveQueue q[2];
+    float4 *hit = nullptr;
+    uint32_t *retest = nullptr;
+    float4 *ps0 = nullptr, *ps1 = nullptr, *rad = nullptr;
+    uint32_t *flags = nullptr;
+    Counters *cnt = nullptr;
+    double *ray64 = nullptr;
+    std::vector<void *> allocs;
+    uint64_t bytes = 0;
+};
+
+struct AccumBuffers {
+    size_t npix = 0;
+    float4 *accf = nullptr;
+    double2 *acc64 = nullptr;
+    float4 *out = nullptr;
+    std::vector<void *> allocs;
+    uint64_t bytes = 0;
+};
+
+struct ServeState;  // continuous-batching server (xt_serve.hip)
+
+struct xt_dev {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, used = 0;
+    std::map<uint32_t, DevScene> scenes;
+    uint32_t next_scene = 1;
+    WaveBuffers wave;
+    AccumBuffers accum;
+    Counters *host_counters = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t tev[XT_TRACE_EVENTS];
+    int trace_launches = 0;
+    std::vector<int> timed_pairs;
+    ServeState *serve = nullptr;
+    std::string err;
+};
+
+int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64);
+int xt_dev_ensure_accum(xt_dev *d, size_t npix);
+void xt_dev_free_wave(xt_dev *d);
+void xt_dev_free_accum(xt_dev *d);
+int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const double *ray64, uint32_t *flags,
+                    size_t cap, bool time_it, float *ms);
+void xt_serve_shutdown(xt_dev *d);

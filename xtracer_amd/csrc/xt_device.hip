@@ -1,0 +1,389 @@
+// Device half of the C ABI: resident scenes, closest-hit queries and the
+// wavefront render loop on one HIP device and stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+#include "device/xt_kernels.h"
+#include "xt_device.h"
+
+#define XT_TRY(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            d->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+            return XT_E_DEVICE;                                                       \
+        }                                                                             \
+    } while (0)
+
+#define XT_LAUNCHED()                                                                 \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess) {                                                       \
+            d->err = std::string("kernel launch: ") + hipGetErrorString(e_);          \
+            return XT_E_DEVICE;                                                       \
+        }                                                                             \
+    } while (0)
+
+namespace {
+
+template <class T>
+int dev_alloc(xt_dev *d, T **p, size_t count, std::vector<void *> *track, uint64_t *bytes) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    const size_t b = count * sizeof(T);
+    if (d->budget && d->used + b > d->budget) {
+        d->err = "HBM budget exceeded";
+        return XT_E_BUDGET;
+    }
+    hipError_t e = hipMalloc((void **)p, b);
+    if (e != hipSuccess) {
+        d->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return XT_E_DEVICE;
+    }
+    d->used += b;
+    if (bytes) *bytes += b;
+    if (track) track->push_back(*p);
+    return XT_OK;
+}
+
+template <class T>
+int upload(xt_dev *d, T **p, const std::vector<T> &v, std::vector<void *> *track, uint64_t *bytes) {
+    int rc = dev_alloc(d, p, v.size(), track, bytes);
+    if (rc) return rc;
+    if (!v.empty()) XT_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return XT_OK;
+}
+
+unsigned grid_for(size_t n, unsigned block, unsigned cap) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+void xt_dev_free_wave(xt_dev *d) {
+    for (void *p : d->wave.allocs) hipFree(p);
+    d->wave.allocs.clear();
+    d->used -= std::min<uint64_t>(d->used, d->wave.bytes);
+    d->wave = WaveBuffers();
+}
+
+void xt_dev_free_accum(xt_dev *d) {
+    for (void *p : d->accum.allocs) hipFree(p);
+    d->accum.allocs.clear();
+    d->used -= std::min<uint64_t>(d->used, d->accum.bytes);
+    d->accum = AccumBuffers();
+}
+
+int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64) {
+    if (d->wave.cap >= cap && (!need64 || d->wave.ray64)) return XT_OK;
+    cap = std::max(cap, d->wave.cap);
+    xt_dev_free_wave(d);
+    WaveBuffers &w = d->wave;
+    std::vector<void *> *t = &w.allocs;
+    uint64_t *b = &w.bytes;
+    int rc = 0;
+    for (int k = 0; k < 2 && !rc; ++k) {
+        rc = rc ? rc : dev_alloc(d, &w.q[k].r0, cap, t, b);
+        rc = rc ? rc : dev_alloc(d, &w.q[k].r1, cap, t, b);
+        rc = rc ? rc : dev_alloc(d, &w.q[k].path, cap, t, b);
+    }
+    rc = rc ? rc : dev_alloc(d, &w.hit, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.retest, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.ps0, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.ps1, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.rad, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.flags, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.cnt, 1, t, b);
+    if (!rc && need64) rc = dev_alloc(d, &w.ray64, 6 * cap, t, b);
+    if (rc) {
+        xt_dev_free_wave(d);
+        return rc;
+    }
+    w.cap = cap;
+    return XT_OK;
+}
+
+int xt_dev_ensure_accum(xt_dev *d, size_t npix) {
+    if (d->accum.npix >= npix) return XT_OK;
+    xt_dev_free_accum(d);
+    AccumBuffers &a = d->accum;
+    int rc = dev_alloc(d, &a.accf, npix, &a.allocs, &a.bytes);
+    rc = rc ? rc : dev_alloc(d, &a.acc64, 3 * npix, &a.allocs, &a.bytes);
+    rc = rc ? rc : dev_alloc(d, &a.out, npix, &a.allocs, &a.bytes);
+    if (rc) {
+        xt_dev_free_accum(d);
+        return rc;
+    }
+    a.npix = npix;
+    return XT_OK;
+}
+
+extern "C" {
+
+int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes) {
+    if (!out) return XT_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return XT_E_DEVICE;
+    if (hip_device < 0 || hip_device >= n) return XT_E_INVALID;
+    xt_dev *d = new xt_dev;
+    d->device = hip_device;
+    d->budget = hbm_budget_bytes;
+    if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete d;
+        return XT_E_DEVICE;
+    }
+    hipEventCreate(&d->ev0);
+    hipEventCreate(&d->ev1);
+    for (int i = 0; i < XT_TRACE_EVENTS; ++i) hipEventCreate(&d->tev[i]);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) d->n_cu = prop.multiProcessorCount;
+    *out = d;
+    return XT_OK;
+}
+
+void xt_dev_destroy(xt_dev *d) {
+    if (!d) return;
+    hipSetDevice(d->device);
+    hipStreamSynchronize(d->stream);
+    xt_serve_shutdown(d);
+    for (auto &kv : d->scenes)
+        for (void *p : kv.second.allocs) hipFree(p);
+    xt_dev_free_wave(d);
+    xt_dev_free_accum(d);
+    if (d->host_counters) hipHostFree(d->host_counters);
+    hipEventDestroy(d->ev0);
+    hipEventDestroy(d->ev1);
+    for (int i = 0; i < XT_TRACE_EVENTS; ++i) hipEventDestroy(d->tev[i]);
+    hipStreamDestroy(d->stream);
+    delete d;
+}
+
+const char *xt_last_error(const xt_dev *d) { return d ? d->err.c_str() : "null device"; }
+
+int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
+    if (!d || !s || !scene_id) return XT_E_INVALID;
+    if (!s->built) {
+        d->err = "scene not built";
+        return XT_E_STATE;
+    }
+    XT_TRY(hipSetDevice(d->device));
+    const xt::FlatScene &f = s->flat;
+    DevScene ds;
+    std::vector<void *> *t = &ds.allocs;
+    uint64_t *b = &ds.bytes;
+    std::vector<uint32_t> obj_mat(f.objects.size());
+    ds.missing_material = false;
+    for (size_t i = 0; i < f.objects.size(); ++i) {
+        obj_mat[i] = f.objects[i].material;
+        if (f.objects[i].material == XT_NO_MATERIAL && f.objects[i].geom_type != 0xffffffffu) ds.missing_material = true;
+    }
+    ds.pt_supported = f.env_supported && !ds.missing_material;
+    for (size_t i = 0; i < f.objects.size(); ++i) {
+        uint32_t m = f.objects[i].material;
+        if (m != XT_NO_MATERIAL && !f.material_supported[m]) ds.pt_supported = false;
+    }
+    XtPrim *prims = nullptr;
+    XtPrim64 *p64 = nullptr;
+    XtPlane *planes = nullptr;
+    XtKdNode *kd = nullptr;
+    uint32_t *kdr = nullptr, *octr = nullptr, *om = nullptr;
+    XtOctNode *oct = nullptr;
+    XtMaterial *mats = nullptr;
+    float *vn = nullptr;
+    std::vector<XtMaterial> mv = f.materials;
+    if (mv.empty()) mv.push_back(XtMaterial{});
+    int rc = upload(d, &prims, f.prims, t, b);
+    rc = rc ? rc : upload(d, &p64, f.prims64, t, b);
+    rc = rc ? rc : upload(d, &planes, f.planes, t, b);
+    rc = rc ? rc : upload(d, &kd, f.kd.nodes, t, b);
+    rc = rc ? rc : upload(d, &kdr, f.kd.refs, t, b);
+    rc = rc ? rc : upload(d, &oct, f.oct_nodes, t, b);
+    rc = rc ? rc : upload(d, &octr, f.oct_refs, t, b);
+    rc = rc ? rc : upload(d, &om, obj_mat, t, b);
+    rc = rc ? rc : upload(d, &mats, mv, t, b);
+    if (!rc && !f.vnormals.empty()) rc = upload(d, &vn, f.vnormals, t, b);
+    if (rc) {
+        for (void *p : ds.allocs) hipFree(p);
+        d->used -= std::min<uint64_t>(d->used, ds.bytes);
+        return rc;
+    }
+    SceneView &v = ds.view;
+    std::memset(&v, 0, sizeof v);
+    v.prims = (const float4 *)prims;
+    v.prims64 = p64;
+    v.planes = planes;
+    v.kd = kd;
+    v.kd_refs = kdr;
+    v.oct = oct;
+    v.oct_refs = octr;
+    v.obj_material = om;
+    v.materials = mats;
+    v.vnormals = vn;
+    v.env = f.env;
+    for (int a = 0; a < 3; ++a) {
+        v.kb_lo[a] = f.kd.bounds.lo[a];
+        v.kb_hi[a] = f.kd.bounds.hi[a];
+    }
+    v.n_prims = (uint32_t)f.prims.size();
+    v.n_planes = (uint32_t)f.planes.size();
+    v.n_oct = (uint32_t)f.oct_nodes.size();
+    if (f.kd.nodes.empty()) {  // empty soup: a root leaf with no refs
+        XtKdNode leaf;
+        leaf.word = 3u;
+        leaf.count = 0;
+        XT_TRY(hipMemcpy(kd, &leaf, sizeof leaf, hipMemcpyHostToDevice));
+    }
+    const uint32_t id = d->next_scene++;
+    d->scenes[id] = std::move(ds);
+    *scene_id = id;
+    return XT_OK;
+}
+
+int xt_scene_release(xt_dev *d, uint32_t scene_id) {
+    if (!d) return XT_E_INVALID;
+    auto it = d->scenes.find(scene_id);
+    if (it == d->scenes.end()) return XT_E_NOTFOUND;
+    hipSetDevice(d->device);
+    hipStreamSynchronize(d->stream);
+    for (void *p : it->second.allocs) hipFree(p);
+    d->used -= std::min<uint64_t>(d->used, it->second.bytes);
+    d->scenes.erase(it);
+    return XT_OK;
+}
+
+}  // extern "C"
+
+// Launches trace + FP64 re-test over queue `qi` (count in cnt->n[qi]).
+int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const double *ray64, uint32_t *flags,
+                    size_t cap, bool time_it, float *ms) {
+    WaveBuffers &w = d->wave;
+    Queue q{w.q[qi].r0, w.q[qi].r1, w.q[qi].path};
+    const bool timed = time_it && d->trace_launches < XT_TRACE_EVENTS - 1;
+    if (timed) hipEventRecord(d->tev[d->trace_launches], d->stream);
+    if (accel == XT_ACCEL_OCTREE) {
+        hipLaunchKernelGGL(k_trace<XT_ACCEL_OCTREE>, dim3(grid_for(cap, XT_OCT_BLOCK, 16384)), dim3(XT_OCT_BLOCK), 0,
+                           d->stream, ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
+    } else {
+        hipLaunchKernelGGL(k_trace<XT_ACCEL_KDTREE>, dim3(grid_for(cap, XT_BLOCK, 4096)), dim3(XT_BLOCK), 0,
+                           d->stream, ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
+    }
+    if (timed) {
+        hipEventRecord(d->tev[d->trace_launches + 1], d->stream);
+        d->timed_pairs.push_back(d->trace_launches);
+        d->trace_launches += 2;
+    }
+    XT_LAUNCHED();
+    if (accel == XT_ACCEL_OCTREE) {
+        hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(cap / 64 + 1, XT_OCT_BLOCK, 2048)),
+                           dim3(XT_OCT_BLOCK), 0, d->stream, ds.view, q, w.hit, w.retest, w.cnt, ray64, flags);
+    } else {
+        hipLaunchKernelGGL(k_retest<XT_ACCEL_KDTREE>, dim3(grid_for(cap / 64 + 1, XT_BLOCK, 1024)), dim3(XT_BLOCK),
+                           0, d->stream, ds.view, q, w.hit, w.retest, w.cnt, ray64, flags);
+    }
+    XT_LAUNCHED();
+    (void)ms;
+    return XT_OK;
+}
+
+extern "C" int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const void *ray0, const void *ray1,
+                               void *hit, uint32_t *flags, uint64_t n, float *kernel_ms) {
+    if (!d || !ray0 || !ray1 || !hit) return XT_E_INVALID;
+    if (n >= (1ull << 31)) return XT_E_INVALID;
+    auto it = d->scenes.find(scene_id);
+    if (it == d->scenes.end()) return XT_E_NOTFOUND;
+    if (accel != XT_ACCEL_KDTREE && accel != XT_ACCEL_OCTREE) return XT_E_INVALID;
+    XT_TRY(hipSetDevice(d->device));
+    int rc = xt_dev_ensure_wave(d, std::max<size_t>(n, 1), false);
+    if (rc) return rc;
+    WaveBuffers &w = d->wave;
+    // the caller's buffers are used in place as queue 0 / the hit array
+    float4 *save0 = w.q[0].r0, *save1 = w.q[0].r1, *saveh = w.hit;
+    w.q[0].r0 = (float4 *)ray0;
+    w.q[0].r1 = (float4 *)ray1;
+    w.hit = (float4 *)hit;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x800000ffu);
+    // paths are only read by the FP64 re-test when FP64 rays are given; set
+    // the count with a one-thread kernel instead of a memset (graph-safe)
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, w.cnt, 0u, (uint32_t)n);
+    if (flags) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, flags, 0u, (size_t)n);
+    d->trace_launches = 0;
+    d->timed_pairs.clear();
+    rc = xt_launch_trace(d, it->second, accel, 0, nullptr, flags, (size_t)n, kernel_ms != nullptr, nullptr);
+    w.q[0].r0 = save0;
+    w.q[0].r1 = save1;
+    w.hit = saveh;
+    if (rc) return rc;
+    XT_TRY(hipStreamSynchronize(d->stream));
+    if (kernel_ms && !d->timed_pairs.empty()) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, d->tev[d->timed_pairs[0]], d->tev[d->timed_pairs[0] + 1]);
+        *kernel_ms = ms;
+    }
+    return XT_OK;
+}
+
+extern "C" int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *rays, xt_hits *hits, uint64_t n) {
+    if (!d || !rays || !hits) return XT_E_INVALID;
+    if (n == 0) return XT_OK;
+    if (n >= (1ull << 31)) return XT_E_INVALID;
+    auto it = d->scenes.find(scene_id);
+    if (it == d->scenes.end()) return XT_E_NOTFOUND;
+    XT_TRY(hipSetDevice(d->device));
+    int rc = xt_dev_ensure_wave(d, n, false);
+    if (rc) return rc;
+    std::vector<void *> tmp;
+    float *soa[6];
+    const float *src[6] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz};
+    for (int k = 0; k < 6; ++k) {
+        if (!src[k]) return XT_E_INVALID;
+        rc = dev_alloc(d, &soa[k], n, &tmp, nullptr);
+        if (rc) break;
+        XT_TRY(hipMemcpyAsync(soa[k], src[k], n * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    }
+    auto release = [&]() {
+        hipStreamSynchronize(d->stream);
+        for (void *p : tmp) {
+            hipFree(p);
+        }
+        d->used -= std::min<uint64_t>(d->used, (uint64_t)tmp.size() * n * sizeof(float));
+    };
+    if (rc) {
+        release();
+        return rc;
+    }
+    WaveBuffers &w = d->wave;
+    Queue q{w.q[0].r0, w.q[0].r1, w.q[0].path};
+    hipLaunchKernelGGL(k_pack_rays, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, soa[0], soa[1], soa[2],
+                       soa[3], soa[4], soa[5], q, (uint32_t)n);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x800000ffu);
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, w.cnt, 0u, (uint32_t)n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, w.flags, 0u, (size_t)n);
+    d->trace_launches = 0;
+    d->timed_pairs.clear();
+    rc = xt_launch_trace(d, it->second, accel, 0, nullptr, w.flags, n, false, nullptr);
+    if (rc) {
+        release();
+        return rc;
+    }
+    hipLaunchKernelGGL(k_unpack_hits, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, w.hit, soa[0],
+                       (uint32_t *)soa[1], soa[2], soa[3], (uint32_t)n);
+    XT_LAUNCHED();
+    float *dst[4] = {hits->t, (float *)hits->prim, hits->u, hits->v};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) XT_TRY(hipMemcpyAsync(dst[k], soa[k], n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    if (hits->flags) XT_TRY(hipMemcpyAsync(hits->flags, w.flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    XT_TRY(hipStreamSynchronize(d->stream));
+    release();
+    return XT_OK;
+}

@@ -1,0 +1,248 @@
+// One-job-per-launch rendering (xt_render / xt_render_device): the wavefront
+// loop that replaces IIntegrator::render()'s OpenMP tile loop
+// (src/xtcore/integrator.cc:29-53) for the stencil, depth, ao and
+// pathtracer integrators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "xt_device.h"
+#include "xt_render.h"
+
+#define XT_TRY(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            d->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+            return XT_E_DEVICE;                                                       \
+        }                                                                             \
+    } while (0)
+
+#define XT_LAUNCHED()                                                                 \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess) {                                                       \
+            d->err = std::string("kernel launch: ") + hipGetErrorString(e_);          \
+            return XT_E_DEVICE;                                                       \
+        }                                                                             \
+    } while (0)
+
+namespace {
+unsigned grid_for(size_t n, unsigned block, unsigned cap) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+}  // namespace
+
+int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_out) {
+    if (!job) return XT_E_INVALID;
+    auto it = d->scenes.find(job->scene_id);
+    if (it == d->scenes.end()) {
+        d->err = "unknown scene id";
+        return XT_E_NOTFOUND;
+    }
+    if (job->width == 0 || job->height == 0 || job->width > 16384 || job->height > 16384) {
+        d->err = "bad resolution";
+        return XT_E_INVALID;
+    }
+    if (job->integrator > XT_INTEGRATOR_PATHTRACER || job->accel > XT_ACCEL_OCTREE || job->aa_mode > XT_AA_RANDOM ||
+        job->accum > XT_ACCUM_FP64) {
+        d->err = "bad job parameter";
+        return XT_E_INVALID;
+    }
+    if (job->integrator == XT_INTEGRATOR_PATHTRACER && !it->second.pt_supported) {
+        d->err = "scene has materials or an environment the GPU pathtracer does not evaluate";
+        return XT_E_SCENE;
+    }
+    std::memset(&j, 0, sizeof j);
+    j.W = job->width;
+    j.H = job->height;
+    j.tiles_x = (j.W + 7) / 8;
+    const uint32_t tiles_y = (j.H + 7) / 8;
+    j.NT = j.tiles_x * tiles_y * 64;
+    j.spp = job->samples;
+    j.aa = job->aa;
+    j.aa_mode = job->aa_mode;
+    // antialiasing::produce (aa.cc:51-77): GRID level L gives L*L positions
+    // of weight 1/L^2 (level 0: none), RANDOM gives L of weight 1/L; each is
+    // repeated `samples` times with its weight divided by `samples`.
+    float aw;
+    if (j.aa_mode == XT_AA_GRID) {
+        j.n_aa = j.aa == 0 ? 0 : j.aa * j.aa;
+        aw = j.aa <= 1 ? 1.f : 1.f / (float)(j.aa * j.aa);
+        j.aa_sub = j.aa ? 1.f / (float)j.aa : 1.f;
+        j.aa_off = j.aa_sub * 0.5f;
+    } else {
+        j.n_aa = j.aa;
+        aw = j.aa ? 1.f / (float)j.aa : 1.f;
+    }
+    j.total = j.n_aa * j.spp;
+    j.weight = (double)aw;
+    if (j.spp) j.weight *= 1. / (double)j.spp;
+    j.weight_f = (float)j.weight;
+    j.rdepth = job->rdepth;
+    j.integrator = job->integrator;
+    j.seed = job->seed;
+    j.job_key = job->job_key;
+    j.accum = job->accum;
+    if (j.accum == XT_ACCUM_DEFAULT)
+        j.accum = (j.integrator == XT_INTEGRATOR_STENCIL || j.integrator == XT_INTEGRATOR_DEPTH) ? XT_ACCUM_REFERENCE
+                                                                                               : XT_ACCUM_FP64;
+    j.wf = (float)j.W;
+    j.hf = (float)j.H;
+    j.cam = job->camera;
+    *ds_out = &it->second;
+    return XT_OK;
+}
+
+uint32_t xt_rounds(const JobDev &j) {
+    switch (j.integrator) {
+        case XT_INTEGRATOR_AO: return 2;
+        case XT_INTEGRATOR_PATHTRACER: return j.rdepth;
+        default: return 1;
+    }
+}
+
+uint32_t xt_auto_spp_pass(const JobDev &j, uint32_t requested) {
+    if (j.total == 0) return 1;
+    if (requested) return std::min(requested, j.total);
+    const uint64_t target = 8u << 20;  // about 8M paths in flight per pass
+    uint64_t s = std::max<uint64_t>(1, target / j.NT);
+    return (uint32_t)std::min<uint64_t>(s, j.total);
+}
+
+// Launches one shading step for query round r over queue qi into queue qi^1.
+int xt_launch_shade(xt_dev *d, const DevScene &ds, const JobDev &j, uint32_t r, int qi, size_t cap) {
+    WaveBuffers &w = d->wave;
+    Queue in{w.q[qi].r0, w.q[qi].r1, w.q[qi].path};
+    Queue out{w.q[qi ^ 1].r0, w.q[qi ^ 1].r1, w.q[qi ^ 1].path};
+    const unsigned g = grid_for(cap, XT_BLOCK, 8192);
+    switch (j.integrator) {
+        case XT_INTEGRATOR_STENCIL:
+            hipLaunchKernelGGL(k_shade_stencil, dim3(g), dim3(XT_BLOCK), 0, d->stream, in, &w.cnt->n[qi], w.hit, w.rad);
+            break;
+        case XT_INTEGRATOR_DEPTH:
+            hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, &w.cnt->n[qi], w.hit,
+                               w.ray64, w.rad);
+            break;
+        case XT_INTEGRATOR_AO:
+            if (r == 0)
+                hipLaunchKernelGGL(k_shade_ao1, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, j, in, &w.cnt->n[qi],
+                                   w.hit, out, &w.cnt->n[qi ^ 1], w.ps0, w.rad);
+            else
+                hipLaunchKernelGGL(k_shade_ao2, dim3(g), dim3(XT_BLOCK), 0, d->stream, j, in, &w.cnt->n[qi], w.hit,
+                                   w.rad);
+            break;
+        default:
+            hipLaunchKernelGGL(k_shade_pt, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, j, in, &w.cnt->n[qi], w.hit,
+                               out, &w.cnt->n[qi ^ 1], w.ps0, w.ps1, w.rad);
+    }
+    XT_LAUNCHED();
+    return XT_OK;
+}
+
+static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats *stats) {
+    JobDev j;
+    const DevScene *ds = nullptr;
+    int rc = xt_job_check(d, job, j, &ds);
+    if (rc) return rc;
+    XT_TRY(hipSetDevice(d->device));
+    const uint32_t spp_pass = xt_auto_spp_pass(j, job->samples_per_pass);
+    const size_t cap = (size_t)spp_pass * j.NT;
+    if (cap >= (1ull << 31)) {
+        d->err = "pass too large";
+        return XT_E_INVALID;
+    }
+    const bool need64 = j.integrator == XT_INTEGRATOR_DEPTH;
+    rc = xt_dev_ensure_wave(d, cap, need64);
+    if (rc) return rc;
+    rc = xt_dev_ensure_accum(d, (size_t)j.W * j.H);
+    if (rc) return rc;
+    if (!d->host_counters) XT_TRY(hipHostMalloc((void **)&d->host_counters, sizeof(Counters), hipHostMallocDefault));
+    WaveBuffers &w = d->wave;
+    AccumBuffers &a = d->accum;
+    const size_t npix = (size_t)j.W * j.H;
+    const unsigned gfill = grid_for(npix * 4, 256, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(gfill), dim3(256), 0, d->stream, (uint32_t *)a.accf, 0u, npix * 4);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(npix * 12, 256, 8192)), dim3(256), 0, d->stream,
+                       (uint32_t *)a.acc64, 0u, npix * 12);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x800000ffu);
+    XT_LAUNCHED();
+    d->trace_launches = 0;
+    d->timed_pairs.clear();
+    const bool time_trace = stats != nullptr;
+    XT_TRY(hipEventRecord(d->ev0, d->stream));
+    const uint32_t rounds = xt_rounds(j);
+    uint32_t passes = 0;
+    for (uint32_t base = 0; base < j.total; base += spp_pass) {
+        ++passes;
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x7u);
+        Queue q0{w.q[0].r0, w.q[0].r1, w.q[0].path};
+        hipLaunchKernelGGL(k_camera, dim3(grid_for(cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
+                           spp_pass, q0, &w.cnt->n[0], w.ps0, w.ps1, w.rad, need64 ? w.ray64 : nullptr);
+        XT_LAUNCHED();
+        int qi = 0;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, (1u << (qi ^ 1)) | (1u << CNT_RETEST));
+            rc = xt_launch_trace(d, *ds, (int)job->accel, qi, need64 && r == 0 ? w.ray64 : nullptr, nullptr, cap,
+                                 time_trace, nullptr);
+            if (rc) return rc;
+            rc = xt_launch_shade(d, *ds, j, r, qi, cap);
+            if (rc) return rc;
+            qi ^= 1;
+            // stop early once every path of the pass has ended
+            if (rounds > 2 && r + 1 < rounds && (r % 2 == 1)) {
+                XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+                XT_TRY(hipStreamSynchronize(d->stream));
+                if (d->host_counters->n[qi] == 0) break;
+            }
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(j.NT, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
+                           spp_pass, w.rad, a.accf, a.acc64);
+        XT_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, a.accf,
+                       a.acc64, (float4 *)d_out);
+    XT_LAUNCHED();
+    XT_TRY(hipEventRecord(d->ev1, d->stream));
+    XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+    XT_TRY(hipStreamSynchronize(d->stream));
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, d->ev0, d->ev1);
+        stats->ms_total = ms;
+        stats->queries = d->host_counters->queries;
+        stats->retests = d->host_counters->retests;
+        stats->samples = (uint64_t)j.total * j.W * j.H;
+        stats->passes = passes;
+        double tsum = 0;
+        for (int k : d->timed_pairs) {
+            float t = 0.f;
+            hipEventElapsedTime(&t, d->tev[k], d->tev[k + 1]);
+            tsum += t;
+        }
+        stats->ms_trace = tsum;
+        stats->trace_launches = (uint32_t)d->timed_pairs.size();
+    }
+    return XT_OK;
+}
+
+extern "C" int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_stats *stats) {
+    if (!d || !job || !d_rgba_out) return XT_E_INVALID;
+    return render_core(d, job, d_rgba_out, stats);
+}
+
+extern "C" int xt_render(xt_dev *d, const xt_job *job, float *rgba_out, xt_job_stats *stats) {
+    if (!d || !job || !rgba_out) return XT_E_INVALID;
+    const size_t npix = (size_t)job->width * job->height;
+    int rc = xt_dev_ensure_accum(d, npix);
+    if (rc) return rc;
+    rc = render_core(d, job, (float *)d->accum.out, stats);
+    if (rc) return rc;
+    XT_TRY(hipMemcpy(rgba_out, d->accum.out, npix * sizeof(float4), hipMemcpyDeviceToHost));
+    return XT_OK;
+}
