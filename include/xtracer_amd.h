@@ -109,6 +109,10 @@ int xt_scene_get_object_params(const xt_scene *s, uint32_t index, double out[9])
  * XT_E_NOTFOUND when no triangle carries vertex normals. */
 int xt_scene_get_vnormals(const xt_scene *s, float *out);
 
+/* The kd-tree as uploaded: n_kd_nodes * 2 words (XtKdNode), n_kd_refs
+ * primitive references and the root box (lo xyz, hi xyz). */
+int xt_scene_get_kdtree(const xt_scene *s, uint32_t *nodes, uint32_t *refs, float bounds[6]);
+
 /* Per-mesh reference octree statistics (node count, deepest level). */
 int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint64_t *nodes, uint64_t *max_level,
                          uint64_t *triangles);

@@ -44,6 +44,8 @@ def lib():
         L.or_render.restype = ctypes.c_uint64
         L.or_render.argtypes = [P, ctypes.POINTER(OrJob), P, P, P]
         L.or_mesh_octree.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), P]
+        L.or_sample.restype = ctypes.c_uint64
+        L.or_sample.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, P]
         _lib = L
     return _lib
 
@@ -139,6 +141,16 @@ class OracleScene:
                 lib().or_free(self._h)
         except Exception:
             pass
+
+
+def sample(kind, normal, count, seed=0, direction=(0.0, 0.0, 1.0), exponent=1.0):
+    """Directions from the oracle's hemisphere / diffuse / lobe samplers."""
+    kinds = {"hemisphere": 0, "diffuse": 1, "lobe": 2}
+    n = np.ascontiguousarray(normal, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    out = np.zeros((count, 3), np.float64)
+    used = lib().or_sample(kinds[kind], _p(n), _p(d), exponent, seed, count, _p(out))
+    return out, used
 
 
 def ref_available():

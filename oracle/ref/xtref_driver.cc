@@ -4,9 +4,12 @@
 // nothing in the product links it. Modelled on the GUI's render driver
 // (src/frontend/gui/workspace.cc:55-81), since the reference CLI cannot
 // render (its integrator selection is commented out).
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+
+#include <omp.h>
 
 #include <xtcore/context.h>
 #include <xtcore/integrator.h>
@@ -23,6 +26,24 @@ using namespace xtcore;
 
 namespace {
 bool g_pool = false;
+// Per-thread query counters (one cache line each), bumped by the link-time
+// wrapper of Scene::intersection below.
+struct alignas(64) Counter {
+    uint64_t n;
+};
+Counter g_queries[512];
+}  // namespace
+
+// The Makefile links with --wrap on the mangled Scene::intersection symbol,
+// so every call the integrators make lands here first; the reference's own
+// code is unchanged.
+#define XR_ISECT _ZN6xtcore5Scene12intersectionERKNS_3RayERNS_12hit_record_tE
+#define XR_CAT2(a, b) a##b
+#define XR_CAT(a, b) XR_CAT2(a, b)
+extern "C" bool XR_CAT(__real_, XR_ISECT)(void *self, const void *ray, void *hit);
+extern "C" bool XR_CAT(__wrap_, XR_ISECT)(void *self, const void *ray, void *hit) {
+    g_queries[omp_get_thread_num() & 511].n++;
+    return XR_CAT(__real_, XR_ISECT)(self, ray, hit);
 }
 
 extern "C" {
@@ -182,6 +203,46 @@ int xr_render(void *h, int integrator, int width, int height, int samples, int a
             o[2] = c.b();
             o[3] = c.a();
         }
+    delete in;
+    return 0;
+}
+
+// Renders only every `stride`-th tile (row-major order) and returns the
+// wall time of render() and the number of Scene::intersection calls it made.
+int xr_render_tiles(void *h, int integrator, int width, int height, int samples, int aa, int rdepth, int threads,
+                    unsigned seed, int stride, double *seconds, uint64_t *queries) {
+    render::context_t *ctx = (render::context_t *)h;
+    render::IIntegrator *in = nullptr;
+    switch (integrator) {
+        case 0: in = new integrator::stencil::Integrator; break;
+        case 1: in = new integrator::depth::Integrator; break;
+        case 2: in = new integrator::ao::Integrator; break;
+        case 3: in = new integrator::pathtracer::Integrator; break;
+        default: return 1;
+    }
+    render::params_t p;
+    p.width = width;
+    p.height = height;
+    p.samples = samples;
+    p.aa = aa;
+    p.rdepth = rdepth;
+    p.threads = threads;
+    p.camera = ctx->scene.m_cameras.empty() ? 0 : ctx->scene.m_cameras.begin()->first;
+    ctx->params = p;
+    ctx->init();
+    render::Tileset keep;
+    for (size_t i = 0; i < ctx->tiles.size(); i += (size_t)(stride > 0 ? stride : 1)) keep.push_back(ctx->tiles[i]);
+    ctx->tiles.swap(keep);
+    in->setup(*ctx);
+    srand(seed);
+    for (auto &c : g_queries) c.n = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    in->render();
+    auto t1 = std::chrono::steady_clock::now();
+    uint64_t q = 0;
+    for (auto &c : g_queries) q += c.n;
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (queries) *queries = q;
     delete in;
     return 0;
 }

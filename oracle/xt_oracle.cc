@@ -520,6 +520,24 @@ struct or_scene_desc {
     const float *env;            // a[3], b[3]
 };
 
+// Sampler hooks for distribution tests: kind 0 hemisphere(n, n), 1 diffuse(n),
+// 2 lobe(n, dir, e). out: 3 doubles per sample; draws of sample i come from
+// stream (seed, i). Returns the total number of draws consumed.
+uint64_t or_sample(int kind, const double *n, const double *dir, double e, uint32_t seed, uint64_t count,
+                   double *out) {
+    uint64_t used = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        Draws r{stream_of(path_key(seed, 0, (uint32_t)i, 0), 1), 0};
+        V nn = mk(n[0], n[1], n[2]);
+        V v = kind == 0 ? hemisphere(nn, nn, r) : kind == 1 ? diffuse(nn, r) : lobe(nn, mk(dir[0], dir[1], dir[2]), e, r);
+        out[3 * i] = v.x;
+        out[3 * i + 1] = v.y;
+        out[3 * i + 2] = v.z;
+        used += r.i;
+    }
+    return used;
+}
+
 void *or_build(const or_scene_desc *in) {
     Scene *s = new Scene;
     for (uint32_t i = 0; i < in->n_objects; ++i) {

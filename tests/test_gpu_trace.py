@@ -81,4 +81,6 @@ def test_kdtree_octree_agree(device):
     a = device.trace(sid, o, d, xa.ACCEL_KDTREE)
     b = device.trace(sid, o, d, xa.ACCEL_OCTREE)
     assert np.array_equal(a[1], b[1])
-    np.testing.assert_array_equal(a[0], b[0])
+    # t may differ by FP32 rounding where one traversal sent the ray to the
+    # FP64 re-test and the other did not
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-6)

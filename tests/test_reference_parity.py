@@ -1,0 +1,153 @@
+"""The loader and the oracle against the REAL reference (oracle/_ref, compiled
+from /root/reference by oracle/ref/Makefile). Runs only where that checker is
+built; each reference scene is handled in a fresh process because the
+reference's string pool is process-global and its handle order (the scene
+scan order) depends on every name interned before."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_SCENES = "/root/reference/scene"
+SELF_CONTAINED = ["2.0", "2.1", "2.1.0", "2.1.1", "2.2", "5.2", "5.3", "5.6", "6.2"]
+
+needs_ref = pytest.mark.skipif(not (O.ref_available() and os.path.isdir(REF_SCENES)),
+                               reason="reference checker (oracle/_ref) or /root/reference not present")
+
+_CHILD = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import xtracer_amd as xa
+from oracle import oracle as O
+path = {path!r}
+s = xa.Scene(path)
+r = O.RefScene(path)
+po, ro = s.objects(), r.objects()
+res = {{"names": [p["name"] for p in po] == [q["name"] for q in ro],
+       "kinds": [(p["geom_type"] if p["geom_type"] != 0xffffffff else -1) for p in po] == [q["kind"] for q in ro]}}
+res["params"] = all(np.array_equal(s.object_params(i)[:(9 if q["kind"] == 2 else 4)],
+                                   q["param"][:(9 if q["kind"] == 2 else 4)])
+                    for i, q in enumerate(ro) if q["kind"] in (0, 1, 2))
+orc = O.OracleScene(s)
+rng = np.random.default_rng(5)
+n = 20000
+o = rng.uniform(-12, 12, (n, 3))
+d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+t1, ob1, _, p1, n1 = orc.trace(o, d)
+t2, ob2, p2, n2 = r.trace(o, d)
+res["trace"] = bool(np.array_equal(t1, t2) and np.array_equal(ob1, ob2) and np.array_equal(p1[ob1 >= 0], p2[ob2 >= 0])
+                    and np.array_equal(n1[ob1 >= 0], n2[ob2 >= 0]))
+res["hits"] = int((ob1 >= 0).sum())
+print(json.dumps(res))
+"""
+
+
+def _child(path):
+    out = subprocess.check_output([sys.executable, "-c", _CHILD.format(root=ROOT, path=path)], timeout=300)
+    return json.loads(out.decode().strip().splitlines()[-1])
+
+
+@needs_ref
+@pytest.mark.parametrize("name", SELF_CONTAINED)
+def test_loader_and_trace_match_reference(name):
+    r = _child(os.path.join(REF_SCENES, name + ".scn"))
+    assert r["names"] and r["kinds"] and r["params"], r
+    assert r["trace"], r
+    assert r["hits"] > 1000
+
+
+def _synthetic(kind, **kw):
+    from xtracer_amd import scenes
+    return scenes.SCENES[kind][0](**kw), scenes.camera_dict(**scenes.SCENES[kind][1])
+
+
+_RENDER_CHILD = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import xtracer_amd as xa
+from oracle import oracle as O
+s = xa.Scene({path!r})
+cam = json.loads({cam!r})
+ref = O.RefScene({path!r}).render({integ}, {w}, {h}, samples={spp}, threads=1)
+mine, _ = O.OracleScene(s).render({integ}, {w}, {h}, cam, samples={spp})
+bad = int((ref.view(np.uint32) != mine.view(np.uint32)).any(axis=2).sum())
+print(json.dumps({{"bad": bad, "nonzero": int((ref[..., 3] > 0).sum())}}))
+"""
+
+
+@needs_ref
+@pytest.mark.parametrize("integrator", [0, 1])
+@pytest.mark.parametrize("kind,kw", [("spheres", {}), ("icosphere", {"subdiv": 3}), ("terrain", {"n": 21})])
+def test_oracle_deterministic_renders_bit_exact(kind, kw, integrator):
+    """stencil and depth use no random numbers with a pinhole camera and GRID
+    aa=1, so the oracle's image must equal the reference's bit for bit."""
+    path, cam = _synthetic(kind, **kw)
+    if cam["flength"] > 0:
+        cam = dict(cam, flength=0.0, aperture=0.0)
+        pytest.skip("thin-lens scene: primary rays use rand()")
+    code = _RENDER_CHILD.format(root=ROOT, path=path, cam=json.dumps(cam), integ=integrator, w=48, h=32, spp=4)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=300).decode().splitlines()[-1])
+    assert out["bad"] == 0, out
+    assert out["nonzero"] > 100
+
+
+@needs_ref
+def test_octree_choice_matches_reference():
+    """Octree<Triangle>::intersection of the real reference picks the same
+    triangle as the oracle's octree (including the pre-order tie rule)."""
+    import xtracer_amd as xa
+    path, _ = _synthetic("icosphere", subdiv=4)
+    s = xa.Scene(path)
+    orc = O.OracleScene(s)
+    k = [i for i, o in enumerate(s.objects()) if o["geom_type"] == 3 and o["prim_count"] > 100][0]
+    lo, cnt = int(orc.mesh_first[k]), int(orc.mesh_count[k])
+    tris = orc.mesh_v[lo:lo + cnt]
+    rng = np.random.default_rng(9)
+    o = rng.uniform(-3, 3, (30000, 3))
+    d = (np.array([0.0, 1.0, 0.0]) - o)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    idx, t = O.ref_octree_trace(tris, o, d)
+    t2, ob2, p2, _, _ = orc.trace(o, d)
+    mine = np.where(ob2 == k, p2, -1)
+    only_mesh = (ob2 == k) | (ob2 == -1)
+    assert (idx >= 0).sum() > 5000
+    np.testing.assert_array_equal(idx[only_mesh], mine[only_mesh])
+    np.testing.assert_array_equal(t[only_mesh & (idx >= 0)], t2[only_mesh & (idx >= 0)])
+
+
+_STAT_CHILD = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import xtracer_amd as xa
+from oracle import oracle as O
+s = xa.Scene({path!r})
+cam = json.loads({cam!r})
+ref = O.RefScene({path!r}).render({integ}, {w}, {h}, samples={spp}, rdepth={rd}, threads=1, seed=7)
+_, _, mean, m2 = O.OracleScene(s).render({integ}, {w}, {h}, cam, samples={spp}, rdepth={rd}, seed=3, stats=True)
+var = m2 / ({spp} - 1)
+se = np.sqrt(2 * var / {spp})
+inside = np.abs(ref[..., :3] - mean) <= 2.576 * se + 1e-5
+print(json.dumps({{"inside": float(inside.mean()), "mean_ref": float(ref[..., :3].mean()), "mean_orc": float(mean.mean())}}))
+"""
+
+
+@needs_ref
+@pytest.mark.parametrize("integrator,kind,kw,rdepth", [(2, "spheres", {}, 100), (3, "spheres", {}, 6),
+                                                       (3, "terrain", {"n": 21}, 6)])
+def test_oracle_estimators_match_reference_statistically(integrator, kind, kw, rdepth):
+    """ao / pathtracer: the reference draws glibc rand(), the oracle its own
+    streams; per-pixel means must agree within the 99% MC bound."""
+    path, cam = _synthetic(kind, **kw)
+    code = _STAT_CHILD.format(root=ROOT, path=path, cam=json.dumps(cam), integ=integrator, w=24, h=16, spp=256,
+                              rd=rdepth)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
+    assert out["inside"] >= 0.97, out

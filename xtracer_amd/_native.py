@@ -27,7 +27,7 @@ HIT_PLANE_BASE = 0xFFFF0000
 SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
     "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals",
-    "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
+    "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
     "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
 ]
@@ -119,6 +119,7 @@ def lib():
         "xt_scene_get_arrays": (I, [P, P, P, P, P, P, P]),
         "xt_scene_get_object_params": (I, [P, U32, P]),
         "xt_scene_get_vnormals": (I, [P, P]),
+        "xt_scene_get_kdtree": (I, [P, P, P, P]),
         "xt_scene_mesh_octree": (I, [P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64),
                                      ctypes.POINTER(U64)]),
         "xt_scene_camera": (I, [P, ctypes.c_char_p, ctypes.POINTER(Camera)]),
@@ -226,6 +227,16 @@ class Scene:
         if rc:
             raise XtError(rc)
         return out
+
+    def kdtree(self):
+        i = self.info()
+        nodes = np.zeros((i.n_kd_nodes, 2), np.uint32)
+        refs = np.zeros(i.n_kd_refs, np.uint32)
+        bounds = np.zeros(6, np.float32)
+        rc = lib().xt_scene_get_kdtree(self._h, _ptr(nodes), _ptr(refs), _ptr(bounds))
+        if rc:
+            raise XtError(rc)
+        return nodes, refs, bounds
 
     def mesh_octrees(self):
         res = []

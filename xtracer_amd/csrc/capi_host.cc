@@ -149,6 +149,19 @@ int xt_scene_get_vnormals(const xt_scene *s, float *out) {
     return XT_OK;
 }
 
+int xt_scene_get_kdtree(const xt_scene *s, uint32_t *nodes, uint32_t *refs, float bounds[6]) {
+    if (!s || !s->built) return XT_E_INVALID;
+    const KdTree &k = s->flat.kd;
+    if (nodes && !k.nodes.empty()) std::memcpy(nodes, k.nodes.data(), k.nodes.size() * sizeof(XtKdNode));
+    if (refs && !k.refs.empty()) std::memcpy(refs, k.refs.data(), k.refs.size() * sizeof(uint32_t));
+    if (bounds)
+        for (int a = 0; a < 3; ++a) {
+            bounds[a] = k.bounds.lo[a];
+            bounds[3 + a] = k.bounds.hi[a];
+        }
+    return XT_OK;
+}
+
 int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint64_t *nodes, uint64_t *max_level,
                          uint64_t *triangles) {
     if (!s || !s->built) return XT_E_INVALID;

@@ -91,29 +91,33 @@ __device__ __forceinline__ void tri32(const f3 o, const f3 d, float tsmall, cons
     offer(best, t, prim, __float_as_uint(b.w), u, v);
 }
 
-// FP32 sphere test. The reference keeps only the smaller root and needs it
-// positive (sphere.cc:35-55), so rays starting inside a sphere miss it.
+// Sphere test. The reference keeps only the smaller root and needs it
+// positive (sphere.cc:35-55), so rays starting inside a sphere miss it. The
+// arithmetic is FP64 on the FP32 inputs: origin - centre is then exact and
+// |oc|^2 - r^2 keeps its accuracy for origins near the surface (spheres are
+// few, so this costs little).
 __device__ __forceinline__ void sph32(const f3 o, const f3 d, float tsmall, const float4 a, const float4 b,
                                       uint32_t prim, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
     if (XT_PRIM_FLAGS(tf) & XT_PF_TINY_SPHERE) return;
-    const f3 oc = sub(o, mk(a.x, a.y, a.z));
-    const float bb = dot3(oc, d);
-    const float cc = dot3(oc, oc) - b.y;
-    const float disc = bb * bb - cc;
-    const float scale = fmaxf(bb * bb, b.y);
-    if (fabsf(disc) <= XT_GRAZE * scale) {
-        best.mark_t = fminf(best.mark_t, fmaxf(-bb, 0.f));
-        if (disc <= 0.f) return;
-    } else if (disc <= 0.f) {
+    const double ox = (double)o.x - (double)a.x, oy = (double)o.y - (double)a.y, oz = (double)o.z - (double)a.z;
+    const double bb = ox * (double)d.x + oy * (double)d.y + oz * (double)d.z;
+    const double r2 = (double)b.x * (double)b.x;
+    const double cc = (ox * ox + oy * oy + oz * oz) - r2;
+    const double disc = bb * bb - cc;
+    const double scale = fmax(bb * bb, r2);
+    if (fabs(disc) <= 1e-6 * scale) {
+        best.mark_t = fminf(best.mark_t, fmaxf((float)-bb, 0.f));
+        if (disc <= 0.0) return;
+    } else if (disc <= 0.0) {
         return;
     }
-    if (fabsf(cc) <= XT_GRAZE * b.y) {  // origin on the surface
+    if (fabs(cc) <= 1e-6 * r2) {  // origin on the surface
         best.mark_t = fminf(best.mark_t, 0.f);
         return;
     }
-    if (cc < 0.f || bb >= 0.f) return;  // inside, or sphere behind
-    const float t = cc / (sqrtf(disc) - bb);
+    if (cc < 0.0 || bb >= 0.0) return;  // inside, or sphere behind
+    const float t = (float)(cc / (sqrt(disc) - bb));
     if (t > best.t * (1.f + XT_TIE)) return;
     if (t < tsmall) {
         best.mark_t = fminf(best.mark_t, t);

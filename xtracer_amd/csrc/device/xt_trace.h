@@ -136,7 +136,9 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             const float tp = (split - oo) * ii;
             const bool below = oo < split || (oo == split && dd <= 0.f);
             const uint32_t first = below ? c : c + 1, second = below ? c + 1 : c;
-            if (tp > tmax * (1.f + eps) || tp < 0.f) {
+            // tp <= 0: the plane is behind the origin, or the origin lies on
+            // it and `below` already picked the side the ray moves into
+            if (tp > tmax * (1.f + eps) || tp <= 0.f) {
                 node = first;
             } else if (tp < tmin * (1.f - eps)) {
                 node = second;
