@@ -28,6 +28,10 @@ PIPE_BYTES_PER_QUERY = 184      # ray/hit/path-state streams of trace + shade + 
 PIPE_BYTES_PER_SAMPLE = 152     # camera ray + state write, FP64 accumulate read + write
 
 
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +45,7 @@ def parse():
     ap.add_argument("--accel", choices=["kdtree", "octree"], default="kdtree")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per wavefront pass (0: auto)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -58,18 +63,20 @@ def cpu_baseline(scene_path, args, threads):
         t0 = time.perf_counter()
         r = O.RefScene(scene_path)
         load_s = time.perf_counter() - t0
+        log(f"cpu baseline: reference scene loaded in {load_s:.1f}s")
         sec, q = ctypes.c_double(), ctypes.c_uint64()
-        # calibrate on a few tiles, then size the sample to the budget
+        # calibrate on one tile per thread, then size the sample to the budget
         stride = max(1, tiles // threads)
         L.xr_render_tiles(r._h, 3, W, H, args.spp, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec),
                           ctypes.byref(q))
-        per_tile = sec.value / max(1, (tiles + stride - 1) // stride)
-        want = max(threads, int(args.cpu_seconds / max(per_tile, 1e-9) * threads))
-        stride2 = max(1, tiles // max(1, want))
-        if stride2 < stride:
+        log(f"cpu baseline: calibration {(tiles + stride - 1) // stride} tiles in {sec.value:.2f}s")
+        rounds = int(args.cpu_seconds / max(sec.value, 1e-3))
+        if rounds >= 2:
+            stride2 = max(1, tiles // (threads * rounds))
             L.xr_render_tiles(r._h, 3, W, H, args.spp, 1, args.rdepth, threads, 1, stride2, ctypes.byref(sec),
                               ctypes.byref(q))
             stride = stride2
+            log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles in {sec.value:.2f}s")
         n_tiles = (tiles + stride - 1) // stride
         return {"value": q.value / sec.value / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
                 "sample": f"reference pathtracer (OpenMP, -O3) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) "
@@ -102,12 +109,15 @@ def main():
     t0 = time.perf_counter()
     path = scenes.icosphere_scene(seed=0, subdiv=args.subdiv)
     gen_s = time.perf_counter() - t0
+    log(f"scene generated in {gen_s:.1f}s: {path}")
     t0 = time.perf_counter()
     scene = xa.Scene(path)
     build_s = time.perf_counter() - t0
     info = scene.info()
+    log(f"scene loaded + trees built in {build_s:.1f}s: {info.n_prims} prims, {info.n_kd_nodes} kd nodes")
     dev = xa.Device(0)
     sid = dev.upload(scene)
+    log("scene uploaded")
     accel = xa.ACCEL_KDTREE if args.accel == "kdtree" else xa.ACCEL_OCTREE
     out = torch.empty((args.height, args.width, 4), dtype=torch.float32, device="cuda")
     cam = scene.camera()
@@ -117,7 +127,9 @@ def main():
                            accel=accel, seed=1000 + k, samples_per_pass=args.spp_pass)
 
     for k in range(args.warmup):
-        dev.render_device(job(k), out.data_ptr())
+        t0 = time.perf_counter()
+        st = dev.render_device(job(k), out.data_ptr())
+        log(f"warmup {k}: {time.perf_counter() - t0:.3f}s, {st.queries} queries, {st.retests} retests")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
@@ -125,6 +137,7 @@ def main():
         stats.append(dev.render_device(job(args.warmup + k), out.data_ptr()))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {wall:.3f}s")
     queries = sum(s.queries for s in stats)
     samples = sum(s.samples for s in stats)
     trace_ms = sum(s.ms_trace for s in stats)
@@ -134,7 +147,7 @@ def main():
     achieved = queries * TRACE_BYTES_PER_QUERY / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     pipe_bytes = queries * PIPE_BYTES_PER_QUERY + samples * PIPE_BYTES_PER_SAMPLE
     pipe = pipe_bytes / (wall) / 1e9
-    threads = min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline(path, args, threads)
     line = {
         "metric": "Mrays/s at 1080p 16spp (pathtracer, plain one-job-per-launch)",

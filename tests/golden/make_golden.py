@@ -1,0 +1,70 @@
+"""Writes the golden fixtures in this directory from the REAL reference
+(oracle/_ref/libxtref.so, built from /root/reference by oracle/ref/Makefile).
+
+Each fixture holds inputs and the reference's outputs on this repo's seeded
+synthetic scenes (regenerated identically anywhere by xtracer_amd.scenes):
+  ref_trace_<scene>.npz   FP32 rays -> Scene::intersection t, object index, point
+  ref_render_<scene>.npz  stencil and depth framebuffers (pinhole, GRID aa=1)
+  ref_mc_<scene>.npz      high-spp ao / pathtracer framebuffers (glibc rand())
+Every scene is loaded in a fresh process: the reference's string pool is
+process-global, and its handle order decides the scan order.
+
+Usage: python tests/golden/make_golden.py
+"""
+import json
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+SCENES = {
+    "spheres": ("spheres", {}),
+    "terrain": ("terrain", {"n": 41}),
+    "ico3": ("icosphere", {"subdiv": 3}),
+}
+
+CHILD = r"""
+import sys, json
+import numpy as np
+sys.path.insert(0, {root!r})
+from oracle import oracle as O
+from xtracer_amd import scenes
+import xtracer_amd as xa
+from tests.helpers import camera_rays
+name, kind, kw = {name!r}, {kind!r}, json.loads({kw!r})
+path = scenes.SCENES[kind][0](**kw)
+r = O.RefScene(path)
+s = xa.Scene(path)
+rng = np.random.default_rng(1234)
+o1, d1 = camera_rays(s.camera(), 64, 48)
+pick = rng.choice(len(o1), 1024, replace=False)
+o2 = rng.uniform(-3, 6, (1024, 3))
+d2 = rng.normal(size=(1024, 3)); d2 /= np.linalg.norm(d2, axis=1)[:, None]
+o = np.concatenate([o1[pick], o2]).astype(np.float32)
+d = np.concatenate([d1[pick], d2]).astype(np.float32)
+t, obj, pt, nm = r.trace(o.astype(np.float64), d.astype(np.float64))
+np.savez_compressed({out!r} + "/ref_trace_" + name + ".npz", o=o, d=d, t=t, obj=obj, point=pt)
+cam = s.camera()
+if cam.thin_lens == 0:
+    st = r.render(0, 48, 32, samples=4, threads=1)
+    dp = r.render(1, 48, 32, samples=4, threads=1)
+    np.savez_compressed({out!r} + "/ref_render_" + name + ".npz", stencil=st, depth=dp, width=48, height=32, spp=4)
+if name in ("spheres", "terrain"):
+    integ = 2 if name == "spheres" else 3
+    img = r.render(integ, 24, 16, samples=2048, rdepth=6, threads=8, seed=99)
+    np.savez_compressed({out!r} + "/ref_mc_" + name + ".npz", image=img, integrator=integ, width=24, height=16,
+                        spp=2048, rdepth=6)
+print("ok", name)
+"""
+
+
+def main():
+    for name, (kind, kw) in SCENES.items():
+        code = CHILD.format(root=ROOT, name=name, kind=kind, kw=json.dumps(kw), out=HERE)
+        print(subprocess.check_output([sys.executable, "-c", code], timeout=1800).decode().strip())
+
+
+if __name__ == "__main__":
+    main()

@@ -224,4 +224,16 @@ __device__ __noinline__ double plane64(const d3 o, const d3 d, const XtPlane &pl
     return t;
 }
 
+// Plane-equation t of a triangle in FP64 from its FP32 vertices (the
+// reference's t = -n.(o - v0) / n.d, triangle.cc:84-94). The FP32 t loses
+// about 1e-7 / |n.d| relative at grazing angles; hits with |n.d| < 0.05 get
+// this value instead.
+__device__ __forceinline__ float tri_t64(const f3 o, const f3 d, const float4 a, const float4 b, const float4 c) {
+    const d3 A = {a.x, a.y, a.z};
+    const d3 n = dnorm(dcross(dsub(d3{b.x, b.y, b.z}, A), dsub(d3{c.x, c.y, c.z}, A)));
+    const d3 D = {d.x, d.y, d.z};
+    const double ndd = ddot(n, D);
+    return (float)(-ddot(n, dsub(d3{o.x, o.y, o.z}, A)) / ndd);
+}
+
 #pragma clang fp contract(on)

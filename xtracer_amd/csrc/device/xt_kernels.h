@@ -263,8 +263,16 @@ static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT
             bool ok;
             if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd(S, tst, r.o, r.d, kstk + threadIdx.x, BS, 1e-6f);
             else ok = walk_oct(S, tst, r.o, r.d, ostk + threadIdx.x, BS, 1e-6f);
-            const Best &bb = tst.best;
+            Best &bb = tst.best;
             mark = !ok || bb.mark_t <= bb.t * (1.f + XT_TIE);
+            if (!mark && bb.prim < S.n_prims && !(r.self_t > 0.f && bb.prim == r.origin)) {
+                const float4 a = S.prims[3 * bb.prim], b = S.prims[3 * bb.prim + 1];
+                if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
+                    const float4 c = S.prims[3 * bb.prim + 2];
+                    const f3 n = cross3(sub(mk(b.x, b.y, b.z), mk(a.x, a.y, a.z)), sub(mk(c.x, c.y, c.z), mk(a.x, a.y, a.z)));
+                    if (fabsf(dot3(n, r.d)) * c.w < 0.05f) bb.t = tri_t64(r.o, r.d, a, b, c);
+                }
+            }
             hit[i] = make_float4(bb.t, __uint_as_float(bb.prim), bb.u, bb.v);
         }
         const uint32_t slot = wave_append(mark, &cnt->n[CNT_RETEST]);
