@@ -45,8 +45,11 @@ def parse():
     ap.add_argument("--accel", choices=["kdtree", "octree"], default="kdtree")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per wavefront pass (0: auto)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, cores))")
+    ap.add_argument("--cpu-threads", type=int, default=1,
+                    help="CPU baseline threads (the reference's glibc rand() lock makes more threads slower)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tuning", default="", help="launch shape as JSON (xt_tuning fields); skips the autotuner")
     return ap.parse_args()
 
 
@@ -65,23 +68,27 @@ def cpu_baseline(scene_path, args, threads):
         load_s = time.perf_counter() - t0
         log(f"cpu baseline: reference scene loaded in {load_s:.1f}s")
         sec, q = ctypes.c_double(), ctypes.c_uint64()
-        # calibrate on one tile per thread, then size the sample to the budget
-        stride = max(1, tiles // threads)
+        # calibrate on 8 tiles per thread spread over the frame (the cost per
+        # tile varies widely: sky tiles are nearly free), then size the sample
+        stride = max(1, tiles // (8 * threads))
         L.xr_render_tiles(r._h, 3, W, H, args.spp, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec),
                           ctypes.byref(q))
-        log(f"cpu baseline: calibration {(tiles + stride - 1) // stride} tiles in {sec.value:.2f}s")
-        rounds = int(args.cpu_seconds / max(sec.value, 1e-3))
-        if rounds >= 2:
-            stride2 = max(1, tiles // (threads * rounds))
+        log(f"cpu baseline: calibration {(tiles + stride - 1) // stride} tiles on {threads} threads in {sec.value:.2f}s")
+        n_cal = (tiles + stride - 1) // stride
+        per_tile = sec.value / n_cal
+        want = int(args.cpu_seconds * threads / max(per_tile, 1e-4))
+        if want >= 2 * n_cal:
+            stride2 = max(2, tiles // want)
             L.xr_render_tiles(r._h, 3, W, H, args.spp, 1, args.rdepth, threads, 1, stride2, ctypes.byref(sec),
                               ctypes.byref(q))
             stride = stride2
             log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles in {sec.value:.2f}s")
         n_tiles = (tiles + stride - 1) // stride
         return {"value": q.value / sec.value / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
-                "sample": f"reference pathtracer (OpenMP, -O3) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) "
-                          f"at full spp; {q.value} Scene::intersection calls in {sec.value:.2f}s; scene load + "
-                          f"octree build {load_s:.1f}s not timed"}
+                "sample": f"reference pathtracer (compiled -O3 from its sources, OpenMP dynamic tiles, {threads} "
+                          f"thread(s)) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) at full spp; "
+                          f"{q.value} Scene::intersection calls in {sec.value:.2f}s; scene load + octree build "
+                          f"{load_s:.1f}s not timed"}
     import xtracer_amd as xa
     from xtracer_amd import scenes
     s = xa.Scene(scene_path, build=True)
@@ -126,6 +133,14 @@ def main():
         return xa.make_job(sid, xa.PATHTRACER, args.width, args.height, cam, samples=args.spp, rdepth=args.rdepth,
                            accel=accel, seed=1000 + k, samples_per_pass=args.spp_pass)
 
+    if args.tuning:
+        dev.set_tuning(**json.loads(args.tuning))
+    tuning = dev.tuning()
+    if not args.no_autotune and not args.tuning:
+        from xtracer_amd.autotune import autotune
+        t0 = time.perf_counter()
+        tuning, _ = autotune(dev, job(999), out.data_ptr(), wave_paths=(4 << 20, 16 << 20), log=log)
+        log(f"autotune picked {tuning} in {time.perf_counter() - t0:.1f}s")
     for k in range(args.warmup):
         t0 = time.perf_counter()
         st = dev.render_device(job(k), out.data_ptr())
@@ -147,7 +162,7 @@ def main():
     achieved = queries * TRACE_BYTES_PER_QUERY / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     pipe_bytes = queries * PIPE_BYTES_PER_QUERY + samples * PIPE_BYTES_PER_SAMPLE
     pipe = pipe_bytes / (wall) / 1e9
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline(path, args, threads)
     line = {
         "metric": "Mrays/s at 1080p 16spp (pathtracer, plain one-job-per-launch)",
@@ -177,7 +192,7 @@ def main():
                   "device_ms_per_frame": round(dev_ms / args.steps, 3), "trace_ms_per_frame":
                   round(trace_ms / args.steps, 3), "scene_gen_s": round(gen_s, 2), "load_build_s": round(build_s, 2),
                   "kd_nodes": info.n_kd_nodes, "kd_refs": info.n_kd_refs, "kd_depth": info.kd_depth,
-                  "passes": stats[0].passes if stats else 0},
+                  "passes": stats[0].passes if stats else 0, "tuning": tuning},
     }
     if cpu:
         line["extra"]["speedup_vs_cpu"] = round(value / cpu["value"], 1)

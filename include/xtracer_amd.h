@@ -139,6 +139,19 @@ int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes);
 void xt_dev_destroy(xt_dev *d);
 const char *xt_last_error(const xt_dev *d);
 
+/* Launch-shape knobs (the autotuner's search space). In xt_dev_set_tuning
+ * a 0 field keeps the current value; xt_dev_get_tuning reports the values in
+ * force (trace_blocks_per_cu 0: as many blocks as a CU keeps resident). */
+#define XT_TUNING_AUTO 0xffffffffu /* trace_blocks_per_cu: back to resident fill */
+typedef struct xt_tuning {
+    uint32_t kd_stack;        /* kd short-stack entries per lane: 8, 12, 16 or 24 (16) */
+    uint32_t trace_block;     /* lanes per trace block: 128 or 256 (256) */
+    uint32_t wave_paths;      /* paths in flight per wavefront pass, 4096..2^28 (8M) */
+    uint32_t trace_blocks_per_cu; /* persistent trace grid: blocks per CU, <= 64 (resident fill) */
+} xt_tuning;
+int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t);
+int xt_dev_get_tuning(const xt_dev *d, xt_tuning *t);
+
 /* Makes a built scene resident in HBM. The scene may be freed afterwards. */
 int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id);
 int xt_scene_release(xt_dev *d, uint32_t scene_id);

@@ -1,0 +1,64 @@
+"""Launch-shape knobs change speed, never results: every kd stack depth, trace
+block size, persistent grid size and wavefront size gives the same hits and
+the same bit-exact images as the defaults."""
+import numpy as np
+import pytest
+
+import xtracer_amd as xa
+from oracle.oracle import OracleScene
+from xtracer_amd import scenes
+from tests.helpers import mixed_rays, scene_path
+
+DEFAULT = {"kd_stack": 16, "trace_block": 256, "wave_paths": 8 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO}
+
+
+@pytest.fixture
+def tuned(device):
+    yield device
+    device.set_tuning(**DEFAULT)
+
+
+@pytest.mark.gpu
+def test_tuning_roundtrip_and_rejects(tuned):
+    t = tuned.set_tuning(**DEFAULT)
+    assert t == {"kd_stack": 16, "trace_block": 256, "wave_paths": 8 << 20, "trace_blocks_per_cu": 0}
+    for bad in ({"kd_stack": 10}, {"trace_block": 64}, {"wave_paths": 100}, {"trace_blocks_per_cu": 65}):
+        with pytest.raises(xa.XtError):
+            tuned.set_tuning(**bad)
+    assert tuned.tuning() == t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stack", [8, 12, 16, 24])
+@pytest.mark.parametrize("block", [128, 256])
+@pytest.mark.parametrize("per_cu", [1, xa.TUNING_AUTO])
+def test_trace_invariant(tuned, stack, block, per_cu):
+    s = xa.Scene(scene_path("icosphere", subdiv=5))
+    sid = tuned.upload(s)
+    o, d = mixed_rays(None, 1 << 16, seed=5)
+    tuned.set_tuning(**DEFAULT)
+    base = tuned.trace(sid, o, d, xa.ACCEL_KDTREE)
+    tuned.set_tuning(kd_stack=stack, trace_block=block, trace_blocks_per_cu=per_cu)
+    got = tuned.trace(sid, o, d, xa.ACCEL_KDTREE)
+    tuned.release(sid)
+    assert np.array_equal(base[1], got[1])
+    # a ray may take the FP64 re-test under one stack depth and not another
+    np.testing.assert_allclose(got[0], base[0], rtol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", [xa.STENCIL, xa.DEPTH])
+def test_render_bit_exact_small_shapes(tuned, integrator):
+    """Several wavefront passes, one resident block per CU and an 8-entry
+    stack still reproduce the oracle's image bit for bit."""
+    s = xa.Scene(scene_path("icosphere", subdiv=4))
+    sid = tuned.upload(s)
+    cam = scenes.camera_dict(**scenes.SCENES["icosphere"][1])
+    W, H, spp = 96, 64, 4
+    tuned.set_tuning(kd_stack=8, trace_block=128, trace_blocks_per_cu=1, wave_paths=4096)
+    img, st = tuned.render(xa.make_job(sid, integrator, W, H, s.camera(), samples=spp, seed=5))
+    tuned.release(sid)
+    assert st.passes == spp  # 4096 paths per pass: one sample per pass
+    ref, nq = OracleScene(s).render(integrator, W, H, cam, samples=spp, seed=5)
+    assert st.queries == nq
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

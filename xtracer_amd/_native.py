@@ -22,13 +22,15 @@ AA_GRID, AA_RANDOM = 0, 1
 ACCUM_DEFAULT, ACCUM_REFERENCE, ACCUM_FP64 = 0, 1, 2
 HIT_NONE = 0xFFFFFFFF
 HIT_PLANE_BASE = 0xFFFF0000
+TUNING_AUTO = 0xFFFFFFFF  # xt_tuning.trace_blocks_per_cu: resident fill
 
 # Exported symbols, in the order include/xtracer_amd.h declares them.
 SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
     "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals",
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
-    "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_scene_upload", "xt_scene_release", "xt_trace",
+    "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
+    "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
 ]
 
@@ -61,6 +63,13 @@ class Camera(ctypes.Structure):
     _fields_ = [("position", ctypes.c_double * 3), ("basis", ctypes.c_double * 9), ("dir_z", ctypes.c_double),
                 ("aperture", ctypes.c_double), ("flength", ctypes.c_double), ("thin_lens", ctypes.c_uint32),
                 ("pad", ctypes.c_uint32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("kd_stack", "trace_block", "wave_paths", "trace_blocks_per_cu")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class Job(ctypes.Structure):
@@ -128,6 +137,8 @@ def lib():
         "xt_dev_create": (I, [ctypes.POINTER(P), I, U64]),
         "xt_dev_destroy": (None, [P]),
         "xt_last_error": (ctypes.c_char_p, [P]),
+        "xt_dev_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "xt_dev_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "xt_scene_upload": (I, [P, P, ctypes.POINTER(U32)]),
         "xt_scene_release": (I, [P, U32]),
         "xt_trace": (I, [P, U32, I, ctypes.POINTER(Rays), ctypes.POINTER(Hits), U64]),
@@ -295,6 +306,18 @@ class Device:
     @property
     def handle(self):
         return self._d
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().xt_dev_get_tuning(self._d, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **knobs):
+        """Sets launch-shape knobs (kd_stack, trace_block, wave_paths,
+        trace_blocks_per_cu); unnamed knobs keep their value."""
+        t = Tuning(**{k: int(v) for k, v in knobs.items()})
+        self._check(lib().xt_dev_set_tuning(self._d, ctypes.byref(t)))
+        return self.tuning()
 
     def upload(self, scene):
         sid = ctypes.c_uint32()

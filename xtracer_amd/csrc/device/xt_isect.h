@@ -46,6 +46,9 @@ struct Best {
 };
 
 __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t obj, float u, float v) {
+    // a kd-tree references a primitive from every leaf it overlaps: meeting
+    // the current best again is not a tie
+    if (prim == b.prim) return;
     // ties go to the earlier object, then to the earlier soup index
     if (t < b.t || (t == b.t && (obj < b.obj || (obj == b.obj && prim < b.prim)))) {
         if (b.t - t <= XT_TIE * t) b.mark_t = fminf(b.mark_t, t);  // runner-up within the tie band

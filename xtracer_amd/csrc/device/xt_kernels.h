@@ -24,6 +24,7 @@ enum {
     CNT_Q0 = 0,      // ray count of queue 0
     CNT_Q1 = 1,      // ray count of queue 1
     CNT_RETEST = 2,  // rays listed for the FP64 re-test
+    CNT_CURSOR = 3,  // work cursor of the persistent trace kernel
     CNT_WORDS = 8
 };
 struct Counters {
@@ -236,18 +237,23 @@ __device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Bes
     }
 }
 
-template <int ACCEL>
-static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK) void k_trace(
-    const SceneView S, const Queue in, const uint32_t *in_count, float4 *hit, uint32_t *retest_list,
-    Counters *cnt) {
-    constexpr int BS = ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK;
-    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? XT_KD_STACK * XT_BLOCK : 1];
-    __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * XT_OCT_BLOCK : 1];
+// Persistent trace: each wave takes 64 consecutive rays at a time from a
+// device cursor (one atomic per wave) until the queue is drained, so waves
+// that drew short traversals pick up more work instead of idling.
+template <int ACCEL, int STACK, int BS>
+static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
+                                                     float4 *hit, uint32_t *retest_list, Counters *cnt) {
+    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? STACK * BS : 1];
+    __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * BS : 1];
     const uint32_t n = *in_count;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->queries, (unsigned long long)n);
-    const uint32_t stride = gridDim.x * BS;
-    for (uint32_t base = blockIdx.x * BS; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+    const uint32_t lane = lane_id();
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&cnt->n[CNT_CURSOR], 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
         bool mark = false;
         if (i < n) {
             const float4 a = in.r0[i], b = in.r1[i];
@@ -261,10 +267,12 @@ static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT
             for (uint32_t k = 0; k < S.n_planes; ++k)
                 if (XT_PLANE_BASE + k != r.origin) plane32(r.o, r.d, tst.tsmall, S.planes[k], XT_PLANE_BASE + k, tst.best);
             bool ok;
-            if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd(S, tst, r.o, r.d, kstk + threadIdx.x, BS, 1e-6f);
+            if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd<STACK>(S, tst, r.o, r.d, kstk + threadIdx.x, BS, 1e-6f);
             else ok = walk_oct(S, tst, r.o, r.d, ostk + threadIdx.x, BS, 1e-6f);
             Best &bb = tst.best;
-            mark = !ok || bb.mark_t <= bb.t * (1.f + XT_TIE);
+            // a deferred candidate matters only up to the closest hit (a miss
+            // has t = INF, so compare only when something was deferred)
+            mark = !ok || (bb.mark_t != INFINITY && bb.mark_t <= bb.t * (1.f + XT_TIE));
             if (!mark && bb.prim < S.n_prims && !(r.self_t > 0.f && bb.prim == r.origin)) {
                 const float4 a = S.prims[3 * bb.prim], b = S.prims[3 * bb.prim + 1];
                 if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
@@ -322,7 +330,7 @@ static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT
         const f3 o32 = mk((float)tst.o.x, (float)tst.o.y, (float)tst.o.z);
         const f3 d32 = mk((float)tst.d.x, (float)tst.d.y, (float)tst.d.z);
         bool ok;
-        if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd(S, tst, o32, d32, kstk + threadIdx.x, BS, 1e-4f);
+        if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd<XT_KD_STACK>(S, tst, o32, d32, kstk + threadIdx.x, BS, 1e-4f);
         else ok = walk_oct(S, tst, o32, d32, ostk + threadIdx.x, BS, 1e-4f);
         if (!ok) atomicAdd(&cnt->steps_exceeded, 1ull);
         const bool is_tri = tst.prim < S.n_prims;

@@ -6,7 +6,7 @@
 #pragma once
 #include "xt_isect.h"
 
-#define XT_KD_STACK 16    // kd-tree short stack entries per lane (8 B each, LDS)
+#define XT_KD_STACK 16    // default kd-tree short stack entries per lane (8 B each, LDS)
 #define XT_OCT_STACK 80   // octree stack entries per lane (4 B each, LDS); depth <= 10 needs <= 78
 #define XT_MAX_STEPS 200000
 
@@ -113,7 +113,7 @@ struct Tester64 {
 // kd-tree traversal with a short stack of (node, t_far) entries in LDS and a
 // restart from the root when entries were dropped. `stk` points at this
 // lane's first entry; entries are `stride` uint2 apart.
-template <class T>
+template <int STACK, class T>
 __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, const f3 d, uint2 *stk, int stride,
                                         float eps) {
     const f3 inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
@@ -143,9 +143,9 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             } else if (tp < tmin * (1.f - eps)) {
                 node = second;
             } else {
-                head = head + 1 == XT_KD_STACK ? 0 : head + 1;
+                head = head + 1 == STACK ? 0 : head + 1;
                 stk[head * stride] = make_uint2(second, __float_as_uint(tmax));
-                if (count < XT_KD_STACK) ++count;
+                if (count < STACK) ++count;
                 else dropped = true;
                 node = first;
                 tmax = fminf(tmax, tp * (1.f + eps));
@@ -168,7 +168,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             continue;
         }
         const uint2 e = stk[head * stride];
-        head = head == 0 ? XT_KD_STACK - 1 : head - 1;
+        head = head == 0 ? STACK - 1 : head - 1;
         --count;
         node = e.x;
         tmin = tmax;

@@ -64,6 +64,7 @@ struct xt_dev {
     int trace_launches = 0;
     std::vector<int> timed_pairs;
     ServeState *serve = nullptr;
+    xt_tuning tuning{16, 256, 8u << 20, 0};
     std::string err;
 };
 

@@ -171,6 +171,50 @@ void xt_dev_destroy(xt_dev *d) {
 
 const char *xt_last_error(const xt_dev *d) { return d ? d->err.c_str() : "null device"; }
 
+int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t) {
+    if (!d || !t) return XT_E_INVALID;
+    xt_tuning n = d->tuning;
+    if (t->kd_stack) {
+        if (t->kd_stack != 8 && t->kd_stack != 12 && t->kd_stack != 16 && t->kd_stack != 24) {
+            d->err = "kd_stack must be 8, 12, 16 or 24";
+            return XT_E_INVALID;
+        }
+        n.kd_stack = t->kd_stack;
+    }
+    if (t->trace_block) {
+        if (t->trace_block != 128 && t->trace_block != 256) {
+            d->err = "trace_block must be 128 or 256";
+            return XT_E_INVALID;
+        }
+        n.trace_block = t->trace_block;
+    }
+    if (t->wave_paths) {
+        if (t->wave_paths < 4096 || t->wave_paths > (1u << 28)) {
+            d->err = "wave_paths must lie in [4096, 2^28]";
+            return XT_E_INVALID;
+        }
+        n.wave_paths = t->wave_paths;
+    }
+    if (t->trace_blocks_per_cu) {
+        if (t->trace_blocks_per_cu == XT_TUNING_AUTO) {
+            n.trace_blocks_per_cu = 0;
+        } else if (t->trace_blocks_per_cu > 64) {
+            d->err = "trace_blocks_per_cu must be at most 64";
+            return XT_E_INVALID;
+        } else {
+            n.trace_blocks_per_cu = t->trace_blocks_per_cu;
+        }
+    }
+    d->tuning = n;
+    return XT_OK;
+}
+
+int xt_dev_get_tuning(const xt_dev *d, xt_tuning *t) {
+    if (!d || !t) return XT_E_INVALID;
+    *t = d->tuning;
+    return XT_OK;
+}
+
 int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     if (!d || !s || !scene_id) return XT_E_INVALID;
     if (!s->built) {
@@ -264,6 +308,47 @@ int xt_scene_release(xt_dev *d, uint32_t scene_id) {
 
 }  // extern "C"
 
+// Instantiations of the kd-tree trace kernel over the tuning space. The
+// persistent grid is `per_cu` blocks per CU, or (per_cu == 0) exactly the
+// blocks the CU keeps resident, from the occupancy calculator.
+template <int STACK, int BS>
+static void launch_kd_trace_sb(unsigned per_cu, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
+                               const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, Counters *cnt) {
+    static int resident = 0;
+    if (per_cu == 0) {
+        if (resident == 0) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<XT_ACCEL_KDTREE, STACK, BS>, BS, 0) !=
+                    hipSuccess ||
+                nb < 1)
+                nb = 1;
+            resident = nb;
+        }
+        per_cu = (unsigned)resident;
+    }
+    const size_t need = (cap + BS - 1) / BS;
+    const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>(need, (size_t)n_cu * per_cu));
+    hipLaunchKernelGGL((k_trace<XT_ACCEL_KDTREE, STACK, BS>), dim3(g), dim3(BS), 0, st, v, q, count, hit, retest, cnt);
+}
+
+template <int STACK>
+static void launch_kd_trace_s(unsigned bs, unsigned per_cu, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
+                              const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, Counters *cnt) {
+    if (bs == 128) launch_kd_trace_sb<STACK, 128>(per_cu, n_cu, cap, st, v, q, count, hit, retest, cnt);
+    else launch_kd_trace_sb<STACK, 256>(per_cu, n_cu, cap, st, v, q, count, hit, retest, cnt);
+}
+
+static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
+                            const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, Counters *cnt) {
+    const unsigned bs = tn.trace_block, pc = tn.trace_blocks_per_cu;
+    switch (tn.kd_stack) {
+        case 8: launch_kd_trace_s<8>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, cnt); break;
+        case 12: launch_kd_trace_s<12>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, cnt); break;
+        case 24: launch_kd_trace_s<24>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, cnt); break;
+        default: launch_kd_trace_s<16>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, cnt); break;
+    }
+}
+
 // Launches trace + FP64 re-test over queue `qi` (count in cnt->n[qi]).
 int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const double *ray64, uint32_t *flags,
                     size_t cap, bool time_it, float *ms) {
@@ -271,12 +356,14 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const doub
     Queue q{w.q[qi].r0, w.q[qi].r1, w.q[qi].path};
     const bool timed = time_it && d->trace_launches < XT_TRACE_EVENTS - 1;
     if (timed) hipEventRecord(d->tev[d->trace_launches], d->stream);
+    const xt_tuning &tn = d->tuning;
     if (accel == XT_ACCEL_OCTREE) {
-        hipLaunchKernelGGL(k_trace<XT_ACCEL_OCTREE>, dim3(grid_for(cap, XT_OCT_BLOCK, 16384)), dim3(XT_OCT_BLOCK), 0,
-                           d->stream, ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
+        // 16 resident 64-lane blocks per CU (LDS: 20 KB each)
+        const unsigned g = (unsigned)std::min<size_t>(cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
+        hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, d->stream,
+                           ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
     } else {
-        hipLaunchKernelGGL(k_trace<XT_ACCEL_KDTREE>, dim3(grid_for(cap, XT_BLOCK, 4096)), dim3(XT_BLOCK), 0,
-                           d->stream, ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
+        launch_kd_trace(tn, d->n_cu, cap, d->stream, ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
     }
     if (timed) {
         hipEventRecord(d->tev[d->trace_launches + 1], d->stream);

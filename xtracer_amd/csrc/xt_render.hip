@@ -106,10 +106,9 @@ uint32_t xt_rounds(const JobDev &j) {
     }
 }
 
-uint32_t xt_auto_spp_pass(const JobDev &j, uint32_t requested) {
+uint32_t xt_auto_spp_pass(const JobDev &j, uint32_t requested, uint32_t target) {
     if (j.total == 0) return 1;
     if (requested) return std::min(requested, j.total);
-    const uint64_t target = 8u << 20;  // about 8M paths in flight per pass
     uint64_t s = std::max<uint64_t>(1, target / j.NT);
     return (uint32_t)std::min<uint64_t>(s, j.total);
 }
@@ -150,7 +149,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
     int rc = xt_job_check(d, job, j, &ds);
     if (rc) return rc;
     XT_TRY(hipSetDevice(d->device));
-    const uint32_t spp_pass = xt_auto_spp_pass(j, job->samples_per_pass);
+    const uint32_t spp_pass = xt_auto_spp_pass(j, job->samples_per_pass, d->tuning.wave_paths);
     const size_t cap = (size_t)spp_pass * j.NT;
     if (cap >= (1ull << 31)) {
         d->err = "pass too large";
@@ -186,7 +185,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         XT_LAUNCHED();
         int qi = 0;
         for (uint32_t r = 0; r < rounds; ++r) {
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, (1u << (qi ^ 1)) | (1u << CNT_RETEST));
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR));
             rc = xt_launch_trace(d, *ds, (int)job->accel, qi, need64 && r == 0 ? w.ray64 : nullptr, nullptr, cap,
                                  time_trace, nullptr);
             if (rc) return rc;
