@@ -135,6 +135,8 @@ int xt_camera_make(const double position[3], const double target[3], const doubl
 
 /* ----------------------------------------------------------------- device */
 
+/* hbm_budget_bytes caps the device memory the handle allocates (scenes,
+ * wavefront and job buffers); 0 means 90% of the device's memory. */
 int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes);
 void xt_dev_destroy(xt_dev *d);
 const char *xt_last_error(const xt_dev *d);
@@ -214,14 +216,33 @@ int xt_render(xt_dev *d, const xt_job *job, float *rgba_out, xt_job_stats *stats
 /* Same with a device-resident output buffer. */
 int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_stats *stats);
 
-/* Serving: continuous batching of jobs on the device. xt_submit admits the
- * job when its buffers fit the HBM budget (else XT_E_BUDGET); xt_poll reports
- * completion and copies the image out once done. xt_serve_step advances the
- * device work by one bounce round; xt_drain runs until every job is done. */
+/* Serving: continuous batching of many jobs on one device, in place of the
+ * reference's one blocking IIntegrator::render() per context.
+ * xt_submit queues the job and returns a ticket; it is admitted (FIFO) once
+ * its buffers (80 B per pixel) fit the HBM budget, and refused with
+ * XT_E_BUDGET when they never could. xt_serve_step runs one wavefront pass
+ * shared by all admitted jobs (each advances by a fair share of samples) and
+ * reports the jobs not yet finished. xt_poll sets *done once the job's image
+ * is ready, copies it to rgba_out (width*height*4 floats, may be NULL) and
+ * forgets the ticket; stats->latency_ms = completion - submission and
+ * stats->queue_ms = admission - submission. A served image is bit-identical
+ * to the same job rendered with xt_render. xt_drain steps until every
+ * admitted and waiting job has finished. */
 int xt_submit(xt_dev *d, const xt_job *job, uint64_t *ticket);
 int xt_poll(xt_dev *d, uint64_t ticket, int *done, float *rgba_out, xt_job_stats *stats);
 int xt_serve_step(xt_dev *d, int *active_jobs);
 int xt_drain(xt_dev *d);
+
+typedef struct xt_serve_info {
+    uint64_t queries;      /* closest-hit queries of all passes */
+    uint64_t retests;
+    uint64_t samples;      /* camera samples of all passes */
+    uint64_t passes;
+    uint32_t jobs_waiting, jobs_active, jobs_finished, pad;
+    double ms_device;      /* device time of all passes, HIP events */
+    uint64_t bytes_held;   /* job buffers held in HBM (in use + pooled) */
+} xt_serve_info;
+int xt_serve_get_info(const xt_dev *d, xt_serve_info *info);
 
 #ifdef __cplusplus
 }

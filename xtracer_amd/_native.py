@@ -32,6 +32,7 @@ SYMBOLS = [
     "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
+    "xt_serve_get_info",
 ]
 
 
@@ -70,6 +71,16 @@ class Tuning(ctypes.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ServeInfo(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_uint64), ("retests", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+                ("passes", ctypes.c_uint64), ("jobs_waiting", ctypes.c_uint32), ("jobs_active", ctypes.c_uint32),
+                ("jobs_finished", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("ms_device", ctypes.c_double),
+                ("bytes_held", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
 
 
 class Job(ctypes.Structure):
@@ -149,6 +160,7 @@ def lib():
         "xt_poll": (I, [P, U64, ctypes.POINTER(I), P, ctypes.POINTER(JobStats)]),
         "xt_serve_step": (I, [P, ctypes.POINTER(I)]),
         "xt_drain": (I, [P]),
+        "xt_serve_get_info": (I, [P, ctypes.POINTER(ServeInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -368,11 +380,17 @@ class Device:
         return t.value
 
     def poll(self, ticket, width, height):
+        """(done, image or None, JobStats); a finished ticket is forgotten."""
         done = ctypes.c_int()
         img = np.zeros((height, width, 4), np.float32)
         st = JobStats()
         self._check(lib().xt_poll(self._d, ticket, ctypes.byref(done), _ptr(img), ctypes.byref(st)))
-        return bool(done.value), img, st
+        return bool(done.value), (img if done.value else None), st
+
+    def serve_info(self):
+        info = ServeInfo()
+        self._check(lib().xt_serve_get_info(self._d, ctypes.byref(info)))
+        return info.as_dict()
 
     def serve_step(self):
         n = ctypes.c_int()

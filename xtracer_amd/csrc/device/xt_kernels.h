@@ -81,6 +81,18 @@ static __global__ void k_reset(Counters *c, uint32_t mask) {
     }
 }
 
+// k_reset over `gridDim.x` consecutive counter blocks (one per lane).
+static __global__ void k_reset_lanes(Counters *c, uint32_t mask) {
+    Counters *b = c + blockIdx.x;
+    const uint32_t i = threadIdx.x;
+    if (i < CNT_WORDS && (mask >> i) & 1u) b->n[i] = 0;
+    if (i == 0 && (mask >> 31) & 1u) {
+        b->queries = 0;
+        b->retests = 0;
+        b->steps_exceeded = 0;
+    }
+}
+
 static __global__ void k_set_count(Counters *c, uint32_t which, uint32_t value) { c->n[which] = value; }
 
 static __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n) {
@@ -171,22 +183,28 @@ __device__ __forceinline__ void aa_coords(const JobDev &j, uint32_t px, uint32_t
 }
 #pragma clang fp contract(on)
 
-// Slots [0, spp_pass * NT): slot i renders sample pass_base + i / NT of the
-// pixel at tile position i % NT.
+// Slots [0, spp_pass * qn): slot i renders sample pass_base + i / qn of the
+// pixel at tile position q0 + i % qn (a whole pass has q0 = 0, qn = NT); its
+// path id is path_base + i (path state, radiance and FP64 rays are indexed
+// by path id). The path state word ps1.w holds the query count (low 16 bits)
+// and the job's rdepth (high 16 bits).
 static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
-                                                     Queue out, uint32_t *out_count, float4 *ps0, float4 *ps1,
-                                                     float4 *rad, double *ray64) {
-    const uint32_t n = spp_pass * j.NT;
+                                                     uint32_t q0, uint32_t qn, uint32_t path_base, Queue out,
+                                                     uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad,
+                                                     double *ray64) {
+    const uint32_t n = spp_pass * qn;
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t qword = 1u | (j.rdepth << 16);
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
+        const uint32_t path = path_base + i;
         bool live = false;
         uint32_t x = 0, y = 0, s = 0;
         if (i < n) {
-            const uint32_t sl = i / j.NT, q = i - sl * j.NT;
+            const uint32_t sl = i / qn, q = q0 + (i - sl * qn);
             s = pass_base + sl;
             live = slot_pixel(j, q, x, y) && s < j.total;
-            rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         double o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
         uint32_t key = 0;
@@ -196,10 +214,10 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint
             float cx, cy;
             aa_coords(j, x, y, s / j.spp, cx, cy);
             camera_ray(j, cx, cy, key, o, d);
-            ps0[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
-            ps1[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(1u));
+            ps0[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
+            ps1[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(qword));
             if (ray64) {
-                double *r = ray64 + 6 * (size_t)i;
+                double *r = ray64 + 6 * (size_t)path;
                 r[0] = o[0];
                 r[1] = o[1];
                 r[2] = o[2];
@@ -212,7 +230,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint
         if (live) {
             out.r0[slot] = make_float4((float)o[0], (float)o[1], (float)o[2], __uint_as_float(XT_NONE));
             out.r1[slot] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
-            out.path[slot] = i;
+            out.path[slot] = path;
         }
     }
 }
@@ -398,7 +416,7 @@ __device__ __forceinline__ void push_ray(Queue &out, uint32_t slot, f3 o, f3 d, 
 // X is the value of the current query: miss -> env, emissive -> F * Le,
 // continue -> F * f * X'. Materials: lambert.cc:30-40, phong.cc:35-55,
 // blinnphong.cc:34-61, emissive.cc:20-27.
-static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const JobDev j, const Queue in,
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, Queue out,
                                                        uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad) {
     const uint32_t n = *in_count;
@@ -416,7 +434,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
             const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
             const float4 s0 = ps0[path], s1 = ps1[path];
             const f3 T = mk(s0.x, s0.y, s0.z), F = mk(s1.x, s1.y, s1.z);
-            const uint32_t key = __float_as_uint(s0.w), q = __float_as_uint(s1.w);
+            const uint32_t key = __float_as_uint(s0.w), qword = __float_as_uint(s1.w);
+            const uint32_t q = qword & 0xffffu, rdepth = qword >> 16;
             const uint32_t prim = __float_as_uint(h.y);
             if (prim == XT_NONE) {
                 const f3 L = cmul(T, env_sample(S.env, d));
@@ -430,7 +449,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                 if (m.type == XT_MAT_EMISSIVE) {
                     const f3 L = cmul(cmul(T, F), mk(m.ke[0], m.ke[1], m.ke[2]));
                     rad[path] = make_float4(L.x, L.y, L.z, 0.f);
-                } else if (q < j.rdepth) {
+                } else if (q < rdepth) {
                     Draws r{xt_stream(key, q), 0};
                     f3 f, dir;
                     if (m.type == XT_MAT_LAMBERT) {
@@ -450,7 +469,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                     }
                     const f3 T2 = cmul(cmul(T, F), f);
                     ps0[path] = make_float4(T2.x, T2.y, T2.z, s0.w);
-                    ps1[path] = make_float4(f.x, f.y, f.z, __uint_as_float(q + 1));
+                    ps1[path] = make_float4(f.x, f.y, f.z, __uint_as_float(qword + 1));
                     cont = true;
                     no = p;
                     nd = dir;
@@ -467,7 +486,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
 
 // AO, first query (src/xtcore/integrator/ao/integrator.cc:12-53): a hit
 // spawns one hemisphere ray; a miss scores w = 1.
-static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao1(const SceneView S, const JobDev j, const Queue in,
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao1(const SceneView S, const Queue in,
                                                         const uint32_t *in_count, const float4 *hit, Queue out,
                                                         uint32_t *out_count, const float4 *ps0, float4 *rad) {
     const uint32_t n = *in_count;
@@ -504,7 +523,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao1(const SceneView S
     }
 }
 
-static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao2(const JobDev j, const Queue in, const uint32_t *in_count,
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao2(const Queue in, const uint32_t *in_count,
                                                         const float4 *hit, float4 *rad) {
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -570,13 +589,17 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_stencil(const Queue i
 
 // --------------------------------------------------------------- accumulation
 
-// Per pixel, samples of this pass in sample order. REFERENCE reproduces the
-// float read-modify-write of each integrator (tile_t::read/write,
-// lib/nimg/color.cc:35-40 clamps); FP64 keeps a Neumaier sum per channel.
+// Per pixel of tile positions [q0, q0 + qn), samples of this pass in sample
+// order (radiance of sample pass_base + sl at rad[sl * qn + q - q0]).
+// REFERENCE reproduces the float read-modify-write of each integrator
+// (tile_t::read/write, lib/nimg/color.cc:35-40 clamps); FP64 keeps a
+// Neumaier sum per channel.
 static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
-                                                         const float4 *rad, float4 *accf, double2 *acc64) {
+                                                         uint32_t q0, uint32_t qn, const float4 *rad, float4 *accf,
+                                                         double2 *acc64) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < j.NT; q += stride) {
+    for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < qn; qi += stride) {
+        const uint32_t q = q0 + qi;
         uint32_t x, y;
         if (!slot_pixel(j, q, x, y)) continue;
         const uint32_t pix = y * j.W + x;
@@ -584,7 +607,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, 
         double2 s0 = acc64[3 * (size_t)pix], s1 = acc64[3 * (size_t)pix + 1], s2 = acc64[3 * (size_t)pix + 2];
         for (uint32_t sl = 0; sl < spp_pass; ++sl) {
             if (pass_base + sl >= j.total) break;
-            const float4 v = rad[sl * j.NT + q];
+            const float4 v = rad[sl * qn + qi];
             if (j.accum == XT_ACCUM_REFERENCE) {
                 if (j.integrator == XT_INTEGRATOR_STENCIL) {
                     if (v.x != 0.f) {

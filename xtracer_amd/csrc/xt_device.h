@@ -25,6 +25,8 @@ struct WaveQueue {
     uint32_t *path = nullptr;
 };
 
+#define XT_MAX_LANES 16
+
 struct WaveBuffers {
     size_t cap = 0;
     WaveQueue q[2];
@@ -32,7 +34,7 @@ struct WaveBuffers {
     uint32_t *retest = nullptr;
     float4 *ps0 = nullptr, *ps1 = nullptr, *rad = nullptr;
     uint32_t *flags = nullptr;
-    Counters *cnt = nullptr;
+    Counters *cnt = nullptr;  // XT_MAX_LANES counter blocks
     double *ray64 = nullptr;
     std::vector<void *> allocs;
     uint64_t bytes = 0;
@@ -58,7 +60,7 @@ struct xt_dev {
     uint32_t next_scene = 1;
     WaveBuffers wave;
     AccumBuffers accum;
-    Counters *host_counters = nullptr;
+    Counters *host_counters = nullptr;  // pinned, XT_MAX_LANES blocks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t tev[XT_TRACE_EVENTS];
     int trace_launches = 0;
@@ -68,10 +70,27 @@ struct xt_dev {
     std::string err;
 };
 
+// Blocks for a grid-stride launch over n items, at least 1, at most cap.
+static inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
 int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64);
 int xt_dev_ensure_accum(xt_dev *d, size_t npix);
 void xt_dev_free_wave(xt_dev *d);
 void xt_dev_free_accum(xt_dev *d);
-int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const double *ray64, uint32_t *flags,
-                    size_t cap, bool time_it, float *ms);
+// A sub-queue of the wavefront buffers: slots [off, off + cap) of the ray
+// queues, the hit and the re-test arrays, counted by its own counter block.
+// A one-job render uses one lane over the whole buffers; the server gives
+// each (scene, integrator, accel) group of a pass a lane of its own.
+struct WaveLane {
+    size_t off = 0, cap = 0;
+    Counters *cnt = nullptr;
+};
+int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &lane, const double *ray64,
+                    uint32_t *flags, bool time_it);
 void xt_serve_shutdown(xt_dev *d);
+bool xt_serve_uses_scene(const xt_dev *d, uint32_t scene_id);

@@ -61,12 +61,6 @@ int upload(xt_dev *d, T **p, const std::vector<T> &v, std::vector<void *> *track
     return XT_OK;
 }
 
-unsigned grid_for(size_t n, unsigned block, unsigned cap) {
-    size_t g = (n + block - 1) / block;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (unsigned)g;
-}
 
 }  // namespace
 
@@ -103,7 +97,7 @@ int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64) {
     rc = rc ? rc : dev_alloc(d, &w.ps1, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.rad, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.flags, cap, t, b);
-    rc = rc ? rc : dev_alloc(d, &w.cnt, 1, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.cnt, XT_MAX_LANES, t, b);
     if (!rc && need64) rc = dev_alloc(d, &w.ray64, 6 * cap, t, b);
     if (rc) {
         xt_dev_free_wave(d);
@@ -147,7 +141,11 @@ int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes) {
     hipEventCreate(&d->ev1);
     for (int i = 0; i < XT_TRACE_EVENTS; ++i) hipEventCreate(&d->tev[i]);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) d->n_cu = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) {
+        d->n_cu = prop.multiProcessorCount;
+        // default budget: 90% of the device memory
+        if (!d->budget) d->budget = (uint64_t)(0.9 * (double)prop.totalGlobalMem);
+    }
     *out = d;
     return XT_OK;
 }
@@ -298,6 +296,10 @@ int xt_scene_release(xt_dev *d, uint32_t scene_id) {
     if (!d) return XT_E_INVALID;
     auto it = d->scenes.find(scene_id);
     if (it == d->scenes.end()) return XT_E_NOTFOUND;
+    if (xt_serve_uses_scene(d, scene_id)) {
+        d->err = "scene is in use by served jobs";
+        return XT_E_STATE;
+    }
     hipSetDevice(d->device);
     hipStreamSynchronize(d->stream);
     for (void *p : it->second.allocs) hipFree(p);
@@ -349,21 +351,24 @@ static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream
     }
 }
 
-// Launches trace + FP64 re-test over queue `qi` (count in cnt->n[qi]).
-int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const double *ray64, uint32_t *flags,
-                    size_t cap, bool time_it, float *ms) {
+// Launches trace + FP64 re-test over queue `qi` of a lane (count in
+// lane.cnt->n[qi]).
+int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &L, const double *ray64,
+                    uint32_t *flags, bool time_it) {
     WaveBuffers &w = d->wave;
-    Queue q{w.q[qi].r0, w.q[qi].r1, w.q[qi].path};
+    Queue q{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
+    float4 *hit = w.hit + L.off;
+    uint32_t *retest = w.retest + L.off;
     const bool timed = time_it && d->trace_launches < XT_TRACE_EVENTS - 1;
     if (timed) hipEventRecord(d->tev[d->trace_launches], d->stream);
     const xt_tuning &tn = d->tuning;
     if (accel == XT_ACCEL_OCTREE) {
         // 16 resident 64-lane blocks per CU (LDS: 20 KB each)
-        const unsigned g = (unsigned)std::min<size_t>(cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
+        const unsigned g = (unsigned)std::min<size_t>(L.cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
         hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, d->stream,
-                           ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
+                           ds.view, q, &L.cnt->n[qi], hit, retest, L.cnt);
     } else {
-        launch_kd_trace(tn, d->n_cu, cap, d->stream, ds.view, q, &w.cnt->n[qi], w.hit, w.retest, w.cnt);
+        launch_kd_trace(tn, d->n_cu, L.cap, d->stream, ds.view, q, &L.cnt->n[qi], hit, retest, L.cnt);
     }
     if (timed) {
         hipEventRecord(d->tev[d->trace_launches + 1], d->stream);
@@ -372,14 +377,13 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const doub
     }
     XT_LAUNCHED();
     if (accel == XT_ACCEL_OCTREE) {
-        hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(cap / 64 + 1, XT_OCT_BLOCK, 2048)),
-                           dim3(XT_OCT_BLOCK), 0, d->stream, ds.view, q, w.hit, w.retest, w.cnt, ray64, flags);
+        hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(L.cap / 64 + 1, XT_OCT_BLOCK, 2048)),
+                           dim3(XT_OCT_BLOCK), 0, d->stream, ds.view, q, hit, retest, L.cnt, ray64, flags);
     } else {
-        hipLaunchKernelGGL(k_retest<XT_ACCEL_KDTREE>, dim3(grid_for(cap / 64 + 1, XT_BLOCK, 1024)), dim3(XT_BLOCK),
-                           0, d->stream, ds.view, q, w.hit, w.retest, w.cnt, ray64, flags);
+        hipLaunchKernelGGL(k_retest<XT_ACCEL_KDTREE>, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 1024)), dim3(XT_BLOCK),
+                           0, d->stream, ds.view, q, hit, retest, L.cnt, ray64, flags);
     }
     XT_LAUNCHED();
-    (void)ms;
     return XT_OK;
 }
 
@@ -406,7 +410,7 @@ extern "C" int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const vo
     if (flags) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, flags, 0u, (size_t)n);
     d->trace_launches = 0;
     d->timed_pairs.clear();
-    rc = xt_launch_trace(d, it->second, accel, 0, nullptr, flags, (size_t)n, kernel_ms != nullptr, nullptr);
+    rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, flags, kernel_ms != nullptr);
     w.q[0].r0 = save0;
     w.q[0].r1 = save1;
     w.hit = saveh;
@@ -458,7 +462,7 @@ extern "C" int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, w.flags, 0u, (size_t)n);
     d->trace_launches = 0;
     d->timed_pairs.clear();
-    rc = xt_launch_trace(d, it->second, accel, 0, nullptr, w.flags, n, false, nullptr);
+    rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, w.flags, false);
     if (rc) {
         release();
         return rc;

@@ -5,4 +5,15 @@
 int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_out);
 uint32_t xt_rounds(const JobDev &j);
 uint32_t xt_auto_spp_pass(const JobDev &j, uint32_t requested, uint32_t target_paths);
-int xt_launch_shade(xt_dev *d, const DevScene &ds, const JobDev &j, uint32_t r, int qi, size_t cap);
+
+// One (scene, integrator, accel) group of a pass and the lane its rays use.
+struct PassGroup {
+    const DevScene *ds = nullptr;
+    uint32_t integrator = 0, accel = 0, rounds = 0;
+    WaveLane lane;
+    bool live = true;
+};
+int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &lane);
+int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace);
+int xt_ensure_host_counters(xt_dev *d);
+

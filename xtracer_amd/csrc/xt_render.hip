@@ -29,12 +29,6 @@
     } while (0)
 
 namespace {
-unsigned grid_for(size_t n, unsigned block, unsigned cap) {
-    size_t g = (n + block - 1) / block;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (unsigned)g;
-}
 }  // namespace
 
 int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_out) {
@@ -51,6 +45,10 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
     if (job->integrator > XT_INTEGRATOR_PATHTRACER || job->accel > XT_ACCEL_OCTREE || job->aa_mode > XT_AA_RANDOM ||
         job->accum > XT_ACCUM_FP64) {
         d->err = "bad job parameter";
+        return XT_E_INVALID;
+    }
+    if (job->rdepth > 0xffffu) {
+        d->err = "rdepth above 65535";
         return XT_E_INVALID;
     }
     if (job->integrator == XT_INTEGRATOR_PATHTRACER && !it->second.pt_supported) {
@@ -113,33 +111,84 @@ uint32_t xt_auto_spp_pass(const JobDev &j, uint32_t requested, uint32_t target) 
     return (uint32_t)std::min<uint64_t>(s, j.total);
 }
 
-// Launches one shading step for query round r over queue qi into queue qi^1.
-int xt_launch_shade(xt_dev *d, const DevScene &ds, const JobDev &j, uint32_t r, int qi, size_t cap) {
+// Launches one shading step of a lane for query round r over queue qi into
+// queue qi^1.
+int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &L) {
     WaveBuffers &w = d->wave;
-    Queue in{w.q[qi].r0, w.q[qi].r1, w.q[qi].path};
-    Queue out{w.q[qi ^ 1].r0, w.q[qi ^ 1].r1, w.q[qi ^ 1].path};
-    const unsigned g = grid_for(cap, XT_BLOCK, 8192);
-    switch (j.integrator) {
+    Queue in{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
+    Queue out{w.q[qi ^ 1].r0 + L.off, w.q[qi ^ 1].r1 + L.off, w.q[qi ^ 1].path + L.off};
+    const float4 *hit = w.hit + L.off;
+    const uint32_t *n_in = &L.cnt->n[qi];
+    uint32_t *n_out = &L.cnt->n[qi ^ 1];
+    const unsigned g = grid_for(L.cap, XT_BLOCK, 8192);
+    switch (integrator) {
         case XT_INTEGRATOR_STENCIL:
-            hipLaunchKernelGGL(k_shade_stencil, dim3(g), dim3(XT_BLOCK), 0, d->stream, in, &w.cnt->n[qi], w.hit, w.rad);
+            hipLaunchKernelGGL(k_shade_stencil, dim3(g), dim3(XT_BLOCK), 0, d->stream, in, n_in, hit, w.rad);
             break;
         case XT_INTEGRATOR_DEPTH:
-            hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, &w.cnt->n[qi], w.hit,
-                               w.ray64, w.rad);
+            hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, n_in, hit, w.ray64,
+                               w.rad);
             break;
         case XT_INTEGRATOR_AO:
             if (r == 0)
-                hipLaunchKernelGGL(k_shade_ao1, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, j, in, &w.cnt->n[qi],
-                                   w.hit, out, &w.cnt->n[qi ^ 1], w.ps0, w.rad);
+                hipLaunchKernelGGL(k_shade_ao1, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, n_in, hit, out,
+                                   n_out, w.ps0, w.rad);
             else
-                hipLaunchKernelGGL(k_shade_ao2, dim3(g), dim3(XT_BLOCK), 0, d->stream, j, in, &w.cnt->n[qi], w.hit,
-                                   w.rad);
+                hipLaunchKernelGGL(k_shade_ao2, dim3(g), dim3(XT_BLOCK), 0, d->stream, in, n_in, hit, w.rad);
             break;
         default:
-            hipLaunchKernelGGL(k_shade_pt, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, j, in, &w.cnt->n[qi], w.hit,
-                               out, &w.cnt->n[qi ^ 1], w.ps0, w.ps1, w.rad);
+            hipLaunchKernelGGL(k_shade_pt, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, n_in, hit, out, n_out,
+                               w.ps0, w.ps1, w.rad);
     }
     XT_LAUNCHED();
+    return XT_OK;
+}
+
+int xt_ensure_host_counters(xt_dev *d) {
+    if (!d->host_counters)
+        XT_TRY(hipHostMalloc((void **)&d->host_counters, XT_MAX_LANES * sizeof(Counters), hipHostMallocDefault));
+    return XT_OK;
+}
+
+// Query rounds of one pass over every group: trace + shade per round, queue
+// 0 holding the camera rays. Every second round past the second, the host
+// reads the queue counts and drops groups whose paths have all ended.
+int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
+    WaveBuffers &w = d->wave;
+    uint32_t max_rounds = 0;
+    for (int k = 0; k < n_groups; ++k) {
+        groups[k].live = true;
+        max_rounds = std::max(max_rounds, groups[k].rounds);
+    }
+    int qi = 0;
+    for (uint32_t r = 0; r < max_rounds; ++r) {
+        for (int k = 0; k < n_groups; ++k) {
+            PassGroup &G = groups[k];
+            if (!G.live || r >= G.rounds) continue;
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, G.lane.cnt,
+                               (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR));
+            const bool fp64_rays = G.integrator == XT_INTEGRATOR_DEPTH && r == 0;
+            int rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
+                                     time_trace);
+            if (rc) return rc;
+            rc = xt_launch_shade(d, *G.ds, G.integrator, r, qi, G.lane);
+            if (rc) return rc;
+        }
+        qi ^= 1;
+        // stop early once every path of the pass has ended
+        if (max_rounds > 2 && r + 1 < max_rounds && (r % 2 == 1)) {
+            XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, XT_MAX_LANES * sizeof(Counters), hipMemcpyDeviceToHost,
+                                  d->stream));
+            XT_TRY(hipStreamSynchronize(d->stream));
+            bool any = false;
+            for (int k = 0; k < n_groups; ++k) {
+                PassGroup &G = groups[k];
+                if (G.live && (r + 1 >= G.rounds || d->host_counters[G.lane.cnt - w.cnt].n[qi] == 0)) G.live = false;
+                any = any || G.live;
+            }
+            if (!any) break;
+        }
+    }
     return XT_OK;
 }
 
@@ -160,7 +209,8 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
     if (rc) return rc;
     rc = xt_dev_ensure_accum(d, (size_t)j.W * j.H);
     if (rc) return rc;
-    if (!d->host_counters) XT_TRY(hipHostMalloc((void **)&d->host_counters, sizeof(Counters), hipHostMallocDefault));
+    rc = xt_ensure_host_counters(d);
+    if (rc) return rc;
     WaveBuffers &w = d->wave;
     AccumBuffers &a = d->accum;
     const size_t npix = (size_t)j.W * j.H;
@@ -174,33 +224,25 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
     d->timed_pairs.clear();
     const bool time_trace = stats != nullptr;
     XT_TRY(hipEventRecord(d->ev0, d->stream));
-    const uint32_t rounds = xt_rounds(j);
+    PassGroup G;
+    G.ds = ds;
+    G.integrator = j.integrator;
+    G.accel = job->accel;
+    G.rounds = xt_rounds(j);
+    G.lane = WaveLane{0, cap, w.cnt};
     uint32_t passes = 0;
     for (uint32_t base = 0; base < j.total; base += spp_pass) {
         ++passes;
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x7u);
         Queue q0{w.q[0].r0, w.q[0].r1, w.q[0].path};
         hipLaunchKernelGGL(k_camera, dim3(grid_for(cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
-                           spp_pass, q0, &w.cnt->n[0], w.ps0, w.ps1, w.rad, need64 ? w.ray64 : nullptr);
+                           spp_pass, 0u, j.NT, 0u, q0, &w.cnt->n[0], w.ps0, w.ps1, w.rad,
+                           need64 ? w.ray64 : nullptr);
         XT_LAUNCHED();
-        int qi = 0;
-        for (uint32_t r = 0; r < rounds; ++r) {
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR));
-            rc = xt_launch_trace(d, *ds, (int)job->accel, qi, need64 && r == 0 ? w.ray64 : nullptr, nullptr, cap,
-                                 time_trace, nullptr);
-            if (rc) return rc;
-            rc = xt_launch_shade(d, *ds, j, r, qi, cap);
-            if (rc) return rc;
-            qi ^= 1;
-            // stop early once every path of the pass has ended
-            if (rounds > 2 && r + 1 < rounds && (r % 2 == 1)) {
-                XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
-                XT_TRY(hipStreamSynchronize(d->stream));
-                if (d->host_counters->n[qi] == 0) break;
-            }
-        }
+        rc = xt_run_rounds(d, &G, 1, time_trace);
+        if (rc) return rc;
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(j.NT, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
-                           spp_pass, w.rad, a.accf, a.acc64);
+                           spp_pass, 0u, j.NT, w.rad, a.accf, a.acc64);
         XT_LAUNCHED();
     }
     hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, a.accf,
