@@ -48,14 +48,23 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=1,
                     help="CPU baseline threads (the reference's glibc rand() lock makes more threads slower)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="search the launch shapes first (default: the library's defaults, which are the shapes "
+                         "this search picked for this workload on MI355X)")
     ap.add_argument("--tuning", default="", help="launch shape as JSON (xt_tuning fields); skips the autotuner")
+    ap.add_argument("--serve-jobs", type=int, default=64, help="jobs of the mixed batch (0: skip serving)")
+    ap.add_argument("--serve-requests", type=int, default=48, help="Poisson requests per arrival rate")
+    ap.add_argument("--serve-rates", default="0.5,0.8,0.95", help="arrival rates as fractions of capacity")
+    ap.add_argument("--strands", type=int, default=50000, help="strands of the hair scene in the serving mix")
     return ap.parse_args()
 
 
 def cpu_baseline(scene_path, args, threads):
-    """Reference renderer (oracle/_ref) on a tile subset of the same frame;
-    falls back to the oracle port. Returns the cpu_baseline object."""
+    """Reference renderer (oracle/_ref) on a spread subset of the frame's
+    32x32 tiles; falls back to the oracle port. A query's cost does not depend
+    on how many samples a pixel takes, so the sample runs at 1 spp on more
+    tiles (raising spp only once every tile is in). Returns the cpu_baseline
+    object."""
     from oracle import oracle as O
     W, H = args.width, args.height
     tiles = ((W + 31) // 32) * ((H + 31) // 32)
@@ -68,38 +77,157 @@ def cpu_baseline(scene_path, args, threads):
         load_s = time.perf_counter() - t0
         log(f"cpu baseline: reference scene loaded in {load_s:.1f}s")
         sec, q = ctypes.c_double(), ctypes.c_uint64()
-        # calibrate on 8 tiles per thread spread over the frame (the cost per
-        # tile varies widely: sky tiles are nearly free), then size the sample
-        stride = max(1, tiles // (8 * threads))
-        L.xr_render_tiles(r._h, 3, W, H, args.spp, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec),
-                          ctypes.byref(q))
-        log(f"cpu baseline: calibration {(tiles + stride - 1) // stride} tiles on {threads} threads in {sec.value:.2f}s")
+        # calibrate on 4 tiles per thread spread over the frame at 1 spp
+        stride = max(1, tiles // (4 * threads))
+        L.xr_render_tiles(r._h, 3, W, H, 1, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec), ctypes.byref(q))
         n_cal = (tiles + stride - 1) // stride
-        per_tile = sec.value / n_cal
-        want = int(args.cpu_seconds * threads / max(per_tile, 1e-4))
-        if want >= 2 * n_cal:
-            stride2 = max(2, tiles // want)
-            L.xr_render_tiles(r._h, 3, W, H, args.spp, 1, args.rdepth, threads, 1, stride2, ctypes.byref(sec),
+        log(f"cpu baseline: calibration {n_cal} tiles at 1 spp on {threads} threads in {sec.value:.2f}s")
+        spp = 1
+        if sec.value < 0.5 * args.cpu_seconds:
+            per_tile = max(sec.value / n_cal, 1e-6)
+            want = args.cpu_seconds * threads / per_tile  # tile-samples that fill the budget
+            if want >= tiles:
+                stride, spp = 1, int(max(1, min(args.spp, want // tiles)))
+            else:
+                stride = max(1, int(tiles // max(want, 1)))
+            L.xr_render_tiles(r._h, 3, W, H, spp, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec),
                               ctypes.byref(q))
-            stride = stride2
-            log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles in {sec.value:.2f}s")
+            log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles at {spp} spp in {sec.value:.2f}s")
         n_tiles = (tiles + stride - 1) // stride
         return {"value": q.value / sec.value / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
                 "sample": f"reference pathtracer (compiled -O3 from its sources, OpenMP dynamic tiles, {threads} "
-                          f"thread(s)) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) at full spp; "
-                          f"{q.value} Scene::intersection calls in {sec.value:.2f}s; scene load + octree build "
-                          f"{load_s:.1f}s not timed"}
+                          f"thread(s)) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) of the same frame at "
+                          f"{spp} spp, rdepth {args.rdepth}; {q.value} Scene::intersection calls in "
+                          f"{sec.value:.2f}s; scene load + octree build {load_s:.1f}s not timed"}
     import xtracer_amd as xa
     from xtracer_amd import scenes
     s = xa.Scene(scene_path, build=True)
     orc = O.OracleScene(s)
     cam = scenes.camera_dict(**scenes.ICOSPHERE_CAMERA)
-    stride = max(1, tiles // threads)
+    stride = max(1, tiles // (4 * threads))
     t0 = time.perf_counter()
-    _, nq = orc.render(3, W, H, cam, samples=args.spp, rdepth=args.rdepth, threads=threads, tile_stride=stride)
+    _, nq = orc.render(3, W, H, cam, samples=1, rdepth=args.rdepth, threads=threads, tile_stride=stride)
     dt = time.perf_counter() - t0
     return {"value": nq / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle port on every {stride}th tile at full spp; {nq} queries in {dt:.2f}s"}
+            "sample": f"FP64 oracle port on every {stride}th tile at 1 spp; {nq} queries in {dt:.2f}s"}
+
+
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def _orbit_camera(xa, cam, rng):
+    """A random pose on the sphere through the scene camera's position around
+    its target (seeded), same lens."""
+    import numpy as np
+    p, t = np.asarray(cam["position"], float), np.asarray(cam["target"], float)
+    r = float(np.linalg.norm(p - t))
+    az, el = rng.uniform(0, 2 * np.pi), rng.uniform(0.05, 0.6)
+    pos = t + r * np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
+    return xa.make_camera(tuple(pos), tuple(t), (0.0, 1.0, 0.0), cam.get("fov", 45.0), cam.get("aperture", 0.0),
+                          cam.get("flength", 0.0))
+
+
+def serving(dev, xa, scenes, args, ico_sid, ico_scene):
+    """Mixed-job batch (served vs one job per launch) and Poisson arrivals
+    with p50/p99 latency. Scenes: 64 spheres (ao), displaced icosphere,
+    strands and terrain (pathtracer)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(2024)
+    pool = []
+    t0 = time.perf_counter()
+    for kind, fn, kw, integ, rdepth in (
+            ("spheres", scenes.spheres, {}, xa.AO, 1),
+            ("strands", scenes.strands_scene, {"strands": args.strands}, xa.PATHTRACER, 6),
+            ("terrain", scenes.terrain_scene, {}, xa.PATHTRACER, 8)):
+        sc = xa.Scene(fn(**kw))
+        pool.append((kind, sc, dev.upload(sc), integ, rdepth, scenes.SCENES[kind][1]))
+        log(f"serving scene {kind}: {sc.info().n_prims} prims")
+    pool.insert(1, ("icosphere", ico_scene, ico_sid, xa.PATHTRACER, 8, scenes.ICOSPHERE_CAMERA))
+    setup_s = time.perf_counter() - t0
+
+    def draw(n, key0, orbit):
+        jobs = []
+        for k in range(n):
+            kind, sc, sid, integ, rdepth, camd = pool[int(rng.integers(len(pool)))]
+            res = int(rng.choice([128, 256, 512, 1024]))
+            spp = int(rng.choice([1, 4, 16, 64]))
+            cam = _orbit_camera(xa, camd, rng) if orbit else sc.camera()
+            jobs.append(xa.make_job(sid, integ, res, res, cam, samples=spp, rdepth=rdepth, seed=7,
+                                    job_key=key0 + k))
+        return jobs
+
+    def plain(jobs, arrivals=None):
+        out = torch.empty((1024, 1024, 4), dtype=torch.float32, device="cuda")
+        lat, q = [], 0
+        t0 = time.perf_counter()
+        for k, j in enumerate(jobs):
+            if arrivals is not None:
+                wait = t0 + arrivals[k] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+            st = dev.render_device(j, out.data_ptr())
+            q += st.queries
+            lat.append((time.perf_counter() - t0 - (arrivals[k] if arrivals is not None else 0.0)) * 1e3)
+        return time.perf_counter() - t0, q, lat
+
+    def served(jobs, arrivals=None):
+        info0 = dev.serve_info()
+        t0 = time.perf_counter()
+        pending, lat, k = {}, [], 0
+        while k < len(jobs) or pending:
+            now = time.perf_counter() - t0
+            while k < len(jobs) and (arrivals is None or arrivals[k] <= now):
+                j = jobs[k]
+                late = 0.0 if arrivals is None else (time.perf_counter() - t0 - arrivals[k]) * 1e3
+                pending[dev.submit(j)] = (j.width, j.height, late)
+                k += 1
+            if not pending:
+                time.sleep(max(0.0, t0 + arrivals[k] - time.perf_counter()))
+                continue
+            dev.serve_step()
+            for t in list(pending):
+                w, h, late = pending[t]
+                done, _, st = dev.poll(t, w, h)
+                if done:
+                    lat.append(late + st.latency_ms)
+                    del pending[t]
+        wall = time.perf_counter() - t0
+        info1 = dev.serve_info()
+        return wall, info1["queries"] - info0["queries"], lat
+
+    res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1)}
+    jobs = draw(args.serve_jobs, 10000, False)
+    samples = sum(j.width * j.height * j.samples for j in jobs)
+    served(jobs[:4])  # warm the server's buffers
+    pw, pq, plat = plain(jobs)
+    sw, sq, slat = served(jobs)
+    res["batch"] = {"jobs": len(jobs), "samples": samples,
+                    "plain": {"wall_s": round(pw, 3), "Mrays_s": round(pq / pw / 1e6, 1),
+                              "p50_ms": round(_pct(plat, 50), 1), "p99_ms": round(_pct(plat, 99), 1)},
+                    "served": {"wall_s": round(sw, 3), "Mrays_s": round(sq / sw / 1e6, 1),
+                               "p50_ms": round(_pct(slat, 50), 1), "p99_ms": round(_pct(slat, 99), 1)},
+                    "served_over_plain": round(pw / sw, 3)}
+    log(f"serving batch: plain {pw:.2f}s served {sw:.2f}s")
+    mu = len(jobs) / sw
+    res["capacity_jobs_s"] = round(mu, 2)
+    res["poisson"] = []
+    for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+        lam = rate * mu
+        jobs = draw(args.serve_requests, 20000 + int(rate * 1000) * 100, True)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs)))
+        sw, sq, slat = served(jobs, arrivals)
+        pw, pq, plat = plain(jobs, arrivals)
+        res["poisson"].append({"load": rate, "lambda_jobs_s": round(lam, 2), "requests": len(jobs),
+                               "served": {"p50_ms": round(_pct(slat, 50), 1), "p99_ms": round(_pct(slat, 99), 1),
+                                          "Mrays_s": round(sq / sw / 1e6, 1)},
+                               "plain": {"p50_ms": round(_pct(plat, 50), 1), "p99_ms": round(_pct(plat, 99), 1),
+                                         "Mrays_s": round(pq / pw / 1e6, 1)}})
+        log(f"poisson load {rate}: served p99 {res['poisson'][-1]['served']['p99_ms']} ms, "
+            f"plain p99 {res['poisson'][-1]['plain']['p99_ms']} ms")
+    return res
 
 
 def main():
@@ -136,7 +264,7 @@ def main():
     if args.tuning:
         dev.set_tuning(**json.loads(args.tuning))
     tuning = dev.tuning()
-    if not args.no_autotune and not args.tuning:
+    if args.autotune and not args.tuning:
         from xtracer_amd.autotune import autotune
         t0 = time.perf_counter()
         tuning, _ = autotune(dev, job(999), out.data_ptr(), wave_paths=(4 << 20, 16 << 20), log=log)
@@ -163,6 +291,7 @@ def main():
     pipe_bytes = queries * PIPE_BYTES_PER_QUERY + samples * PIPE_BYTES_PER_SAMPLE
     pipe = pipe_bytes / (wall) / 1e9
     threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
+    serve = serving(dev, xa, scenes, args, sid, scene) if args.serve_jobs > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(path, args, threads)
     line = {
         "metric": "Mrays/s at 1080p 16spp (pathtracer, plain one-job-per-launch)",
@@ -194,6 +323,8 @@ def main():
                   "kd_nodes": info.n_kd_nodes, "kd_refs": info.n_kd_refs, "kd_depth": info.kd_depth,
                   "passes": stats[0].passes if stats else 0, "tuning": tuning},
     }
+    if serve:
+        line["serving"] = serve
     if cpu:
         line["extra"]["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)

@@ -146,9 +146,9 @@ const char *xt_last_error(const xt_dev *d);
  * force (trace_blocks_per_cu 0: as many blocks as a CU keeps resident). */
 #define XT_TUNING_AUTO 0xffffffffu /* trace_blocks_per_cu: back to resident fill */
 typedef struct xt_tuning {
-    uint32_t kd_stack;        /* kd short-stack entries per lane: 8, 12, 16 or 24 (16) */
+    uint32_t kd_stack;        /* kd short-stack entries per lane: 8, 12, 16 or 24 (12) */
     uint32_t trace_block;     /* lanes per trace block: 128 or 256 (256) */
-    uint32_t wave_paths;      /* paths in flight per wavefront pass, 4096..2^28 (8M) */
+    uint32_t wave_paths;      /* paths in flight per wavefront pass, 4096..2^28 (16M) */
     uint32_t trace_blocks_per_cu; /* persistent trace grid: blocks per CU, <= 64 (resident fill) */
 } xt_tuning;
 int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t);

@@ -11,7 +11,7 @@ from tests.helpers import scene_path
 def small_passes(device):
     device.set_tuning(wave_paths=1 << 16)
     yield device
-    device.set_tuning(wave_paths=8 << 20)
+    device.set_tuning(wave_paths=16 << 20)
 
 
 def _scenes(device):

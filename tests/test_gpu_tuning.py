@@ -9,7 +9,7 @@ from oracle.oracle import OracleScene
 from xtracer_amd import scenes
 from tests.helpers import mixed_rays, scene_path
 
-DEFAULT = {"kd_stack": 16, "trace_block": 256, "wave_paths": 8 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO}
+DEFAULT = {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO}
 
 
 @pytest.fixture
@@ -21,7 +21,7 @@ def tuned(device):
 @pytest.mark.gpu
 def test_tuning_roundtrip_and_rejects(tuned):
     t = tuned.set_tuning(**DEFAULT)
-    assert t == {"kd_stack": 16, "trace_block": 256, "wave_paths": 8 << 20, "trace_blocks_per_cu": 0}
+    assert t == {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": 0}
     for bad in ({"kd_stack": 10}, {"trace_block": 64}, {"wave_paths": 100}, {"trace_blocks_per_cu": 65}):
         with pytest.raises(xa.XtError):
             tuned.set_tuning(**bad)

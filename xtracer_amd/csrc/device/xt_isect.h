@@ -42,7 +42,8 @@ struct Best {
     uint32_t prim;
     uint32_t obj;
     float u, v;
-    float mark_t;  // smallest t of a deferred candidate (INF: none)
+    float mark_t;    // smallest t of a deferred candidate (INF: none)
+    uint32_t graze;  // a grazing triangle left to the FP64 graze pass (XT_NONE: none)
 };
 
 __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t obj, float u, float v) {
@@ -63,19 +64,18 @@ __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t 
 }
 
 // FP32 triangle test (Moller-Trumbore) with deferral marks.
-__device__ __forceinline__ void tri32(const f3 o, const f3 d, float tsmall, const float4 a, const float4 b,
+// Returns true when the ray grazes the triangle (|n.d| < XT_GRAZE): FP32
+// cannot place that hit and the caller decides it in FP64.
+__device__ __forceinline__ bool tri32(const f3 o, const f3 d, float tsmall, const float4 a, const float4 b,
                                       const float4 c, uint32_t prim, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
-    if (XT_PRIM_FLAGS(tf) & XT_PF_DEGENERATE) return;
+    if (XT_PRIM_FLAGS(tf) & XT_PF_DEGENERATE) return false;
     const f3 v0 = mk(a.x, a.y, a.z);
     const f3 e1 = sub(mk(b.x, b.y, b.z), v0);
     const f3 e2 = sub(mk(c.x, c.y, c.z), v0);
     const f3 pv = cross3(d, e2);
     const float det = dot3(e1, pv);
-    if (fabsf(det) * c.w < XT_GRAZE) {  // |n.d| small: the reference threshold is 1e-8
-        best.mark_t = fminf(best.mark_t, 0.f);
-        return;
-    }
+    if (fabsf(det) * c.w < XT_GRAZE) return true;  // |n.d| small: the reference threshold is 1e-8
     const float inv = 1.f / det;
     const f3 tv = sub(o, v0);
     const float u = dot3(tv, pv) * inv;
@@ -84,14 +84,15 @@ __device__ __forceinline__ void tri32(const f3 o, const f3 d, float tsmall, cons
     const float t = dot3(e2, qv) * inv;
     const float w = 1.f - u - v;
     const float m = fminf(fminf(u, v), w);
-    if (t < -tsmall || t > best.t * (1.f + XT_TIE)) return;
+    if (t < -tsmall || t > best.t * (1.f + XT_TIE)) return false;
     if (m > -XT_BAND && m < XT_BAND) best.mark_t = fminf(best.mark_t, t);
-    if (m < 0.f) return;
+    if (m < 0.f) return false;
     if (t < tsmall) {
         best.mark_t = fminf(best.mark_t, fmaxf(t, 0.f));
-        return;
+        return false;
     }
     offer(best, t, prim, __float_as_uint(b.w), u, v);
+    return false;
 }
 
 // Sphere test. The reference keeps only the smaller root and needs it

@@ -25,6 +25,7 @@ enum {
     CNT_Q1 = 1,      // ray count of queue 1
     CNT_RETEST = 2,  // rays listed for the FP64 re-test
     CNT_CURSOR = 3,  // work cursor of the persistent trace kernel
+    CNT_GRAZE = 4,   // rays listed for the FP64 test of one grazing triangle
     CNT_WORDS = 8
 };
 struct Counters {
@@ -247,6 +248,7 @@ __device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Bes
     b.obj = XT_NONE;
     b.u = b.v = 0.f;
     b.mark_t = INFINITY;
+    b.graze = XT_NONE;
     if (r.self_t > 0.f && r.origin != XT_NONE) {
         b.t = r.self_t;
         b.prim = r.origin;
@@ -260,7 +262,8 @@ __device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Bes
 // that drew short traversals pick up more work instead of idling.
 template <int ACCEL, int STACK, int BS>
 static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
-                                                     float4 *hit, uint32_t *retest_list, Counters *cnt) {
+                                                     float4 *hit, uint32_t *retest_list, uint2 *graze_list,
+                                                     Counters *cnt, uint32_t defer_graze) {
     __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? STACK * BS : 1];
     __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * BS : 1];
     const uint32_t n = *in_count;
@@ -273,6 +276,7 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
         if (base >= n) break;
         const uint32_t i = base + lane;
         bool mark = false;
+        uint32_t gprim = XT_NONE;
         if (i < n) {
             const float4 a = in.r0[i], b = in.r1[i];
             RayF r;
@@ -280,7 +284,7 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             r.d = mk(b.x, b.y, b.z);
             r.origin = __float_as_uint(a.w);
             r.self_t = b.w;
-            Tester32 tst{S, r, tsmall_of(r.o), {}};
+            Tester32 tst{S, r, tsmall_of(r.o), {}, defer_graze != 0};
             seed_best(S, r, tst.best);
             for (uint32_t k = 0; k < S.n_planes; ++k)
                 if (XT_PLANE_BASE + k != r.origin) plane32(r.o, r.d, tst.tsmall, S.planes[k], XT_PLANE_BASE + k, tst.best);
@@ -291,6 +295,7 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             // a deferred candidate matters only up to the closest hit (a miss
             // has t = INF, so compare only when something was deferred)
             mark = !ok || (bb.mark_t != INFINITY && bb.mark_t <= bb.t * (1.f + XT_TIE));
+            if (!mark) gprim = bb.graze;
             if (!mark && bb.prim < S.n_prims && !(r.self_t > 0.f && bb.prim == r.origin)) {
                 const float4 a = S.prims[3 * bb.prim], b = S.prims[3 * bb.prim + 1];
                 if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
@@ -303,6 +308,45 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
         }
         const uint32_t slot = wave_append(mark, &cnt->n[CNT_RETEST]);
         if (mark) retest_list[slot] = i;
+        const bool g = gprim != XT_NONE;
+        const uint32_t gslot = wave_append(g, &cnt->n[CNT_GRAZE]);
+        if (g) graze_list[gslot] = make_uint2(i, gprim);
+    }
+}
+
+// FP64 decision of the one grazing triangle a ray left undecided
+// (Triangle::intersection, triangle.cc:82-123, on the same FP32 ray). The
+// FP32 closest hit stands unless the grazing hit is nearer; a grazing hit
+// near the origin or within the tie band of the closest sends the ray to the
+// full FP64 re-test.
+static __global__ __launch_bounds__(XT_BLOCK) void k_graze(const SceneView S, const Queue in, float4 *hit,
+                                                      const uint2 *list, uint32_t *retest_list, Counters *cnt) {
+    const uint32_t n = cnt->n[CNT_GRAZE];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t k = base + threadIdx.x;
+        bool need = false;
+        uint32_t i = 0;
+        if (k < n) {
+            const uint2 e = list[k];
+            i = e.x;
+            const float4 a = in.r0[i], b = in.r1[i];
+            double bc[3];
+            const double t = tri64(d3{a.x, a.y, a.z}, d3{b.x, b.y, b.z}, S.prims64[e.y], bc);
+            if (t > 0.0) {
+                const float tf = (float)t;
+                const float4 h = hit[i];
+                if (tf < tsmall_of(mk(a.x, a.y, a.z))) {
+                    need = true;
+                } else if (__float_as_uint(h.y) == XT_NONE || tf < h.x * (1.f - XT_TIE)) {
+                    hit[i] = make_float4(tf, __uint_as_float(e.y), (float)bc[1], (float)bc[2]);
+                } else if (tf <= h.x * (1.f + XT_TIE)) {
+                    need = true;
+                }
+            }
+        }
+        const uint32_t slot = wave_append(need, &cnt->n[CNT_RETEST]);
+        if (need) retest_list[slot] = i;
     }
 }
 

@@ -16,12 +16,72 @@ struct Local {
     size_t leaves = 0;
 };
 
+
+// A reference: primitive id and its bounds inside the current node.
+struct Ref {
+    uint32_t id;
+    Box32 b;
+};
+
 struct Task {
     uint32_t slot;
     Box32 box;
-    std::vector<uint32_t> refs;
+    std::vector<Ref> refs;
     int depth;
 };
+
+// Bounds of triangle v (9 doubles) clipped to box (Sutherland-Hodgman in
+// FP64, inclusive at the planes), widened outward by a relative 1e-6 of the
+// box extent and intersected with the box. False when the triangle does not
+// reach the box.
+bool clip_triangle(const double *v, const Box32 &box, Box32 &out) {
+    double poly[2][12][3];
+    int n = 3, cur = 0;
+    for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) poly[0][k][a] = v[3 * k + a];
+    for (int a = 0; a < 3 && n > 0; ++a) {
+        for (int side = 0; side < 2 && n > 0; ++side) {
+            const double plane = side == 0 ? (double)box.lo[a] : (double)box.hi[a];
+            const double sgn = side == 0 ? 1.0 : -1.0;  // inside: sgn * (x - plane) >= 0
+            int m = 0;
+            double(*src)[3] = poly[cur];
+            double(*dst)[3] = poly[cur ^ 1];
+            for (int k = 0; k < n && m < 11; ++k) {
+                const double *p = src[k], *q = src[(k + 1) % n];
+                const double dp = sgn * (p[a] - plane), dq = sgn * (q[a] - plane);
+                if (dp >= 0) {
+                    for (int c = 0; c < 3; ++c) dst[m][c] = p[c];
+                    ++m;
+                }
+                if ((dp >= 0) != (dq >= 0) && m < 11) {
+                    const double t = dp / (dp - dq);
+                    for (int c = 0; c < 3; ++c) dst[m][c] = p[c] + t * (q[c] - p[c]);
+                    dst[m][a] = plane;
+                    ++m;
+                }
+            }
+            n = m;
+            cur ^= 1;
+        }
+    }
+    if (n == 0) return false;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], poly[cur][k][a]);
+            hi[a] = std::max(hi[a], poly[cur][k][a]);
+        }
+    for (int a = 0; a < 3; ++a) {
+        const double ext = (double)box.hi[a] - (double)box.lo[a];
+        const double pad = 1e-6 * ext + 1e-12 * (std::fabs(lo[a]) + std::fabs(hi[a]));
+        const float l = std::nextafter((float)(lo[a] - pad), -INFINITY);
+        const float h = std::nextafter((float)(hi[a] + pad), INFINITY);
+        out.lo[a] = std::max(l, box.lo[a]);
+        out.hi[a] = std::min(h, box.hi[a]);
+        if (out.lo[a] > out.hi[a]) return false;
+    }
+    return true;
+}
 
 inline float area(const Box32 &b) {
     float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
@@ -29,22 +89,23 @@ inline float area(const Box32 &b) {
 }
 
 struct Builder {
-    const std::vector<Box32> &pb;
+    const double *verts;  // 9 per prim, or null
+    const uint8_t *is_tri;
     KdParams p;
     int spawn_depth;  // -1: never spawn
     std::vector<Task> *spawned;
 
-    void leaf(Local &L, uint32_t slot, const std::vector<uint32_t> &refs, int depth) {
+    void leaf(Local &L, uint32_t slot, const std::vector<Ref> &refs, int depth) {
         XtKdNode n;
         n.word = ((uint32_t)L.refs.size() << 2) | 3u;
         n.count = (uint32_t)refs.size();
         L.nodes[slot] = n;
-        L.refs.insert(L.refs.end(), refs.begin(), refs.end());
+        for (const Ref &r : refs) L.refs.push_back(r.id);
         L.depth = std::max(L.depth, depth);
         L.leaves++;
     }
 
-    bool find_split(const Box32 &box, const std::vector<uint32_t> &refs, int &axis, float &split) {
+    bool find_split(const Box32 &box, const std::vector<Ref> &refs, int &axis, float &split) {
         const size_t n = refs.size();
         const float sa = area(box);
         if (!(sa > 0.f)) return false;
@@ -58,8 +119,8 @@ struct Builder {
             std::fill(st.begin(), st.end(), 0u);
             std::fill(en.begin(), en.end(), 0u);
             const float scale = (float)K / ext;
-            for (uint32_t r : refs) {
-                float l = std::max(pb[r].lo[a], lo), h = std::min(pb[r].hi[a], box.hi[a]);
+            for (const Ref &r : refs) {
+                float l = std::max(r.b.lo[a], lo), h = std::min(r.b.hi[a], box.hi[a]);
                 int s = (int)((l - lo) * scale), e = (int)((h - lo) * scale);
                 s = std::min(std::max(s, 0), K - 1);
                 e = std::min(std::max(e, 0), K - 1);
@@ -88,7 +149,7 @@ struct Builder {
         return found;
     }
 
-    void build(Local &L, uint32_t slot, const Box32 &box, std::vector<uint32_t> &refs, int depth) {
+    void build(Local &L, uint32_t slot, const Box32 &box, std::vector<Ref> &refs, int depth) {
         if (spawn_depth >= 0 && depth == spawn_depth && refs.size() > (size_t)p.leaf_prims) {
             spawned->push_back(Task{slot, box, std::move(refs), depth});
             return;
@@ -99,26 +160,41 @@ struct Builder {
             leaf(L, slot, refs, depth);
             return;
         }
-        std::vector<uint32_t> left, right;
+        std::vector<Ref> left, right;
         left.reserve(refs.size());
         right.reserve(refs.size());
-        for (uint32_t r : refs) {
-            float l = pb[r].lo[axis], h = pb[r].hi[axis];
-            bool to_l = l < split || (l <= split && h <= split);
-            bool to_r = h > split || (l >= split && h >= split);
-            if (to_l) left.push_back(r);
-            if (to_r) right.push_back(r);
+        Box32 lb = box, rb = box;
+        lb.hi[axis] = split;
+        rb.lo[axis] = split;
+        for (const Ref &r : refs) {
+            const float l = r.b.lo[axis], h = r.b.hi[axis];
+            const bool to_l = l < split || (l <= split && h <= split);
+            const bool to_r = h > split || (l >= split && h >= split);
+            if (to_l && to_r && l < split && h > split && verts && is_tri[r.id]) {
+                // straddler: keep only the sides its surface reaches
+                Box32 cb;
+                if (clip_triangle(verts + 9 * (size_t)r.id, lb, cb)) left.push_back(Ref{r.id, cb});
+                if (clip_triangle(verts + 9 * (size_t)r.id, rb, cb)) right.push_back(Ref{r.id, cb});
+                continue;
+            }
+            if (to_l) {
+                Ref c = r;
+                c.b.hi[axis] = std::min(c.b.hi[axis], split);
+                left.push_back(c);
+            }
+            if (to_r) {
+                Ref c = r;
+                c.b.lo[axis] = std::max(c.b.lo[axis], split);
+                right.push_back(c);
+            }
         }
-        std::vector<uint32_t>().swap(refs);
+        std::vector<Ref>().swap(refs);
         uint32_t c = (uint32_t)L.nodes.size();
         L.nodes.resize(c + 2);
         XtKdNode in;
         in.word = (c << 2) | (uint32_t)axis;
         in.split = split;
         L.nodes[slot] = in;
-        Box32 lb = box, rb = box;
-        lb.hi[axis] = split;
-        rb.lo[axis] = split;
         build(L, c, lb, left, depth + 1);
         build(L, c + 1, rb, right, depth + 1);
     }
@@ -126,7 +202,8 @@ struct Builder {
 
 }  // namespace
 
-void kdtree_build(const std::vector<Box32> &pb, const KdParams &params, KdTree &out) {
+void kdtree_build(const std::vector<Box32> &pb, const double *tri_verts, const uint8_t *is_tri,
+                  const KdParams &params, KdTree &out) {
     KdParams p = params;
     const size_t n = pb.size();
     if (p.max_depth <= 0) p.max_depth = (int)std::lround(8.0 + 1.3 * std::log2((double)std::max<size_t>(n, 1)));
@@ -146,15 +223,15 @@ void kdtree_build(const std::vector<Box32> &pb, const KdParams &params, KdTree &
     if (n == 0)
         for (int a = 0; a < 3; ++a) root.lo[a] = root.hi[a] = 0.f;
 
-    std::vector<uint32_t> refs(n);
-    for (size_t i = 0; i < n; ++i) refs[i] = (uint32_t)i;
+    std::vector<Ref> refs(n);
+    for (size_t i = 0; i < n; ++i) refs[i] = Ref{(uint32_t)i, pb[i]};
 
     int spawn = 0;
     while ((1 << spawn) < threads * 4 && spawn < 10) ++spawn;
     if (n < 50000 || threads == 1) spawn = -1;
 
     std::vector<Task> tasks;
-    Builder top{pb, p, spawn, &tasks};
+    Builder top{tri_verts, is_tri, p, spawn, &tasks};
     Local G;
     G.nodes.resize(1);
     top.build(G, 0, root, refs, 0);
@@ -172,7 +249,7 @@ void kdtree_build(const std::vector<Box32> &pb, const KdParams &params, KdTree &
                 size_t k = cursor.fetch_add(1);
                 if (k >= order.size()) return;
                 Task &t = tasks[order[k]];
-                Builder b{pb, p, -1, nullptr};
+                Builder b{tri_verts, is_tri, p, -1, nullptr};
                 Local &L = locals[order[k]];
                 L.nodes.resize(1);
                 b.build(L, 0, t.box, t.refs, t.depth);

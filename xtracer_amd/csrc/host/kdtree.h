@@ -1,8 +1,10 @@
 // SAH kd-tree over the primitive soup. The reference has no kd-tree (its
 // README lists one, src/xtcore/bvh.cc is an empty stub), so the structure is
 // this project's own; only its closest-hit results are checked against the
-// reference. Binned SAH with an empty-space bonus; straddling primitives are
-// referenced from both children.
+// reference. Binned SAH with an empty-space bonus. A triangle that straddles
+// a split is clipped to each child's box (in FP64, bounds widened outward)
+// and is referenced only by the children its surface reaches, with the
+// clipped bounds driving the deeper splits.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -34,6 +36,9 @@ struct KdTree {
     size_t leaves = 0;
 };
 
-void kdtree_build(const std::vector<Box32> &prim_boxes, const KdParams &p, KdTree &out);
+// tri_verts: 9 doubles per primitive (v0 v1 v2) for the triangles marked in
+// is_tri; null disables clipping (every straddler goes to both children).
+void kdtree_build(const std::vector<Box32> &prim_boxes, const double *tri_verts, const uint8_t *is_tri,
+                  const KdParams &p, KdTree &out);
 
 }  // namespace xt

@@ -32,6 +32,7 @@ struct WaveBuffers {
     WaveQueue q[2];
     float4 *hit = nullptr;
     uint32_t *retest = nullptr;
+    uint2 *graze = nullptr;
     float4 *ps0 = nullptr, *ps1 = nullptr, *rad = nullptr;
     uint32_t *flags = nullptr;
     Counters *cnt = nullptr;  // XT_MAX_LANES counter blocks
@@ -66,7 +67,7 @@ struct xt_dev {
     int trace_launches = 0;
     std::vector<int> timed_pairs;
     ServeState *serve = nullptr;
-    xt_tuning tuning{16, 256, 8u << 20, 0};
+    xt_tuning tuning{12, 256, 16u << 20, 0};
     std::string err;
 };
 

@@ -16,6 +16,7 @@ cached under a directory keyed by their parameters.
 import hashlib
 import json
 import os
+import sys
 
 import numpy as np
 
@@ -59,6 +60,34 @@ def _mark(path):
         f.write("ok\n")
 
 
+def _fmt_verts(chunk):
+    return "".join("v %.9g %.9g %.9g\n" % tuple(v) for v in chunk)
+
+
+def _fmt_faces(chunk):
+    return "".join("f %d %d %d\n" % tuple(t) for t in chunk)
+
+
+def _gpu_initialised():
+    torch = sys.modules.get("torch")
+    try:
+        return bool(torch is not None and torch.cuda.is_initialized())
+    except Exception:
+        return True
+
+
+def _format(fn, rows):
+    """Formats rows with fn, in forked worker processes for big meshes (only
+    while this process has not initialised the GPU runtime)."""
+    if len(rows) < (1 << 20) or _gpu_initialised():
+        return fn(rows)
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    chunks = np.array_split(rows, workers * 4)
+    with mp.get_context("fork").Pool(workers) as pool:
+        return "".join(pool.map(fn, chunks))
+
+
 def write_obj(path, verts, faces, normals=None):
     """Writes vertices (float32-exact) and 1-based triangle faces."""
     verts = np.asarray(verts, np.float32)
@@ -66,13 +95,13 @@ def write_obj(path, verts, faces, normals=None):
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         f.write("# xtracer_amd synthetic mesh\n")
-        f.write("".join("v %.9g %.9g %.9g\n" % tuple(v) for v in verts.astype(np.float64)))
+        f.write(_format(_fmt_verts, verts.astype(np.float64)))
         if normals is not None:
             normals = np.asarray(normals, np.float32).astype(np.float64)
             f.write("".join("vn %.9g %.9g %.9g\n" % tuple(n) for n in normals))
             f.write("".join("f %d//%d %d//%d %d//%d\n" % (a, a, b, b, c, c) for a, b, c in (faces + 1)))
         else:
-            f.write("".join("f %d %d %d\n" % tuple(t) for t in (faces + 1)))
+            f.write(_format(_fmt_faces, faces + 1))
     os.replace(tmp, path)
 
 
