@@ -26,6 +26,11 @@ HBM_PEAK_GBS = 8000.0
 TRACE_BYTES_PER_QUERY = 48      # read ray (2 x float4) + write hit (float4)
 PIPE_BYTES_PER_QUERY = 184      # ray/hit/path-state streams of trace + shade + compaction
 PIPE_BYTES_PER_SAMPLE = 152     # camera ray + state write, FP64 accumulate read + write
+# HBM bytes the trace kernel moves per query on this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
+# (profiles/hbm_traffic_pathtracer_icosphere_1080p16.md); a measured figure,
+# re-taken whenever the kernel or the tree changes.
+TRACE_TRAFFIC_PER_QUERY = 383
 
 
 def log(msg):
@@ -56,6 +61,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=48, help="Poisson requests per arrival rate")
     ap.add_argument("--serve-rates", default="0.5,0.8,0.95", help="arrival rates as fractions of capacity")
     ap.add_argument("--strands", type=int, default=50000, help="strands of the hair scene in the serving mix")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, copy MIB MiB device-to-device once (a kernel of known traffic "
+                         "that scales hardware byte counters)")
     return ap.parse_args()
 
 
@@ -281,6 +289,14 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {wall:.3f}s")
+    if args.copy_probe > 0:
+        src = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
     queries = sum(s.queries for s in stats)
     samples = sum(s.samples for s in stats)
     trace_ms = sum(s.ms_trace for s in stats)
@@ -311,7 +327,10 @@ def main():
                    "width": args.width, "height": args.height, "spp": args.spp, "rdepth": args.rdepth,
                    "prims": info.n_prims, "accel": args.accel},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": int(TRACE_TRAFFIC_PER_QUERY * queries / max(1, launches)) if launches else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per k_trace dispatch "
+                                       "(copy-calibrated), per query, times queries per launch",
                      "kernel": "k_trace (kd-tree)", "bytes_per_query": TRACE_BYTES_PER_QUERY,
                      "avg_launch_ms": round(trace_ms / max(1, launches), 4), "launches": launches},
         "roofline_pipeline": {"achieved": round(pipe, 2), "frac": round(pipe / HBM_PEAK_GBS, 5),
