@@ -1,6 +1,7 @@
 // Primitive intersection on the device.
 //
-// FP32 tests (used by every traversal) decide robust cases and raise a
+// FP32 tests (used by every traversal; triangles with the watertight test)
+// decide robust cases and raise a
 // "re-test" mark for a ray whenever a decision might differ from the
 // reference's FP64 predicate: a barycentric coordinate within XT_BAND of an
 // edge, a grazing angle, a near-tangent sphere, a tiny t, or two candidates
@@ -64,25 +65,76 @@ __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t 
 }
 
 // FP32 triangle test (Moller-Trumbore) with deferral marks.
-// Returns true when the ray grazes the triangle (|n.d| < XT_GRAZE): FP32
-// cannot place that hit and the caller decides it in FP64.
-__device__ __forceinline__ bool tri32(const f3 o, const f3 d, float tsmall, const float4 a, const float4 b,
+// Per-ray shear of the watertight ray/triangle test (Woop, Benthin, Wald,
+// "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013): with kz the
+// largest direction component and (kx, ky) the others in winding order, the
+// ray becomes the +z axis of a sheared frame in which p - o sits at
+// (q[kx] - sx q[kz], q[ky] - sy q[kz], sz q[kz]). Triangles that share an
+// edge evaluate its 2D edge function from the same transformed vertices, so
+// a ray can cross a mesh between two triangles only by hitting one of them.
+struct Shear {
+    float sx, sy, sz;
+    float dz;      // |d[kz]|: |n.d| = |det| * dz / |e1 x e2|
+    uint32_t k;    // kx | ky << 2 | kz << 4
+};
+
+__device__ __forceinline__ float comp3(f3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ Shear make_shear(f3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const uint32_t kz = ax > ay ? (ax > az ? 0u : 2u) : (ay > az ? 1u : 2u);
+    uint32_t kx = kz == 2u ? 0u : kz + 1u;
+    uint32_t ky = kx == 2u ? 0u : kx + 1u;
+    const float dk = comp3(d, kz);
+    if (dk < 0.f) {  // keep the winding
+        const uint32_t t = kx;
+        kx = ky;
+        ky = t;
+    }
+    Shear s;
+    s.k = kx | ky << 2 | kz << 4;
+    if (dk == 0.f) {  // zero direction: no triangle is hit
+        s.sx = s.sy = s.sz = s.dz = 0.f;
+        return s;
+    }
+    s.sz = 1.f / dk;
+    s.sx = comp3(d, kx) * s.sz;
+    s.sy = comp3(d, ky) * s.sz;
+    s.dz = fabsf(dk);
+    return s;
+}
+
+// A point relative to the origin in the sheared frame: (x, y) and the
+// unscaled depth p[kz].
+__device__ __forceinline__ void shear3(const Shear &sh, f3 p, float &x, float &y, float &z) {
+    z = comp3(p, sh.k >> 4);
+    x = comp3(p, sh.k & 3u) - sh.sx * z;
+    y = comp3(p, (sh.k >> 2) & 3u) - sh.sy * z;
+}
+
+#pragma clang fp contract(off)
+// 2D edge function p*q - r*s with both products rounded: the triangle on the
+// other side of an edge computes the exact negation.
+__device__ __forceinline__ float edge2(float p, float q, float r, float s) { return p * q - r * s; }
+#pragma clang fp contract(on)
+
+// FP32 watertight test of one triangle. Returns true when the ray grazes it
+// (|n.d| < XT_GRAZE): FP32 cannot place that hit and the caller decides it
+// in FP64. (u, v) offered with a hit are the weights of v1 and v2.
+__device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall, const float4 a, const float4 b,
                                       const float4 c, uint32_t prim, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
-    if (XT_PRIM_FLAGS(tf) & XT_PF_DEGENERATE) return false;
-    const f3 v0 = mk(a.x, a.y, a.z);
-    const f3 e1 = sub(mk(b.x, b.y, b.z), v0);
-    const f3 e2 = sub(mk(c.x, c.y, c.z), v0);
-    const f3 pv = cross3(d, e2);
-    const float det = dot3(e1, pv);
-    if (fabsf(det) * c.w < XT_GRAZE) return true;  // |n.d| small: the reference threshold is 1e-8
+    if ((XT_PRIM_FLAGS(tf) & XT_PF_DEGENERATE) || sh.dz == 0.f) return false;
+    float ax, ay, az, bx, by, bz, cx, cy, cz;
+    shear3(sh, sub(mk(a.x, a.y, a.z), o), ax, ay, az);
+    shear3(sh, sub(mk(b.x, b.y, b.z), o), bx, by, bz);
+    shear3(sh, sub(mk(c.x, c.y, c.z), o), cx, cy, cz);
+    const float U = edge2(cx, by, cy, bx), V = edge2(ax, cy, ay, cx), W = edge2(bx, ay, by, ax);
+    const float det = U + V + W;
+    if (fabsf(det) * sh.dz * c.w < XT_GRAZE) return true;  // |n.d| small: the reference threshold is 1e-8
     const float inv = 1.f / det;
-    const f3 tv = sub(o, v0);
-    const float u = dot3(tv, pv) * inv;
-    const f3 qv = cross3(tv, e1);
-    const float v = dot3(d, qv) * inv;
-    const float t = dot3(e2, qv) * inv;
-    const float w = 1.f - u - v;
+    const float t = (U * az + V * bz + W * cz) * sh.sz * inv;
+    const float u = V * inv, v = W * inv, w = U * inv;
     const float m = fminf(fminf(u, v), w);
     if (t < -tsmall || t > best.t * (1.f + XT_TIE)) return false;
     if (m > -XT_BAND && m < XT_BAND) best.mark_t = fminf(best.mark_t, t);
@@ -153,6 +205,29 @@ __device__ __forceinline__ void plane32(const f3 o, const f3 d, float tsmall, co
 struct d3 {
     double x, y, z;
 };
+
+// Conservative FP32 screen for the FP64 re-test: true only when the
+// watertight FP32 test puts the triangle clearly outside (5% of an edge
+// beyond it), clearly behind the origin, or clearly beyond `limit`, with the
+// FP32 error (about 4e-7 * distance / (edge * |n.d|) relative) under 1%.
+__device__ __forceinline__ bool screen_out32(const f3 o, const Shear &sh, const float4 a, const float4 b,
+                                             const float4 c, float limit) {
+    if (sh.dz == 0.f) return false;
+    float ax, ay, az, bx, by, bz, cx, cy, cz;
+    const f3 A = sub(mk(a.x, a.y, a.z), o);
+    shear3(sh, A, ax, ay, az);
+    shear3(sh, sub(mk(b.x, b.y, b.z), o), bx, by, bz);
+    shear3(sh, sub(mk(c.x, c.y, c.z), o), cx, cy, cz);
+    const float U = edge2(cx, by, cy, bx), V = edge2(ax, cy, ay, cx), W = edge2(bx, ay, by, ax);
+    const float det = U + V + W;
+    const float g = fabsf(det) * sh.dz * c.w;  // |n.d|
+    const float dist = fmaxf(fmaxf(fabsf(A.x), fabsf(A.y)), fabsf(A.z)) + sqrtf(1.f / c.w);
+    if (!(g > 0.f) || 4e-7f * dist * sqrtf(c.w) > 0.01f * g) return false;
+    const float inv = 1.f / det;
+    const float m = fminf(fminf(U * inv, V * inv), W * inv);
+    const float t = (U * az + V * bz + W * cz) * sh.sz * inv;
+    return m < -0.05f || t < -0.05f * dist || t > limit * 1.05f;
+}
 
 #pragma clang fp contract(off)
 

@@ -284,7 +284,7 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             r.d = mk(b.x, b.y, b.z);
             r.origin = __float_as_uint(a.w);
             r.self_t = b.w;
-            Tester32 tst{S, r, tsmall_of(r.o), {}, defer_graze != 0};
+            Tester32 tst{S, r, tsmall_of(r.o), {}, defer_graze != 0, make_shear(r.d)};
             seed_best(S, r, tst.best);
             for (uint32_t k = 0; k < S.n_planes; ++k)
                 if (XT_PLANE_BASE + k != r.origin) plane32(r.o, r.d, tst.tsmall, S.planes[k], XT_PLANE_BASE + k, tst.best);
@@ -372,7 +372,8 @@ static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT
         r.d = mk(b.x, b.y, b.z);
         r.origin = __float_as_uint(a.w);
         r.self_t = b.w;
-        Tester64 tst{S, r, {r.o.x, r.o.y, r.o.z}, {r.d.x, r.d.y, r.d.z}, INFINITY, XT_NONE, XT_NONE, 0, 0, 0};
+        Tester64 tst{S, r, {r.o.x, r.o.y, r.o.z}, {r.d.x, r.d.y, r.d.z}, INFINITY, XT_NONE, XT_NONE, 0, 0, 0,
+                     r.o, Shear{}};
         if (ray64) {
             const double *q = ray64 + 6 * (size_t)in.path[i];
             tst.o = {q[0], q[1], q[2]};
@@ -391,6 +392,8 @@ static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT
         }
         const f3 o32 = mk((float)tst.o.x, (float)tst.o.y, (float)tst.o.z);
         const f3 d32 = mk((float)tst.d.x, (float)tst.d.y, (float)tst.d.z);
+        tst.o32 = o32;
+        tst.sh = make_shear(d32);
         bool ok;
         if (ACCEL == XT_ACCEL_KDTREE) ok = walk_kd<XT_KD_STACK>(S, tst, o32, d32, kstk + threadIdx.x, BS, 1e-4f);
         else ok = walk_oct(S, tst, o32, d32, ostk + threadIdx.x, BS, 1e-4f);

@@ -62,13 +62,14 @@ struct Tester32 {
     float tsmall;
     Best best;
     bool defer_graze;  // the ray stands for an FP64 ray: grazing triangles take the full re-test
+    Shear sh;
     __device__ __forceinline__ float limit() const { return best.t; }
     __device__ __forceinline__ void test(uint32_t prim) {
         if (prim == r.origin) return;
         const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1];
         if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
             const float4 c = S.prims[3 * prim + 2];
-            if (tri32(r.o, r.d, tsmall, a, b, c, prim, best)) {
+            if (tri32(r.o, sh, tsmall, a, b, c, prim, best)) {
                 // one grazing triangle per ray is decided in FP64 after the
                 // traversal (k_graze); a second one, or a ray that stands
                 // for an FP64 ray (whose small difference moves a grazing
@@ -89,6 +90,8 @@ struct Tester64 {
     double bt;
     uint32_t prim, obj;
     double b0, b1, b2;
+    f3 o32;    // the ray rounded to FP32, for the FP32 screen
+    Shear sh;
     __device__ __forceinline__ float limit() const {
         return bt == INFINITY ? INFINITY : (float)bt * (1.f + 1e-4f) + 1e-30f;
     }
@@ -106,6 +109,9 @@ struct Tester64 {
         if (p == r.origin) return;
         const float4 a = S.prims[3 * p], b = S.prims[3 * p + 1];
         const uint32_t tf = __float_as_uint(a.w);
+        if (XT_PRIM_TYPE(tf) == XT_PRIM_TRIANGLE &&
+            screen_out32(o32, sh, a, b, S.prims[3 * p + 2], limit()))
+            return;
         const XtPrim64 q = S.prims64[p];
         if (XT_PRIM_TYPE(tf) == XT_PRIM_TRIANGLE) {
             double bc[3];
