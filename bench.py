@@ -280,7 +280,8 @@ def main():
     for k in range(args.warmup):
         t0 = time.perf_counter()
         st = dev.render_device(job(k), out.data_ptr())
-        log(f"warmup {k}: {time.perf_counter() - t0:.3f}s, {st.queries} queries, {st.retests} retests")
+        log(f"warmup {k}: {time.perf_counter() - t0:.3f}s, {st.queries} queries, {st.retests} retests, "
+            f"{st.trace_fail} FP32 walks and {st.retest_fail} FP64 walks out of steps")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()

@@ -209,6 +209,8 @@ typedef struct xt_job_stats {
     uint32_t passes;
     double latency_ms;     /* serving: completion - submission */
     double queue_ms;       /* serving: admission - submission */
+    uint64_t trace_fail;   /* FP32 traversals that ran out of steps or stack (re-tested in FP64) */
+    uint64_t retest_fail;  /* FP64 re-test traversals that ran out of steps (closest hit so far kept) */
 } xt_job_stats;
 
 /* Plain one-job-per-launch render. rgba_out: width*height*4 floats, host. */
@@ -223,8 +225,9 @@ int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_sta
  * XT_E_BUDGET when they never could. xt_serve_step runs one wavefront pass
  * shared by all admitted jobs (each advances by a fair share of samples) and
  * reports the jobs not yet finished. xt_poll sets *done once the job's image
- * is ready, copies it to rgba_out (width*height*4 floats, may be NULL) and
- * forgets the ticket; stats->latency_ms = completion - submission and
+ * is ready; with rgba_out (width*height*4 floats) it then copies the image
+ * and forgets the ticket, with rgba_out NULL it only reports (the image is
+ * kept for a later call); stats->latency_ms = completion - submission and
  * stats->queue_ms = admission - submission. A served image is bit-identical
  * to the same job rendered with xt_render. xt_drain steps until every
  * admitted and waiting job has finished. */

@@ -92,7 +92,8 @@ class Job(ctypes.Structure):
 class JobStats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_uint64), ("retests", ctypes.c_uint64), ("samples", ctypes.c_uint64),
                 ("ms_total", ctypes.c_double), ("ms_trace", ctypes.c_double), ("trace_launches", ctypes.c_uint32),
-                ("passes", ctypes.c_uint32), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double)]
+                ("passes", ctypes.c_uint32), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("trace_fail", ctypes.c_uint64), ("retest_fail", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -382,10 +383,13 @@ class Device:
     def poll(self, ticket, width, height):
         """(done, image or None, JobStats); a finished ticket is forgotten."""
         done = ctypes.c_int()
-        img = np.zeros((height, width, 4), np.float32)
         st = JobStats()
+        self._check(lib().xt_poll(self._d, ticket, ctypes.byref(done), None, ctypes.byref(st)))
+        if not done.value:
+            return False, None, st
+        img = np.empty((height, width, 4), np.float32)
         self._check(lib().xt_poll(self._d, ticket, ctypes.byref(done), _ptr(img), ctypes.byref(st)))
-        return bool(done.value), (img if done.value else None), st
+        return True, img, st
 
     def serve_info(self):
         info = ServeInfo()

@@ -30,7 +30,7 @@ enum {
 };
 struct Counters {
     uint32_t n[CNT_WORDS];
-    unsigned long long queries, retests, steps_exceeded, pad;
+    unsigned long long queries, retests, steps_exceeded, trace_fail;
 };
 
 struct Queue {
@@ -79,6 +79,7 @@ static __global__ void k_reset(Counters *c, uint32_t mask) {
         c->queries = 0;
         c->retests = 0;
         c->steps_exceeded = 0;
+        c->trace_fail = 0;
     }
 }
 
@@ -91,6 +92,7 @@ static __global__ void k_reset_lanes(Counters *c, uint32_t mask) {
         b->queries = 0;
         b->retests = 0;
         b->steps_exceeded = 0;
+        b->trace_fail = 0;
     }
 }
 
@@ -294,15 +296,16 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             Best &bb = tst.best;
             // a deferred candidate matters only up to the closest hit (a miss
             // has t = INF, so compare only when something was deferred)
+            if (!ok) atomicAdd(&cnt->trace_fail, 1ull);
             mark = !ok || (bb.mark_t != INFINITY && bb.mark_t <= bb.t * (1.f + XT_TIE));
             if (!mark) gprim = bb.graze;
             if (!mark && bb.prim < S.n_prims && !(r.self_t > 0.f && bb.prim == r.origin)) {
                 const float4 a = S.prims[3 * bb.prim], b = S.prims[3 * bb.prim + 1];
-                if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
-                    const float4 c = S.prims[3 * bb.prim + 2];
-                    const f3 n = cross3(sub(mk(b.x, b.y, b.z), mk(a.x, a.y, a.z)), sub(mk(c.x, c.y, c.z), mk(a.x, a.y, a.z)));
-                    if (fabsf(dot3(n, r.d)) * c.w < 0.05f) bb.t = tri_t64(r.o, r.d, a, b, c);
-                }
+                // the sheared-frame t of the FP32 test loses up to ~1e-7 * distance /
+                // (edge * |n.d|) relative; the closest hit gets the plane-equation
+                // t in FP64 instead
+                if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE)
+                    bb.t = tri_t64(r.o, r.d, a, b, S.prims[3 * bb.prim + 2]);
             }
             hit[i] = make_float4(bb.t, __uint_as_float(bb.prim), bb.u, bb.v);
         }

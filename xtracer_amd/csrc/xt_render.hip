@@ -258,6 +258,8 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         stats->ms_total = ms;
         stats->queries = d->host_counters->queries;
         stats->retests = d->host_counters->retests;
+        stats->trace_fail = d->host_counters->trace_fail;
+        stats->retest_fail = d->host_counters->steps_exceeded;
         stats->samples = (uint64_t)j.total * j.W * j.H;
         stats->passes = passes;
         double tsum = 0;

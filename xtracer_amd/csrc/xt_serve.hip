@@ -463,12 +463,13 @@ extern "C" int xt_poll(xt_dev *d, uint64_t ticket, int *done, float *rgba_out, x
         stats->queue_ms = J.state == WAITING ? now_ms() - J.t_submit : J.t_admit - J.t_submit;
     }
     if (J.state != FINISHED) return XT_OK;
-    if (rgba_out) std::memcpy(rgba_out, J.host.p, J.npix * sizeof(float4));
     if (stats) {
         stats->latency_ms = J.t_done - J.t_submit;
         stats->ms_total = J.t_done - J.t_admit;
     }
     *done = 1;
+    if (!rgba_out) return XT_OK;  // report only: the image stays until it is fetched
+    std::memcpy(rgba_out, J.host.p, J.npix * sizeof(float4));
     S.free_host.push_back(J.host);
     S.free_events.push_back(J.done_ev);
     S.jobs.erase(it);
