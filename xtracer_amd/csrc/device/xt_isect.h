@@ -43,9 +43,19 @@ struct Best {
     uint32_t prim;
     uint32_t obj;
     float u, v;
-    float mark_t;    // smallest t of a deferred candidate (INF: none)
-    uint32_t graze;  // a grazing triangle left to the FP64 graze pass (XT_NONE: none)
+    float mark_t;        // smallest t of a deferred candidate (INF: none)
+    uint32_t c0, c1;     // deferred candidates, for the FP64 re-check
+    uint32_t nc;         // candidates deferred (> 2: too many to list)
 };
+
+// Defers primitive `prim` at about t to the FP64 decision.
+__device__ __forceinline__ void defer(Best &b, float t, uint32_t prim) {
+    b.mark_t = fminf(b.mark_t, t);
+    if (prim == b.c0 || prim == b.c1) return;
+    if (b.nc == 0) b.c0 = prim;
+    else if (b.nc == 1) b.c1 = prim;
+    b.nc = b.nc < 3 ? b.nc + 1 : 3;
+}
 
 __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t obj, float u, float v) {
     // a kd-tree references a primitive from every leaf it overlaps: meeting
@@ -53,14 +63,14 @@ __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t 
     if (prim == b.prim) return;
     // ties go to the earlier object, then to the earlier soup index
     if (t < b.t || (t == b.t && (obj < b.obj || (obj == b.obj && prim < b.prim)))) {
-        if (b.t - t <= XT_TIE * t) b.mark_t = fminf(b.mark_t, t);  // runner-up within the tie band
+        if (b.t - t <= XT_TIE * t) defer(b, t, b.prim);  // the displaced best is within the tie band
         b.t = t;
         b.prim = prim;
         b.obj = obj;
         b.u = u;
         b.v = v;
     } else if (t - b.t <= XT_TIE * b.t) {
-        b.mark_t = fminf(b.mark_t, b.t);
+        defer(b, b.t, prim);
     }
 }
 
@@ -137,10 +147,10 @@ __device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall,
     const float u = V * inv, v = W * inv, w = U * inv;
     const float m = fminf(fminf(u, v), w);
     if (t < -tsmall || t > best.t * (1.f + XT_TIE)) return false;
-    if (m > -XT_BAND && m < XT_BAND) best.mark_t = fminf(best.mark_t, t);
+    if (m > -XT_BAND && m < XT_BAND) defer(best, t, prim);
     if (m < 0.f) return false;
     if (t < tsmall) {
-        best.mark_t = fminf(best.mark_t, fmaxf(t, 0.f));
+        defer(best, fmaxf(t, 0.f), prim);
         return false;
     }
     offer(best, t, prim, __float_as_uint(b.w), u, v);
@@ -163,20 +173,20 @@ __device__ __forceinline__ void sph32(const f3 o, const f3 d, float tsmall, cons
     const double disc = bb * bb - cc;
     const double scale = fmax(bb * bb, r2);
     if (fabs(disc) <= 1e-6 * scale) {
-        best.mark_t = fminf(best.mark_t, fmaxf((float)-bb, 0.f));
+        defer(best, fmaxf((float)-bb, 0.f), prim);
         if (disc <= 0.0) return;
     } else if (disc <= 0.0) {
         return;
     }
     if (fabs(cc) <= 1e-6 * r2) {  // origin on the surface
-        best.mark_t = fminf(best.mark_t, 0.f);
+        defer(best, 0.f, prim);
         return;
     }
     if (cc < 0.0 || bb >= 0.0) return;  // inside, or sphere behind
     const float t = (float)(cc / (sqrt(disc) - bb));
     if (t > best.t * (1.f + XT_TIE)) return;
     if (t < tsmall) {
-        best.mark_t = fminf(best.mark_t, t);
+        defer(best, t, prim);
         return;
     }
     offer(best, t, prim, __float_as_uint(b.w), 0.f, 0.f);
@@ -188,13 +198,13 @@ __device__ __forceinline__ void plane32(const f3 o, const f3 d, float tsmall, co
     const float ndd = dot3(n, d);
     const float nl = sqrtf(dot3(n, n));
     if (fabsf(ndd) < XT_GRAZE * nl) {
-        if (nl > 0.f) best.mark_t = fminf(best.mark_t, 0.f);
+        if (nl > 0.f) defer(best, 0.f, id);
         return;
     }
     const float t = dot3(sub(mk(pl.point[0], pl.point[1], pl.point[2]), o), n) / ndd;
     if (t < -tsmall || t > best.t * (1.f + XT_TIE)) return;
     if (t < tsmall) {
-        best.mark_t = fminf(best.mark_t, fmaxf(t, 0.f));
+        defer(best, fmaxf(t, 0.f), id);
         return;
     }
     offer(best, t, id, pl.object, 0.f, 0.f);

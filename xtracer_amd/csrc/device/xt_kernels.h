@@ -25,7 +25,7 @@ enum {
     CNT_Q1 = 1,      // ray count of queue 1
     CNT_RETEST = 2,  // rays listed for the FP64 re-test
     CNT_CURSOR = 3,  // work cursor of the persistent trace kernel
-    CNT_GRAZE = 4,   // rays listed for the FP64 test of one grazing triangle
+    CNT_RECHECK = 4, // rays listed for the FP64 re-check of their deferred candidates
     CNT_WORDS = 8
 };
 struct Counters {
@@ -250,7 +250,8 @@ __device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Bes
     b.obj = XT_NONE;
     b.u = b.v = 0.f;
     b.mark_t = INFINITY;
-    b.graze = XT_NONE;
+    b.c0 = b.c1 = XT_NONE;
+    b.nc = 0;
     if (r.self_t > 0.f && r.origin != XT_NONE) {
         b.t = r.self_t;
         b.prim = r.origin;
@@ -263,9 +264,9 @@ __device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Bes
 // device cursor (one atomic per wave) until the queue is drained, so waves
 // that drew short traversals pick up more work instead of idling.
 template <int ACCEL, int STACK, int BS>
-static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
-                                                     float4 *hit, uint32_t *retest_list, uint2 *graze_list,
-                                                     Counters *cnt, uint32_t defer_graze) {
+static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
+                                                     float4 *hit, uint32_t *retest_list, uint4 *recheck_list,
+                                                     Counters *cnt) {
     __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? STACK * BS : 1];
     __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * BS : 1];
     const uint32_t n = *in_count;
@@ -277,8 +278,8 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
         base = __shfl(base, 0);
         if (base >= n) break;
         const uint32_t i = base + lane;
-        bool mark = false;
-        uint32_t gprim = XT_NONE;
+        bool mark = false, full = false;
+        uint32_t c0 = XT_NONE, c1 = XT_NONE;
         if (i < n) {
             const float4 a = in.r0[i], b = in.r1[i];
             RayF r;
@@ -286,7 +287,7 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             r.d = mk(b.x, b.y, b.z);
             r.origin = __float_as_uint(a.w);
             r.self_t = b.w;
-            Tester32 tst{S, r, tsmall_of(r.o), {}, defer_graze != 0, make_shear(r.d)};
+            Tester32 tst{S, r, tsmall_of(r.o), {}, make_shear(r.d)};
             seed_best(S, r, tst.best);
             for (uint32_t k = 0; k < S.n_planes; ++k)
                 if (XT_PLANE_BASE + k != r.origin) plane32(r.o, r.d, tst.tsmall, S.planes[k], XT_PLANE_BASE + k, tst.best);
@@ -298,7 +299,9 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             // has t = INF, so compare only when something was deferred)
             if (!ok) atomicAdd(&cnt->trace_fail, 1ull);
             mark = !ok || (bb.mark_t != INFINITY && bb.mark_t <= bb.t * (1.f + XT_TIE));
-            if (!mark) gprim = bb.graze;
+            full = !ok || bb.nc > 2;
+            c0 = bb.c0;
+            c1 = bb.c1;
             if (!mark && bb.prim < S.n_prims && !(r.self_t > 0.f && bb.prim == r.origin)) {
                 const float4 a = S.prims[3 * bb.prim], b = S.prims[3 * bb.prim + 1];
                 // the sheared-frame t of the FP32 test loses up to ~1e-7 * distance /
@@ -309,43 +312,78 @@ static __global__ __launch_bounds__(BS) void k_trace(const SceneView S, const Qu
             }
             hit[i] = make_float4(bb.t, __uint_as_float(bb.prim), bb.u, bb.v);
         }
-        const uint32_t slot = wave_append(mark, &cnt->n[CNT_RETEST]);
-        if (mark) retest_list[slot] = i;
-        const bool g = gprim != XT_NONE;
-        const uint32_t gslot = wave_append(g, &cnt->n[CNT_GRAZE]);
-        if (g) graze_list[gslot] = make_uint2(i, gprim);
+        // rays with a few deferred candidates get the FP64 re-check of just
+        // those; rays with more, or whose traversal gave up, the full re-test
+        const bool rf = mark && full, rc = mark && !full;
+        const uint32_t slot = wave_append(rf, &cnt->n[CNT_RETEST]);
+        if (rf) retest_list[slot] = i;
+        const uint32_t cslot = wave_append(rc, &cnt->n[CNT_RECHECK]);
+        if (rc) recheck_list[cslot] = make_uint4(i, c0, c1, XT_NONE);
     }
 }
 
-// FP64 decision of the one grazing triangle a ray left undecided
-// (Triangle::intersection, triangle.cc:82-123, on the same FP32 ray). The
-// FP32 closest hit stands unless the grazing hit is nearer; a grazing hit
-// near the origin or within the tie band of the closest sends the ray to the
-// full FP64 re-test.
-static __global__ __launch_bounds__(XT_BLOCK) void k_graze(const SceneView S, const Queue in, float4 *hit,
-                                                      const uint2 *list, uint32_t *retest_list, Counters *cnt) {
-    const uint32_t n = cnt->n[CNT_GRAZE];
+__device__ __forceinline__ uint32_t object_of(const SceneView &S, uint32_t p) {
+    return p >= XT_PLANE_BASE ? S.planes[p - XT_PLANE_BASE].object : __float_as_uint(S.prims[3 * p + 1].w);
+}
+
+// FP64 re-check of a ray's FP32 closest hit and its deferred candidates: the
+// reference's predicates (triangle.cc:82-123, sphere.cc:35-61,
+// plane.cc:19-61) on those primitives alone, ordered by (t, object, soup
+// index) like Scene::intersection. The FP32 traversal stopped at its closest
+// hit, so when the FP64 winner lies beyond it (or the FP32 closest hit is
+// rejected and nothing else hits) the ray goes on to the full re-test.
+static __global__ __launch_bounds__(XT_BLOCK) void k_recheck(const SceneView S, const Queue in, float4 *hit,
+                                                        const uint4 *list, uint32_t *retest_list, Counters *cnt,
+                                                        const double *ray64, uint32_t *flags) {
+    const uint32_t n = cnt->n[CNT_RECHECK];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t k = base + threadIdx.x;
         bool need = false;
         uint32_t i = 0;
         if (k < n) {
-            const uint2 e = list[k];
+            const uint4 e = list[k];
             i = e.x;
             const float4 a = in.r0[i], b = in.r1[i];
-            double bc[3];
-            const double t = tri64(d3{a.x, a.y, a.z}, d3{b.x, b.y, b.z}, S.prims64[e.y], bc);
-            if (t > 0.0) {
-                const float tf = (float)t;
-                const float4 h = hit[i];
-                if (tf < tsmall_of(mk(a.x, a.y, a.z))) {
-                    need = true;
-                } else if (__float_as_uint(h.y) == XT_NONE || tf < h.x * (1.f - XT_TIE)) {
-                    hit[i] = make_float4(tf, __uint_as_float(e.y), (float)bc[1], (float)bc[2]);
-                } else if (tf <= h.x * (1.f + XT_TIE)) {
-                    need = true;
+            const uint32_t origin = __float_as_uint(a.w);
+            d3 o = {a.x, a.y, a.z}, d = {b.x, b.y, b.z};
+            if (ray64) {
+                const double *q = ray64 + 6 * (size_t)in.path[i];
+                o = {q[0], q[1], q[2]};
+                d = {q[3], q[4], q[5]};
+            }
+            const float4 h = hit[i];
+            const uint32_t B = __float_as_uint(h.y);
+            Tester64 tst{S, RayF{}, o, d, INFINITY, XT_NONE, XT_NONE, 0, 0, 0, mk(a.x, a.y, a.z), Shear{}};
+            if (b.w > 0.f && origin != XT_NONE) tst.offer64(b.w, origin, object_of(S, origin), nullptr);
+            const uint32_t cand[4] = {B, e.y, e.z, e.w};
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t p = cand[c];
+                if (p == XT_NONE || p == origin) continue;
+                bool seen = false;
+                for (int c2 = 0; c2 < c; ++c2) seen = seen || cand[c2] == p;
+                if (seen) continue;
+                double t, bc[3] = {0, 0, 0};
+                bool tri = false;
+                if (p >= XT_PLANE_BASE) {
+                    t = plane64(o, d, S.planes[p - XT_PLANE_BASE]);
+                } else if (XT_PRIM_TYPE(__float_as_uint(S.prims[3 * p].w)) == XT_PRIM_TRIANGLE) {
+                    t = tri64(o, d, S.prims64[p], bc);
+                    tri = true;
+                } else {
+                    t = sph64(o, d, S.prims64[p]);
                 }
+                if (t > 0.0) tst.offer64(t, p, object_of(S, p), tri ? bc : nullptr);
+            }
+            // the FP32 traversal explored everything up to its closest hit (all
+            // of space when it found none)
+            if (B != XT_NONE && !(tst.bt <= (double)h.x * (1.0 + 4.0 * XT_TIE))) {
+                need = true;
+            } else {
+                const bool is_tri = tst.prim < S.n_prims;
+                hit[i] = make_float4(tst.prim == XT_NONE ? INFINITY : (float)tst.bt, __uint_as_float(tst.prim),
+                                     is_tri ? (float)tst.b1 : 0.f, is_tri ? (float)tst.b2 : 0.f);
+                if (flags) flags[i] |= 1u;
             }
         }
         const uint32_t slot = wave_append(need, &cnt->n[CNT_RETEST]);

@@ -61,7 +61,6 @@ struct Tester32 {
     RayF r;
     float tsmall;
     Best best;
-    bool defer_graze;  // the ray stands for an FP64 ray: grazing triangles take the full re-test
     Shear sh;
     __device__ __forceinline__ float limit() const { return best.t; }
     __device__ __forceinline__ void test(uint32_t prim) {
@@ -69,14 +68,8 @@ struct Tester32 {
         const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1];
         if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
             const float4 c = S.prims[3 * prim + 2];
-            if (tri32(r.o, sh, tsmall, a, b, c, prim, best)) {
-                // one grazing triangle per ray is decided in FP64 after the
-                // traversal (k_graze); a second one, or a ray that stands
-                // for an FP64 ray (whose small difference moves a grazing
-                // hit a long way), takes the full FP64 re-test
-                if (defer_graze || (best.graze != XT_NONE && best.graze != prim)) best.mark_t = fminf(best.mark_t, 0.f);
-                else best.graze = prim;
-            }
+            // a grazing triangle is decided by the FP64 re-check
+            if (tri32(r.o, sh, tsmall, a, b, c, prim, best)) defer(best, 0.f, prim);
         } else {
             sph32(r.o, r.d, tsmall, a, b, prim, best);
         }

@@ -32,7 +32,7 @@ struct WaveBuffers {
     WaveQueue q[2];
     float4 *hit = nullptr;
     uint32_t *retest = nullptr;
-    uint2 *graze = nullptr;
+    uint4 *recheck = nullptr;
     float4 *ps0 = nullptr, *ps1 = nullptr, *rad = nullptr;
     uint32_t *flags = nullptr;
     Counters *cnt = nullptr;  // XT_MAX_LANES counter blocks

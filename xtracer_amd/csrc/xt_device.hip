@@ -93,7 +93,7 @@ int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64) {
     }
     rc = rc ? rc : dev_alloc(d, &w.hit, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.retest, cap, t, b);
-    rc = rc ? rc : dev_alloc(d, &w.graze, cap, t, b);
+    rc = rc ? rc : dev_alloc(d, &w.recheck, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.ps0, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.ps1, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.rad, cap, t, b);
@@ -316,8 +316,7 @@ int xt_scene_release(xt_dev *d, uint32_t scene_id) {
 // blocks the CU keeps resident, from the occupancy calculator.
 template <int STACK, int BS>
 static void launch_kd_trace_sb(unsigned per_cu, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
-                               const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint2 *graze, Counters *cnt,
-                               uint32_t defer) {
+                               const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt) {
     static int resident = 0;
     if (per_cu == 0) {
         if (resident == 0) {
@@ -332,27 +331,25 @@ static void launch_kd_trace_sb(unsigned per_cu, int n_cu, size_t cap, hipStream_
     }
     const size_t need = (cap + BS - 1) / BS;
     const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>(need, (size_t)n_cu * per_cu));
-    hipLaunchKernelGGL((k_trace<XT_ACCEL_KDTREE, STACK, BS>), dim3(g), dim3(BS), 0, st, v, q, count, hit, retest, graze,
-                       cnt, defer);
+    hipLaunchKernelGGL((k_trace<XT_ACCEL_KDTREE, STACK, BS>), dim3(g), dim3(BS), 0, st, v, q, count, hit, retest, recheck,
+                       cnt);
 }
 
 template <int STACK>
 static void launch_kd_trace_s(unsigned bs, unsigned per_cu, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
-                              const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint2 *graze, Counters *cnt,
-                              uint32_t defer) {
-    if (bs == 128) launch_kd_trace_sb<STACK, 128>(per_cu, n_cu, cap, st, v, q, count, hit, retest, graze, cnt, defer);
-    else launch_kd_trace_sb<STACK, 256>(per_cu, n_cu, cap, st, v, q, count, hit, retest, graze, cnt, defer);
+                              const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt) {
+    if (bs == 128) launch_kd_trace_sb<STACK, 128>(per_cu, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt);
+    else launch_kd_trace_sb<STACK, 256>(per_cu, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt);
 }
 
 static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
-                            const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint2 *graze, Counters *cnt,
-                            uint32_t defer) {
+                            const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt) {
     const unsigned bs = tn.trace_block, pc = tn.trace_blocks_per_cu;
     switch (tn.kd_stack) {
-        case 8: launch_kd_trace_s<8>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, graze, cnt, defer); break;
-        case 12: launch_kd_trace_s<12>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, graze, cnt, defer); break;
-        case 24: launch_kd_trace_s<24>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, graze, cnt, defer); break;
-        default: launch_kd_trace_s<16>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, graze, cnt, defer); break;
+        case 8: launch_kd_trace_s<8>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
+        case 12: launch_kd_trace_s<12>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
+        case 24: launch_kd_trace_s<24>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
+        default: launch_kd_trace_s<16>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
     }
 }
 
@@ -364,9 +361,7 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const Wave
     Queue q{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
     float4 *hit = w.hit + L.off;
     uint32_t *retest = w.retest + L.off;
-    uint2 *graze = w.graze + L.off;
-    // rays with FP64 originals defer grazing triangles to the FP64 re-test
-    const uint32_t defer = ray64 != nullptr;
+    uint4 *recheck = w.recheck + L.off;
     const bool timed = time_it && d->trace_launches < XT_TRACE_EVENTS - 1;
     if (timed) hipEventRecord(d->tev[d->trace_launches], d->stream);
     const xt_tuning &tn = d->tuning;
@@ -374,9 +369,9 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const Wave
         // 16 resident 64-lane blocks per CU (LDS: 20 KB each)
         const unsigned g = (unsigned)std::min<size_t>(L.cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
         hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, d->stream,
-                           ds.view, q, &L.cnt->n[qi], hit, retest, graze, L.cnt, defer);
+                           ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
     } else {
-        launch_kd_trace(tn, d->n_cu, L.cap, d->stream, ds.view, q, &L.cnt->n[qi], hit, retest, graze, L.cnt, defer);
+        launch_kd_trace(tn, d->n_cu, L.cap, d->stream, ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
     }
     if (timed) {
         hipEventRecord(d->tev[d->trace_launches + 1], d->stream);
@@ -384,8 +379,8 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const Wave
         d->trace_launches += 2;
     }
     XT_LAUNCHED();
-    hipLaunchKernelGGL(k_graze, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 256)), dim3(XT_BLOCK), 0, d->stream, ds.view,
-                       q, hit, graze, retest, L.cnt);
+    hipLaunchKernelGGL(k_recheck, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 256)), dim3(XT_BLOCK), 0, d->stream,
+                       ds.view, q, hit, recheck, retest, L.cnt, ray64, flags);
     XT_LAUNCHED();
     if (accel == XT_ACCEL_OCTREE) {
         hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(L.cap / 64 + 1, XT_OCT_BLOCK, 2048)),

@@ -166,7 +166,7 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
             PassGroup &G = groups[k];
             if (!G.live || r >= G.rounds) continue;
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, G.lane.cnt,
-                               (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_GRAZE));
+                               (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
             const bool fp64_rays = G.integrator == XT_INTEGRATOR_DEPTH && r == 0;
             int rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
                                      time_trace);
