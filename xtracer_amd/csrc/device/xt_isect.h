@@ -48,6 +48,8 @@ struct Best {
     uint32_t nc;         // candidates deferred (> 2: too many to list)
 };
 
+#define XT_NONE_OBJ 0xffffffffu
+
 // Defers primitive `prim` at about t to the FP64 decision.
 __device__ __forceinline__ void defer(Best &b, float t, uint32_t prim) {
     b.mark_t = fminf(b.mark_t, t);
@@ -61,8 +63,10 @@ __device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t 
     // a kd-tree references a primitive from every leaf it overlaps: meeting
     // the current best again is not a tie
     if (prim == b.prim) return;
-    // ties go to the earlier object, then to the earlier soup index
-    if (t < b.t || (t == b.t && (obj < b.obj || (obj == b.obj && prim < b.prim)))) {
+    // ties go to the earlier object, then to the earlier soup index; an exact
+    // tie with an object not at hand is deferred
+    if (t < b.t || (t == b.t && obj != XT_NONE_OBJ && b.obj != XT_NONE_OBJ &&
+                    (obj < b.obj || (obj == b.obj && prim < b.prim)))) {
         if (b.t - t <= XT_TIE * t) defer(b, t, b.prim);  // the displaced best is within the tie band
         b.t = t;
         b.prim = prim;
@@ -131,8 +135,8 @@ __device__ __forceinline__ float edge2(float p, float q, float r, float s) { ret
 // FP32 watertight test of one triangle. Returns true when the ray grazes it
 // (|n.d| < XT_GRAZE): FP32 cannot place that hit and the caller decides it
 // in FP64. (u, v) offered with a hit are the weights of v1 and v2.
-__device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall, const float4 a, const float4 b,
-                                      const float4 c, uint32_t prim, Best &best) {
+__device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall, const float4 a, uint32_t obj,
+                                      const float4 b, const float4 c, uint32_t prim, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
     if ((XT_PRIM_FLAGS(tf) & XT_PF_DEGENERATE) || sh.dz == 0.f) return false;
     float ax, ay, az, bx, by, bz, cx, cy, cz;
@@ -153,7 +157,7 @@ __device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall,
         defer(best, fmaxf(t, 0.f), prim);
         return false;
     }
-    offer(best, t, prim, __float_as_uint(b.w), u, v);
+    offer(best, t, prim, obj, u, v);
     return false;
 }
 
@@ -163,7 +167,7 @@ __device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall,
 // |oc|^2 - r^2 keeps its accuracy for origins near the surface (spheres are
 // few, so this costs little).
 __device__ __forceinline__ void sph32(const f3 o, const f3 d, float tsmall, const float4 a, const float4 b,
-                                      uint32_t prim, Best &best) {
+                                      uint32_t prim, uint32_t obj, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
     if (XT_PRIM_FLAGS(tf) & XT_PF_TINY_SPHERE) return;
     const double ox = (double)o.x - (double)a.x, oy = (double)o.y - (double)a.y, oz = (double)o.z - (double)a.z;
@@ -189,7 +193,7 @@ __device__ __forceinline__ void sph32(const f3 o, const f3 d, float tsmall, cons
         defer(best, t, prim);
         return;
     }
-    offer(best, t, prim, __float_as_uint(b.w), 0.f, 0.f);
+    offer(best, t, prim, obj, 0.f, 0.f);
 }
 
 __device__ __forceinline__ void plane32(const f3 o, const f3 d, float tsmall, const XtPlane &pl, uint32_t id,

@@ -16,6 +16,7 @@ struct SceneView {
     const XtPlane *planes;
     const XtKdNode *kd;
     const uint32_t *kd_refs;
+    const float4 *kd_leaf;  // 3 float4 per leaf reference: the XtPrim with the soup index as object word
     const XtOctNode *oct;
     const uint32_t *oct_refs;
     const uint32_t *obj_material;
@@ -63,16 +64,30 @@ struct Tester32 {
     Best best;
     Shear sh;
     __device__ __forceinline__ float limit() const { return best.t; }
-    __device__ __forceinline__ void test(uint32_t prim) {
+    // one record (a, b, c) of primitive `prim`; `obj` its object, or XT_NONE
+    // where the record does not carry it (exact t ties then go to the FP64
+    // re-check, which knows the objects)
+    __device__ __forceinline__ void test_rec(uint32_t prim, uint32_t obj, const float4 a, const float4 b,
+                                             const float4 c) {
         if (prim == r.origin) return;
-        const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1];
         if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
-            const float4 c = S.prims[3 * prim + 2];
             // a grazing triangle is decided by the FP64 re-check
-            if (tri32(r.o, sh, tsmall, a, b, c, prim, best)) defer(best, 0.f, prim);
+            if (tri32(r.o, sh, tsmall, a, obj, b, c, prim, best)) defer(best, 0.f, prim);
         } else {
-            sph32(r.o, r.d, tsmall, a, b, prim, best);
+            sph32(r.o, r.d, tsmall, a, b, prim, obj, best);
         }
+    }
+    __device__ __forceinline__ void test(uint32_t prim) {
+        const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1], c = S.prims[3 * prim + 2];
+        test_rec(prim, __float_as_uint(b.w), a, b, c);
+    }
+    // kd leaf reference k: the three loads go out together (the empty asm
+    // keeps the compiler from sinking the third into the triangle branch,
+    // which would cost a second memory round trip)
+    __device__ __forceinline__ void test_leaf(uint32_t k) {
+        const float4 a = S.kd_leaf[3 * k], b = S.kd_leaf[3 * k + 1], c = S.kd_leaf[3 * k + 2];
+        asm volatile("" ::"v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
+        test_rec(__float_as_uint(b.w), XT_NONE, a, b, c);
     }
 };
 
@@ -98,6 +113,7 @@ struct Tester64 {
             b2 = bc ? bc[2] : 0.0;
         }
     }
+    __device__ __forceinline__ void test_leaf(uint32_t k) { test(S.kd_refs[k]); }
     __device__ __forceinline__ void test(uint32_t p) {
         if (p == r.origin) return;
         const float4 a = S.prims[3 * p], b = S.prims[3 * p + 1];
@@ -161,7 +177,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             axis = nd.word & 3u;
         }
         const uint32_t first_ref = nd.word >> 2, n = nd.count;
-        for (uint32_t k = 0; k < n; ++k) tst.test(S.kd_refs[first_ref + k]);
+        for (uint32_t k = 0; k < n; ++k) tst.test_leaf(first_ref + k);
         const float lim = tst.limit();
         if (lim <= tmax) return true;
         if (count == 0) {

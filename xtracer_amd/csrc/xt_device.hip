@@ -236,7 +236,15 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
         uint32_t m = f.objects[i].material;
         if (m != XT_NO_MATERIAL && !f.material_supported[m]) ds.pt_supported = false;
     }
-    XtPrim *prims = nullptr;
+    // kd leaves' primitives copied in leaf order, the object word replaced
+    // by the soup index, so the FP32 traversal reads a leaf's records
+    // without going through the reference list
+    std::vector<XtPrim> leaf(f.kd.refs.size());
+    for (size_t r = 0; r < f.kd.refs.size(); ++r) {
+        leaf[r] = f.prims[f.kd.refs[r]];
+        leaf[r].object = f.kd.refs[r];
+    }
+    XtPrim *prims = nullptr, *kdl = nullptr;
     XtPrim64 *p64 = nullptr;
     XtPlane *planes = nullptr;
     XtKdNode *kd = nullptr;
@@ -251,6 +259,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     rc = rc ? rc : upload(d, &planes, f.planes, t, b);
     rc = rc ? rc : upload(d, &kd, f.kd.nodes, t, b);
     rc = rc ? rc : upload(d, &kdr, f.kd.refs, t, b);
+    rc = rc ? rc : upload(d, &kdl, leaf, t, b);
     rc = rc ? rc : upload(d, &oct, f.oct_nodes, t, b);
     rc = rc ? rc : upload(d, &octr, f.oct_refs, t, b);
     rc = rc ? rc : upload(d, &om, obj_mat, t, b);
@@ -268,6 +277,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     v.planes = planes;
     v.kd = kd;
     v.kd_refs = kdr;
+    v.kd_leaf = (const float4 *)kdl;
     v.oct = oct;
     v.oct_refs = octr;
     v.obj_material = om;
