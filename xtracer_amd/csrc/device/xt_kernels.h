@@ -18,6 +18,8 @@
 
 #define XT_BLOCK 256
 #define XT_OCT_BLOCK 64
+#define XT_TRACE_CHUNK 4  // 64-ray groups a trace wave takes per cursor fetch
+#define XT_DEAD 0xfffffffeu  // origin word of a queue entry without a ray
 
 // Indices into the device counter block.
 enum {
@@ -60,6 +62,31 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
     uint32_t base = 0;
     if (lane == leader && n) base = atomicAdd(counter, n);
     base = __shfl(base, (int)leader);
+    const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
+    return base + (uint32_t)__popcll(below);
+}
+
+// Block-aggregated append (one atomic per block): every thread of the block
+// must call it, with `lds` pointing at BS / 64 + 1 shared words.
+template <int BS>
+__device__ __forceinline__ uint32_t block_append(bool want, uint32_t *counter, uint32_t *lds) {
+    constexpr int NW = BS / 64;
+    const unsigned long long m = __ballot(want);
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = lds[w];
+            lds[w] = tot;
+            tot += c;
+        }
+        lds[NW] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = lds[NW] + lds[wid];
+    __syncthreads();
     const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
     return base + (uint32_t)__popcll(below);
 }
@@ -187,54 +214,51 @@ __device__ __forceinline__ void aa_coords(const JobDev &j, uint32_t px, uint32_t
 #pragma clang fp contract(on)
 
 // Slots [0, spp_pass * qn): slot i renders sample pass_base + i / qn of the
-// pixel at tile position q0 + i % qn (a whole pass has q0 = 0, qn = NT); its
-// path id is path_base + i (path state, radiance and FP64 rays are indexed
-// by path id). The path state word ps1.w holds the query count (low 16 bits)
-// and the job's rdepth (high 16 bits).
+// pixel at tile position q0 + i % qn (a whole pass has q0 = 0, qn = NT) and
+// writes queue entry i; its path id is path_base + i (path state, radiance
+// and FP64 rays are indexed by path id). Slots without a sample (pixels past
+// the image edge, samples past the job's count) get a dead ray, which the
+// trace turns into a miss whose radiance is never accumulated. The path
+// state word ps1.w holds the query count (low 16 bits) and the job's rdepth
+// (high 16 bits).
 static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
                                                      uint32_t q0, uint32_t qn, uint32_t path_base, Queue out,
-                                                     uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad,
-                                                     double *ray64) {
+                                                     float4 *ps0, float4 *ps1, float4 *rad, double *ray64) {
     const uint32_t n = spp_pass * qn;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t qword = 1u | (j.rdepth << 16);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t path = path_base + i;
-        bool live = false;
-        uint32_t x = 0, y = 0, s = 0;
-        if (i < n) {
-            const uint32_t sl = i / qn, q = q0 + (i - sl * qn);
-            s = pass_base + sl;
-            live = slot_pixel(j, q, x, y) && s < j.total;
-            rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint32_t sl = i / qn, q = q0 + (i - sl * qn);
+        const uint32_t s = pass_base + sl;
+        uint32_t x = 0, y = 0;
+        const bool live = slot_pixel(j, q, x, y) && s < j.total;
+        rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
+        out.path[i] = path;
+        if (!live) {
+            out.r0[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(XT_DEAD));
+            out.r1[i] = make_float4(0.f, 0.f, 1.f, 0.f);
+            continue;
         }
-        double o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
-        uint32_t key = 0;
-        if (live) {
-            const uint32_t pixel = y * j.W + x;
-            key = xt_path_key(j.seed, j.job_key, pixel, s);
-            float cx, cy;
-            aa_coords(j, x, y, s / j.spp, cx, cy);
-            camera_ray(j, cx, cy, key, o, d);
-            ps0[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
-            ps1[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(qword));
-            if (ray64) {
-                double *r = ray64 + 6 * (size_t)path;
-                r[0] = o[0];
-                r[1] = o[1];
-                r[2] = o[2];
-                r[3] = d[0];
-                r[4] = d[1];
-                r[5] = d[2];
-            }
+        double o[3], d[3];
+        const uint32_t pixel = y * j.W + x;
+        const uint32_t key = xt_path_key(j.seed, j.job_key, pixel, s);
+        float cx, cy;
+        aa_coords(j, x, y, s / j.spp, cx, cy);
+        camera_ray(j, cx, cy, key, o, d);
+        ps0[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
+        ps1[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(qword));
+        if (ray64) {
+            double *r = ray64 + 6 * (size_t)path;
+            r[0] = o[0];
+            r[1] = o[1];
+            r[2] = o[2];
+            r[3] = d[0];
+            r[4] = d[1];
+            r[5] = d[2];
         }
-        const uint32_t slot = wave_append(live, out_count);
-        if (live) {
-            out.r0[slot] = make_float4((float)o[0], (float)o[1], (float)o[2], __uint_as_float(XT_NONE));
-            out.r1[slot] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
-            out.path[slot] = path;
-        }
+        out.r0[i] = make_float4((float)o[0], (float)o[1], (float)o[2], __uint_as_float(XT_NONE));
+        out.r1[i] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
     }
 }
 
@@ -260,9 +284,10 @@ __device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Bes
     }
 }
 
-// Persistent trace: each wave takes 64 consecutive rays at a time from a
-// device cursor (one atomic per wave) until the queue is drained, so waves
-// that drew short traversals pick up more work instead of idling.
+// Persistent trace: each wave takes XT_TRACE_CHUNK x 64 consecutive rays at
+// a time from a device cursor (one atomic per chunk) until the queue is
+// drained, so waves that drew short traversals pick up more work instead of
+// idling.
 template <int ACCEL, int STACK, int BS>
 static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
                                                      float4 *hit, uint32_t *retest_list, uint4 *recheck_list,
@@ -272,15 +297,25 @@ static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) 
     const uint32_t n = *in_count;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->queries, (unsigned long long)n);
     const uint32_t lane = lane_id();
+    uint32_t chunk = 0, part = XT_TRACE_CHUNK;
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&cnt->n[CNT_CURSOR], 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
+        if (part == XT_TRACE_CHUNK) {  // next chunk of XT_TRACE_CHUNK x 64 rays for this wave
+            if (lane == 0) chunk = atomicAdd(&cnt->n[CNT_CURSOR], 64u * XT_TRACE_CHUNK);
+            chunk = __shfl(chunk, 0);
+            part = 0;
+        }
+        const uint32_t base = chunk + 64u * part++;
+        if (base >= n) {
+            if (chunk >= n) break;
+            part = XT_TRACE_CHUNK;
+            continue;
+        }
         const uint32_t i = base + lane;
         bool mark = false, full = false;
         uint32_t c0 = XT_NONE, c1 = XT_NONE;
-        if (i < n) {
+        if (i < n && __float_as_uint(in.r0[i].w) == XT_DEAD) {
+            hit[i] = make_float4(INFINITY, __uint_as_float(XT_NONE), 0.f, 0.f);
+        } else if (i < n) {
             const float4 a = in.r0[i], b = in.r1[i];
             RayF r;
             r.o = mk(a.x, a.y, a.z);
@@ -507,6 +542,7 @@ __device__ __forceinline__ void push_ray(Queue &out, uint32_t slot, f3 o, f3 d, 
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, Queue out,
                                                        uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad) {
+    __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -567,7 +603,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                 // q == rdepth with a continuing material: eval(0) == 0
             }
         }
-        const uint32_t slot = wave_append(cont, out_count);
+        const uint32_t slot = block_append<XT_BLOCK>(cont, out_count, lds);
         if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
     }
 }
@@ -577,6 +613,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao1(const SceneView S, const Queue in,
                                                         const uint32_t *in_count, const float4 *hit, Queue out,
                                                         uint32_t *out_count, const float4 *ps0, float4 *rad) {
+    __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -606,7 +643,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao1(const SceneView S
                 nself = self_hit_t(g, N, dir);
             }
         }
-        const uint32_t slot = wave_append(cont, out_count);
+        const uint32_t slot = block_append<XT_BLOCK>(cont, out_count, lds);
         if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
     }
 }

@@ -318,13 +318,16 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
     XT_LAUNCHED();
     for (size_t g = 0; g < grps.size(); ++g) {
         const WaveLane &L = groups[g].lane;
-        Queue q0{w.q[0].r0 + L.off, w.q[0].r1 + L.off, w.q[0].path + L.off};
+        // camera rays fill the lane's queue 0 in slot order
+        hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, L.cnt, 0u, (uint32_t)L.cap);
         for (size_t k : grps[g].segs) {
             const SJob &J = S.jobs[segs[k].ticket];
             const size_t n = (size_t)segs[k].samples * segs[k].qn;
+            const size_t at = placed[k].path_base;
+            Queue q0{w.q[0].r0 + at, w.q[0].r1 + at, w.q[0].path + at};
             hipLaunchKernelGGL(k_camera, dim3(grid_for(n, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, J.jd,
-                               J.issued, segs[k].samples, segs[k].q0, segs[k].qn, (uint32_t)placed[k].path_base, q0,
-                               &L.cnt->n[0], w.ps0, w.ps1, w.rad, need64 ? w.ray64 : nullptr);
+                               J.issued, segs[k].samples, segs[k].q0, segs[k].qn, (uint32_t)at, q0, w.ps0, w.ps1,
+                               w.rad, need64 ? w.ray64 : nullptr);
             XT_LAUNCHED();
         }
     }
