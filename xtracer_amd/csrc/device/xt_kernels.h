@@ -18,7 +18,7 @@
 
 #define XT_BLOCK 256
 #define XT_OCT_BLOCK 64
-#define XT_TRACE_CHUNK 4  // 64-ray groups a trace wave takes per cursor fetch
+#define XT_TRACE_CHUNK 1  // 64-ray groups a trace wave takes per cursor fetch
 #define XT_DEAD 0xfffffffeu  // origin word of a queue entry without a ray
 
 // Indices into the device counter block.
