@@ -81,13 +81,27 @@ struct Tester32 {
         const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1], c = S.prims[3 * prim + 2];
         test_rec(prim, __float_as_uint(b.w), a, b, c);
     }
-    // kd leaf reference k: the three loads go out together (the empty asm
-    // keeps the compiler from sinking the third into the triangle branch,
-    // which would cost a second memory round trip)
-    __device__ __forceinline__ void test_leaf(uint32_t k) {
-        const float4 a = S.kd_leaf[3 * k], b = S.kd_leaf[3 * k + 1], c = S.kd_leaf[3 * k + 2];
-        asm volatile("" ::"v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
-        test_rec(__float_as_uint(b.w), XT_NONE, a, b, c);
+    // kd leaf references [k, k + n): each record's three loads go out
+    // together, and the next record's while this one is tested (the empty
+    // asm keeps the compiler from sinking loads into the branches that use
+    // them, which would cost extra memory round trips)
+    __device__ __forceinline__ void test_leaf_range(uint32_t k, uint32_t n) {
+        if (n == 0) return;
+        float4 a = S.kd_leaf[3 * k], b = S.kd_leaf[3 * k + 1], c = S.kd_leaf[3 * k + 2];
+        for (uint32_t j = 1; j <= n; ++j) {
+            float4 na = a, nb = b, nc = c;
+            if (j < n) {
+                const uint32_t q = 3 * (k + j);
+                na = S.kd_leaf[q];
+                nb = S.kd_leaf[q + 1];
+                nc = S.kd_leaf[q + 2];
+            }
+            asm volatile("" ::"v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
+            test_rec(__float_as_uint(b.w), XT_NONE, a, b, c);
+            a = na;
+            b = nb;
+            c = nc;
+        }
     }
 };
 
@@ -113,7 +127,9 @@ struct Tester64 {
             b2 = bc ? bc[2] : 0.0;
         }
     }
-    __device__ __forceinline__ void test_leaf(uint32_t k) { test(S.kd_refs[k]); }
+    __device__ __forceinline__ void test_leaf_range(uint32_t k, uint32_t n) {
+        for (uint32_t j = 0; j < n; ++j) test(S.kd_refs[k + j]);
+    }
     __device__ __forceinline__ void test(uint32_t p) {
         if (p == r.origin) return;
         const float4 a = S.prims[3 * p], b = S.prims[3 * p + 1];
@@ -177,7 +193,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             axis = nd.word & 3u;
         }
         const uint32_t first_ref = nd.word >> 2, n = nd.count;
-        for (uint32_t k = 0; k < n; ++k) tst.test_leaf(first_ref + k);
+        tst.test_leaf_range(first_ref, n);
         const float lim = tst.limit();
         if (lim <= tmax) return true;
         if (count == 0) {
