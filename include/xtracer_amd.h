@@ -52,7 +52,7 @@ extern "C" {
 
 /* hit primitive ids returned by xt_trace */
 #define XT_HIT_NONE 0xffffffffu
-#define XT_HIT_PLANE_BASE 0xffff0000u /* XT_HIT_PLANE_BASE + k: plane k (k < 65534) */
+#define XT_HIT_PLANE_BASE 0xffff0000u /* XT_HIT_PLANE_BASE + k: plane k (k < 65533) */
 
 typedef struct xt_scene xt_scene; /* host scene: loaded, flattened, trees built */
 typedef struct xt_dev xt_dev;     /* one HIP device with resident scenes */

@@ -41,44 +41,37 @@ __device__ __forceinline__ f3 norm3(f3 a) {
 struct Best {
     float t;
     uint32_t prim;
-    uint32_t obj;
-    float u, v;
-    float mark_t;        // smallest t of a deferred candidate (INF: none)
-    uint32_t c0, c1;     // deferred candidates, for the FP64 re-check
-    uint32_t nc;         // candidates deferred (> 2: too many to list)
+    float mark_t;     // smallest t of a deferred candidate (INF: none)
+    uint32_t c0, c1;  // deferred candidates for the FP64 re-check (c1 = XT_CAND_OVERFLOW: more than two)
 };
 
-#define XT_NONE_OBJ 0xffffffffu
+#define XT_CAND_OVERFLOW 0xfffffffdu
 
 // Defers primitive `prim` at about t to the FP64 decision.
 __device__ __forceinline__ void defer(Best &b, float t, uint32_t prim) {
     b.mark_t = fminf(b.mark_t, t);
     if (prim == b.c0 || prim == b.c1) return;
-    if (b.nc == 0) b.c0 = prim;
-    else if (b.nc == 1) b.c1 = prim;
-    b.nc = b.nc < 3 ? b.nc + 1 : 3;
+    if (b.c0 == 0xffffffffu) b.c0 = prim;
+    else if (b.c1 == 0xffffffffu) b.c1 = prim;
+    else b.c1 = XT_CAND_OVERFLOW;
 }
 
-__device__ __forceinline__ void offer(Best &b, float t, uint32_t prim, uint32_t obj, float u, float v) {
+// A candidate hit at t. The reference breaks exact ties by object, then by
+// soup index; the FP32 pass defers every candidate within the tie band,
+// exact ties included, to the FP64 re-check, which orders them that way.
+__device__ __forceinline__ void offer(Best &b, float t, uint32_t prim) {
     // a kd-tree references a primitive from every leaf it overlaps: meeting
     // the current best again is not a tie
     if (prim == b.prim) return;
-    // ties go to the earlier object, then to the earlier soup index; an exact
-    // tie with an object not at hand is deferred
-    if (t < b.t || (t == b.t && obj != XT_NONE_OBJ && b.obj != XT_NONE_OBJ &&
-                    (obj < b.obj || (obj == b.obj && prim < b.prim)))) {
+    if (t < b.t) {
         if (b.t - t <= XT_TIE * t) defer(b, t, b.prim);  // the displaced best is within the tie band
         b.t = t;
         b.prim = prim;
-        b.obj = obj;
-        b.u = u;
-        b.v = v;
     } else if (t - b.t <= XT_TIE * b.t) {
         defer(b, b.t, prim);
     }
 }
 
-// FP32 triangle test (Moller-Trumbore) with deferral marks.
 // Per-ray shear of the watertight ray/triangle test (Woop, Benthin, Wald,
 // "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013): with kz the
 // largest direction component and (kx, ky) the others in winding order, the
@@ -134,9 +127,9 @@ __device__ __forceinline__ float edge2(float p, float q, float r, float s) { ret
 
 // FP32 watertight test of one triangle. Returns true when the ray grazes it
 // (|n.d| < XT_GRAZE): FP32 cannot place that hit and the caller decides it
-// in FP64. (u, v) offered with a hit are the weights of v1 and v2.
-__device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall, const float4 a, uint32_t obj,
-                                      const float4 b, const float4 c, uint32_t prim, Best &best) {
+// in FP64.
+__device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall, const float4 a, const float4 b,
+                                      const float4 c, uint32_t prim, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
     if ((XT_PRIM_FLAGS(tf) & XT_PF_DEGENERATE) || sh.dz == 0.f) return false;
     float ax, ay, az, bx, by, bz, cx, cy, cz;
@@ -148,8 +141,7 @@ __device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall,
     if (fabsf(det) * sh.dz * c.w < XT_GRAZE) return true;  // |n.d| small: the reference threshold is 1e-8
     const float inv = 1.f / det;
     const float t = (U * az + V * bz + W * cz) * sh.sz * inv;
-    const float u = V * inv, v = W * inv, w = U * inv;
-    const float m = fminf(fminf(u, v), w);
+    const float m = fminf(fminf(V * inv, W * inv), U * inv);
     if (t < -tsmall || t > best.t * (1.f + XT_TIE)) return false;
     if (m > -XT_BAND && m < XT_BAND) defer(best, t, prim);
     if (m < 0.f) return false;
@@ -157,7 +149,7 @@ __device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall,
         defer(best, fmaxf(t, 0.f), prim);
         return false;
     }
-    offer(best, t, prim, obj, u, v);
+    offer(best, t, prim);
     return false;
 }
 
@@ -167,7 +159,7 @@ __device__ __forceinline__ bool tri32(const f3 o, const Shear &sh, float tsmall,
 // |oc|^2 - r^2 keeps its accuracy for origins near the surface (spheres are
 // few, so this costs little).
 __device__ __forceinline__ void sph32(const f3 o, const f3 d, float tsmall, const float4 a, const float4 b,
-                                      uint32_t prim, uint32_t obj, Best &best) {
+                                      uint32_t prim, Best &best) {
     const uint32_t tf = __float_as_uint(a.w);
     if (XT_PRIM_FLAGS(tf) & XT_PF_TINY_SPHERE) return;
     const double ox = (double)o.x - (double)a.x, oy = (double)o.y - (double)a.y, oz = (double)o.z - (double)a.z;
@@ -193,7 +185,7 @@ __device__ __forceinline__ void sph32(const f3 o, const f3 d, float tsmall, cons
         defer(best, t, prim);
         return;
     }
-    offer(best, t, prim, obj, 0.f, 0.f);
+    offer(best, t, prim);
 }
 
 __device__ __forceinline__ void plane32(const f3 o, const f3 d, float tsmall, const XtPlane &pl, uint32_t id,
@@ -211,7 +203,7 @@ __device__ __forceinline__ void plane32(const f3 o, const f3 d, float tsmall, co
         defer(best, fmaxf(t, 0.f), id);
         return;
     }
-    offer(best, t, id, pl.object, 0.f, 0.f);
+    offer(best, t, id);
 }
 
 // ------------------------------------------------------------ FP64 predicates
@@ -317,16 +309,23 @@ __device__ __noinline__ double plane64(const d3 o, const d3 d, const XtPlane &pl
     return t;
 }
 
-// Plane-equation t of a triangle in FP64 from its FP32 vertices (the
-// reference's t = -n.(o - v0) / n.d, triangle.cc:84-94). The FP32 t loses
-// about 1e-7 / |n.d| relative at grazing angles; hits with |n.d| < 0.05 get
-// this value instead.
-__device__ __forceinline__ float tri_t64(const f3 o, const f3 d, const float4 a, const float4 b, const float4 c) {
+// Closest-hit record of a triangle in FP64 from its FP32 vertices: the
+// reference's plane-equation t = -n.(o - v0) / n.d (triangle.cc:84-94) and
+// the weights (u, v) of v1 and v2 at o + t d. The FP32 test's t loses up to
+// about 1e-7 * distance / (edge * |n.d|) relative.
+__device__ __forceinline__ void tri_hit64(const f3 o, const f3 d, const float4 a, const float4 b, const float4 c,
+                                          float &t, float &u, float &v) {
     const d3 A = {a.x, a.y, a.z};
-    const d3 n = dnorm(dcross(dsub(d3{b.x, b.y, b.z}, A), dsub(d3{c.x, c.y, c.z}, A)));
-    const d3 D = {d.x, d.y, d.z};
-    const double ndd = ddot(n, D);
-    return (float)(-ddot(n, dsub(d3{o.x, o.y, o.z}, A)) / ndd);
+    const d3 e1 = dsub(d3{b.x, b.y, b.z}, A), e2 = dsub(d3{c.x, c.y, c.z}, A);
+    const d3 x = dcross(e1, e2);
+    const d3 n = dnorm(x);
+    const d3 D = {d.x, d.y, d.z}, O = {o.x, o.y, o.z};
+    const double tt = -ddot(n, dsub(O, A)) / ddot(n, D);
+    const d3 pa = dsub(dadd(O, dscl(D, tt)), A);
+    const double xx = ddot(x, x);
+    t = (float)tt;
+    u = xx > 0.0 ? (float)(ddot(dcross(pa, e2), x) / xx) : 0.f;
+    v = xx > 0.0 ? (float)(ddot(dcross(e1, pa), x) / xx) : 0.f;
 }
 
 #pragma clang fp contract(on)

@@ -268,19 +268,14 @@ __device__ __forceinline__ float tsmall_of(f3 o) {
     return XT_TSMALL * (1.f + fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z))));
 }
 
-__device__ __forceinline__ void seed_best(const SceneView &S, const RayF &r, Best &b) {
+__device__ __forceinline__ void seed_best(const RayF &r, Best &b) {
     b.t = INFINITY;
     b.prim = XT_NONE;
-    b.obj = XT_NONE;
-    b.u = b.v = 0.f;
     b.mark_t = INFINITY;
     b.c0 = b.c1 = XT_NONE;
-    b.nc = 0;
     if (r.self_t > 0.f && r.origin != XT_NONE) {
         b.t = r.self_t;
         b.prim = r.origin;
-        b.obj = r.origin >= XT_PLANE_BASE ? S.planes[r.origin - XT_PLANE_BASE].object
-                                          : __float_as_uint(S.prims[3 * r.origin + 1].w);
     }
 }
 
@@ -323,7 +318,7 @@ static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) 
             r.origin = __float_as_uint(a.w);
             r.self_t = b.w;
             Tester32 tst{S, r, tsmall_of(r.o), {}, make_shear(r.d)};
-            seed_best(S, r, tst.best);
+            seed_best(r, tst.best);
             for (uint32_t k = 0; k < S.n_planes; ++k)
                 if (XT_PLANE_BASE + k != r.origin) plane32(r.o, r.d, tst.tsmall, S.planes[k], XT_PLANE_BASE + k, tst.best);
             bool ok;
@@ -334,18 +329,17 @@ static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) 
             // has t = INF, so compare only when something was deferred)
             if (!ok) atomicAdd(&cnt->trace_fail, 1ull);
             mark = !ok || (bb.mark_t != INFINITY && bb.mark_t <= bb.t * (1.f + XT_TIE));
-            full = !ok || bb.nc > 2;
+            full = !ok || bb.c1 == XT_CAND_OVERFLOW;
             c0 = bb.c0;
             c1 = bb.c1;
+            float ht = bb.t, hu = 0.f, hv = 0.f;
             if (!mark && bb.prim < S.n_prims && !(r.self_t > 0.f && bb.prim == r.origin)) {
                 const float4 a = S.prims[3 * bb.prim], b = S.prims[3 * bb.prim + 1];
-                // the sheared-frame t of the FP32 test loses up to ~1e-7 * distance /
-                // (edge * |n.d|) relative; the closest hit gets the plane-equation
-                // t in FP64 instead
+                // the closest triangle's t and barycentrics in FP64
                 if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE)
-                    bb.t = tri_t64(r.o, r.d, a, b, S.prims[3 * bb.prim + 2]);
+                    tri_hit64(r.o, r.d, a, b, S.prims[3 * bb.prim + 2], ht, hu, hv);
             }
-            hit[i] = make_float4(bb.t, __uint_as_float(bb.prim), bb.u, bb.v);
+            hit[i] = make_float4(ht, __uint_as_float(bb.prim), hu, hv);
         }
         // rays with a few deferred candidates get the FP64 re-check of just
         // those; rays with more, or whose traversal gave up, the full re-test

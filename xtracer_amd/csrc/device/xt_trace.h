@@ -64,22 +64,19 @@ struct Tester32 {
     Best best;
     Shear sh;
     __device__ __forceinline__ float limit() const { return best.t; }
-    // one record (a, b, c) of primitive `prim`; `obj` its object, or XT_NONE
-    // where the record does not carry it (exact t ties then go to the FP64
-    // re-check, which knows the objects)
-    __device__ __forceinline__ void test_rec(uint32_t prim, uint32_t obj, const float4 a, const float4 b,
-                                             const float4 c) {
+    // one record (a, b, c) of primitive `prim`
+    __device__ __forceinline__ void test_rec(uint32_t prim, const float4 a, const float4 b, const float4 c) {
         if (prim == r.origin) return;
         if (XT_PRIM_TYPE(__float_as_uint(a.w)) == XT_PRIM_TRIANGLE) {
             // a grazing triangle is decided by the FP64 re-check
-            if (tri32(r.o, sh, tsmall, a, obj, b, c, prim, best)) defer(best, 0.f, prim);
+            if (tri32(r.o, sh, tsmall, a, b, c, prim, best)) defer(best, 0.f, prim);
         } else {
-            sph32(r.o, r.d, tsmall, a, b, prim, obj, best);
+            sph32(r.o, r.d, tsmall, a, b, prim, best);
         }
     }
     __device__ __forceinline__ void test(uint32_t prim) {
         const float4 a = S.prims[3 * prim], b = S.prims[3 * prim + 1], c = S.prims[3 * prim + 2];
-        test_rec(prim, __float_as_uint(b.w), a, b, c);
+        test_rec(prim, a, b, c);
     }
     // kd leaf references [k, k + n): each record's three loads go out
     // together, and the next record's while this one is tested (the empty
@@ -97,7 +94,7 @@ struct Tester32 {
                 nc = S.kd_leaf[q + 2];
             }
             asm volatile("" ::"v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
-            test_rec(__float_as_uint(b.w), XT_NONE, a, b, c);
+            test_rec(__float_as_uint(b.w), a, b, c);
             a = na;
             b = nb;
             c = nc;

@@ -222,7 +222,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     }
     XT_TRY(hipSetDevice(d->device));
     const xt::FlatScene &f = s->flat;
-    if (f.planes.size() >= 0xfffeu || f.prims.size() >= 0xffff0000u) {
+    if (f.planes.size() >= 0xfffdu || f.prims.size() >= 0xffff0000u) {
         d->err = "too many primitives for the hit id space";
         return XT_E_SCENE;
     }
