@@ -30,7 +30,7 @@ PIPE_BYTES_PER_SAMPLE = 152     # camera ray + state write, FP64 accumulate read
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
 # (profiles/hbm_traffic_pathtracer_icosphere_1080p16.md); a measured figure,
 # re-taken whenever the kernel or the tree changes.
-TRACE_TRAFFIC_PER_QUERY = 378
+TRACE_TRAFFIC_PER_QUERY = 327
 
 
 def log(msg):
