@@ -206,35 +206,41 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene):
         info1 = dev.serve_info()
         return wall, info1["queries"] - info0["queries"], lat
 
-    res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1)}
+    def summary(wall, q, lat):
+        return {"wall_s": round(wall, 3), "Mrays_s": round(q / wall / 1e6, 1),
+                "p50_ms": round(_pct(lat, 50), 1), "p99_ms": round(_pct(lat, 99), 1)}
+
+    policies = ("fair", "srpt")
+    res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1), "policies": list(policies)}
     jobs = draw(args.serve_jobs, 10000, False)
     samples = sum(j.width * j.height * j.samples for j in jobs)
     served(jobs[:4])  # warm the server's buffers
     pw, pq, plat = plain(jobs)
-    sw, sq, slat = served(jobs)
-    res["batch"] = {"jobs": len(jobs), "samples": samples,
-                    "plain": {"wall_s": round(pw, 3), "Mrays_s": round(pq / pw / 1e6, 1),
-                              "p50_ms": round(_pct(plat, 50), 1), "p99_ms": round(_pct(plat, 99), 1)},
-                    "served": {"wall_s": round(sw, 3), "Mrays_s": round(sq / sw / 1e6, 1),
-                               "p50_ms": round(_pct(slat, 50), 1), "p99_ms": round(_pct(slat, 99), 1)},
-                    "served_over_plain": round(pw / sw, 3)}
-    log(f"serving batch: plain {pw:.2f}s served {sw:.2f}s")
-    mu = len(jobs) / sw
+    res["batch"] = {"jobs": len(jobs), "samples": samples, "plain": summary(pw, pq, plat)}
+    walls = {}
+    for pol in policies:
+        dev.set_serve_policy(pol)
+        sw, sq, slat = served(jobs)
+        walls[pol] = sw
+        res["batch"]["served_" + pol] = summary(sw, sq, slat)
+        log(f"serving batch ({pol}): plain {pw:.2f}s served {sw:.2f}s")
+    res["batch"]["served_over_plain"] = {pol: round(pw / walls[pol], 3) for pol in policies}
+    mu = len(jobs) / min(walls.values())
     res["capacity_jobs_s"] = round(mu, 2)
     res["poisson"] = []
     for rate in [float(x) for x in args.serve_rates.split(",") if x]:
         lam = rate * mu
         jobs = draw(args.serve_requests, 20000 + int(rate * 1000) * 100, True)
         arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs)))
-        sw, sq, slat = served(jobs, arrivals)
-        pw, pq, plat = plain(jobs, arrivals)
-        res["poisson"].append({"load": rate, "lambda_jobs_s": round(lam, 2), "requests": len(jobs),
-                               "served": {"p50_ms": round(_pct(slat, 50), 1), "p99_ms": round(_pct(slat, 99), 1),
-                                          "Mrays_s": round(sq / sw / 1e6, 1)},
-                               "plain": {"p50_ms": round(_pct(plat, 50), 1), "p99_ms": round(_pct(plat, 99), 1),
-                                         "Mrays_s": round(pq / pw / 1e6, 1)}})
-        log(f"poisson load {rate}: served p99 {res['poisson'][-1]['served']['p99_ms']} ms, "
-            f"plain p99 {res['poisson'][-1]['plain']['p99_ms']} ms")
+        entry = {"load": rate, "lambda_jobs_s": round(lam, 2), "requests": len(jobs)}
+        for pol in policies:
+            dev.set_serve_policy(pol)
+            entry["served_" + pol] = summary(*served(jobs, arrivals))
+        entry["plain"] = summary(*plain(jobs, arrivals))
+        res["poisson"].append(entry)
+        log(f"poisson load {rate}: p99 plain {entry['plain']['p99_ms']} ms, "
+            + ", ".join(f"{pol} {entry['served_' + pol]['p99_ms']} ms" for pol in policies))
+    dev.set_serve_policy("srpt")
     return res
 
 

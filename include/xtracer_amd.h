@@ -247,6 +247,13 @@ typedef struct xt_serve_info {
 } xt_serve_info;
 int xt_serve_get_info(const xt_dev *d, xt_serve_info *info);
 
+/* Pass scheduling: XT_SERVE_SRPT (default) fills each pass with the jobs
+ * that have the least work left; XT_SERVE_FAIR lets the oldest job take half
+ * of each pass and shares the rest equally. Images do not depend on it. */
+#define XT_SERVE_FAIR 0
+#define XT_SERVE_SRPT 1
+int xt_serve_set_policy(xt_dev *d, int policy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,8 +33,10 @@ def _drain_all(device, tickets):
 
 
 @pytest.mark.gpu
-def test_served_equals_alone(small_passes):
+@pytest.mark.parametrize("policy", ["srpt", "fair"])
+def test_served_equals_alone(small_passes, policy):
     dev = small_passes
+    dev.set_serve_policy(policy)
     sc = _scenes(dev)
     jobs = []
     for k, (kind, integ, w, h, spp) in enumerate([
@@ -47,6 +49,7 @@ def test_served_equals_alone(small_passes):
     tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
     got = _drain_all(dev, tickets)
     assert dev.serve_info()["jobs_active"] == 0
+    dev.set_serve_policy("srpt")
     for (t, (img, st)), j in zip(got.items(), jobs):
         ref, _ = dev.render(j)
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), f"job {t} differs from its one-job render"

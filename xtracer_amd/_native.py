@@ -23,6 +23,7 @@ ACCUM_DEFAULT, ACCUM_REFERENCE, ACCUM_FP64 = 0, 1, 2
 HIT_NONE = 0xFFFFFFFF
 HIT_PLANE_BASE = 0xFFFF0000
 TUNING_AUTO = 0xFFFFFFFF  # xt_tuning.trace_blocks_per_cu: resident fill
+SERVE_FAIR, SERVE_SRPT = 0, 1
 
 # Exported symbols, in the order include/xtracer_amd.h declares them.
 SYMBOLS = [
@@ -32,7 +33,7 @@ SYMBOLS = [
     "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
-    "xt_serve_get_info",
+    "xt_serve_get_info", "xt_serve_set_policy",
 ]
 
 
@@ -162,6 +163,7 @@ def lib():
         "xt_serve_step": (I, [P, ctypes.POINTER(I)]),
         "xt_drain": (I, [P]),
         "xt_serve_get_info": (I, [P, ctypes.POINTER(ServeInfo)]),
+        "xt_serve_set_policy": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -390,6 +392,12 @@ class Device:
         img = np.empty((height, width, 4), np.float32)
         self._check(lib().xt_poll(self._d, ticket, ctypes.byref(done), _ptr(img), ctypes.byref(st)))
         return True, img, st
+
+    def set_serve_policy(self, policy):
+        """SERVE_FAIR or SERVE_SRPT (or "fair" / "srpt")."""
+        if isinstance(policy, str):
+            policy = {"fair": SERVE_FAIR, "srpt": SERVE_SRPT}[policy]
+        self._check(lib().xt_serve_set_policy(self._d, int(policy)))
 
     def serve_info(self):
         info = ServeInfo()

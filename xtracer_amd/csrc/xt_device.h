@@ -26,6 +26,7 @@ struct WaveQueue {
 };
 
 #define XT_MAX_LANES 16
+#define XT_LANE_STREAMS 4  // streams the groups of a served pass spread over
 
 struct WaveBuffers {
     size_t cap = 0;
@@ -56,6 +57,8 @@ struct xt_dev {
     int device = 0;
     int n_cu = 256;
     hipStream_t stream = nullptr;
+    hipStream_t lane_stream[XT_LANE_STREAMS] = {};  // default priority, like `stream`
+    hipEvent_t lane_ev[XT_LANE_STREAMS] = {}, fork_ev = nullptr;
     uint64_t budget = 0, used = 0;
     std::map<uint32_t, DevScene> scenes;
     uint32_t next_scene = 1;
@@ -92,6 +95,6 @@ struct WaveLane {
     Counters *cnt = nullptr;
 };
 int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &lane, const double *ray64,
-                    uint32_t *flags, bool time_it);
+                    uint32_t *flags, bool time_it, hipStream_t st);
 void xt_serve_shutdown(xt_dev *d);
 bool xt_serve_uses_scene(const xt_dev *d, uint32_t scene_id);

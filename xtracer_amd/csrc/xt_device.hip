@@ -161,6 +161,11 @@ void xt_dev_destroy(xt_dev *d) {
     xt_dev_free_wave(d);
     xt_dev_free_accum(d);
     if (d->host_counters) hipHostFree(d->host_counters);
+    for (int k = 0; k < XT_LANE_STREAMS; ++k) {
+        if (d->lane_stream[k]) hipStreamDestroy(d->lane_stream[k]);
+        if (d->lane_ev[k]) hipEventDestroy(d->lane_ev[k]);
+    }
+    if (d->fork_ev) hipEventDestroy(d->fork_ev);
     hipEventDestroy(d->ev0);
     hipEventDestroy(d->ev1);
     for (int i = 0; i < XT_TRACE_EVENTS; ++i) hipEventDestroy(d->tev[i]);
@@ -370,38 +375,38 @@ static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream
 // Launches trace + FP64 re-test over queue `qi` of a lane (count in
 // lane.cnt->n[qi]).
 int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &L, const double *ray64,
-                    uint32_t *flags, bool time_it) {
+                    uint32_t *flags, bool time_it, hipStream_t st) {
     WaveBuffers &w = d->wave;
     Queue q{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
     float4 *hit = w.hit + L.off;
     uint32_t *retest = w.retest + L.off;
     uint4 *recheck = w.recheck + L.off;
     const bool timed = time_it && d->trace_launches < XT_TRACE_EVENTS - 1;
-    if (timed) hipEventRecord(d->tev[d->trace_launches], d->stream);
+    if (timed) hipEventRecord(d->tev[d->trace_launches], st);
     const xt_tuning &tn = d->tuning;
     if (accel == XT_ACCEL_OCTREE) {
         // 16 resident 64-lane blocks per CU (LDS: 20 KB each)
         const unsigned g = (unsigned)std::min<size_t>(L.cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
-        hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, d->stream,
+        hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, st,
                            ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
     } else {
-        launch_kd_trace(tn, d->n_cu, L.cap, d->stream, ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
+        launch_kd_trace(tn, d->n_cu, L.cap, st, ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
     }
     if (timed) {
-        hipEventRecord(d->tev[d->trace_launches + 1], d->stream);
+        hipEventRecord(d->tev[d->trace_launches + 1], st);
         d->timed_pairs.push_back(d->trace_launches);
         d->trace_launches += 2;
     }
     XT_LAUNCHED();
-    hipLaunchKernelGGL(k_recheck, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 256)), dim3(XT_BLOCK), 0, d->stream,
+    hipLaunchKernelGGL(k_recheck, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 256)), dim3(XT_BLOCK), 0, st,
                        ds.view, q, hit, recheck, retest, L.cnt, ray64, flags);
     XT_LAUNCHED();
     if (accel == XT_ACCEL_OCTREE) {
         hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(L.cap / 64 + 1, XT_OCT_BLOCK, 2048)),
-                           dim3(XT_OCT_BLOCK), 0, d->stream, ds.view, q, hit, retest, L.cnt, ray64, flags);
+                           dim3(XT_OCT_BLOCK), 0, st, ds.view, q, hit, retest, L.cnt, ray64, flags);
     } else {
         hipLaunchKernelGGL(k_retest<XT_ACCEL_KDTREE>, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 1024)), dim3(XT_BLOCK),
-                           0, d->stream, ds.view, q, hit, retest, L.cnt, ray64, flags);
+                           0, st, ds.view, q, hit, retest, L.cnt, ray64, flags);
     }
     XT_LAUNCHED();
     return XT_OK;
@@ -430,7 +435,7 @@ extern "C" int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const vo
     if (flags) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, flags, 0u, (size_t)n);
     d->trace_launches = 0;
     d->timed_pairs.clear();
-    rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, flags, kernel_ms != nullptr);
+    rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, flags, kernel_ms != nullptr, d->stream);
     w.q[0].r0 = save0;
     w.q[0].r1 = save1;
     w.hit = saveh;
@@ -482,7 +487,7 @@ extern "C" int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, w.flags, 0u, (size_t)n);
     d->trace_launches = 0;
     d->timed_pairs.clear();
-    rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, w.flags, false);
+    rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, w.flags, false, d->stream);
     if (rc) {
         release();
         return rc;

@@ -13,7 +13,8 @@ struct PassGroup {
     WaveLane lane;
     bool live = true;
 };
-int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &lane);
+int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &lane,
+                    hipStream_t st);
 int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace);
 int xt_ensure_host_counters(xt_dev *d);
 

@@ -113,7 +113,8 @@ uint32_t xt_auto_spp_pass(const JobDev &j, uint32_t requested, uint32_t target) 
 
 // Launches one shading step of a lane for query round r over queue qi into
 // queue qi^1.
-int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &L) {
+int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &L,
+                    hipStream_t st) {
     WaveBuffers &w = d->wave;
     Queue in{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
     Queue out{w.q[qi ^ 1].r0 + L.off, w.q[qi ^ 1].r1 + L.off, w.q[qi ^ 1].path + L.off};
@@ -123,21 +124,21 @@ int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t
     const unsigned g = grid_for(L.cap, XT_BLOCK, 8192);
     switch (integrator) {
         case XT_INTEGRATOR_STENCIL:
-            hipLaunchKernelGGL(k_shade_stencil, dim3(g), dim3(XT_BLOCK), 0, d->stream, in, n_in, hit, w.rad);
+            hipLaunchKernelGGL(k_shade_stencil, dim3(g), dim3(XT_BLOCK), 0, st, in, n_in, hit, w.rad);
             break;
         case XT_INTEGRATOR_DEPTH:
-            hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, n_in, hit, w.ray64,
+            hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64,
                                w.rad);
             break;
         case XT_INTEGRATOR_AO:
             if (r == 0)
-                hipLaunchKernelGGL(k_shade_ao1, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, n_in, hit, out,
+                hipLaunchKernelGGL(k_shade_ao1, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out,
                                    n_out, w.ps0, w.rad);
             else
-                hipLaunchKernelGGL(k_shade_ao2, dim3(g), dim3(XT_BLOCK), 0, d->stream, in, n_in, hit, w.rad);
+                hipLaunchKernelGGL(k_shade_ao2, dim3(g), dim3(XT_BLOCK), 0, st, in, n_in, hit, w.rad);
             break;
         default:
-            hipLaunchKernelGGL(k_shade_pt, dim3(g), dim3(XT_BLOCK), 0, d->stream, ds.view, in, n_in, hit, out, n_out,
+            hipLaunchKernelGGL(k_shade_pt, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out, n_out,
                                w.ps0, w.ps1, w.rad);
     }
     XT_LAUNCHED();
@@ -151,8 +152,11 @@ int xt_ensure_host_counters(xt_dev *d) {
 }
 
 // Query rounds of one pass over every group: trace + shade per round, queue
-// 0 holding the camera rays. Every second round past the second, the host
-// reads the queue counts and drops groups whose paths have all ended.
+// 0 holding the camera rays. With several groups, each group's kernels run
+// on a stream of its own so that one group's launch tails overlap another's
+// work; the device stream forks to them and joins them again. Every second
+// round past the second, the host reads the queue counts and drops groups
+// whose paths have all ended.
 int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
     WaveBuffers &w = d->wave;
     uint32_t max_rounds = 0;
@@ -160,23 +164,50 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
         groups[k].live = true;
         max_rounds = std::max(max_rounds, groups[k].rounds);
     }
+    const int n_st = n_groups > 1 ? std::min(n_groups, XT_LANE_STREAMS) : 0;
+    for (int k = 0; k < n_st; ++k) {
+        if (!d->lane_stream[k]) {
+            XT_TRY(hipStreamCreateWithFlags(&d->lane_stream[k], hipStreamNonBlocking));
+            XT_TRY(hipEventCreateWithFlags(&d->lane_ev[k], hipEventDisableTiming));
+        }
+    }
+    if (n_st && !d->fork_ev) XT_TRY(hipEventCreateWithFlags(&d->fork_ev, hipEventDisableTiming));
+    auto stream_of = [&](int k) { return n_st ? d->lane_stream[k % n_st] : d->stream; };
+    auto fork = [&]() -> int {
+        if (!n_st) return XT_OK;
+        XT_TRY(hipEventRecord(d->fork_ev, d->stream));
+        for (int k = 0; k < n_st; ++k) XT_TRY(hipStreamWaitEvent(d->lane_stream[k], d->fork_ev, 0));
+        return XT_OK;
+    };
+    auto join = [&]() -> int {
+        for (int k = 0; k < n_st; ++k) {
+            XT_TRY(hipEventRecord(d->lane_ev[k], d->lane_stream[k]));
+            XT_TRY(hipStreamWaitEvent(d->stream, d->lane_ev[k], 0));
+        }
+        return XT_OK;
+    };
+    int rc = fork();
+    if (rc) return rc;
     int qi = 0;
     for (uint32_t r = 0; r < max_rounds; ++r) {
         for (int k = 0; k < n_groups; ++k) {
             PassGroup &G = groups[k];
             if (!G.live || r >= G.rounds) continue;
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, G.lane.cnt,
+            const hipStream_t st = stream_of(k);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, G.lane.cnt,
                                (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
             const bool fp64_rays = G.integrator == XT_INTEGRATOR_DEPTH && r == 0;
-            int rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
-                                     time_trace);
+            rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
+                                 time_trace && !n_st, st);
             if (rc) return rc;
-            rc = xt_launch_shade(d, *G.ds, G.integrator, r, qi, G.lane);
+            rc = xt_launch_shade(d, *G.ds, G.integrator, r, qi, G.lane, st);
             if (rc) return rc;
         }
         qi ^= 1;
         // stop early once every path of the pass has ended
         if (max_rounds > 2 && r + 1 < max_rounds && (r % 2 == 1)) {
+            rc = join();
+            if (rc) return rc;
             XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, XT_MAX_LANES * sizeof(Counters), hipMemcpyDeviceToHost,
                                   d->stream));
             XT_TRY(hipStreamSynchronize(d->stream));
@@ -186,10 +217,12 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
                 if (G.live && (r + 1 >= G.rounds || d->host_counters[G.lane.cnt - w.cnt].n[qi] == 0)) G.live = false;
                 any = any || G.live;
             }
-            if (!any) break;
+            if (!any) return XT_OK;
+            rc = fork();
+            if (rc) return rc;
         }
     }
-    return XT_OK;
+    return join();
 }
 
 static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats *stats) {

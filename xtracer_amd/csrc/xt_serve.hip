@@ -7,11 +7,14 @@
 // (FP32 + FP64 accumulators and the resolved image, 80 B per pixel) fit the
 // device's HBM budget; a job that could never fit is refused at submission.
 //
-// Scheduling: each step runs one pass. Jobs with samples left get a fair
-// share of the pass's path slots (the wavefront size), at least one sample
-// each, taken in round-robin order so that a job skipped for lack of room
-// goes first in the next pass; leftover room goes to the same jobs in the
-// same order. Jobs join at any pass boundary (iteration-level batching).
+// Scheduling: each step runs one pass. XT_SERVE_SRPT (the default) fills it
+// with the jobs that have the least work left. XT_SERVE_FAIR lets the oldest unfinished job take
+// half of the pass's path slots (the wavefront size), which bounds how much
+// sharing slows a large job down; the other jobs with samples left get equal
+// shares of the rest, at least one 8x8 tile each, in round-robin order so
+// that a job skipped for lack of room goes first in the next pass; leftover
+// room goes to the same jobs in the same order. Jobs join at any pass
+// boundary (iteration-level batching).
 //
 // Layout: the pass's slots are grouped by (scene, integrator, accel). Each
 // group is a lane of the wavefront buffers with its own counters, so one
@@ -111,6 +114,7 @@ struct ServeState {
     hipEvent_t epoch = nullptr, p0 = nullptr, p1 = nullptr;
     double epoch_ms = 0;
     xt_serve_info info{};
+    int policy = XT_SERVE_SRPT;
 };
 
 namespace {
@@ -211,6 +215,43 @@ int admit(xt_dev *d, ServeState &S) {
     return XT_OK;
 }
 
+// Shortest remaining work first: jobs sorted by the path slots they still
+// need, each given all the room left (whole samples, or a slice of its
+// current sample), so small jobs finish in the pass after they arrive.
+void plan_srpt(xt_dev *d, ServeState &S, std::vector<Segment> &segs) {
+    std::vector<std::pair<uint64_t, uint64_t>> order;  // (remaining slots, ticket)
+    for (uint64_t t : S.active) {
+        const SJob &J = S.jobs[t];
+        if (J.issued < J.jd.total)
+            order.push_back({(uint64_t)(J.jd.total - J.issued) * J.jd.NT - J.qc, t});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<uint64_t, uint64_t> &a, const std::pair<uint64_t, uint64_t> &b) {
+                         return a.first < b.first;
+                     });
+    int64_t room = std::max<int64_t>(64, d->tuning.wave_paths & ~63u);
+    std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> keys;
+    for (const auto &e : order) {
+        if (room < 64) break;
+        const SJob &J = S.jobs[e.second];
+        const auto key = std::make_tuple(J.job.scene_id, J.jd.integrator, J.job.accel);
+        if (std::find(keys.begin(), keys.end(), key) == keys.end()) {
+            if (keys.size() >= XT_MAX_LANES) continue;
+            keys.push_back(key);
+        }
+        const int64_t nt = J.jd.NT;
+        if (J.qc == 0 && nt <= room) {
+            const int64_t k = std::min<int64_t>(J.jd.total - J.issued, room / nt);
+            segs.push_back({e.second, (uint32_t)k, 0u, (uint32_t)nt, 0});
+            room -= k * nt;
+        } else {
+            const int64_t len = std::min<int64_t>(nt - J.qc, room & ~63ll);
+            segs.push_back({e.second, 1u, J.qc, (uint32_t)len, 0});
+            room -= len;
+        }
+    }
+}
+
 // Picks the segments of the next pass (see the scheduling note above). A
 // pass never exceeds the wavefront size: a job whose image has more tile
 // positions than its share gets a slice of its current sample (a multiple
@@ -218,18 +259,35 @@ int admit(xt_dev *d, ServeState &S) {
 // before starting the next, so every pixel still sees its samples in order.
 void plan_pass(xt_dev *d, ServeState &S, std::vector<Segment> &segs) {
     segs.clear();
+    if (S.policy == XT_SERVE_SRPT) {
+        plan_srpt(d, S, segs);
+        return;
+    }
+    // the oldest unfinished job first (it may take half of the pass), then
+    // the others in round-robin order sharing the rest
     std::vector<uint64_t> order;
+    uint64_t oldest = 0;
+    for (uint64_t t : S.active) {
+        const SJob &J = S.jobs[t];
+        if (J.issued < J.jd.total) {
+            oldest = t;
+            break;
+        }
+    }
+    if (!oldest) return;
+    order.push_back(oldest);
     for (size_t k = 0; k < S.active.size(); ++k) {
         const uint64_t t = S.active[(S.rr + k) % S.active.size()];
         const SJob &J = S.jobs[t];
-        if (J.issued < J.jd.total) order.push_back(t);
+        if (t != oldest && J.issued < J.jd.total) order.push_back(t);
     }
-    if (order.empty()) return;
     const int64_t C = std::max<int64_t>(64, d->tuning.wave_paths & ~63u);
     const int64_t share = std::max<int64_t>(64, (C / (int64_t)order.size()) & ~63ll);
+    const int64_t first_share = std::max<int64_t>(share, (C / 2) & ~63ll);
     int64_t room = C;
     std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> keys;  // groups in use
-    for (uint64_t t : order) {
+    for (size_t oi = 0; oi < order.size(); ++oi) {
+        const uint64_t t = order[oi];
         if (room < 64) break;
         const SJob &J = S.jobs[t];
         const auto key = std::make_tuple(J.job.scene_id, J.jd.integrator, J.job.accel);
@@ -237,7 +295,7 @@ void plan_pass(xt_dev *d, ServeState &S, std::vector<Segment> &segs) {
             if (keys.size() >= XT_MAX_LANES) continue;
             keys.push_back(key);
         }
-        const int64_t nt = J.jd.NT, give = std::min(share, room);
+        const int64_t nt = J.jd.NT, give = std::min(oi == 0 ? first_share : share, room);
         if (J.qc == 0 && nt <= give) {
             const int64_t k = std::min<int64_t>(J.jd.total - J.issued, give / nt);
             segs.push_back({t, (uint32_t)k, 0u, (uint32_t)nt, 0});
@@ -526,6 +584,14 @@ extern "C" int xt_drain(xt_dev *d) {
         if (rc) return rc;
         if (n == 0) return XT_OK;
     }
+}
+
+extern "C" int xt_serve_set_policy(xt_dev *d, int policy) {
+    if (!d || (policy != XT_SERVE_FAIR && policy != XT_SERVE_SRPT)) return XT_E_INVALID;
+    int rc = serve_init(d);
+    if (rc) return rc;
+    d->serve->policy = policy;
+    return XT_OK;
 }
 
 extern "C" int xt_serve_get_info(const xt_dev *d, xt_serve_info *info) {
