@@ -35,18 +35,24 @@ extern "C" {
 #define XT_ACCEL_KDTREE 0
 #define XT_ACCEL_OCTREE 1
 
-/* integrators, src/xtcore/integrator/{stencil,depth,ao,pathtracer} */
+/* integrators, src/xtcore/integrator/{stencil,depth,ao,pathtracer,normal,uv,
+ * emission,wireframe}; uv is refused with XT_E_SCENE (the loader keeps no
+ * texture coordinates) */
 #define XT_INTEGRATOR_STENCIL 0
 #define XT_INTEGRATOR_DEPTH 1
 #define XT_INTEGRATOR_AO 2
 #define XT_INTEGRATOR_PATHTRACER 3
+#define XT_INTEGRATOR_NORMAL 4
+#define XT_INTEGRATOR_UV 5
+#define XT_INTEGRATOR_EMISSION 6
+#define XT_INTEGRATOR_WIREFRAME 7
 
 /* antialiasing::SAMPLE_DISTRIBUTION (src/xtcore/aa.h) */
 #define XT_AA_GRID 0
 #define XT_AA_RANDOM 1
 
 /* per-pixel accumulation */
-#define XT_ACCUM_DEFAULT 0   /* REFERENCE for stencil/depth, FP64 for ao/pathtracer */
+#define XT_ACCUM_DEFAULT 0   /* FP64 for ao/pathtracer, REFERENCE for the others (normal: always) */
 #define XT_ACCUM_REFERENCE 1 /* the reference's float read-modify-write per sample */
 #define XT_ACCUM_FP64 2      /* compensated (Neumaier) FP64 sum, rounded once */
 

@@ -21,6 +21,7 @@
 #include <xtcore/parseutil.h>
 #include <xtcore/strpool.h>
 #include <xtcore/log.h>
+#include <xtcore/integrator/wireframe/integrator.h>
 
 using namespace xtcore;
 
@@ -166,8 +167,8 @@ void xr_octree_trace(uint64_t n_tris, const double *tris, uint64_t n, const doub
     }
 }
 
-// Renders with a reference integrator (0 stencil, 1 depth, 2 ao,
-// 3 pathtracer) and assembles the framebuffer into rgba (W*H*4 floats).
+// Renders with a reference integrator (0 stencil, 1 depth, 2 ao, 3 pathtracer,
+// 4 normal, 5 uv, 6 emission, 7 wireframe) and assembles the framebuffer into rgba (W*H*4 floats).
 int xr_render(void *h, int integrator, int width, int height, int samples, int aa, int rdepth, int threads,
               unsigned seed, float *rgba) {
     render::context_t *ctx = (render::context_t *)h;
@@ -177,6 +178,10 @@ int xr_render(void *h, int integrator, int width, int height, int samples, int a
         case 1: in = new integrator::depth::Integrator; break;
         case 2: in = new integrator::ao::Integrator; break;
         case 3: in = new integrator::pathtracer::Integrator; break;
+        case 4: in = new integrator::normal::Integrator; break;
+        case 5: in = new integrator::uv::Integrator; break;
+        case 6: in = new integrator::emission::Integrator; break;
+        case 7: in = new integrator::wireframe::Integrator; break;
         default: return 1;
     }
     render::params_t p;
@@ -218,6 +223,10 @@ int xr_render_tiles(void *h, int integrator, int width, int height, int samples,
         case 1: in = new integrator::depth::Integrator; break;
         case 2: in = new integrator::ao::Integrator; break;
         case 3: in = new integrator::pathtracer::Integrator; break;
+        case 4: in = new integrator::normal::Integrator; break;
+        case 5: in = new integrator::uv::Integrator; break;
+        case 6: in = new integrator::emission::Integrator; break;
+        case 7: in = new integrator::wireframe::Integrator; break;
         default: return 1;
     }
     render::params_t p;

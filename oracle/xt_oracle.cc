@@ -38,6 +38,7 @@ static inline V operator*(V a, double s) { return mk(a.x * s, a.y * s, a.z * s);
 static inline V operator*(double s, V a) { return mk(a.x * s, a.y * s, a.z * s); }
 static inline V operator/(V a, double s) { return mk(a.x / s, a.y / s, a.z / s); }
 static inline V operator+(V a, double s) { return mk(a.x + s, a.y + s, a.z + s); }
+static inline V operator-(V a, double s) { return mk(a.x - s, a.y - s, a.z - s); }
 static inline V mul(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 static inline double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
@@ -644,7 +645,7 @@ void or_trace(void *h, uint64_t n, const double *rays, double *t, int32_t *obj, 
 }
 
 struct or_job {
-    uint32_t integrator;  // 0 stencil, 1 depth, 2 ao, 3 pathtracer
+    uint32_t integrator;  // 0 stencil, 1 depth, 2 ao, 3 pathtracer, 4 normal, 6 emission, 7 wireframe
     uint32_t width, height, samples, aa, aa_mode, rdepth, seed, job_key;
     uint32_t tile_size;   // 32 (context.cc defaults)
     uint32_t tile_stride; // render every k-th tile in row-major order (1: all)
@@ -759,6 +760,48 @@ uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double 
                                 val[c] = w;
                             }
                             px[3] = 1.f;
+                        } else if (jb->integrator == 4) {  // normal/integrator.cc:11-41
+                            Hit hh;
+                            ++nq;
+                            V acc = mk((double)px[0], (double)px[1], (double)px[2]) * 2.0 - 1.0;
+                            if (scene_hit(*s, o, d, hh)) {
+                                V nn = (hh.normal + acc) * 0.5;
+                                acc = (nrm(nn) + 1.0) * 0.5;
+                                val[0] = hh.normal.x;
+                                val[1] = hh.normal.y;
+                                val[2] = hh.normal.z;
+                            }
+                            // ColorRGBAf(acc, 1): negative channels clamp to 0
+                            const float fr = (float)acc.x, fg = (float)acc.y, fb = (float)acc.z;
+                            px[0] = fr < 0.f ? 0.f : fr;
+                            px[1] = fg < 0.f ? 0.f : fg;
+                            px[2] = fb < 0.f ? 0.f : fb;
+                            px[3] = 1.f;
+                        } else if (jb->integrator == 6) {  // emission/integrator.cc:11-40
+                            Hit hh;
+                            ++nq;
+                            if (scene_hit(*s, o, d, hh)) {
+                                const Obj &ob = s->objs[hh.obj];
+                                if (ob.mat != 0xffffffffu) {
+                                    const Mat &m = s->mats[ob.mat];
+                                    const float wf = (float)weight;  // operator*(ColorRGBf, float)
+                                    for (int c = 0; c < 3; ++c) {
+                                        const float e = cl(m.ke[c] * wf);
+                                        px[c] = cl(px[c] + e);
+                                        val[c] = m.ke[c];
+                                    }
+                                }
+                            }
+                            px[3] = 1.f;  // ColorRGBAf = ColorRGBf sets alpha 1
+                        } else if (jb->integrator == 7) {  // wireframe/integrator.cc:11-36
+                            Hit hh;
+                            ++nq;
+                            if (scene_hit(*s, o, d, hh)) {
+                                float na = (float)((double)px[3] + weight);
+                                px[0] = px[1] = px[2] = 1.f;
+                                px[3] = na < 0.f ? 0.f : (na > 1.f ? 1.f : na);
+                                val[0] = val[1] = val[2] = 1;
+                            }
                         } else {  // pathtracer/integrator.cc:51-74
                             C3 L3 = eval(cx, jb->rdepth, C3{1, 1, 1}, o, d, key, 1, nq);
                             C3 add = cscale(L3, (float)weight);

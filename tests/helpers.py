@@ -54,3 +54,27 @@ def mixed_rays(scene_path, n, seed=0, box=(-3, 6)):
 
 def scene_path(kind, **kw):
     return scenes.SCENES[kind][0](**kw)
+
+
+def glow_scene(path):
+    """A pinhole scene with emissive and non-emissive materials in view: an
+    emissive sphere, a Lambert sphere that also carries an emissive sampler,
+    a plain Lambert sphere, a Phong triangle and a Lambert floor plane."""
+    from tests import scn
+    scn.write(path,
+              geometry={"floor": dict(type="plane", normal=(0, 1, 0), distance=0.0),
+                        "a": dict(type="sphere", position=(-1.2, 0.6, 0.0), radius=0.6),
+                        "b": dict(type="sphere", position=(0.2, 0.5, 0.4), radius=0.5),
+                        "c": dict(type="sphere", position=(1.4, 0.4, -0.3), radius=0.4),
+                        "tri": dict(type="triangle", v=[(-2.0, 0.0, 1.5), (2.0, 0.0, 1.5), (0.0, 2.2, 1.8)])},
+              objects={"floor": ("floor", "ground"), "a": ("a", "lamp"), "b": ("b", "glowing"),
+                       "c": ("c", "matte"), "tri": ("tri", "shiny")},
+              materials={"ground": dict(type="lambert", kd=(0.5, 0.5, 0.5)),
+                         "lamp": dict(type="emissive", ke=(4.0, 3.5, 2.0)),
+                         "glowing": dict(type="lambert", kd=(0.2, 0.6, 0.2), ke=(0.3, 1.25, 0.5)),
+                         "matte": dict(type="lambert", kd=(0.7, 0.2, 0.2)),
+                         "shiny": dict(type="phong", kd=(0.3, 0.3, 0.6), ks=(0.5, 0.5, 0.5), reflectance=0.5,
+                                       exponent=20.0)},
+              camera=dict(position=(0.3, 1.4, -4.5), target=(0.0, 0.6, 0.5), fov=50.0),
+              env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
+    return path

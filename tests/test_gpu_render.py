@@ -22,11 +22,19 @@ def _setup(device, kind, **kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("integrator", [xa.STENCIL, xa.DEPTH])
-@pytest.mark.parametrize("kind,kw,samples", [("spheres", {}, 16), ("icosphere", {"subdiv": 4}, 4)])
+@pytest.mark.parametrize("integrator", [xa.STENCIL, xa.DEPTH, xa.NORMAL, xa.EMISSION, xa.WIREFRAME])
+@pytest.mark.parametrize("kind,kw,samples", [("spheres", {}, 16), ("icosphere", {"subdiv": 4}, 4), ("glow", {}, 4)])
 @pytest.mark.parametrize("accel", [xa.ACCEL_KDTREE, xa.ACCEL_OCTREE])
-def test_bit_exact(device, kind, kw, samples, integrator, accel):
-    s, sid, cam = _setup(device, kind, **kw)
+def test_bit_exact(device, kind, kw, samples, integrator, accel, tmp_path):
+    """The deterministic integrators (stencil, depth, normal, emission,
+    wireframe) reproduce the oracle's image bit for bit."""
+    if kind == "glow":
+        from tests.helpers import glow_scene
+        s = xa.Scene(glow_scene(str(tmp_path / "glow.scn")))
+        sid = device.upload(s)
+        cam = scenes.camera_dict(position=(0.3, 1.4, -4.5), target=(0.0, 0.6, 0.5), fov=50.0)
+    else:
+        s, sid, cam = _setup(device, kind, **kw)
     W, H = 96, 64
     job = xa.make_job(sid, integrator, W, H, s.camera(), samples=samples, seed=5, accel=accel)
     img, st = device.render(job)
@@ -66,3 +74,11 @@ def test_monte_carlo_bound(device, integrator, kind, kw):
     z = 2.576
     inside = np.abs(img[..., :3] - mean) <= z * se + 1e-6
     assert inside.mean() >= 0.97, f"{inside.mean():.4f} of channels inside the 99% bound"
+
+
+@pytest.mark.gpu
+def test_uv_refused(device):
+    s, sid, _ = _setup(device, "spheres")
+    with pytest.raises(xa.XtError) as e:
+        device.render(xa.make_job(sid, xa.UV, 16, 16, s.camera()))
+    assert e.value.code == 3

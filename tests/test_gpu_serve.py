@@ -43,7 +43,9 @@ def test_served_equals_alone(small_passes, policy):
             ("spheres", xa.STENCIL, 64, 48, 4), ("spheres", xa.DEPTH, 80, 40, 2),
             ("icosphere", xa.AO, 72, 56, 8), ("icosphere", xa.PATHTRACER, 96, 64, 16),
             ("spheres", xa.PATHTRACER, 40, 40, 5), ("icosphere", xa.PATHTRACER, 33, 17, 3),
-            ("icosphere", xa.PATHTRACER, 320, 240, 3)]):  # more tile positions than a pass: sliced samples
+            ("icosphere", xa.PATHTRACER, 320, 240, 3),  # more tile positions than a pass: sliced samples
+            ("icosphere", xa.NORMAL, 64, 64, 2), ("spheres", xa.EMISSION, 48, 32, 4),
+            ("icosphere", xa.WIREFRAME, 40, 24, 3)]):
         s, sid = sc[kind]
         jobs.append(xa.make_job(sid, integ, w, h, s.camera(), samples=spp, rdepth=5, seed=7 + k, job_key=k))
     tickets = {dev.submit(j): (j.width, j.height) for j in jobs}

@@ -84,12 +84,20 @@ print(json.dumps({{"bad": bad, "nonzero": int((ref[..., 3] > 0).sum())}}))
 
 
 @needs_ref
-@pytest.mark.parametrize("integrator", [0, 1])
-@pytest.mark.parametrize("kind,kw", [("spheres", {}), ("icosphere", {"subdiv": 3}), ("terrain", {"n": 21})])
-def test_oracle_deterministic_renders_bit_exact(kind, kw, integrator):
-    """stencil and depth use no random numbers with a pinhole camera and GRID
-    aa=1, so the oracle's image must equal the reference's bit for bit."""
-    path, cam = _synthetic(kind, **kw)
+@pytest.mark.parametrize("integrator", [0, 1, 4, 6, 7])
+@pytest.mark.parametrize("kind,kw", [("spheres", {}), ("icosphere", {"subdiv": 3}), ("terrain", {"n": 21}),
+                                     ("glow", {})])
+def test_oracle_deterministic_renders_bit_exact(kind, kw, integrator, tmp_path):
+    """stencil, depth, normal, emission and wireframe use no random numbers
+    with a pinhole camera and GRID aa=1, so the oracle's image must equal the
+    reference's bit for bit."""
+    if kind == "glow":
+        from tests.helpers import glow_scene
+        path = glow_scene(str(tmp_path / "glow.scn"))
+        from xtracer_amd.scenes import camera_dict
+        cam = camera_dict(position=(0.3, 1.4, -4.5), target=(0.0, 0.6, 0.5), fov=50.0)
+    else:
+        path, cam = _synthetic(kind, **kw)
     if cam["flength"] > 0:
         cam = dict(cam, flength=0.0, aperture=0.0)
         pytest.skip("thin-lens scene: primary rays use rand()")

@@ -16,8 +16,9 @@ ERRORS = {1: "XT_E_INVALID", 2: "XT_E_IO", 3: "XT_E_SCENE", 4: "XT_E_DEVICE", 5:
           6: "XT_E_NOTFOUND", 7: "XT_E_STATE"}
 
 ACCEL_KDTREE, ACCEL_OCTREE = 0, 1
-STENCIL, DEPTH, AO, PATHTRACER = 0, 1, 2, 3
-INTEGRATORS = {"stencil": STENCIL, "depth": DEPTH, "ao": AO, "pathtracer": PATHTRACER}
+STENCIL, DEPTH, AO, PATHTRACER, NORMAL, UV, EMISSION, WIREFRAME = range(8)
+INTEGRATORS = {"stencil": STENCIL, "depth": DEPTH, "ao": AO, "pathtracer": PATHTRACER, "normal": NORMAL, "uv": UV,
+               "emission": EMISSION, "wireframe": WIREFRAME}
 AA_GRID, AA_RANDOM = 0, 1
 ACCUM_DEFAULT, ACCUM_REFERENCE, ACCUM_FP64 = 0, 1, 2
 HIT_NONE = 0xFFFFFFFF

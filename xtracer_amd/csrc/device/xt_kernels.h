@@ -694,6 +694,106 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_depth(const SceneView
     }
 }
 
+#pragma clang fp contract(off)
+// Normal (src/xtcore/integrator/normal/integrator.cc:26-36): the hit record's
+// normal in FP64 as the reference forms it. Triangles interpolate their
+// vertex normals with the area-sum barycentrics and fall back to the
+// normalised geometric normal when that sum has zero length
+// (triangle.cc:110-113); spheres give (p - c) / r (sphere.cc:55-58); planes
+// their normal as given. The normal goes to the FP64 ray slot of the path
+// (the camera ray is not needed after this query); rad.y marks a hit.
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_normal(const SceneView S, const Queue in,
+                                                           const uint32_t *in_count, const float4 *hit,
+                                                           double *ray64, float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = in.path[i];
+        const float4 h = hit[i];
+        const uint32_t prim = __float_as_uint(h.y);
+        if (prim == XT_NONE) {
+            rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        double *q = ray64 + 6 * (size_t)path;
+        const d3 o = {q[0], q[1], q[2]}, d = {q[3], q[4], q[5]};
+        d3 nn;
+        if (prim >= XT_PLANE_BASE) {
+            const XtPlane &pl = S.planes[prim - XT_PLANE_BASE];
+            nn = {pl.normal64[0], pl.normal64[1], pl.normal64[2]};
+        } else {
+            const XtPrim64 &p = S.prims64[prim];
+            if (XT_PRIM_TYPE(__float_as_uint(S.prims[3 * prim].w)) == XT_PRIM_TRIANGLE) {
+                double bc[3];
+                double t = tri64(o, d, p, bc);
+                if (!(t > 0.0)) {  // the FP64 predicate disagrees: FP32 barycentrics
+                    bc[1] = h.z;
+                    bc[2] = h.w;
+                    bc[0] = 1.0 - bc[1] - bc[2];
+                }
+                const d3 A = {p.a[0], p.a[1], p.a[2]};
+                const d3 geo = dnorm(dcross(dsub(d3{p.b[0], p.b[1], p.b[2]}, A), dsub(d3{p.c[0], p.c[1], p.c[2]}, A)));
+                nn = geo;
+                if (S.vnormals) {
+                    const float *vn = S.vnormals + 9 * (size_t)prim;
+                    const d3 pn = dadd(dadd(dscl(d3{vn[0], vn[1], vn[2]}, bc[0]), dscl(d3{vn[3], vn[4], vn[5]}, bc[1])),
+                                       dscl(d3{vn[6], vn[7], vn[8]}, bc[2]));
+                    if (dlen(pn) != 0.0) nn = pn;
+                }
+            } else {
+                double t = sph64(o, d, p);
+                if (!(t > 0.0)) t = (double)h.x;
+                const d3 pt = dadd(o, dscl(d, t));
+                const double r = p.b[0];
+                nn = {(pt.x - p.a[0]) / r, (pt.y - p.a[1]) / r, (pt.z - p.a[2]) / r};
+            }
+        }
+        q[0] = nn.x;
+        q[1] = nn.y;
+        q[2] = nn.z;
+        rad[path] = make_float4(0.f, 1.f, 0.f, 0.f);
+    }
+}
+
+// The normal integrator's per-sample update of the stored pixel
+// (normal/integrator.cc:24-37): acc = rgb * 2 - 1; on a hit
+// acc = (normalize((n + acc) * 0.5) + 1) * 0.5; stored as ColorRGBAf(acc, 1),
+// whose constructor clamps negative channels to 0.
+__device__ __forceinline__ float4 normal_update(float4 a, bool hit, d3 n) {
+    d3 acc = {(double)a.x * 2.0 - 1.0, (double)a.y * 2.0 - 1.0, (double)a.z * 2.0 - 1.0};
+    if (hit) {
+        const d3 m = dscl(dadd(n, acc), 0.5);
+        const d3 u = dnorm(m);
+        acc = {(u.x + 1.0) * 0.5, (u.y + 1.0) * 0.5, (u.z + 1.0) * 0.5};
+    }
+    return make_float4(fmaxf((float)acc.x, 0.f), fmaxf((float)acc.y, 0.f), fmaxf((float)acc.z, 0.f), 1.f);
+}
+#pragma clang fp contract(on)
+
+// Emission (src/xtcore/integrator/emission/integrator.cc:26-35): the hit
+// object's emissive colour; rad.w marks a hit on an object with a material.
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_emission(const SceneView S, const Queue in,
+                                                             const uint32_t *in_count, const float4 *hit,
+                                                             float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = in.path[i];
+        const uint32_t prim = __float_as_uint(hit[i].y);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (prim != XT_NONE) {
+            const uint32_t obj = prim >= XT_PLANE_BASE ? S.planes[prim - XT_PLANE_BASE].object
+                                                       : __float_as_uint(S.prims[3 * prim + 1].w);
+            const uint32_t mi = S.obj_material[obj];
+            if (mi != XT_NONE) {
+                const XtMaterial &m = S.materials[mi];
+                v = make_float4(m.ke[0], m.ke[1], m.ke[2], 1.f);
+            }
+        }
+        rad[path] = v;
+    }
+}
+
 // Stencil (src/xtcore/integrator/stencil/integrator.cc:28-30).
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_stencil(const Queue in, const uint32_t *in_count,
                                                             const float4 *hit, float4 *rad) {
@@ -714,8 +814,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_stencil(const Queue i
 // (tile_t::read/write, lib/nimg/color.cc:35-40 clamps); FP64 keeps a
 // Neumaier sum per channel.
 static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
-                                                         uint32_t q0, uint32_t qn, const float4 *rad, float4 *accf,
-                                                         double2 *acc64) {
+                                                         uint32_t q0, uint32_t qn, const float4 *rad,
+                                                         const double *nrm64, float4 *accf, double2 *acc64) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < qn; qi += stride) {
         const uint32_t q = q0 + qi;
@@ -728,7 +828,18 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, 
             if (pass_base + sl >= j.total) break;
             const float4 v = rad[sl * qn + qi];
             if (j.accum == XT_ACCUM_REFERENCE) {
-                if (j.integrator == XT_INTEGRATOR_STENCIL) {
+                if (j.integrator == XT_INTEGRATOR_NORMAL) {
+                    const double *q = nrm64 + 6 * ((size_t)sl * qn + qi);
+                    a = normal_update(a, v.y != 0.f, d3{q[0], q[1], q[2]});
+                } else if (j.integrator == XT_INTEGRATOR_EMISSION) {
+                    // acc += emissive * float(weight) (ColorRGBf operators clamp at 0); alpha 1
+                    if (v.w != 0.f) {
+                        a.x = fmaxf(a.x + fmaxf(v.x * j.weight_f, 0.f), 0.f);
+                        a.y = fmaxf(a.y + fmaxf(v.y * j.weight_f, 0.f), 0.f);
+                        a.z = fmaxf(a.z + fmaxf(v.z * j.weight_f, 0.f), 0.f);
+                    }
+                    a.w = 1.f;
+                } else if (j.integrator == XT_INTEGRATOR_STENCIL || j.integrator == XT_INTEGRATOR_WIREFRAME) {
                     if (v.x != 0.f) {
                         const float na = (float)((double)a.w + j.weight);
                         a = make_float4(1.f, 1.f, 1.f, fminf(fmaxf(na, 0.f), 1.f));
@@ -754,7 +865,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, 
             } else {
                 const double w = j.weight;
                 double xs[3] = {(double)v.x * w, (double)v.y * w, (double)v.z * w};
-                if (j.integrator == XT_INTEGRATOR_STENCIL) xs[1] = xs[2] = xs[0];
+                if (j.integrator == XT_INTEGRATOR_STENCIL || j.integrator == XT_INTEGRATOR_WIREFRAME) xs[1] = xs[2] = xs[0];
+                if (j.integrator == XT_INTEGRATOR_EMISSION && v.w == 0.f) xs[0] = xs[1] = xs[2] = 0.0;
                 if (j.integrator == XT_INTEGRATOR_DEPTH && v.y == 0.f) xs[0] = xs[1] = xs[2] = 0.0;
                 if (j.integrator == XT_INTEGRATOR_DEPTH) xs[1] = xs[2] = xs[0];
                 double2 *acc[3] = {&s0, &s1, &s2};
@@ -787,7 +899,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_resolve(const JobDev j, con
             const double2 s = acc64[3 * (size_t)p + k];
             c[k] = fmaxf((float)(s.x + s.y), 0.f);
         }
-        if (j.integrator == XT_INTEGRATOR_STENCIL) {
+        if (j.integrator == XT_INTEGRATOR_STENCIL || j.integrator == XT_INTEGRATOR_WIREFRAME) {
             const float a = fminf(c[0], 1.f);
             const float rgb = c[0] > 0.f ? 1.f : 0.f;
             out[p] = make_float4(rgb, rgb, rgb, a);

@@ -17,6 +17,7 @@ struct DevScene {
     std::vector<void *> allocs;
     uint64_t bytes = 0;
     bool pt_supported = true;
+    bool materials_supported = true;
     bool missing_material = false;
 };
 

@@ -243,7 +243,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     ds.pt_supported = f.env_supported && !ds.missing_material;
     for (size_t i = 0; i < f.objects.size(); ++i) {
         uint32_t m = f.objects[i].material;
-        if (m != XT_NO_MATERIAL && !f.material_supported[m]) ds.pt_supported = false;
+        if (m != XT_NO_MATERIAL && !f.material_supported[m]) ds.pt_supported = ds.materials_supported = false;
     }
     // kd leaves' primitives copied in leaf order, the object word replaced
     // by the soup index, so the FP32 traversal reads a leaf's records

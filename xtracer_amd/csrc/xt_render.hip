@@ -42,7 +42,7 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
         d->err = "bad resolution";
         return XT_E_INVALID;
     }
-    if (job->integrator > XT_INTEGRATOR_PATHTRACER || job->accel > XT_ACCEL_OCTREE || job->aa_mode > XT_AA_RANDOM ||
+    if (job->integrator > XT_INTEGRATOR_WIREFRAME || job->accel > XT_ACCEL_OCTREE || job->aa_mode > XT_AA_RANDOM ||
         job->accum > XT_ACCUM_FP64) {
         d->err = "bad job parameter";
         return XT_E_INVALID;
@@ -50,6 +50,14 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
     if (job->rdepth > 0xffffu) {
         d->err = "rdepth above 65535";
         return XT_E_INVALID;
+    }
+    if (job->integrator == XT_INTEGRATOR_UV) {
+        d->err = "uv integrator: the scene loader keeps no texture coordinates";
+        return XT_E_SCENE;
+    }
+    if (job->integrator == XT_INTEGRATOR_EMISSION && !it->second.materials_supported) {
+        d->err = "scene has material samplers the GPU path does not evaluate";
+        return XT_E_SCENE;
     }
     if (job->integrator == XT_INTEGRATOR_PATHTRACER && !it->second.pt_supported) {
         d->err = "scene has materials or an environment the GPU pathtracer does not evaluate";
@@ -87,8 +95,10 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
     j.job_key = job->job_key;
     j.accum = job->accum;
     if (j.accum == XT_ACCUM_DEFAULT)
-        j.accum = (j.integrator == XT_INTEGRATOR_STENCIL || j.integrator == XT_INTEGRATOR_DEPTH) ? XT_ACCUM_REFERENCE
-                                                                                               : XT_ACCUM_FP64;
+        j.accum = (j.integrator == XT_INTEGRATOR_AO || j.integrator == XT_INTEGRATOR_PATHTRACER) ? XT_ACCUM_FP64
+                                                                                              : XT_ACCUM_REFERENCE;
+    // the normal integrator blends into the stored pixel; there is no sum
+    if (j.integrator == XT_INTEGRATOR_NORMAL) j.accum = XT_ACCUM_REFERENCE;
     j.wf = (float)j.W;
     j.hf = (float)j.H;
     j.cam = job->camera;
@@ -124,7 +134,14 @@ int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t
     const unsigned g = grid_for(L.cap, XT_BLOCK, 8192);
     switch (integrator) {
         case XT_INTEGRATOR_STENCIL:
+        case XT_INTEGRATOR_WIREFRAME:  // wireframe/integrator.cc is the stencil rule
             hipLaunchKernelGGL(k_shade_stencil, dim3(g), dim3(XT_BLOCK), 0, st, in, n_in, hit, w.rad);
+            break;
+        case XT_INTEGRATOR_NORMAL:
+            hipLaunchKernelGGL(k_shade_normal, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64, w.rad);
+            break;
+        case XT_INTEGRATOR_EMISSION:
+            hipLaunchKernelGGL(k_shade_emission, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.rad);
             break;
         case XT_INTEGRATOR_DEPTH:
             hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64,
@@ -196,7 +213,7 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
             const hipStream_t st = stream_of(k);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, G.lane.cnt,
                                (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
-            const bool fp64_rays = G.integrator == XT_INTEGRATOR_DEPTH && r == 0;
+            const bool fp64_rays = (G.integrator == XT_INTEGRATOR_DEPTH || G.integrator == XT_INTEGRATOR_NORMAL) && r == 0;
             rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
                                  time_trace && !n_st, st);
             if (rc) return rc;
@@ -237,7 +254,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         d->err = "pass too large";
         return XT_E_INVALID;
     }
-    const bool need64 = j.integrator == XT_INTEGRATOR_DEPTH;
+    const bool need64 = j.integrator == XT_INTEGRATOR_DEPTH || j.integrator == XT_INTEGRATOR_NORMAL;
     rc = xt_dev_ensure_wave(d, cap, need64);
     if (rc) return rc;
     rc = xt_dev_ensure_accum(d, (size_t)j.W * j.H);
@@ -275,7 +292,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         rc = xt_run_rounds(d, &G, 1, time_trace);
         if (rc) return rc;
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(j.NT, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
-                           spp_pass, 0u, j.NT, w.rad, a.accf, a.acc64);
+                           spp_pass, 0u, j.NT, w.rad, need64 ? w.ray64 : nullptr, a.accf, a.acc64);
         XT_LAUNCHED();
     }
     hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, a.accf,

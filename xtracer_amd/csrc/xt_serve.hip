@@ -361,7 +361,7 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
         groups[g].accel = grps[g].accel;
         groups[g].rounds = rounds;
         groups[g].ds = &d->scenes.at(grps[g].scene);
-        need64 = need64 || grps[g].integrator == XT_INTEGRATOR_DEPTH;
+        need64 = need64 || grps[g].integrator == XT_INTEGRATOR_DEPTH || grps[g].integrator == XT_INTEGRATOR_NORMAL;
     }
     if (total >= (1ull << 31)) {
         d->err = "pass too large";
@@ -395,7 +395,7 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
         SJob &J = S.jobs[segs[k].ticket];
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(segs[k].qn, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream,
                            J.jd, J.issued, segs[k].samples, segs[k].q0, segs[k].qn, w.rad + placed[k].path_base,
-                           J.accf, J.acc64);
+                           need64 ? w.ray64 + 6 * placed[k].path_base : nullptr, J.accf, J.acc64);
         XT_LAUNCHED();
         if (segs[k].qn == J.jd.NT) {
             J.issued += segs[k].samples;
