@@ -139,6 +139,17 @@ int xt_scene_camera(const xt_scene *s, const char *camera_name, xt_camera *out);
 int xt_camera_make(const double position[3], const double target[3], const double up[3], float fov_deg,
                    float aperture, float flength, xt_camera *out);
 
+/* Writes an assembled RGBA float framebuffer (width*height*4, row-major) as
+ * the file type its extension names: .png .bmp .tga (8-bit, the reference's
+ * conversion: v*255, set to 255 above 255, truncated; BMP composites alpha
+ * over magenta like the reference's writer), .hdr (Radiance RGBE), .exr
+ * (OpenEXR, FLOAT B G R, uncompressed), .pfm, .ppm. Replaces
+ * nimg::io::save::{png,bmp,tga,hdr,exr} (lib/nimg/img.cc:72-318) as the GUI
+ * (src/frontend/gui/action.cc:95-110) and CLI (src/frontend/cli/xtracer.cc:
+ * 140-150) call them. XT_E_INVALID for an unknown extension, XT_E_IO when the
+ * file cannot be written. */
+int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t height);
+
 /* ----------------------------------------------------------------- device */
 
 /* hbm_budget_bytes caps the device memory the handle allocates (scenes,

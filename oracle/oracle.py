@@ -175,8 +175,21 @@ def ref():
         L.xr_octree_trace.argtypes = [ctypes.c_uint64, P, ctypes.c_uint64, P, P, P, P]
         L.xr_render.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_uint, P]
+        L.xr_save_image.argtypes = [ctypes.c_int, ctypes.c_char_p, P, ctypes.c_int, ctypes.c_int]
         _ref = L
     return _ref
+
+
+REF_IMAGE_FORMATS = {"png": 0, "bmp": 1, "tga": 2, "hdr": 3, "exr": 4}
+
+
+def ref_save_image(fmt, path, rgba):
+    """Writes an (H, W, 4) float framebuffer through the reference's own
+    nimg::io::save::<fmt> (lib/nimg/img.cc:72-318), via a Pixmap."""
+    a = np.ascontiguousarray(rgba, np.float32)
+    rc = ref().xr_save_image(REF_IMAGE_FORMATS[fmt], str(path).encode(), _p(a), a.shape[1], a.shape[0])
+    if rc:
+        raise RuntimeError(f"reference {fmt} writer returned {rc}")
 
 
 class RefScene:

@@ -22,6 +22,8 @@
 #include <xtcore/strpool.h>
 #include <xtcore/log.h>
 #include <xtcore/integrator/wireframe/integrator.h>
+#include <nimg/img.h>
+#include <nimg/pixmap.h>
 
 using namespace xtcore;
 
@@ -254,6 +256,26 @@ int xr_render_tiles(void *h, int integrator, int width, int height, int samples,
     if (queries) *queries = q;
     delete in;
     return 0;
+}
+
+// Writes rgba (W*H*4 floats) through the reference's own image writers
+// (lib/nimg/img.cc): 0 png, 1 bmp, 2 tga, 3 hdr, 4 exr.
+int xr_save_image(int fmt, const char *path, const float *rgba, int w, int h) {
+    nimg::Pixmap pm;
+    if (pm.init((size_t)w, (size_t)h)) return 1;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float *p = rgba + 4 * ((size_t)y * w + x);
+            pm.pixel((size_t)x, (size_t)y) = nimg::ColorRGBAf(p[0], p[1], p[2], p[3]);
+        }
+    switch (fmt) {
+        case 0: return nimg::io::save::png(path, pm);
+        case 1: return nimg::io::save::bmp(path, pm);
+        case 2: return nimg::io::save::tga(path, pm);
+        case 3: return nimg::io::save::hdr(path, pm);
+        case 4: return nimg::io::save::exr(path, pm);
+        default: return 1;
+    }
 }
 
 }  // extern "C"

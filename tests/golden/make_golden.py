@@ -6,6 +6,8 @@ synthetic scenes (regenerated identically anywhere by xtracer_amd.scenes):
   ref_trace_<scene>.npz   FP32 rays -> Scene::intersection t, object index, point
   ref_render_<scene>.npz  stencil and depth framebuffers (pinhole, GRID aa=1)
   ref_mc_<scene>.npz      high-spp ao / pathtracer framebuffers (glibc rand())
+  ref_images.npz          a framebuffer and the bytes of the reference's
+                          png / bmp / tga / hdr / exr files of it (nimg::io::save)
 Every scene is loaded in a fresh process: the reference's string pool is
 process-global, and its handle order decides the scan order.
 
@@ -60,7 +62,43 @@ print("ok", name)
 """
 
 
+def image_framebuffer():
+    """A 37x23 RGBA float framebuffer with flat stretches (run-length
+    packets), values below 0 and above 1, alpha outside [0, 1], very small
+    and very large HDR values, and an odd width (BMP row padding)."""
+    import numpy as np
+    rng = np.random.default_rng(4321)
+    h, w = 23, 37
+    fb = rng.uniform(-0.2, 1.5, (h, w, 4)).astype(np.float32)
+    fb[..., 3] = rng.uniform(-0.1, 1.1, (h, w))
+    fb[2:6, :] = (0.25, 0.5, 0.75, 1.0)           # long runs
+    fb[7, 3:20] = (1.0, 0.0, 1.0, 0.5)
+    fb[9, ::2] = (0.3, 0.3, 0.3, 1.0)             # alternating
+    fb[11, 5] = (1e3, 2e-3, 7.0, 1.0)             # large and small HDR values
+    fb[11, 6] = (1e-35, 0.0, 1e-36, 1.0)          # below the RGBE floor
+    fb[12, :] = (1.0, 1.0, 1.0, 1.0)
+    fb[13, :8] = (0.0, 0.0, 0.0, 0.0)
+    return fb
+
+
+def make_images():
+    import tempfile
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    fb = image_framebuffer()
+    out = {"rgba": fb}
+    with tempfile.TemporaryDirectory() as d:
+        for fmt in O.REF_IMAGE_FORMATS:
+            path = os.path.join(d, "img." + fmt)
+            O.ref_save_image(fmt, path, fb)
+            out[fmt] = np.fromfile(path, np.uint8)
+    np.savez_compressed(os.path.join(HERE, "ref_images.npz"), **out)
+    print("ok images")
+
+
 def main():
+    make_images()
     for name, (kind, kw) in SCENES.items():
         code = CHILD.format(root=ROOT, name=name, kind=kind, kw=json.dumps(kw), out=HERE)
         print(subprocess.check_output([sys.executable, "-c", code], timeout=1800).decode().strip())

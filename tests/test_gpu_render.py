@@ -82,3 +82,26 @@ def test_uv_refused(device):
     with pytest.raises(xa.XtError) as e:
         device.render(xa.make_job(sid, xa.UV, 16, 16, s.camera()))
     assert e.value.code == 3
+
+
+@pytest.mark.gpu
+def test_xtrender_exports(tmp_path):
+    """The headless driver's export: -outdir names the file as the reference
+    CLI does (<scene>_<renderer>.png) and its pixels are the 8-bit conversion
+    of the same render written raw."""
+    import os
+    import shutil
+    import subprocess
+    exe = os.path.join(os.path.dirname(xa.__file__), "xtrender")
+    scn = tmp_path / "spheres.scn"
+    shutil.copy(scene_path("spheres"), scn)
+    args = [exe, str(scn), "-renderer", "stencil", "-res", "40x24", "-samples", "2"]
+    for extra in (["-out", str(tmp_path / "frame.raw")], ["-outdir", str(tmp_path / "out")]):
+        os.makedirs(tmp_path / "out", exist_ok=True)
+        r = subprocess.run(args + extra, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    raw = np.fromfile(tmp_path / "frame.raw", np.float32).reshape(24, 40, 4)
+    from tests.test_image_io import read_png, to8
+    png = read_png((tmp_path / "out" / "spheres_stencil.png").read_bytes())
+    assert np.array_equal(png, to8(raw))
+    assert raw[..., 3].max() > 0

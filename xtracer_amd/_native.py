@@ -31,7 +31,7 @@ SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
     "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals",
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
-    "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
+    "xt_image_write", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
     "xt_serve_get_info", "xt_serve_set_policy",
@@ -148,6 +148,7 @@ def lib():
         "xt_scene_camera": (I, [P, ctypes.c_char_p, ctypes.POINTER(Camera)]),
         "xt_camera_make": (I, [ctypes.POINTER(ctypes.c_double)] * 3 + [ctypes.c_float] * 3 +
                            [ctypes.POINTER(Camera)]),
+        "xt_image_write": (I, [ctypes.c_char_p, P, U32, U32]),
         "xt_dev_create": (I, [ctypes.POINTER(P), I, U64]),
         "xt_dev_destroy": (None, [P]),
         "xt_last_error": (ctypes.c_char_p, [P]),
@@ -303,6 +304,19 @@ def make_camera(position, target, up, fov, aperture=0.0, flength=0.0):
     if rc:
         raise XtError(rc)
     return c
+
+
+def write_image(path, rgba):
+    """Writes an (H, W, 4) float framebuffer to `path`; the format follows the
+    extension (.png .bmp .tga .hdr .exr .pfm .ppm). Host-only: needs the
+    library, not a GPU."""
+    import numpy as np
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError(f"expected an (H, W, 4) array, got {a.shape}")
+    rc = lib().xt_image_write(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0])
+    if rc:
+        raise XtError(rc, f"xt_image_write({path})")
 
 
 class Device:

@@ -6,6 +6,7 @@
 
 #include "../../include/xtracer_amd.h"
 #include "capi_internal.h"
+#include "host/image_io.h"
 
 using namespace xt;
 
@@ -211,6 +212,13 @@ int xt_scene_camera(const xt_scene *s, const char *camera_name, xt_camera *out) 
     const double t[3] = {cam->target.x, cam->target.y, cam->target.z};
     const double u[3] = {cam->up.x, cam->up.y, cam->up.z};
     return xt_camera_make(p, t, u, cam->fov, cam->aperture, cam->flength, out);
+}
+
+
+int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t height) {
+    if (!path || !rgba || !width || !height) return XT_E_INVALID;
+    const int rc = xt::write_image(path, rgba, width, height);
+    return rc == 0 ? XT_OK : rc == 2 ? XT_E_INVALID : XT_E_IO;
 }
 
 }  // extern "C"

@@ -106,7 +106,8 @@ public:
 namespace gpu {
 
 enum Kind { STENCIL = XT_INTEGRATOR_STENCIL, DEPTH = XT_INTEGRATOR_DEPTH, AO = XT_INTEGRATOR_AO,
-            PATHTRACER = XT_INTEGRATOR_PATHTRACER };
+            PATHTRACER = XT_INTEGRATOR_PATHTRACER, NORMAL = XT_INTEGRATOR_NORMAL, UV = XT_INTEGRATOR_UV,
+            EMISSION = XT_INTEGRATOR_EMISSION, WIREFRAME = XT_INTEGRATOR_WIREFRAME };
 
 // The MI355X integrator: setup_auxiliary() creates the device and uploads the
 // scene (the role the reference's Embree integrator gives it), render()

@@ -1,10 +1,14 @@
 // Headless driver over the xtcore-shaped C++ API: loads a .scn, renders with
-// the MI355X integrator and writes the assembled framebuffer as a PFM (RGB,
-// float) or raw RGBA floats. Flags follow the reference CLI
-// (src/frontend/cli/argdefs.h:7-17) where they apply:
-//   xtrender scene.scn [-renderer stencil|depth|ao|pathtracer] [-res WxH]
+// the MI355X integrator and writes the assembled framebuffer. Flags follow
+// the reference CLI (src/frontend/cli/argdefs.h:7-17) where they apply:
+//   xtrender scene.scn [-renderer stencil|depth|ao|pathtracer|normal|emission|wireframe] [-res WxH]
 //            [-samples N] [-rdepth N] [-aa N] [-cam NAME] [-mod a.b.c:v]
-//            [-accel kdtree|octree] [-seed N] [-out FILE(.pfm|.raw)]
+//            [-accel kdtree|octree] [-seed N] [-outdir DIR] [-out FILE]
+// With neither -out nor -outdir the image goes to <scene>_<renderer>.png in
+// the current directory, with -outdir to DIR/<scene>_<renderer>.png, as the
+// reference CLI names it (src/frontend/cli/xtracer.cc:125-146). -out picks
+// the format by extension (.png .bmp .tga .hdr .exr .pfm .ppm, or .raw for
+// the RGBA floats as they are).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,14 +21,15 @@ using namespace xtamd;
 
 static int usage() {
     std::fprintf(stderr,
-                 "usage: xtrender scene.scn [-renderer stencil|depth|ao|pathtracer] [-res WxH] [-samples N]\n"
-                 "       [-rdepth N] [-aa N] [-cam NAME] [-mod a.b.c:v] [-accel kdtree|octree] [-seed N] [-out FILE]\n");
+                 "usage: xtrender scene.scn [-renderer stencil|depth|ao|pathtracer|normal|emission|wireframe] [-res WxH] [-samples N]\n"
+                 "       [-rdepth N] [-aa N] [-cam NAME] [-mod a.b.c:v] [-accel kdtree|octree] [-seed N]\n"
+                 "       [-outdir DIR] [-out FILE(.png|.bmp|.tga|.hdr|.exr|.pfm|.ppm|.raw)]\n");
     return 1;
 }
 
 int main(int argc, char **argv) {
     if (argc < 2) return usage();
-    std::string scene = argv[1], out = "out.pfm", renderer = "pathtracer";
+    std::string scene = argv[1], out, outdir, renderer = "pathtracer";
     std::vector<std::string> mods;
     render::context_t ctx;
     for (int i = 2; i < argc; ++i) {
@@ -45,12 +50,17 @@ int main(int argc, char **argv) {
         else if (a == "-accel") ctx.params.accel = std::strcmp(v, "octree") ? XT_ACCEL_KDTREE : XT_ACCEL_OCTREE;
         else if (a == "-seed") ctx.params.seed = (uint32_t)std::strtoul(v, nullptr, 10);
         else if (a == "-out") out = v;
+        else if (a == "-outdir") outdir = v;
         else return usage();
         ++i;
     }
     gpu::Kind kind = renderer == "stencil" ? gpu::STENCIL
                      : renderer == "depth" ? gpu::DEPTH
                      : renderer == "ao" ? gpu::AO
+                     : renderer == "normal" ? gpu::NORMAL
+                     : renderer == "uv" ? gpu::UV
+                     : renderer == "emission" ? gpu::EMISSION
+                     : renderer == "wireframe" ? gpu::WIREFRAME
                      : gpu::PATHTRACER;
     int rc = ctx.load(scene, mods);
     if (rc) {
@@ -71,21 +81,25 @@ int main(int argc, char **argv) {
     }
     std::vector<float> rgba;
     render::assemble(rgba, ctx);
-    FILE *f = std::fopen(out.c_str(), "wb");
-    if (!f) return 5;
     const size_t W = ctx.params.width, H = ctx.params.height;
-    if (out.size() > 4 && out.substr(out.size() - 4) == ".raw") {
-        std::fwrite(rgba.data(), sizeof(float), rgba.size(), f);
-    } else {
-        std::fprintf(f, "PF\n%zu %zu\n-1.0\n", W, H);
-        std::vector<float> row(3 * W);
-        for (size_t y = H; y-- > 0;) {  // PFM stores rows bottom to top
-            for (size_t x = 0; x < W; ++x)
-                for (int c = 0; c < 3; ++c) row[3 * x + c] = rgba[4 * (y * W + x) + c];
-            std::fwrite(row.data(), sizeof(float), row.size(), f);
-        }
+    if (out.empty()) {
+        std::string base = scene.substr(scene.find_last_of('/') == std::string::npos ? 0 : scene.find_last_of('/') + 1);
+        base = base.substr(0, base.find_last_of('.'));
+        if (!outdir.empty() && outdir.back() != '/') outdir += '/';
+        out = outdir + base + "_" + renderer + ".png";
     }
-    std::fclose(f);
+    if (out.size() > 4 && out.substr(out.size() - 4) == ".raw") {
+        FILE *f = std::fopen(out.c_str(), "wb");
+        const bool ok = f && std::fwrite(rgba.data(), sizeof(float), rgba.size(), f) == rgba.size();
+        if (f) std::fclose(f);
+        if (!ok) {
+            std::fprintf(stderr, "failed to write %s\n", out.c_str());
+            return 5;
+        }
+    } else if (int wr = xt_image_write(out.c_str(), rgba.data(), (uint32_t)W, (uint32_t)H)) {
+        std::fprintf(stderr, "failed to export %s (%d)\n", out.c_str(), wr);
+        return 5;
+    }
     const xt_job_stats &st = in.stats();
     std::printf("%s %zux%zu: %llu queries in %.3f ms (%.1f Mrays/s)\n", renderer.c_str(), W, H,
                 (unsigned long long)st.queries, st.ms_total, st.ms_total > 0 ? st.queries / st.ms_total / 1e3 : 0.0);
