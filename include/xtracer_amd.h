@@ -123,19 +123,31 @@ int xt_scene_get_kdtree(const xt_scene *s, uint32_t *nodes, uint32_t *refs, floa
 int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint64_t *nodes, uint64_t *max_level,
                          uint64_t *triangles);
 
-/* Pinhole / thin-lens camera, evaluated on the host in FP64 exactly as
- * Perspective::get_primary_ray (src/xtcore/camera/perspective.cc:26-98). */
+/* Camera kinds, as the .scn "type" names them (parseutil.cc:262-275). */
+#define XT_CAMERA_THIN_LENS 0u  /* Perspective, src/xtcore/camera/perspective.cc:26-98 */
+#define XT_CAMERA_ODS 1u        /* omni-directional stereo, camera/ods.cc:15-33 */
+#define XT_CAMERA_ERP 2u        /* equirectangular, camera/erp.cc:11-25 */
+#define XT_CAMERA_CUBEMAP 3u    /* vertical strip +x -x +y -y +z -z, camera/cubemap.cc:17-60;
+                                   the image height must be a multiple of 6 */
+
+/* A camera, set up on the host in FP64 the way the reference sets it up;
+ * the device evaluates its primary rays in FP64 with the reference's
+ * float/double mix. */
 typedef struct xt_camera {
     double position[3];
-    double basis[9];      /* row-major m_transform 3x3 */
-    double dir_z;         /* 1 / tan(fov * RADIAN / 2) */
+    double basis[9];      /* thin-lens: row-major m_transform 3x3; ODS / ERP: the
+                             orientation's rotation Rx*Ry*Rz (matrix.cc:485-492),
+                             row-major; cubemap: unused */
+    double dir_z;         /* thin-lens: 1 / tan(fov * RADIAN / 2) */
     double aperture, flength;
-    uint32_t thin_lens;   /* flength > 0 */
-    uint32_t pad;
+    uint32_t thin_lens;   /* thin-lens with flength > 0 */
+    uint32_t kind;        /* XT_CAMERA_* */
+    double ipd;           /* ODS: the float inter-pupillary distance */
 } xt_camera;
 /* camera_name NULL or "": the first camera in name order (workspace.cc:33-38) */
 int xt_scene_camera(const xt_scene *s, const char *camera_name, xt_camera *out);
-/* A camera from an explicit pose (serving requests with their own views). */
+/* A thin-lens camera from an explicit pose (serving requests with their own
+ * views). */
 int xt_camera_make(const double position[3], const double target[3], const double up[3], float fov_deg,
                    float aperture, float flength, xt_camera *out);
 

@@ -27,7 +27,31 @@ class OrJob(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("integrator", "width", "height", "samples", "aa", "aa_mode",
                                                 "rdepth", "seed", "job_key", "tile_size", "tile_stride")] + [
         ("threads", ctypes.c_int), ("cam", ctypes.c_double * 9), ("fov", ctypes.c_float),
-        ("aperture", ctypes.c_float), ("flength", ctypes.c_float)]
+        ("aperture", ctypes.c_float), ("flength", ctypes.c_float), ("cam_kind", ctypes.c_uint32),
+        ("orientation", ctypes.c_double * 3), ("ipd", ctypes.c_float)]
+
+
+CAMERA_KINDS = {"thin-lens": 0, "ods": 1, "erp": 2, "cubemap": 3}
+
+
+def _job(integrator, width, height, cam, samples=1, aa=1, aa_mode=0, rdepth=100, seed=1, job_key=0, threads=0,
+         tile_stride=1):
+    """cam: dict(position, target, up, fov, aperture, flength) for a thin lens, or
+    dict(type=ods|erp|cubemap, position, orientation, ipd)."""
+    kind = CAMERA_KINDS[cam.get("type", "thin-lens")]
+    return OrJob(integrator, width, height, samples, aa, aa_mode, rdepth, seed, job_key, 32, tile_stride, threads,
+                 (ctypes.c_double * 9)(*cam["position"], *cam.get("target", (0, 0, 1)), *cam.get("up", (0, 1, 0))),
+                 cam.get("fov", 0.0), cam.get("aperture", 0.0), cam.get("flength", 0.0), kind,
+                 (ctypes.c_double * 3)(*cam.get("orientation", (0, 0, 0))), cam.get("ipd", 0.0))
+
+
+def camera_rays(cam, width, height, xy):
+    """The oracle's primary rays of camera dict `cam` at sample coordinates xy (n x 2 float32)."""
+    xy = np.ascontiguousarray(xy, np.float32)
+    o = np.zeros((len(xy), 3))
+    d = np.zeros((len(xy), 3))
+    lib().or_camera_rays(ctypes.byref(_job(0, width, height, cam)), len(xy), _p(xy), _p(o), _p(d))
+    return o, d
 
 
 def lib():
@@ -43,6 +67,7 @@ def lib():
         L.or_trace.argtypes = [P, ctypes.c_uint64, P, P, P, P, P, P, ctypes.c_int]
         L.or_render.restype = ctypes.c_uint64
         L.or_render.argtypes = [P, ctypes.POINTER(OrJob), P, P, P]
+        L.or_camera_rays.argtypes = [ctypes.POINTER(OrJob), ctypes.c_uint64, P, P, P]
         L.or_mesh_octree.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), P]
         L.or_sample.restype = ctypes.c_uint64
         L.or_sample.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, P]
@@ -115,10 +140,8 @@ class OracleScene:
 
     def render(self, integrator, width, height, cam, samples=1, aa=1, aa_mode=0, rdepth=100, seed=1, job_key=0,
                threads=0, tile_stride=1, stats=False):
-        """cam: dict(position, target, up, fov, aperture, flength)."""
-        j = OrJob(integrator, width, height, samples, aa, aa_mode, rdepth, seed, job_key, 32, tile_stride, threads,
-                  (ctypes.c_double * 9)(*cam["position"], *cam["target"], *cam["up"]), cam["fov"],
-                  cam.get("aperture", 0.0), cam.get("flength", 0.0))
+        """cam: a camera dict as _job takes it."""
+        j = _job(integrator, width, height, cam, samples, aa, aa_mode, rdepth, seed, job_key, threads, tile_stride)
         img = np.zeros((height, width, 4), np.float32)
         mean = np.zeros((height, width, 3), np.float64) if stats else None
         m2 = np.zeros((height, width, 3), np.float64) if stats else None
@@ -176,6 +199,7 @@ def ref():
         L.xr_render.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_uint, P]
         L.xr_save_image.argtypes = [ctypes.c_int, ctypes.c_char_p, P, ctypes.c_int, ctypes.c_int]
+        L.xr_camera_rays.argtypes = [P, ctypes.c_uint64, P, ctypes.c_float, ctypes.c_float, P, P]
         _ref = L
     return _ref
 
@@ -220,6 +244,15 @@ class RefScene:
         normal = np.zeros((n, 3), np.float64)
         ref().xr_trace(self._h, n, _p(rays), _p(t), _p(obj), _p(point), _p(normal))
         return t, obj, point, normal
+
+    def camera_rays(self, width, height, xy):
+        """Primary rays of the scene's first camera at sample coordinates xy (n x 2 float32)."""
+        xy = np.ascontiguousarray(xy, np.float32)
+        o = np.zeros((len(xy), 3))
+        d = np.zeros((len(xy), 3))
+        if ref().xr_camera_rays(self._h, len(xy), _p(xy), width, height, _p(o), _p(d)):
+            raise RuntimeError("scene has no camera")
+        return o, d
 
     def render(self, integrator, width, height, samples=1, aa=1, rdepth=100, threads=1, seed=1):
         img = np.zeros((height, width, 4), np.float32)

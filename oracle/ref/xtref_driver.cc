@@ -258,6 +258,25 @@ int xr_render_tiles(void *h, int integrator, int width, int height, int samples,
     return 0;
 }
 
+// Primary rays of the scene's first camera (the one xr_render uses) at n
+// sample coordinates xy (float pairs), through ICamera::get_primary_ray.
+int xr_camera_rays(void *h, uint64_t n, const float *xy, float width, float height, double *o, double *d) {
+    render::context_t *ctx = (render::context_t *)h;
+    if (ctx->scene.m_cameras.empty()) return 1;
+    asset::ICamera *cam = ctx->scene.get_camera(ctx->scene.m_cameras.begin()->first);
+    if (!cam) return 1;
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r = cam->get_primary_ray(xy[2 * i], xy[2 * i + 1], width, height);
+        o[3 * i] = r.origin.x;
+        o[3 * i + 1] = r.origin.y;
+        o[3 * i + 2] = r.origin.z;
+        d[3 * i] = r.direction.x;
+        d[3 * i + 1] = r.direction.y;
+        d[3 * i + 2] = r.direction.z;
+    }
+    return 0;
+}
+
 // Writes rgba (W*H*4 floats) through the reference's own image writers
 // (lib/nimg/img.cc): 0 png, 1 bmp, 2 tga, 3 hdr, 4 exr.
 int xr_save_image(int fmt, const char *path, const float *rgba, int w, int h) {

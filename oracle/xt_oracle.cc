@@ -414,17 +414,91 @@ static C3 environment(const Scene &s, V dir) {
 
 // ---------------------------------------------------------------- camera
 struct Cam {
+    uint32_t kind = 0;  // 0 thin-lens, 1 ods, 2 erp, 3 cubemap (parseutil.cc:262-275)
     V pos, target, up;
     float fov, aperture, flength;
     V rx, ry, rz;
+    V orientation;      // ods / erp
+    float ipd = 0.f;    // ods
+    double rot[3][3];   // Matrix4x4f m; m.rotate(orientation)
 };
+typedef double M3[3][3];
+static void mat3_mul(const M3 &a, const M3 &b, M3 &r) {  // matrix.cc:57-66
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j];
+}
 static void cam_setup(Cam &c) {  // perspective.cc:26-36
     c.rz = nrm(c.target - c.pos);
     c.rx = nrm(cross(c.up, c.rz));
     c.ry = nrm(cross(c.rx, c.rz));
+    // Matrix4x4f::rotate(euler), matrix.cc:485-492: identity *= Rx*Ry*Rz
+    // (the 4x4 product of matrix.cc:414-427, its zero terms included)
+    const V &e = c.orientation;
+    const M3 xr = {{1, 0, 0}, {0, std::cos(e.x), -std::sin(e.x)}, {0, std::sin(e.x), std::cos(e.x)}};
+    const M3 yr = {{std::cos(e.y), 0, std::sin(e.y)}, {0, 1, 0}, {-std::sin(e.y), 0, std::cos(e.y)}};
+    const M3 zr = {{std::cos(e.z), -std::sin(e.z), 0}, {std::sin(e.z), std::cos(e.z), 0}, {0, 0, 1}};
+    M3 xy, r;
+    mat3_mul(xr, yr, xy);
+    mat3_mul(xy, zr, r);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            c.rot[i][j] = (i == 0 ? 1.0 : 0.0) * r[0][j] + (i == 1 ? 1.0 : 0.0) * r[1][j] +
+                          (i == 2 ? 1.0 : 0.0) * r[2][j] + 0.0 * 0.0;
 }
+static V rotated(const double m[3][3], V v) {  // Vector3f::transformed, vector.inl:517-523
+    return mk(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z + 0.0, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z + 0.0,
+              m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z + 0.0);
+}
+static const double PI_D = 3.14159265358979323846;  // nmath::PI, precision.h:101
+
+// erp.cc:11-25
+static void primary_erp(const Cam &c, float x, float y, float width, float height, V &o, V &d) {
+    float phi = (x / width) * PI_D * 2.f;
+    float theta = (y / height) * PI_D;
+    o = c.pos;
+    d = mk(std::sin((double)theta) * std::cos((double)phi), std::cos((double)theta),
+           std::sin((double)theta) * std::sin((double)phi));
+    d = rotated(c.rot, d);
+}
+
+// ods.cc:15-33
+static void primary_ods(const Cam &c, float x, float y, float width, float height, V &o, V &d) {
+    float h = height * .5f;
+    float scale = c.ipd * .5f * ((y < h) ? -1 : 1);
+    if (y > h) y = y - h;
+    float theta = (x / width) * 2.f * PI_D;
+    float phi = PI_D * .5f - (y / h) * PI_D;
+    V u = mk(std::cos((double)theta), 0, std::sin((double)theta));
+    o = mk(u.x * (double)scale, u.y * (double)scale, u.z * (double)scale) + c.pos;
+    d = mk(std::sin((double)theta) * std::cos((double)phi), std::sin((double)phi),
+           -std::cos((double)theta) * std::cos((double)phi));
+    d = rotated(c.rot, d);
+}
+
+// cubemap.cc:17-60
+static void primary_cubemap(const Cam &c, float x, float y, float width, float height, V &o, V &d) {
+    size_t face_size = (size_t)(height) / 6;
+    size_t face = (size_t)(y) / face_size;
+    V up, camdir;
+    switch (face) {
+        case 0: camdir = mk(1, 0, 0); up = mk(0, 1, 0); break;
+        case 1: camdir = mk(-1, 0, 0); up = mk(0, 1, 0); break;
+        case 2: camdir = mk(0, 1, 0); up = mk(0, 0, -1); break;
+        case 3: camdir = mk(0, -1, 0); up = mk(0, 0, 1); break;
+        case 4: camdir = mk(0, 0, 1); up = mk(0, 1, 0); break;
+        default: camdir = mk(0, 0, -1); up = mk(0, 1, 0); break;
+    }
+    y -= (float)(face * face_size);
+    o = c.pos;
+    V v = mk((2.f * (double)x / (double)width) - 1.f, (2.f * (double)y / (double)face_size) - 1.f, 1.f);
+    v = nrm(v);
+    V rz = camdir, rx = nrm(cross(up, rz)), ry = nrm(cross(rx, rz));
+    const double m[3][3] = {{rx.x, ry.x, rz.x}, {rx.y, ry.y, rz.y}, {rx.z, ry.z, rz.z}};
+    d = nrm(rotated(m, v));
+}
+
 // perspective.cc:38-98
-static void primary(const Cam &c, float x, float y, float w, float h, uint32_t key, V &o, V &d) {
+static void primary_persp(const Cam &c, float x, float y, float w, float h, uint32_t key, V &o, V &d) {
     double aspect = (double)w / (double)h;
     o = c.pos;
     V dir = mk((2.0 * (double)x / (double)w) - 1.0, ((2.0 * (double)y / (double)h) - 1.0) / aspect,
@@ -446,6 +520,16 @@ static void primary(const Cam &c, float x, float y, float w, float h, uint32_t k
     };
     d = nrm(xf(dir));
     if (c.flength > 0) o = xf(org) + c.pos;
+}
+
+// ICamera::get_primary_ray of the camera's kind.
+static void primary(const Cam &c, float x, float y, float w, float h, uint32_t key, V &o, V &d) {
+    switch (c.kind) {
+        case 1: primary_ods(c, x, y, w, h, o, d); break;
+        case 2: primary_erp(c, x, y, w, h, o, d); break;
+        case 3: primary_cubemap(c, x, y, w, h, o, d); break;
+        default: primary_persp(c, x, y, w, h, key, o, d); break;
+    }
 }
 
 struct Job {
@@ -475,6 +559,13 @@ static C3 eval(const Ctx &cx, size_t depth, C3 in_i, V o, V d, uint32_t key, uin
     bool cont = true;
     if (m.type == 3) {
         f = C3{m.ke[0], m.ke[1], m.ke[2]};
+        cont = false;
+    } else if (m.type == 4) {
+        // dielectric.cc:24-32 sets only the ray: the intensity keeps
+        // ColorRGBf's default (0, 0, 0), so value = in.I * 0 and the product
+        // with the refracted ray's (finite) value stays 0; the recursion's
+        // queries change nothing and are not made here.
+        f = C3{0.f, 0.f, 0.f};
         cont = false;
     } else if (m.type == 0) {  // lambert.cc:30-40
         no = h.point + h.normal * 1e-8;
@@ -652,6 +743,9 @@ struct or_job {
     int threads;
     double cam[9];        // position, target, up
     float fov, aperture, flength;
+    uint32_t cam_kind;    // 0 thin-lens, 1 ods, 2 erp, 3 cubemap
+    double orientation[3];
+    float ipd;
 };
 
 // Renders with the reference's per-sample float read-modify-write (tile.cc,
@@ -659,8 +753,7 @@ struct or_job {
 // mean/m2 are given (W*H*3 doubles each) the per-pixel sample mean and sum of
 // squared deviations (Welford, FP64) of the per-sample values are returned
 // too. Returns the number of executed scene queries.
-uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double *m2) {
-    Scene *s = (Scene *)h;
+static Cam cam_of(const or_job *jb) {
     Cam cam;
     cam.pos = mk(jb->cam[0], jb->cam[1], jb->cam[2]);
     cam.target = mk(jb->cam[3], jb->cam[4], jb->cam[5]);
@@ -668,7 +761,32 @@ uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double 
     cam.fov = jb->fov;
     cam.aperture = jb->aperture;
     cam.flength = jb->flength;
+    cam.kind = jb->cam_kind;
+    cam.orientation = mk(jb->orientation[0], jb->orientation[1], jb->orientation[2]);
+    cam.ipd = jb->ipd;
     cam_setup(cam);
+    return cam;
+}
+
+// Primary rays of the job's camera at n sample coordinates xy (float pairs);
+// a thin lens draws from the stream of key 0.
+void or_camera_rays(const or_job *jb, uint64_t n, const float *xy, double *o, double *d) {
+    const Cam cam = cam_of(jb);
+    for (uint64_t i = 0; i < n; ++i) {
+        V a, b;
+        primary(cam, xy[2 * i], xy[2 * i + 1], (float)jb->width, (float)jb->height, 0u, a, b);
+        o[3 * i] = a.x;
+        o[3 * i + 1] = a.y;
+        o[3 * i + 2] = a.z;
+        d[3 * i] = b.x;
+        d[3 * i + 1] = b.y;
+        d[3 * i + 2] = b.z;
+    }
+}
+
+uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double *m2) {
+    Scene *s = (Scene *)h;
+    const Cam cam = cam_of(jb);
     const uint32_t W = jb->width, H = jb->height, T = jb->tile_size ? jb->tile_size : 32;
     const uint32_t tx = W / T + (W % T ? 1 : 0), ty = H / T + (H % T ? 1 : 0);
     const uint32_t ntiles = tx * ty, stride = jb->tile_stride ? jb->tile_stride : 1;

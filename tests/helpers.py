@@ -78,3 +78,51 @@ def glow_scene(path):
               camera=dict(position=(0.3, 1.4, -4.5), target=(0.0, 0.6, 0.5), fov=50.0),
               env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
     return path
+
+
+CAMERAS = {
+    "pinhole": dict(position=(0.3, 1.4, -6.5), target=(0.0, 0.6, 0.5), fov=50.0),
+    "erp": dict(type="erp", position=(0.1, 0.9, -1.6), orientation=(0.3, -0.7, 1.1)),
+    "ods": dict(type="ods", position=(0.1, 0.9, -1.6), orientation=(-0.2, 0.4, 0.05), ipd=0.25),
+    "cubemap": dict(type="cubemap", position=(0.1, 0.9, -1.6)),
+}
+
+
+def camera_dict_of(kind):
+    """CAMERAS[kind] in the form the oracle takes, float32-rounded like the loader."""
+    from xtracer_amd.scenes import camera_dict
+    if kind == "pinhole":
+        return camera_dict(**CAMERAS[kind])
+    cam = dict(CAMERAS[kind])
+    for k in ("position", "orientation"):
+        if k in cam:
+            cam[k] = [float(np.float32(c)) for c in cam[k]]
+    if "ipd" in cam:
+        cam["ipd"] = float(np.float32(cam["ipd"]))
+    return cam
+
+
+def camera_scene(path, kind):
+    """The glow scene's objects and materials, plus a dielectric sphere, seen
+    by camera CAMERAS[kind]: a pinhole, or the all-around erp, ods or cubemap
+    from inside the scene."""
+    from tests import scn
+    scn.write(path,
+              geometry={"floor": dict(type="plane", normal=(0, 1, 0), distance=0.0),
+                        "a": dict(type="sphere", position=(-1.2, 0.6, 0.0), radius=0.6),
+                        "b": dict(type="sphere", position=(0.2, 0.5, 0.4), radius=0.5),
+                        "c": dict(type="sphere", position=(1.4, 0.4, -0.3), radius=0.4),
+                        "glass": dict(type="sphere", position=(-0.4, 0.8, -3.0), radius=0.7),
+                        "tri": dict(type="triangle", v=[(-2.0, 0.0, 1.5), (2.0, 0.0, 1.5), (0.0, 2.2, 1.8)])},
+              objects={"floor": ("floor", "ground"), "a": ("a", "lamp"), "b": ("b", "glowing"),
+                       "c": ("c", "matte"), "glass": ("glass", "glass"), "tri": ("tri", "shiny")},
+              materials={"ground": dict(type="lambert", kd=(0.5, 0.5, 0.5)),
+                         "lamp": dict(type="emissive", ke=(4.0, 3.5, 2.0)),
+                         "glowing": dict(type="lambert", kd=(0.2, 0.6, 0.2), ke=(0.3, 1.25, 0.5)),
+                         "matte": dict(type="lambert", kd=(0.7, 0.2, 0.2)),
+                         "glass": dict(type="dielectric", ior=1.5),
+                         "shiny": dict(type="phong", kd=(0.3, 0.3, 0.6), ks=(0.5, 0.5, 0.5), reflectance=0.5,
+                                       exponent=20.0)},
+              camera=CAMERAS[kind],
+              env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
+    return path

@@ -7,7 +7,8 @@ from xtracer_amd.scenes import col3, f32, vec3
 def write(path, geometry, objects, materials, camera=None, env=None):
     """geometry: {name: dict(type=..., **props)}; objects: {name: (geometry, material)};
     materials: {name: dict(type=..., kd=, ks=, ke=, reflectance=, exponent=)};
-    camera: dict(position, target, fov, aperture, flength); env: None | ("color", c) | ("gradient", a, b)."""
+    camera: dict(position, target, fov, aperture, flength), or dict(type=ods|erp|cubemap,
+    position, orientation, ipd); env: None | ("color", c) | ("gradient", a, b)."""
     out = ["title = test"]
     if env is not None:
         if env[0] == "color":
@@ -16,10 +17,19 @@ def write(path, geometry, objects, materials, camera=None, env=None):
             out.append("environment = {\n\ttype = gradient\n\tconfig = {\n\t\ta = %s\n\t\tb = %s\n\t}\n}"
                        % (col3(env[1]), col3(env[2])))
     cam = camera or dict(position=(0, 0, -5), target=(0, 0, 0), fov=45.0)
-    out.append("camera = {\n\tdefault = {\n\t\ttype = thin-lens\n\t\tfov = %s\n\t\taperture = %s\n\t\tflength = %s\n"
-               "\t\tposition = %s\n\t\ttarget = %s\n\t\tup = vec3(0,1,0)\n\t}\n}"
-               % (f32(cam.get("fov", 45.0)), f32(cam.get("aperture", 0.0)), f32(cam.get("flength", 0.0)),
-                  vec3(cam["position"]), vec3(cam["target"])))
+    kind = cam.get("type", "thin-lens")
+    if kind == "thin-lens":
+        out.append("camera = {\n\tdefault = {\n\t\ttype = thin-lens\n\t\tfov = %s\n\t\taperture = %s\n\t\tflength = %s\n"
+                   "\t\tposition = %s\n\t\ttarget = %s\n\t\tup = vec3(0,1,0)\n\t}\n}"
+                   % (f32(cam.get("fov", 45.0)), f32(cam.get("aperture", 0.0)), f32(cam.get("flength", 0.0)),
+                      vec3(cam["position"]), vec3(cam["target"])))
+    else:  # ods / erp / cubemap (parseutil.cc:222-258)
+        c = [f"\t\ttype = {kind}", f"\t\tposition = {vec3(cam['position'])}"]
+        if "orientation" in cam:
+            c.append(f"\t\torientation = {vec3(cam['orientation'])}")
+        if "ipd" in cam:
+            c.append(f"\t\tipd = {f32(cam['ipd'])}")
+        out.append("camera = {\n\tdefault = {\n" + "\n".join(c) + "\n\t}\n}")
     g = ["geometry = {"]
     for name, p in geometry.items():
         g.append(f"\t{name} = {{\n\t\ttype = {p['type']}")
@@ -48,7 +58,7 @@ def write(path, geometry, objects, materials, camera=None, env=None):
             if key in p:
                 m.append(f"\t\t\t\t{sname} = {{\n\t\t\t\t\ttype = color\n\t\t\t\t\tvalue = {col3(p[key])}\n\t\t\t\t}}")
         m.append("\t\t\t}")
-        sc = [(k, p[k]) for k in ("reflectance", "exponent") if k in p]
+        sc = [(k, p[k]) for k in ("reflectance", "exponent", "ior") if k in p]
         if sc:
             m.append("\t\t\tscalars = {")
             m += [f"\t\t\t\t{k} = {f32(v)}" for k, v in sc]

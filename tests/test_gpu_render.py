@@ -105,3 +105,64 @@ def test_xtrender_exports(tmp_path):
     png = read_png((tmp_path / "out" / "spheres_stencil.png").read_bytes())
     assert np.array_equal(png, to8(raw))
     assert raw[..., 3].max() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", [xa.STENCIL, xa.DEPTH, xa.NORMAL, xa.EMISSION])
+@pytest.mark.parametrize("kind", ["erp", "ods", "cubemap"])
+@pytest.mark.parametrize("accel", [xa.ACCEL_KDTREE, xa.ACCEL_OCTREE])
+def test_camera_kinds(device, kind, integrator, accel, tmp_path):
+    """ODS / ERP / cubemap primary rays, evaluated in FP64 on the device with
+    the reference's float/double mix. The cubemap has no transcendental
+    functions: its images are bit-exact against the oracle. ERP and ODS take
+    double sines and cosines, which the device's math library and the host's
+    may round differently in the last bit: pixels must agree to 1e-5
+    relative, and all but 0.2% of them bit for bit."""
+    from tests.helpers import camera_dict_of, camera_scene
+    s = xa.Scene(camera_scene(str(tmp_path / "cam.scn"), kind))
+    sid = device.upload(s)
+    W, H = 96, 72
+    job = xa.make_job(sid, integrator, W, H, s.camera(), samples=4, seed=5, accel=accel)
+    img, st = device.render(job)
+    ref, nq = OracleScene(s).render(integrator, W, H, camera_dict_of(kind), samples=4, seed=5)
+    assert st.queries == nq
+    exact = (img.view(np.uint32) == ref.view(np.uint32)).all(axis=2)
+    close = np.isclose(img, ref, rtol=1e-5, atol=1e-6).all(axis=2)
+    if kind == "cubemap":
+        assert exact.all(), f"{(~exact).sum()} pixels differ"
+    else:
+        assert close.all(), f"{(~close).sum()} pixels differ by more than 1e-5"
+        assert exact.mean() >= 0.998, f"only {exact.mean():.4f} of pixels bit-exact"
+    assert (ref[..., 3] > 0).sum() > 1000
+    device.release(sid)
+
+
+@pytest.mark.gpu
+def test_cubemap_height_refused(device, tmp_path):
+    from tests.helpers import camera_scene
+    s = xa.Scene(camera_scene(str(tmp_path / "cam.scn"), "cubemap"))
+    sid = device.upload(s)
+    with pytest.raises(xa.XtError) as e:
+        device.render(xa.make_job(sid, xa.STENCIL, 32, 40, s.camera()))
+    assert e.value.code == 1
+    device.release(sid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["pinhole", "erp"])
+def test_dielectric_same_stream(device, kind, tmp_path):
+    """A dielectric sphere in view ends its paths with 0, as the reference's
+    zero intensity does; with the same seed device and oracle agree on all
+    but the FP32-decided pixels, and on the query count."""
+    from tests.helpers import camera_dict_of, camera_scene
+    s = xa.Scene(camera_scene(str(tmp_path / "cam.scn"), kind))
+    sid = device.upload(s)
+    W, H, spp = 64, 48, 8
+    job = xa.make_job(sid, xa.PATHTRACER, W, H, s.camera(), samples=spp, rdepth=6, seed=11,
+                      accum=xa.ACCUM_REFERENCE)
+    img, st = device.render(job)
+    ref, nq = OracleScene(s).render(xa.PATHTRACER, W, H, camera_dict_of(kind), samples=spp, rdepth=6, seed=11)
+    close = np.isclose(img[..., :3], ref[..., :3], rtol=1e-3, atol=1e-4).all(axis=2)
+    assert close.mean() >= 0.98, f"only {close.mean():.4f} of pixels agree"
+    assert abs(st.queries - nq) <= 0.01 * nq
+    device.release(sid)

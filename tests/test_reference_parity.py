@@ -159,3 +159,32 @@ def test_oracle_estimators_match_reference_statistically(integrator, kind, kw, r
                               rd=rdepth)
     out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
     assert out["inside"] >= 0.97, out
+
+
+@needs_ref
+@pytest.mark.parametrize("integrator", [0, 1, 4, 6])
+@pytest.mark.parametrize("kind", ["erp", "ods", "cubemap"])
+def test_camera_kinds_deterministic_renders_bit_exact(kind, integrator, tmp_path):
+    """ODS / ERP / cubemap cameras through stencil, depth, normal and
+    emission: the oracle's image equals the reference's bit for bit."""
+    from tests.helpers import camera_dict_of, camera_scene
+    path = camera_scene(str(tmp_path / "cam.scn"), kind)
+    code = _RENDER_CHILD.format(root=ROOT, path=path, cam=json.dumps(camera_dict_of(kind)), integ=integrator,
+                                w=48, h=36, spp=4)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=300).decode().splitlines()[-1])
+    assert out["bad"] == 0, out
+    assert out["nonzero"] > 100
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["pinhole", "erp"])
+def test_dielectric_pathtracer_statistically(kind, tmp_path):
+    """A scene with a dielectric sphere in view: the reference's dielectric
+    leaves the path's intensity at 0 (dielectric.cc:24-32), the oracle ends
+    the path with 0; per-pixel means agree within the 99% MC bound."""
+    from tests.helpers import camera_dict_of, camera_scene
+    path = camera_scene(str(tmp_path / "cam.scn"), kind)
+    code = _STAT_CHILD.format(root=ROOT, path=path, cam=json.dumps(camera_dict_of(kind)), integ=3, w=24, h=16,
+                              spp=256, rd=6)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
+    assert out["inside"] >= 0.97, out

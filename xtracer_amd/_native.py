@@ -25,6 +25,7 @@ HIT_NONE = 0xFFFFFFFF
 HIT_PLANE_BASE = 0xFFFF0000
 TUNING_AUTO = 0xFFFFFFFF  # xt_tuning.trace_blocks_per_cu: resident fill
 SERVE_FAIR, SERVE_SRPT = 0, 1
+CAMERA_THIN_LENS, CAMERA_ODS, CAMERA_ERP, CAMERA_CUBEMAP = 0, 1, 2, 3
 
 # Exported symbols, in the order include/xtracer_amd.h declares them.
 SYMBOLS = [
@@ -65,7 +66,7 @@ class ObjectInfo(ctypes.Structure):
 class Camera(ctypes.Structure):
     _fields_ = [("position", ctypes.c_double * 3), ("basis", ctypes.c_double * 9), ("dir_z", ctypes.c_double),
                 ("aperture", ctypes.c_double), ("flength", ctypes.c_double), ("thin_lens", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32)]
+                ("kind", ctypes.c_uint32), ("ipd", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):

@@ -177,6 +177,36 @@ int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint
 
 // Perspective::calculate_transform (perspective.cc:26-36) and the projection
 // constant 1 / tan(fov * RADIAN / 2) (perspective.cc:53), in FP64.
+}  // extern "C"
+
+namespace {
+// Matrix4x4f m; m.rotate(euler) as ODS / ERP build it (matrix.cc:485-492,
+// 414-427): Rx*Ry*Rz from double cos/sin, the 3x3 products summed in the
+// reference's order, then identity *= R, whose sums turn -0 entries into +0.
+void euler_rotation(const dvec3 &e, double out[9]) {
+    typedef double M3[3][3];
+    const M3 xr = {{1, 0, 0}, {0, std::cos(e.x), -std::sin(e.x)}, {0, std::sin(e.x), std::cos(e.x)}};
+    const M3 yr = {{std::cos(e.y), 0, std::sin(e.y)}, {0, 1, 0}, {-std::sin(e.y), 0, std::cos(e.y)}};
+    const M3 zr = {{std::cos(e.z), -std::sin(e.z), 0}, {std::sin(e.z), std::cos(e.z), 0}, {0, 0, 1}};
+    auto mul = [](const M3 &a, const M3 &b, M3 &r) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) r[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j];
+    };
+    M3 xy, r;
+    mul(xr, yr, xy);
+    mul(xy, zr, r);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {  // row i of the identity times column j of R as a 4x4
+            double v = (i == 0 ? 1.0 : 0.0) * r[0][j];
+            v = v + (i == 1 ? 1.0 : 0.0) * r[1][j];
+            v = v + (i == 2 ? 1.0 : 0.0) * r[2][j];
+            out[3 * i + j] = v + 0.0;  // the fourth term, 0 * 0
+        }
+}
+}  // namespace
+
+extern "C" {
+
 int xt_camera_make(const double position[3], const double target[3], const double up[3], float fov_deg,
                    float aperture, float flength, xt_camera *out) {
     if (!position || !target || !up || !out) return XT_E_INVALID;
@@ -209,9 +239,17 @@ int xt_scene_camera(const xt_scene *s, const char *camera_name, xt_camera *out) 
     }
     if (!cam) return XT_E_NOTFOUND;
     const double p[3] = {cam->position.x, cam->position.y, cam->position.z};
-    const double t[3] = {cam->target.x, cam->target.y, cam->target.z};
-    const double u[3] = {cam->up.x, cam->up.y, cam->up.z};
-    return xt_camera_make(p, t, u, cam->fov, cam->aperture, cam->flength, out);
+    if (cam->kind == XT_CAMERA_THIN_LENS) {
+        const double t[3] = {cam->target.x, cam->target.y, cam->target.z};
+        const double u[3] = {cam->up.x, cam->up.y, cam->up.z};
+        return xt_camera_make(p, t, u, cam->fov, cam->aperture, cam->flength, out);
+    }
+    std::memset(out, 0, sizeof *out);
+    out->kind = cam->kind;
+    for (int i = 0; i < 3; ++i) out->position[i] = p[i];
+    out->ipd = cam->ipd;
+    if (cam->kind == XT_CAMERA_ODS || cam->kind == XT_CAMERA_ERP) euler_rotation(cam->orientation, out->basis);
+    return XT_OK;
 }
 
 

@@ -135,8 +135,8 @@ static __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n) {
 #pragma clang fp contract(off)
 // Perspective::get_primary_ray (src/xtcore/camera/perspective.cc:38-98) in
 // FP64; the lens draws come from query stream 0 of the path.
-__device__ __forceinline__ void camera_ray(const JobDev &j, float x, float y, uint32_t key, double o[3],
-                                           double d[3]) {
+__device__ __forceinline__ void perspective_ray(const JobDev &j, float x, float y, uint32_t key, double o[3],
+                                                double d[3]) {
     const xt_camera &c = j.cam;
     const double aspect = (double)j.wf / (double)j.hf;
     double dir[3] = {(2.0 * (double)x / (double)j.wf) - 1.0, ((2.0 * (double)y / (double)j.hf) - 1.0) / aspect,
@@ -188,6 +188,121 @@ __device__ __forceinline__ void camera_ray(const JobDev &j, float x, float y, ui
         o[0] = org[0];
         o[1] = org[1];
         o[2] = org[2];
+    }
+}
+
+#define XT_PI_D 3.14159265358979323846  // nmath::PI (lib/nmath/precision.h:101), a double
+
+// v transformed by the orientation matrix m (Vector3f::transformed,
+// lib/nmath/vector.inl:517-523; the translation column is 0).
+__device__ __forceinline__ void orient(const double *m, double v[3]) {
+    const double a = m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + 0.0;
+    const double b = m[3] * v[0] + m[4] * v[1] + m[5] * v[2] + 0.0;
+    const double c = m[6] * v[0] + m[7] * v[1] + m[8] * v[2] + 0.0;
+    v[0] = a;
+    v[1] = b;
+    v[2] = c;
+}
+
+// ERP::get_primary_ray (src/xtcore/camera/erp.cc:11-25): the angles are
+// floats, their sines and cosines doubles; the direction is not normalised.
+__device__ __forceinline__ void erp_ray(const JobDev &j, float x, float y, double o[3], double d[3]) {
+    const xt_camera &c = j.cam;
+    const float phi = (float)((double)(x / j.wf) * XT_PI_D * 2.0);
+    const float theta = (float)((double)(y / j.hf) * XT_PI_D);
+    const double st = sin((double)theta);
+    d[0] = st * cos((double)phi);
+    d[1] = cos((double)theta);
+    d[2] = st * sin((double)phi);
+    orient(c.basis, d);
+    o[0] = c.position[0];
+    o[1] = c.position[1];
+    o[2] = c.position[2];
+}
+
+// ODS::get_primary_ray (src/xtcore/camera/ods.cc:15-33): the top half of the
+// image is the left eye (origin offset -ipd/2 along the tangent), the
+// bottom half the right eye; the direction is not normalised.
+__device__ __forceinline__ void ods_ray(const JobDev &j, float x, float y, double o[3], double d[3]) {
+    const xt_camera &c = j.cam;
+    const float h = j.hf * .5f;
+    const float scale = (float)c.ipd * .5f * ((y < h) ? -1.f : 1.f);
+    if (y > h) y = y - h;
+    const float theta = (float)((double)((x / j.wf) * 2.f) * XT_PI_D);
+    const float phi = (float)(XT_PI_D * (double).5f - (double)(y / h) * XT_PI_D);
+    const double ct = cos((double)theta), s_t = sin((double)theta);
+    const double cp = cos((double)phi), sp = sin((double)phi);
+    o[0] = ct * (double)scale + c.position[0];
+    o[1] = 0.0 * (double)scale + c.position[1];
+    o[2] = s_t * (double)scale + c.position[2];
+    d[0] = s_t * cp;
+    d[1] = sp;
+    d[2] = -ct * cp;
+    orient(c.basis, d);
+}
+
+// Cubemap::get_primary_ray (src/xtcore/camera/cubemap.cc:17-60): a vertical
+// strip of six square faces +x -x +y -y +z -z, each face_size = H / 6 rows.
+__device__ __forceinline__ void cubemap_ray(const JobDev &j, float x, float y, double o[3], double d[3]) {
+    const xt_camera &c = j.cam;
+    const uint64_t face_size = (uint64_t)j.hf / 6u;
+    const uint32_t face = (uint32_t)((uint64_t)y / face_size);
+    double cd[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+    switch (face) {
+        case 0: cd[0] = 1; break;
+        case 1: cd[0] = -1; break;
+        case 2: cd[1] = 1; up[1] = 0; up[2] = -1; break;
+        case 3: cd[1] = -1; up[1] = 0; up[2] = 1; break;
+        case 4: cd[2] = 1; break;
+        default: cd[2] = -1; break;
+    }
+    y -= (float)(face * face_size);
+    double v[3] = {(2.f * (double)x / (double)j.wf) - 1.f, (2.f * (double)y / (double)face_size) - 1.f, 1.f};
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l != 0.0) {
+        v[0] /= l;
+        v[1] /= l;
+        v[2] /= l;
+    }
+    // rz = camdir, rx = normalize(up x rz), ry = normalize(rx x rz)
+    double rx[3] = {up[1] * cd[2] - up[2] * cd[1], up[2] * cd[0] - up[0] * cd[2], up[0] * cd[1] - up[1] * cd[0]};
+    l = sqrt(rx[0] * rx[0] + rx[1] * rx[1] + rx[2] * rx[2]);
+    if (l != 0.0) {
+        rx[0] /= l;
+        rx[1] /= l;
+        rx[2] /= l;
+    }
+    double ry[3] = {rx[1] * cd[2] - rx[2] * cd[1], rx[2] * cd[0] - rx[0] * cd[2], rx[0] * cd[1] - rx[1] * cd[0]};
+    l = sqrt(ry[0] * ry[0] + ry[1] * ry[1] + ry[2] * ry[2]);
+    if (l != 0.0) {
+        ry[0] /= l;
+        ry[1] /= l;
+        ry[2] /= l;
+    }
+    const double m[9] = {rx[0], ry[0], cd[0], rx[1], ry[1], cd[1], rx[2], ry[2], cd[2]};
+    orient(m, v);
+    l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l != 0.0) {
+        v[0] /= l;
+        v[1] /= l;
+        v[2] /= l;
+    }
+    d[0] = v[0];
+    d[1] = v[1];
+    d[2] = v[2];
+    o[0] = c.position[0];
+    o[1] = c.position[1];
+    o[2] = c.position[2];
+}
+
+// ICamera::get_primary_ray of the job's camera kind.
+__device__ __forceinline__ void camera_ray(const JobDev &j, float x, float y, uint32_t key, double o[3],
+                                           double d[3]) {
+    switch (j.cam.kind) {
+        case XT_CAMERA_ODS: ods_ray(j, x, y, o, d); break;
+        case XT_CAMERA_ERP: erp_ray(j, x, y, o, d); break;
+        case XT_CAMERA_CUBEMAP: cubemap_ray(j, x, y, o, d); break;
+        default: perspective_ray(j, x, y, key, o, d); break;
     }
 }
 
@@ -532,7 +647,10 @@ __device__ __forceinline__ void push_ray(Queue &out, uint32_t slot, f3 o, f3 d, 
 // state: T (product so far) and F (factor of the last hit); L = T * X where
 // X is the value of the current query: miss -> env, emissive -> F * Le,
 // continue -> F * f * X'. Materials: lambert.cc:30-40, phong.cc:35-55,
-// blinnphong.cc:34-61, emissive.cc:20-27.
+// blinnphong.cc:34-61, emissive.cc:20-27. A dielectric (dielectric.cc:24-32)
+// leaves its hit_result's intensity at ColorRGBf's default 0, so the value
+// of the query is 0 whatever the refracted ray returns: the path ends with
+// L = 0 (the reference traces on and multiplies by 0).
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, Queue out,
                                                        uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad) {
@@ -567,6 +685,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                 if (m.type == XT_MAT_EMISSIVE) {
                     const f3 L = cmul(cmul(T, F), mk(m.ke[0], m.ke[1], m.ke[2]));
                     rad[path] = make_float4(L.x, L.y, L.z, 0.f);
+                } else if (m.type == XT_MAT_DIELECTRIC) {
+                    rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
                 } else if (q < rdepth) {
                     Draws r{xt_stream(key, q), 0};
                     f3 f, dir;

@@ -101,6 +101,7 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
         x.type = m.type == MAT_LAMBERT ? XT_MAT_LAMBERT
                  : m.type == MAT_PHONG ? XT_MAT_PHONG
                  : m.type == MAT_BLINNPHONG ? XT_MAT_BLINNPHONG
+                 : m.type == MAT_DIELECTRIC ? XT_MAT_DIELECTRIC
                  : XT_MAT_EMISSIVE;
         std::memcpy(x.kd, m.diffuse, sizeof x.kd);
         std::memcpy(x.ks, m.specular, sizeof x.ks);

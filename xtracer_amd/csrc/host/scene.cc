@@ -79,21 +79,33 @@ bool parse_bool(const char *v) {
     return s == "1" || s == "yes" || s == "true";
 }
 
+// deserialize_camera and its four kinds (parseutil.cc:205-275): every
+// property read with its default (vec3 0, number 0) when absent.
 int load_camera(Scene &sc, ncf::Node *p) {
     const char *type = p->property("type");
-    if (!type || std::strcmp(type, "thin-lens") != 0) {
+    const std::string t = type ? type : "";
+    Camera cam;
+    if (t == "thin-lens") cam.kind = 0;
+    else if (t == "ods") cam.kind = 1;
+    else if (t == "erp") cam.kind = 2;
+    else if (t == "cubemap") cam.kind = 3;
+    else {
         sc.error = "unsupported camera type for " + p->name();
         return 1;
     }
-    Camera cam;
     cam.name = p->name();
     cam.handle = sc.intern(p->name());
     cam.position = vec3_of(p, "position");
-    cam.target = vec3_of(p, "target");
-    cam.up = vec3_of(p, "up");
-    cam.fov = (float)numf(p->property("fov"));
-    cam.flength = (float)numf(p->property("flength"));
-    cam.aperture = (float)numf(p->property("aperture"));
+    if (cam.kind == 0) {
+        cam.target = vec3_of(p, "target");
+        cam.up = vec3_of(p, "up");
+        cam.fov = (float)numf(p->property("fov"));
+        cam.flength = (float)numf(p->property("flength"));
+        cam.aperture = (float)numf(p->property("aperture"));
+    } else if (cam.kind == 1 || cam.kind == 2) {
+        cam.orientation = vec3_of(p, "orientation");
+        if (cam.kind == 1) cam.ipd = (float)numf(p->property("ipd"));
+    }
     sc.cameras[cam.handle] = cam;
     return 0;
 }
@@ -249,7 +261,6 @@ int load_material(Scene &sc, ncf::Node *p) {
         else if (nm == "ior") m.ior = v;
         else if (nm == "emissive") m.is_emissive = true;
     }
-    if (m.type == MAT_DIELECTRIC) m.unsupported.push_back("dielectric");
     sc.materials[m.handle] = m;
     return 0;
 }

@@ -22,8 +22,11 @@ enum LoadStatus { LOAD_OK = 0, LOAD_FAIL = 1, LOAD_PARSE = 2 };
 struct Camera {
     std::string name;
     uint64_t handle = 0;
+    uint32_t kind = 0;  // XT_CAMERA_*: thin-lens, ods, erp, cubemap
     dvec3 position, target{0, 0, 1}, up{0, 1, 0};
     float fov = 0.f, aperture = 0.f, flength = 0.f;  // float members, perspective.h:22-26
+    dvec3 orientation;                               // ods / erp euler angles (ods.h:18-19, erp.h:17)
+    float ipd = 0.f;                                 // ods
 };
 
 enum GeomType : uint32_t { GEOM_PLANE = 0, GEOM_SPHERE = 1, GEOM_TRIANGLE = 2, GEOM_MESH = 3 };

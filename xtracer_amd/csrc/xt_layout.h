@@ -51,6 +51,7 @@ struct XtPlane {
 #define XT_MAT_PHONG 1u
 #define XT_MAT_BLINNPHONG 2u
 #define XT_MAT_EMISSIVE 3u
+#define XT_MAT_DIELECTRIC 4u
 
 struct XtMaterial {
     uint32_t type;

@@ -51,6 +51,15 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
         d->err = "rdepth above 65535";
         return XT_E_INVALID;
     }
+    if (job->camera.kind > XT_CAMERA_CUBEMAP) {
+        d->err = "unknown camera kind";
+        return XT_E_INVALID;
+    }
+    if (job->camera.kind == XT_CAMERA_CUBEMAP && job->height % 6u != 0) {
+        // rows past the sixth face have no face in the reference (cubemap.cc:26-36)
+        d->err = "cubemap camera: the image height must be a multiple of 6";
+        return XT_E_INVALID;
+    }
     if (job->integrator == XT_INTEGRATOR_UV) {
         d->err = "uv integrator: the scene loader keeps no texture coordinates";
         return XT_E_SCENE;
