@@ -402,7 +402,7 @@ template <int ACCEL, int STACK, int BS>
 static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
                                                      float4 *hit, uint32_t *retest_list, uint4 *recheck_list,
                                                      Counters *cnt) {
-    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? STACK * BS : 1];
+    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? (STACK + 1) * BS : 1];
     __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * BS : 1];
     const uint32_t n = *in_count;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->queries, (unsigned long long)n);
@@ -542,7 +542,7 @@ static __global__ __launch_bounds__(ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT
     const SceneView S, const Queue in, float4 *hit, const uint32_t *list, Counters *cnt, const double *ray64,
     uint32_t *flags) {
     constexpr int BS = ACCEL == XT_ACCEL_KDTREE ? XT_BLOCK : XT_OCT_BLOCK;
-    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? XT_KD_STACK * XT_BLOCK : 1];
+    __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? (XT_KD_STACK + 1) * XT_BLOCK : 1];
     __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * XT_OCT_BLOCK : 1];
     const uint32_t n = cnt->n[CNT_RETEST];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->retests, (unsigned long long)n);

@@ -148,7 +148,10 @@ struct Tester64 {
 
 // kd-tree traversal with a short stack of (node, t_far) entries in LDS and a
 // restart from the root when entries were dropped. `stk` points at this
-// lane's first entry; entries are `stride` uint2 apart.
+// lane's first entry; entries are `stride` uint2 apart, STACK + 1 of them:
+// the ring holds at most STACK entries, so the slot after the head is always
+// free. Each descent step writes the far child there and keeps it only when
+// both children are visited, which leaves the step without branches.
 template <int STACK, class T>
 __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, const f3 d, uint2 *stk, int stride,
                                         float eps) {
@@ -157,6 +160,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
     if (!slab(o, inv, S.kb_lo, S.kb_hi, t_root0, t_root1, eps)) return true;
     float tmin = t_root0, tmax = t_root1;
     uint32_t node = 0;
+    constexpr int SLOTS = STACK + 1;
     int head = -1, count = 0;
     bool dropped = false;
     const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z}, ia[3] = {inv.x, inv.y, inv.z};
@@ -174,18 +178,17 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             const uint32_t first = below ? c : c + 1, second = below ? c + 1 : c;
             // tp <= 0: the plane is behind the origin, or the origin lies on
             // it and `below` already picked the side the ray moves into
-            if (tp > tmax * (1.f + eps) || tp <= 0.f) {
-                node = first;
-            } else if (tp < tmin * (1.f - eps)) {
-                node = second;
-            } else {
-                head = head + 1 == STACK ? 0 : head + 1;
-                stk[head * stride] = make_uint2(second, __float_as_uint(tmax));
-                if (count < STACK) ++count;
-                else dropped = true;
-                node = first;
-                tmax = fminf(tmax, tp * (1.f + eps));
-            }
+            const bool only_first = tp > tmax * (1.f + eps) || tp <= 0.f;
+            const bool only_second = !only_first && tp < tmin * (1.f - eps);
+            const bool both = !only_first && !only_second;
+            const int next = head + 1 == SLOTS ? 0 : head + 1;
+            stk[next * stride] = make_uint2(second, __float_as_uint(tmax));
+            // a push onto a full ring drops its oldest entry
+            dropped = dropped || (both && count == STACK);
+            count = both && count < STACK ? count + 1 : count;
+            head = both ? next : head;
+            tmax = both ? fminf(tmax, tp * (1.f + eps)) : tmax;
+            node = only_second ? second : first;
             nd = S.kd[node];
             axis = nd.word & 3u;
         }
@@ -204,7 +207,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             continue;
         }
         const uint2 e = stk[head * stride];
-        head = head == 0 ? STACK - 1 : head - 1;
+        head = head == 0 ? SLOTS - 1 : head - 1;
         --count;
         node = e.x;
         tmin = tmax;
