@@ -36,8 +36,8 @@ extern "C" {
 #define XT_ACCEL_OCTREE 1
 
 /* integrators, src/xtcore/integrator/{stencil,depth,ao,pathtracer,normal,uv,
- * emission,wireframe}; uv is refused with XT_E_SCENE (the loader keeps no
- * texture coordinates) */
+ * emission,wireframe}; normal and uv always accumulate as the reference does
+ * (XT_ACCUM_REFERENCE): their per-sample updates are not sums */
 #define XT_INTEGRATOR_STENCIL 0
 #define XT_INTEGRATOR_DEPTH 1
 #define XT_INTEGRATOR_AO 2
@@ -114,6 +114,13 @@ int xt_scene_get_object_params(const xt_scene *s, uint32_t index, double out[9])
 /* Per-vertex normals of the soup, 9 floats per prim (zeros where absent);
  * XT_E_NOTFOUND when no triangle carries vertex normals. */
 int xt_scene_get_vnormals(const xt_scene *s, float *out);
+/* Texture coordinates, for the uv integrator: prim_tc (6 doubles per soup
+ * primitive: u, v of each triangle corner, zeros for spheres) as the
+ * reference's Triangle::tc holds them (explicit triangles: parseutil.cc:395-397;
+ * mesh corners: mesh.cc:56-58, Triangle() defaults for corners without one,
+ * triangle.cc:12-17), and uv_scale (2 doubles per object: the geometry's
+ * u_scale, v_scale, parseutil.cc:410-413). Either pointer may be NULL. */
+int xt_scene_get_texcoords(const xt_scene *s, double *prim_tc, double *uv_scale);
 
 /* The kd-tree as uploaded: n_kd_nodes * 2 words (XtKdNode), n_kd_refs
  * primitive references and the root box (lo xyz, hi xyz). */

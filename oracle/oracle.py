@@ -19,7 +19,8 @@ _ref = None
 class SceneDesc(ctypes.Structure):
     P = ctypes.c_void_p
     _fields_ = [("n_objects", ctypes.c_uint32), ("obj_type", P), ("obj_material", P), ("obj_param", P),
-                ("mesh_first", P), ("mesh_count", P), ("mesh_v", P), ("mesh_n", P), ("n_materials", ctypes.c_uint32),
+                ("mesh_first", P), ("mesh_count", P), ("mesh_v", P), ("mesh_n", P), ("obj_tc", P), ("mesh_tc", P),
+                ("obj_uv_scale", P), ("n_materials", ctypes.c_uint32),
                 ("mat_type", P), ("mat_param", P), ("env_type", ctypes.c_uint32), ("env", P)]
 
 
@@ -93,16 +94,22 @@ class OracleScene:
         self.obj_param = np.zeros((n, 9), np.float64)
         self.mesh_first = np.zeros(n, np.uint32)
         self.mesh_count = np.zeros(n, np.uint32)
-        tris, norms = [], []
+        tris, norms, tcs = [], [], []
+        prim_tc, uv_scale = scene.texcoords()
+        self.obj_tc = np.zeros((n, 6), np.float64)
+        self.obj_uv_scale = np.ascontiguousarray(uv_scale, np.float64)
         base = 0
         # soup index -> (object, OBJ-order index) map, for comparing hits
         self.soup_to_local = arr["prim_local"].astype(np.int64)
         for k, o in enumerate(objs):
             self.obj_param[k] = scene.object_params(k)
+            if o["geom_type"] == 2:
+                self.obj_tc[k] = prim_tc[o["first_prim"]]
             if o["geom_type"] == 3:
                 lo, cnt = o["first_prim"], o["prim_count"]
                 order = np.argsort(arr["prim_local"][lo:lo + cnt], kind="stable")
                 tris.append(arr["prims64"][lo:lo + cnt][order])
+                tcs.append(prim_tc[lo:lo + cnt][order])
                 if vn is not None:
                     norms.append(vn[lo:lo + cnt][order])
                 self.mesh_first[k] = base
@@ -110,6 +117,7 @@ class OracleScene:
                 base += cnt
         self.mesh_v = np.ascontiguousarray(np.concatenate(tris) if tris else np.zeros((1, 9)), np.float64)
         self.mesh_n = (np.ascontiguousarray(np.concatenate(norms), np.float32) if (vn is not None and norms) else None)
+        self.mesh_tc = np.ascontiguousarray(np.concatenate(tcs) if tcs else np.zeros((1, 6)), np.float64)
         mats = arr["materials"]
         self.mat_type = np.ascontiguousarray(mats["type"], np.uint32)
         mp = np.zeros((max(1, len(mats)), 11), np.float32)
@@ -122,7 +130,8 @@ class OracleScene:
         self.mat_param = mp
         self.env = np.ascontiguousarray(arr["env"][1:7], np.float32)
         d = SceneDesc(n, _p(self.obj_type), _p(self.obj_material), _p(self.obj_param), _p(self.mesh_first),
-                      _p(self.mesh_count), _p(self.mesh_v), _p(self.mesh_n), len(mats), _p(self.mat_type),
+                      _p(self.mesh_count), _p(self.mesh_v), _p(self.mesh_n), _p(self.obj_tc), _p(self.mesh_tc),
+                      _p(self.obj_uv_scale), len(mats), _p(self.mat_type),
                       _p(self.mat_param), int(arr["env"][0]), _p(self.env))
         self._h = lib().or_build(ctypes.byref(d))
         self.objects = objs

@@ -76,6 +76,7 @@ struct Draws {
 struct Tri {
     V v[3];
     V n[3];
+    double tc[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // Vector2f tc[3] (triangle.h:29)
 };
 struct Box {
     V lo, hi;
@@ -99,6 +100,7 @@ struct Obj {
     double sr;
     Tri tri;
     int mesh;
+    double us = 0, vs = 0;  // uv_scale (surface.h:26)
 };
 struct Mat {
     uint32_t type;  // 0 lambert, 1 phong, 2 blinn-phong, 3 emissive
@@ -141,6 +143,7 @@ static bool box_hit(const Box &b, V o, V d) {
 struct Hit {
     double t = INFINITY;
     V point, normal;
+    double tu = 0, tv = 0;  // texcoord (hitrecord.h:33; z stays 0)
     int obj = -1;
     long prim = -1;  // triangle index inside its mesh (OBJ face order)
 };
@@ -170,6 +173,9 @@ static bool tri_hit(const Tri &tr, V o, V d, Hit *h) {
     if (h) {
         h->t = t;
         h->point = pos;
+        // triangle.cc:113: tc[0] * bc.x + tc[1] * bc.y + tc[2] * bc.z
+        h->tu = tr.tc[0][0] * b0 + tr.tc[1][0] * b1 + tr.tc[2][0] * b2;
+        h->tv = tr.tc[0][1] * b0 + tr.tc[1][1] * b1 + tr.tc[2][1] * b2;
         V pn = tr.n[0] * b0 + tr.n[1] * b1 + tr.n[2] * b2;
         h->normal = len(pn) ? pn : normal;
     }
@@ -177,7 +183,7 @@ static bool tri_hit(const Tri &tr, V o, V d, Hit *h) {
 }
 
 // Sphere::intersection (sphere.cc:35-61).
-static bool sph_hit(V c, double r, V o, V d, Hit *h) {
+static bool sph_hit(V c, double r, V o, V d, Hit *h, double us = 0, double vs = 0) {
     double b = 2 * dot(o - c, d);
     double cq = dot(c, c) + dot(o, o) + 2 * dot(-c, o) - r * r;
     double disc = (b * b - 4 * cq);
@@ -189,6 +195,9 @@ static bool sph_hit(V c, double r, V o, V d, Hit *h) {
             h->t = t;
             h->point = o + d * t;
             h->normal = (h->point - c) / r * (t1 * t2 > 0. ? 1. : -1);
+            // sphere.cc:53-54
+            h->tu = std::asin(h->normal.x / (us != 0.0f ? us : 1.0f)) / 3.14159265358979323846 + 0.5;
+            h->tv = std::asin(h->normal.y / (vs != 0.0f ? vs : 1.0f)) / 3.14159265358979323846 + 0.5;
             return true;
         }
     }
@@ -196,7 +205,7 @@ static bool sph_hit(V c, double r, V o, V d, Hit *h) {
 }
 
 // Plane::intersection (plane.cc:19-61).
-static bool plane_hit(V n, double off, V o, V d, Hit *h) {
+static bool plane_hit(V n, double off, V o, V d, Hit *h, double us = 0, double vs = 0) {
     double ndd = dot(n, d);
     if (std::fabs(ndd) < 1e-8) return false;
     V v = mk(std::fabs(n.x), std::fabs(n.y), std::fabs(n.z)) * off;
@@ -207,6 +216,20 @@ static bool plane_hit(V n, double off, V o, V d, Hit *h) {
         h->t = t;
         h->point = o + d * t;
         h->normal = n;
+        // plane.cc:43-57: the point projected on two tangents, scaled, wrapped
+        V nn = nrm(n);
+        V uvec = mk(nn.y, nn.z, -nn.x);
+        V vvec = cross(uvec, nn);
+        double tu = dot(uvec, (v + h->point)) * us;
+        double tv = dot(vvec, (v + h->point)) * vs;
+        if (tu > 1.f) tu -= (float)(int)tu;
+        if (tv > 1.f) tv -= (float)(int)tv;
+        if (tu < -1.f) tu -= (float)(int)tu;
+        if (tv < -1.f) tv -= (float)(int)tv;
+        if (tu < 0.f) tu = 1.f + tu;
+        if (tv < 0.f) tv = 1.f + tv;
+        h->tu = tu;
+        h->tv = tv;
     }
     return true;
 }
@@ -317,8 +340,8 @@ static bool scene_hit(const Scene &s, V o, V d, Hit &res) {
         bool ok = false;
         long prim = -1;
         switch (ob.type) {
-            case 0: ok = plane_hit(ob.pn, ob.poff, o, d, &h); break;
-            case 1: ok = sph_hit(ob.sc, ob.sr, o, d, &h); break;
+            case 0: ok = plane_hit(ob.pn, ob.poff, o, d, &h, ob.us, ob.vs); break;
+            case 1: ok = sph_hit(ob.sc, ob.sr, o, d, &h, ob.us, ob.vs); break;
             case 2: ok = tri_hit(ob.tri, o, d, &h); break;
             case 3:
                 prim = oct_walk(s.meshes[ob.mesh], 0, o, d, &h);
@@ -605,6 +628,9 @@ struct or_scene_desc {
     const uint32_t *mesh_count;
     const double *mesh_v;        // 9 doubles per triangle, OBJ face order
     const float *mesh_n;         // 9 floats per triangle or NULL
+    const double *obj_tc;        // 6 doubles per object (triangles: u, v per corner) or NULL
+    const double *mesh_tc;       // 6 doubles per mesh triangle or NULL
+    const double *obj_uv_scale;  // 2 doubles per object or NULL
     uint32_t n_materials;
     const uint32_t *mat_type;
     const float *mat_param;      // 11 floats: kd[3], ks[3], ke[3], reflectance, exponent
@@ -646,6 +672,8 @@ void *or_build(const or_scene_desc *in) {
             o.sr = p[3];
         } else if (o.type == 2) {
             for (int k = 0; k < 3; ++k) o.tri.v[k] = mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+            if (in->obj_tc)
+                for (int k = 0; k < 6; ++k) o.tri.tc[k / 2][k % 2] = in->obj_tc[6 * (size_t)i + k];
         } else if (o.type == 3) {
             Mesh m;
             for (uint32_t t = 0; t < in->mesh_count[i]; ++t) {
@@ -655,12 +683,20 @@ void *or_build(const or_scene_desc *in) {
                     tr.v[k] = mk(in->mesh_v[9 * g + 3 * k], in->mesh_v[9 * g + 3 * k + 1], in->mesh_v[9 * g + 3 * k + 2]);
                     if (in->mesh_n)
                         tr.n[k] = mk(in->mesh_n[9 * g + 3 * k], in->mesh_n[9 * g + 3 * k + 1], in->mesh_n[9 * g + 3 * k + 2]);
+                    if (in->mesh_tc) {
+                        tr.tc[k][0] = in->mesh_tc[6 * g + 2 * k];
+                        tr.tc[k][1] = in->mesh_tc[6 * g + 2 * k + 1];
+                    }
                 }
                 m.tris.push_back(tr);
             }
             build_mesh(m);
             o.mesh = (int)s->meshes.size();
             s->meshes.push_back(std::move(m));
+        }
+        if (in->obj_uv_scale) {
+            o.us = in->obj_uv_scale[2 * (size_t)i];
+            o.vs = in->obj_uv_scale[2 * (size_t)i + 1];
         }
         s->objs.push_back(o);
     }
@@ -895,6 +931,26 @@ uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double 
                             px[1] = fg < 0.f ? 0.f : fg;
                             px[2] = fb < 0.f ? 0.f : fb;
                             px[3] = 1.f;
+                        } else if (jb->integrator == 5) {  // uv/integrator.cc:11-44
+                            Hit hh;
+                            ++nq;
+                            V acc = mk((double)px[0], (double)px[1], (double)px[2]);
+                            const double alpha = (double)px[3];
+                            double alpha_sample = 0.f;
+                            if (scene_hit(*s, o, d, hh)) {
+                                // (texcoord * 0.5f + 0.5f) * weight, texcoord.z = 0
+                                acc = acc + mk(hh.tu * 0.5f + 0.5f, hh.tv * 0.5f + 0.5f, 0.0 * 0.5f + 0.5f) * weight;
+                                alpha_sample += weight;
+                                val[0] = hh.tu;
+                                val[1] = hh.tv;
+                            }
+                            // ColorRGBAf(acc, 0.) then a(alpha + weight * alpha_sample)
+                            const float fr = (float)acc.x, fg = (float)acc.y, fb = (float)acc.z;
+                            px[0] = fr < 0.f ? 0.f : fr;
+                            px[1] = fg < 0.f ? 0.f : fg;
+                            px[2] = fb < 0.f ? 0.f : fb;
+                            const float na = (float)(alpha + weight * alpha_sample);
+                            px[3] = na < 0.f ? 0.f : (na > 1.f ? 1.f : na);
                         } else if (jb->integrator == 6) {  // emission/integrator.cc:11-40
                             Hit hh;
                             ++nq;

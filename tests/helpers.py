@@ -126,3 +126,33 @@ def camera_scene(path, kind):
               camera=CAMERAS[kind],
               env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
     return path
+
+
+def uv_scene(path, camera="pinhole"):
+    """Texture-coordinate sources in view: spheres with and without u/v
+    scales (one seen from inside by the all-around cameras), a scaled floor
+    plane, a triangle with corner texcoords, and a mesh whose faces have vt
+    indices, or none (Triangle() defaults)."""
+    import os
+    from tests import scn
+    d = os.path.dirname(str(path))
+    rng = np.random.default_rng(17)
+    tris = []
+    for k in range(12):
+        c = np.array([-2.5 + 0.45 * k, 0.4 + 0.1 * (k % 3), 2.2])
+        tris.append([c + [-0.2, -0.2, 0], c + [0.2, -0.2, 0.05], c + [0.0, 0.25, -0.05]])
+    uvs = [None if k % 4 == 0 else [tuple(rng.uniform(-1.5, 2.5, 2)) for _ in range(3)] for k in range(12)]
+    obj = scn.write_obj_uv(os.path.join(d, "uvmesh.obj"), tris, uvs)
+    scn.write(path,
+              geometry={"floor": dict(type="plane", normal=(0, 1, 0), distance=0.0, u_scale=0.7, v_scale=1.3),
+                        "a": dict(type="sphere", position=(-1.2, 0.6, 0.0), radius=0.6, u_scale=1.5, v_scale=0.8),
+                        "b": dict(type="sphere", position=(0.2, 0.5, 0.4), radius=0.5),
+                        "tri": dict(type="triangle", v=[(-2.0, 0.0, 1.5), (2.0, 0.0, 1.5), (0.0, 2.2, 1.8)],
+                                    tc=[(0.1, 0.9), (0.8, 0.2), (0.45, -0.3)]),
+                        "m": dict(type="mesh", source=os.path.basename(obj))},
+              objects={"floor": ("floor", "ground"), "a": ("a", "ground"), "b": ("b", "ground"),
+                       "tri": ("tri", "ground"), "m": ("m", "ground")},
+              materials={"ground": dict(type="lambert", kd=(0.5, 0.5, 0.5))},
+              camera=CAMERAS[camera],
+              env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
+    return path

@@ -39,10 +39,19 @@ def write(path, geometry, objects, materials, camera=None, env=None):
             g.append(f"\t\tposition = {vec3(p['position'])}")
             if p["type"] == "sphere":
                 g.append(f"\t\tradius = {p['radius']!r}")
+        elif p["type"] == "triangle" and "tc" in p:  # corners as groups with u, v (parseutil.cc:385-397)
+            g.append("\t\tvecdata = {")
+            for k, (v, t) in enumerate(zip(p["v"], p["tc"])):
+                g.append("\t\t\tv%d = {\n\t\t\t\tx = %r\n\t\t\t\ty = %r\n\t\t\t\tz = %r\n\t\t\t\tu = %r\n\t\t\t\tv = %r\n\t\t\t}"
+                         % (k, float(v[0]), float(v[1]), float(v[2]), float(t[0]), float(t[1])))
+            g.append("\t\t}")
         elif p["type"] == "triangle":
             g.append("\t\tvecdata = {\n\t\t\tv0 = %s\n\t\t\tv1 = %s\n\t\t\tv2 = %s\n\t\t}" % tuple(vec3(v) for v in p["v"]))
         elif p["type"] == "mesh":
             g.append(f"\t\tsource = {p['source']}")
+        for k in ("u_scale", "v_scale"):
+            if k in p:
+                g.append(f"\t\t{k} = {p[k]!r}")
         g.append("\t}")
     g.append("}")
     out.append("\n".join(g))
@@ -79,4 +88,26 @@ def write_obj(path, tris):
                 f.write("v %.9g %.9g %.9g\n" % tuple(np.float64(v)))
         for k in range(len(tris)):
             f.write("f %d %d %d\n" % (3 * k + 1, 3 * k + 2, 3 * k + 3))
+    return str(path)
+
+
+def write_obj_uv(path, tris, uvs):
+    """OBJ with texture coordinates: uvs[k] is None (corners without vt) or
+    three (u, v) pairs for triangle k."""
+    tris = np.asarray(tris, np.float32).reshape(-1, 3, 3)
+    with open(path, "w") as f:
+        for t in tris:
+            for v in t:
+                f.write("v %.9g %.9g %.9g\n" % tuple(np.float64(v)))
+        nt = 0
+        for uv in uvs:
+            if uv is not None:
+                for u, v in uv:
+                    f.write("vt %.9g %.9g\n" % (u, v))
+        for k, uv in enumerate(uvs):
+            if uv is None:
+                f.write("f %d %d %d\n" % (3 * k + 1, 3 * k + 2, 3 * k + 3))
+            else:
+                f.write("f %d/%d %d/%d %d/%d\n" % (3 * k + 1, nt + 1, 3 * k + 2, nt + 2, 3 * k + 3, nt + 3))
+                nt += 3
     return str(path)

@@ -188,3 +188,19 @@ def test_dielectric_pathtracer_statistically(kind, tmp_path):
                               spp=256, rd=6)
     out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
     assert out["inside"] >= 0.97, out
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["pinhole", "erp"])
+def test_uv_renders_bit_exact(kind, tmp_path):
+    """The uv integrator (uv/integrator.cc:11-44) with texture coordinates
+    from spheres (asin mapping, u/v scale), a scaled plane (projection and
+    wrap), a triangle's corner texcoords and a mesh's vt indices (and
+    Triangle() defaults for faces without): oracle == reference bit for bit."""
+    from tests.helpers import camera_dict_of, uv_scene
+    path = uv_scene(str(tmp_path / "uv.scn"), kind)
+    code = _RENDER_CHILD.format(root=ROOT, path=path, cam=json.dumps(camera_dict_of(kind)), integ=5, w=64, h=48,
+                                spp=4)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=300).decode().splitlines()[-1])
+    assert out["bad"] == 0, out
+    assert out["nonzero"] > 500

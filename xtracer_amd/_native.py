@@ -30,7 +30,7 @@ CAMERA_THIN_LENS, CAMERA_ODS, CAMERA_ERP, CAMERA_CUBEMAP = 0, 1, 2, 3
 # Exported symbols, in the order include/xtracer_amd.h declares them.
 SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
-    "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals",
+    "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals", "xt_scene_get_texcoords",
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
     "xt_image_write", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
@@ -143,6 +143,7 @@ def lib():
         "xt_scene_get_arrays": (I, [P, P, P, P, P, P, P]),
         "xt_scene_get_object_params": (I, [P, U32, P]),
         "xt_scene_get_vnormals": (I, [P, P]),
+        "xt_scene_get_texcoords": (I, [P, P, P]),
         "xt_scene_get_kdtree": (I, [P, P, P, P]),
         "xt_scene_mesh_octree": (I, [P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64),
                                      ctypes.POINTER(U64)]),
@@ -256,6 +257,16 @@ class Scene:
         if rc:
             raise XtError(rc)
         return out
+
+    def texcoords(self):
+        """(prim_tc: (n_prims, 6) u, v per triangle corner; uv_scale: (n_objects, 2))."""
+        i = self.info()
+        tc = np.zeros((i.n_prims, 6), np.float64)
+        sc = np.zeros((len(self.objects()), 2), np.float64)
+        rc = lib().xt_scene_get_texcoords(self._h, _ptr(tc), _ptr(sc))
+        if rc:
+            raise XtError(rc)
+        return tc, sc
 
     def kdtree(self):
         i = self.info()

@@ -150,6 +150,15 @@ int xt_scene_get_vnormals(const xt_scene *s, float *out) {
     return XT_OK;
 }
 
+int xt_scene_get_texcoords(const xt_scene *s, double *prim_tc, double *uv_scale) {
+    if (!s || !s->built) return XT_E_INVALID;
+    const FlatScene &f = s->flat;
+    if (prim_tc && !f.tc64.empty()) std::memcpy(prim_tc, f.tc64.data(), f.tc64.size() * sizeof(double));
+    if (uv_scale && !f.obj_uv_scale.empty())
+        std::memcpy(uv_scale, f.obj_uv_scale.data(), f.obj_uv_scale.size() * sizeof(double));
+    return XT_OK;
+}
+
 int xt_scene_get_kdtree(const xt_scene *s, uint32_t *nodes, uint32_t *refs, float bounds[6]) {
     if (!s || !s->built) return XT_E_INVALID;
     const KdTree &k = s->flat.kd;

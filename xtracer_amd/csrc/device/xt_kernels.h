@@ -888,6 +888,106 @@ __device__ __forceinline__ float4 normal_update(float4 a, bool hit, d3 n) {
     }
     return make_float4(fmaxf((float)acc.x, 0.f), fmaxf((float)acc.y, 0.f), fmaxf((float)acc.z, 0.f), 1.f);
 }
+
+// Texture coordinates of the closest hit in FP64 (hitrecord.h:33), as the
+// reference's primitives set them: a triangle's corner texcoords weighted by
+// its barycentrics (triangle.cc:110-114), a sphere's asin mapping of the
+// normal (sphere.cc:53-54), a plane's projection on two tangents, scaled
+// and wrapped (plane.cc:43-57).
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_uv(const SceneView S, const Queue in,
+                                                       const uint32_t *in_count, const float4 *hit, double *ray64,
+                                                       float4 *rad) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const double PI = 3.14159265358979323846;  // nmath::PI
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = in.path[i];
+        const float4 h = hit[i];
+        const uint32_t prim = __float_as_uint(h.y);
+        if (prim == XT_NONE) {
+            rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        double *q = ray64 + 6 * (size_t)path;
+        const d3 o = {q[0], q[1], q[2]}, d = {q[3], q[4], q[5]};
+        double tu = 0.0, tv = 0.0;
+        if (prim >= XT_PLANE_BASE) {
+            const XtPlane &pl = S.planes[prim - XT_PLANE_BASE];
+            const double us = S.obj_uv[2 * pl.object], vs = S.obj_uv[2 * pl.object + 1];
+            double t = plane64(o, d, pl);
+            if (!(t > 0.0)) t = (double)h.x;
+            const d3 pt = dadd(o, dscl(d, t));
+            const d3 v = {pl.point64[0], pl.point64[1], pl.point64[2]};
+            const d3 nn = dnorm(d3{pl.normal64[0], pl.normal64[1], pl.normal64[2]});
+            const d3 uvec = {nn.y, nn.z, -nn.x};
+            const d3 vvec = dcross(uvec, nn);
+            const d3 vp = dadd(v, pt);
+            tu = ddot(uvec, vp) * us;
+            tv = ddot(vvec, vp) * vs;
+            if (tu > 1.f) tu -= (float)(int)tu;
+            if (tv > 1.f) tv -= (float)(int)tv;
+            if (tu < -1.f) tu -= (float)(int)tu;
+            if (tv < -1.f) tv -= (float)(int)tv;
+            if (tu < 0.f) tu = 1.f + tu;
+            if (tv < 0.f) tv = 1.f + tv;
+        } else {
+            const XtPrim64 &p = S.prims64[prim];
+            if (XT_PRIM_TYPE(__float_as_uint(S.prims[3 * prim].w)) == XT_PRIM_TRIANGLE) {
+                double bc[3];
+                const double t = tri64(o, d, p, bc);
+                if (!(t > 0.0)) {  // the FP64 predicate disagrees: FP32 barycentrics
+                    bc[1] = h.z;
+                    bc[2] = h.w;
+                    bc[0] = 1.0 - bc[1] - bc[2];
+                }
+                const double *tc = S.tc64 + 6 * (size_t)prim;
+                tu = tc[0] * bc[0] + tc[2] * bc[1] + tc[4] * bc[2];
+                tv = tc[1] * bc[0] + tc[3] * bc[1] + tc[5] * bc[2];
+            } else {
+                const uint32_t obj = __float_as_uint(S.prims[3 * prim + 1].w);
+                const double us = S.obj_uv[2 * obj], vs = S.obj_uv[2 * obj + 1];
+                const d3 c = {p.a[0], p.a[1], p.a[2]};
+                const double r = p.b[0];
+                const double b = 2 * ddot(dsub(o, c), d);
+                const d3 nc = {-c.x, -c.y, -c.z};
+                const double cq = ddot(c, c) + ddot(o, o) + 2 * ddot(nc, o) - r * r;
+                const double disc = (b * b - 4 * cq);
+                const double s = sqrt(disc > 0.0 ? disc : 0.0);
+                const double t1 = (-b + s) / 2.0, t2 = (-b - s) / 2.0;
+                double t = t1 < t2 ? t1 : t2;
+                if (!(disc > 0.0 && t > XT_EPS64)) t = (double)h.x;
+                const d3 pt = dadd(o, dscl(d, t));
+                const double sg = t1 * t2 > 0. ? 1. : -1;
+                const d3 nn = {(pt.x - c.x) / r * sg, (pt.y - c.y) / r * sg, (pt.z - c.z) / r * sg};
+                tu = asin(nn.x / (us != 0.0f ? us : 1.0f)) / PI + 0.5;
+                tv = asin(nn.y / (vs != 0.0f ? vs : 1.0f)) / PI + 0.5;
+            }
+        }
+        q[0] = tu;
+        q[1] = tv;
+        rad[path] = make_float4(0.f, 1.f, 0.f, 0.f);
+    }
+}
+
+// The uv integrator's per-sample update of the stored pixel
+// (uv/integrator.cc:24-40): acc = rgb; on a hit acc += (texcoord * 0.5 +
+// 0.5) * weight with texcoord.z = 0, and the alpha becomes
+// alpha + weight * (hit ? weight : 0); stored as ColorRGBAf(acc, 0) (negative
+// channels to 0) with the alpha setter's clamp to [0, 1].
+__device__ __forceinline__ float4 uv_update(float4 a, bool hit, double tu, double tv, double w) {
+    double ax = (double)a.x, ay = (double)a.y, az = (double)a.z;
+    const double alpha = (double)a.w;
+    double alpha_sample = 0.0;
+    if (hit) {
+        ax = ax + (tu * 0.5 + 0.5) * w;
+        ay = ay + (tv * 0.5 + 0.5) * w;
+        az = az + (0.0 * 0.5 + 0.5) * w;
+        alpha_sample += w;
+    }
+    const float fx = (float)ax, fy = (float)ay, fz = (float)az, fa = (float)(alpha + w * alpha_sample);
+    return make_float4(fx < 0.f ? 0.f : fx, fy < 0.f ? 0.f : fy, fz < 0.f ? 0.f : fz,
+                       fa < 0.f ? 0.f : (fa > 1.f ? 1.f : fa));
+}
 #pragma clang fp contract(on)
 
 // Emission (src/xtcore/integrator/emission/integrator.cc:26-35): the hit
@@ -951,6 +1051,9 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, 
                 if (j.integrator == XT_INTEGRATOR_NORMAL) {
                     const double *q = nrm64 + 6 * ((size_t)sl * qn + qi);
                     a = normal_update(a, v.y != 0.f, d3{q[0], q[1], q[2]});
+                } else if (j.integrator == XT_INTEGRATOR_UV) {
+                    const double *q = nrm64 + 6 * ((size_t)sl * qn + qi);
+                    a = uv_update(a, v.y != 0.f, q[0], q[1], j.weight);
                 } else if (j.integrator == XT_INTEGRATOR_EMISSION) {
                     // acc += emissive * float(weight) (ColorRGBf operators clamp at 0); alpha 1
                     if (v.w != 0.f) {

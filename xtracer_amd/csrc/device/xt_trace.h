@@ -22,6 +22,8 @@ struct SceneView {
     const uint32_t *obj_material;
     const XtMaterial *materials;
     const float *vnormals;  // 9 per prim or null
+    const double *tc64;     // 6 per prim: texcoords (u, v) of a triangle's corners
+    const double *obj_uv;   // 2 per object: u_scale, v_scale
     XtEnv env;
     float kb_lo[3], kb_hi[3];
     uint32_t n_prims, n_planes, n_oct;

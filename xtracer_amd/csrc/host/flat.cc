@@ -53,6 +53,7 @@ void tri_finish(XtPrim &p, const dvec3 v[3], bool has_vn) {
 struct MeshTri {
     dvec3 v[3];
     float n[9];
+    double tc[6];  // u, v per corner
     bool has_n;
 };
 
@@ -62,9 +63,14 @@ void mesh_tris(const MeshData &m, std::vector<MeshTri> &out) {
     for (size_t i = 0; i < nt; ++i) {
         MeshTri &t = out[i];
         t.has_n = false;
+        // a corner without a texture index keeps Triangle()'s texcoord
+        // (triangle.cc:12-17: (0, 0), (1, 0), (0, 0)); mesh.cc:56-58
+        static const double tc_default[6] = {0, 0, 1, 0, 0, 0};
         for (int k = 0; k < 3; ++k) {
-            int vi = m.iv[3 * i + k], ni = m.in[3 * i + k];
+            int vi = m.iv[3 * i + k], ni = m.in[3 * i + k], ti = m.it[3 * i + k];
             t.v[k] = vi >= 0 ? dvec3(m.v[3 * vi], m.v[3 * vi + 1], m.v[3 * vi + 2]) : dvec3(0, 0, 0);
+            t.tc[2 * k] = ti >= 0 ? (double)m.uv[2 * ti] : tc_default[2 * k];
+            t.tc[2 * k + 1] = ti >= 0 ? (double)m.uv[2 * ti + 1] : tc_default[2 * k + 1];
             if (ni >= 0) {
                 t.n[3 * k] = m.n[3 * ni];
                 t.n[3 * k + 1] = m.n[3 * ni + 1];
@@ -140,6 +146,7 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
     }
 
     // objects in scan order
+    out.obj_uv_scale.assign(2 * sc.objects.size(), 0.0);
     for (auto &kv : sc.objects) {
         const Object &o = kv.second;
         FlatObject fo;
@@ -158,6 +165,8 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
         const Geometry &g = gi->second;
         fo.geom_type = g.type;
         fo.first_prim = (uint32_t)out.prims.size();
+        out.obj_uv_scale[2 * oi] = g.u_scale;
+        out.obj_uv_scale[2 * oi + 1] = g.v_scale;
         if (g.type == GEOM_PLANE) {
             XtPlane p{};
             dvec3 pt = dvec3(std::fabs(g.normal.x), std::fabs(g.normal.y), std::fabs(g.normal.z)) * g.offset;
@@ -181,6 +190,7 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
             q.b[0] = g.radius;
             out.prims.push_back(p);
             out.prims64.push_back(q);
+            out.tc64.insert(out.tc64.end(), 6, 0.0);
             out.prim_local.push_back(0);
             fo.prim_count = 1;
         } else if (g.type == GEOM_TRIANGLE) {
@@ -196,6 +206,7 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
             set3d(q.c, g.v[2]);
             out.prims.push_back(p);
             out.prims64.push_back(q);
+            for (int k = 0; k < 3; ++k) out.tc64.insert(out.tc64.end(), {g.tc[k][0], g.tc[k][1]});
             out.prim_local.push_back(0);
             fo.prim_count = 1;
         } else {
@@ -215,6 +226,7 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
                 set3d(q.c, t.v[2]);
                 out.prims.push_back(p);
                 out.prims64.push_back(q);
+                out.tc64.insert(out.tc64.end(), t.tc, t.tc + 6);
                 out.prim_local.push_back(li);
             }
             fo.prim_count = (uint32_t)order.size();

@@ -27,6 +27,8 @@ struct FlatScene {
     std::vector<XtPrim> prims;        // soup in tie order
     std::vector<XtPrim64> prims64;
     std::vector<float> vnormals;      // 9 floats per prim when any triangle has vertex normals
+    std::vector<double> tc64;         // 6 per prim: a triangle's texcoords (u, v per corner), 0 for spheres
+    std::vector<double> obj_uv_scale; // 2 per object: the geometry's u_scale, v_scale
     std::vector<uint32_t> prim_local; // index of the primitive inside its geometry (OBJ face order)
     std::vector<XtPlane> planes;
     std::vector<XtMaterial> materials;

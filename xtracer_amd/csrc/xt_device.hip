@@ -261,6 +261,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     XtOctNode *oct = nullptr;
     XtMaterial *mats = nullptr;
     float *vn = nullptr;
+    double *tc64 = nullptr, *obj_uv = nullptr;
     std::vector<XtMaterial> mv = f.materials;
     if (mv.empty()) mv.push_back(XtMaterial{});
     int rc = upload(d, &prims, f.prims, t, b);
@@ -273,6 +274,12 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     rc = rc ? rc : upload(d, &octr, f.oct_refs, t, b);
     rc = rc ? rc : upload(d, &om, obj_mat, t, b);
     rc = rc ? rc : upload(d, &mats, mv, t, b);
+    if (!rc && (f.tc64.size() != 6 * f.prims.size() || f.obj_uv_scale.size() != 2 * f.objects.size())) {
+        d->err = "texture coordinate arrays do not match the primitives";
+        rc = XT_E_SCENE;
+    }
+    rc = rc ? rc : upload(d, &tc64, f.tc64, t, b);
+    rc = rc ? rc : upload(d, &obj_uv, f.obj_uv_scale, t, b);
     if (!rc && !f.vnormals.empty()) rc = upload(d, &vn, f.vnormals, t, b);
     if (rc) {
         for (void *p : ds.allocs) hipFree(p);
@@ -292,6 +299,8 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     v.obj_material = om;
     v.materials = mats;
     v.vnormals = vn;
+    v.tc64 = tc64;
+    v.obj_uv = obj_uv;
     v.env = f.env;
     for (int a = 0; a < 3; ++a) {
         v.kb_lo[a] = f.kd.bounds.lo[a];

@@ -60,10 +60,6 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
         d->err = "cubemap camera: the image height must be a multiple of 6";
         return XT_E_INVALID;
     }
-    if (job->integrator == XT_INTEGRATOR_UV) {
-        d->err = "uv integrator: the scene loader keeps no texture coordinates";
-        return XT_E_SCENE;
-    }
     if (job->integrator == XT_INTEGRATOR_EMISSION && !it->second.materials_supported) {
         d->err = "scene has material samplers the GPU path does not evaluate";
         return XT_E_SCENE;
@@ -106,8 +102,9 @@ int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_ou
     if (j.accum == XT_ACCUM_DEFAULT)
         j.accum = (j.integrator == XT_INTEGRATOR_AO || j.integrator == XT_INTEGRATOR_PATHTRACER) ? XT_ACCUM_FP64
                                                                                               : XT_ACCUM_REFERENCE;
-    // the normal integrator blends into the stored pixel; there is no sum
-    if (j.integrator == XT_INTEGRATOR_NORMAL) j.accum = XT_ACCUM_REFERENCE;
+    // the normal integrator blends into the stored pixel, and the uv
+    // integrator's alpha adds weight squared: neither is a plain sum
+    if (j.integrator == XT_INTEGRATOR_NORMAL || j.integrator == XT_INTEGRATOR_UV) j.accum = XT_ACCUM_REFERENCE;
     j.wf = (float)j.W;
     j.hf = (float)j.H;
     j.cam = job->camera;
@@ -148,6 +145,9 @@ int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t
             break;
         case XT_INTEGRATOR_NORMAL:
             hipLaunchKernelGGL(k_shade_normal, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64, w.rad);
+            break;
+        case XT_INTEGRATOR_UV:
+            hipLaunchKernelGGL(k_shade_uv, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64, w.rad);
             break;
         case XT_INTEGRATOR_EMISSION:
             hipLaunchKernelGGL(k_shade_emission, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.rad);
@@ -222,7 +222,8 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
             const hipStream_t st = stream_of(k);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, G.lane.cnt,
                                (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
-            const bool fp64_rays = (G.integrator == XT_INTEGRATOR_DEPTH || G.integrator == XT_INTEGRATOR_NORMAL) && r == 0;
+            const bool fp64_rays = (G.integrator == XT_INTEGRATOR_DEPTH || G.integrator == XT_INTEGRATOR_NORMAL ||
+                                    G.integrator == XT_INTEGRATOR_UV) && r == 0;
             rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
                                  time_trace && !n_st, st);
             if (rc) return rc;
@@ -263,7 +264,8 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         d->err = "pass too large";
         return XT_E_INVALID;
     }
-    const bool need64 = j.integrator == XT_INTEGRATOR_DEPTH || j.integrator == XT_INTEGRATOR_NORMAL;
+    const bool need64 = j.integrator == XT_INTEGRATOR_DEPTH || j.integrator == XT_INTEGRATOR_NORMAL ||
+                        j.integrator == XT_INTEGRATOR_UV;
     rc = xt_dev_ensure_wave(d, cap, need64);
     if (rc) return rc;
     rc = xt_dev_ensure_accum(d, (size_t)j.W * j.H);
