@@ -121,6 +121,18 @@ int xt_scene_get_vnormals(const xt_scene *s, float *out);
  * triangle.cc:12-17), and uv_scale (2 doubles per object: the geometry's
  * u_scale, v_scale, parseutil.cc:410-413). Either pointer may be NULL. */
 int xt_scene_get_texcoords(const xt_scene *s, double *prim_tc, double *uv_scale);
+/* Samplers and the textures they read, as io::scn::load builds them
+ * (parseutil.cc:147-203, 428-500; img.cc:17-66 for the texels): counts[0..2]
+ * receive the numbers of textures, texel floats and table samplers; then,
+ * where not NULL: tex_info (offset in texels / 4, width, height, 0 per
+ * texture), texels (4 floats per texel, row 0 at the top), samplers (14
+ * words each: kind (1 colour, 2 texture, 3 gradient, 4 cubemap, 5 erp),
+ * bilinear, 6 texture indices (cubemap faces left, right, top, bottom,
+ * front, back), then a[3], b[3] as floats), mat_smp (3 per material, the
+ * table sampler of diffuse / specular / emissive or 0xffffffff for a colour
+ * or absent one) and env_smp (the environment's table sampler). */
+int xt_scene_get_samplers(const xt_scene *s, uint32_t counts[3], uint32_t *tex_info, float *texels, uint32_t *samplers,
+                          uint32_t *mat_smp, uint32_t *env_smp);
 
 /* The kd-tree as uploaded: n_kd_nodes * 2 words (XtKdNode), n_kd_refs
  * primitive references and the root box (lo xyz, hi xyz). */
@@ -168,6 +180,15 @@ int xt_camera_make(const double position[3], const double target[3], const doubl
  * 140-150) call them. XT_E_INVALID for an unknown extension, XT_E_IO when the
  * file cannot be written. */
 int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t height);
+
+/* Reads a texture image as the scene loader does for `texture`, `cubemap`
+ * and `erp` samplers: RGBA floats (byte / 255, alpha 1 for RGB files), row 0
+ * at the top. Replaces nimg::io::load::image (lib/nimg/img.cc:17-66) for the
+ * files it reads as 8-bit RGB / RGBA: PNG (not interlaced), TGA (raw or
+ * run-length, 24 / 32 bit), BMP (uncompressed 24 / 32 bit), PPM (P6, maxval
+ * 255). With rgba NULL only *width / *height are set. XT_E_IO when the file
+ * cannot be read, XT_E_INVALID for a format or variant not decoded. */
+int xt_image_read(const char *path, float *rgba, uint32_t *width, uint32_t *height);
 
 /* ----------------------------------------------------------------- device */
 

@@ -21,7 +21,9 @@ class SceneDesc(ctypes.Structure):
     _fields_ = [("n_objects", ctypes.c_uint32), ("obj_type", P), ("obj_material", P), ("obj_param", P),
                 ("mesh_first", P), ("mesh_count", P), ("mesh_v", P), ("mesh_n", P), ("obj_tc", P), ("mesh_tc", P),
                 ("obj_uv_scale", P), ("n_materials", ctypes.c_uint32),
-                ("mat_type", P), ("mat_param", P), ("env_type", ctypes.c_uint32), ("env", P)]
+                ("mat_type", P), ("mat_param", P), ("env_type", ctypes.c_uint32), ("env", P),
+                ("n_textures", ctypes.c_uint32), ("tex_info", P), ("texels", P), ("n_samplers", ctypes.c_uint32),
+                ("samplers", P), ("mat_smp", P), ("env_smp", ctypes.c_uint32)]
 
 
 class OrJob(ctypes.Structure):
@@ -129,10 +131,17 @@ class OracleScene:
             mp[:, 10] = mats["exponent"]
         self.mat_param = mp
         self.env = np.ascontiguousarray(arr["env"][1:7], np.float32)
+        smp = scene.samplers()
+        self.tex_info = np.ascontiguousarray(smp["tex_info"])
+        self.texels = np.ascontiguousarray(smp["texels"])
+        self.samplers = np.ascontiguousarray(smp["samplers"])
+        self.mat_smp = np.ascontiguousarray(smp["mat_smp"]) if len(mats) else None
         d = SceneDesc(n, _p(self.obj_type), _p(self.obj_material), _p(self.obj_param), _p(self.mesh_first),
                       _p(self.mesh_count), _p(self.mesh_v), _p(self.mesh_n), _p(self.obj_tc), _p(self.mesh_tc),
                       _p(self.obj_uv_scale), len(mats), _p(self.mat_type),
-                      _p(self.mat_param), int(arr["env"][0]), _p(self.env))
+                      _p(self.mat_param), int(arr["env"][0]), _p(self.env), len(self.tex_info),
+                      _p(self.tex_info), _p(self.texels), len(self.samplers), _p(self.samplers), _p(self.mat_smp),
+                      smp["env_smp"])
         self._h = lib().or_build(ctypes.byref(d))
         self.objects = objs
 
@@ -209,6 +218,7 @@ def ref():
                                 ctypes.c_int, ctypes.c_int, ctypes.c_uint, P]
         L.xr_save_image.argtypes = [ctypes.c_int, ctypes.c_char_p, P, ctypes.c_int, ctypes.c_int]
         L.xr_camera_rays.argtypes = [P, ctypes.c_uint64, P, ctypes.c_float, ctypes.c_float, P, P]
+        L.xr_load_image.argtypes = [ctypes.c_char_p, P, P, P]
         _ref = L
     return _ref
 
@@ -223,6 +233,20 @@ def ref_save_image(fmt, path, rgba):
     rc = ref().xr_save_image(REF_IMAGE_FORMATS[fmt], str(path).encode(), _p(a), a.shape[1], a.shape[0])
     if rc:
         raise RuntimeError(f"reference {fmt} writer returned {rc}")
+
+
+def ref_load_image(path):
+    """(H, W, 4) float32 pixels of `path` as the reference's own loader reads
+    them (nimg::io::load::image, lib/nimg/img.cc:17-66), or None when it
+    refuses the file."""
+    w = np.zeros(1, np.int32)
+    h = np.zeros(1, np.int32)
+    if ref().xr_load_image(str(path).encode(), None, _p(w), _p(h)):
+        return None
+    a = np.zeros((int(h[0]), int(w[0]), 4), np.float32)
+    if ref().xr_load_image(str(path).encode(), _p(a), _p(w), _p(h)):
+        return None
+    return a
 
 
 class RefScene:

@@ -297,4 +297,26 @@ int xr_save_image(int fmt, const char *path, const float *rgba, int w, int h) {
     }
 }
 
+// Loads `path` through the reference's own loader (nimg::io::load::image,
+// lib/nimg/img.cc:17-66, stb_image underneath). With rgba NULL only the size
+// is returned; otherwise rgba receives w*h*4 floats, row 0 at the top.
+int xr_load_image(const char *path, float *rgba, int *w, int *h) {
+    nimg::Pixmap pm;
+    const int rc = nimg::io::load::image(path, pm);
+    if (rc) return rc;
+    *w = (int)pm.width();
+    *h = (int)pm.height();
+    if (!rgba) return 0;
+    for (size_t y = 0; y < pm.height(); ++y)
+        for (size_t x = 0; x < pm.width(); ++x) {
+            const nimg::ColorRGBAf &c = pm.pixel_ro(x, y);
+            float *p = rgba + 4 * (y * pm.width() + x);
+            p[0] = c.r();
+            p[1] = c.g();
+            p[2] = c.b();
+            p[3] = c.a();
+        }
+    return 0;
+}
+
 }  // extern "C"

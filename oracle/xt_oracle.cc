@@ -8,11 +8,16 @@
 //   primitives          src/xtcore/math/{triangle,sphere,plane,aabb}.cc
 //   octree build/walk   src/xtcore/octree.tml:97-249, mesh.cc:30-113
 //   Scene::intersection src/xtcore/scene.cc:191-216
-//   camera              src/xtcore/camera/perspective.cc:26-98
+//   cameras             src/xtcore/camera/{perspective,erp,ods,cubemap}.cc
 //   anti-aliasing       src/xtcore/aa.cc:7-77
-//   integrators         src/xtcore/integrator/{stencil,depth,ao,pathtracer}
+//   integrators         src/xtcore/integrator/{stencil,depth,ao,pathtracer,
+//                       normal,uv,emission,wireframe}
 //   materials/sampling  src/xtcore/material/*.cc, lib/nmath/sample.inl
+//   samplers            src/xtcore/sampler_*.cc, lib/nimg/sample.cc:8-50
 //   colour clamps       lib/nimg/color.inl, lib/nimg/color.cc
+// Every function is pinned against the compiled reference (oracle/_ref) by
+// the tests: rays, hits and deterministic images bit for bit, estimators
+// within their Monte Carlo bounds.
 // Random numbers use the same counter-based streams as the device path
 // (xtracer_amd/csrc/device/xt_rng.h restated below), because the reference's
 // own glibc rand() stream is not reproducible.
@@ -106,8 +111,24 @@ struct Mat {
     uint32_t type;  // 0 lambert, 1 phong, 2 blinn-phong, 3 emissive
     float kd[3], ks[3], ke[3];
     float refl, expo;
+    uint32_t smp[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};  // diffuse, specular, emissive samplers
+};
+// A texture Pixmap (RGBA float, row 0 at the top) and a sampler
+// (sampler_{col,tex,gradient,cubemap,erp}.h).
+struct Tex {
+    size_t w = 0, h = 0;
+    const float *px = nullptr;
+};
+struct Smp {
+    uint32_t kind;  // 1 colour, 2 texture, 3 gradient, 4 cubemap, 5 erp
+    uint32_t bilinear;
+    int32_t tex[6];  // cubemap faces: left, right, top, bottom, front, back
+    float a[3], b[3];
 };
 struct Scene {
+    std::vector<Tex> texs;
+    std::vector<Smp> smps;
+    uint32_t env_smp = 0;
     std::vector<Obj> objs;
     std::vector<Mesh> meshes;
     std::vector<Mat> mats;
@@ -424,7 +445,88 @@ static inline C3 cmul(C3 a, C3 b) { return C3{cl(a.r * b.r), cl(a.g * b.g), cl(a
 static inline C3 cscale(C3 a, float s) { return C3{cl(a.r * s), cl(a.g * s), cl(a.b * s)}; }
 static inline C3 cadd(C3 a, C3 b) { return C3{cl(a.r + b.r), cl(a.g + b.g), cl(a.b + b.b)}; }
 
+// Pixmap::pixel_ro (pixmap.cc:44-50): coordinates past the edge clamp.
+static C3 pixel_ro(const Tex &t, size_t x, size_t y) {
+    size_t nx = x >= t.w ? t.w - 1 : x;
+    size_t ny = y >= t.h ? t.h - 1 : y;
+    const float *p = t.px + 4 * (ny * t.w + nx);
+    return C3{p[0], p[1], p[2]};
+}
+// size_t(x) as x86-64 converts a double: NaN and out-of-range give 2^63
+static size_t to_size(double x) { return x >= 0.0 && x < 18446744073709551616.0 ? (size_t)x : (size_t)1 << 63; }
+// nimg::sample::nearest (sample.cc:8-17)
+static C3 nearest(const Tex &t, double u, double v) {
+    u = u < 0 ? 0 : (u > 1 ? u - std::floor(u) : u);
+    v = v < 0 ? 0 : (v > 1 ? v - std::floor(v) : v);
+    size_t x = to_size(u * (float)t.w);
+    size_t y = to_size(v * (float)t.h);
+    return pixel_ro(t, x, y);
+}
+// nimg::sample::bilinear (sample.cc:19-50), over ColorRGBAf's * and +
+static C3 bilinear(const Tex &t, double u, double v) {
+    size_t w = t.w, h = t.h;
+    float fu = u * w;
+    float fv = v * h;
+    int u1 = ((int)fu) % w;
+    int v1 = ((int)fv) % h;
+    int u2 = (u1 + 1) % w;
+    int v2 = (v1 + 1) % h;
+    float fracu = fu - floorf(fu);
+    float fracv = fv - floorf(fv);
+    float w1 = (1 - fracu) * (1 - fracv);
+    float w2 = fracu * (1 - fracv);
+    float w3 = (1 - fracu) * fracv;
+    float w4 = fracu * fracv;
+    C3 c1 = pixel_ro(t, u1, v1), c2 = pixel_ro(t, u2, v1), c3 = pixel_ro(t, u1, v2), c4 = pixel_ro(t, u2, v2);
+    return cadd(cadd(cadd(cscale(c1, w1), cscale(c2, w2)), cscale(c3, w3)), cscale(c4, w4));
+}
+// ISampler::sample(tc) (sampler_col.cc:23-27, sampler_tex.cc:40-48,
+// sampler_gradient.cc:11-17, sampler_erp.cc:13-25, sampler_cubemap.cc:14-41)
+static C3 sample(const Scene &s, const Smp &sm, V tc) {
+    if (sm.kind == 1) return C3{sm.a[0], sm.a[1], sm.a[2]};
+    if (sm.kind == 2) {
+        const Tex &t = s.texs[sm.tex[0]];
+        return sm.bilinear ? bilinear(t, tc.x, tc.y) : nearest(t, tc.x, tc.y);
+    }
+    V dir = nrm(tc);
+    if (sm.kind == 3) {
+        float t = 0.5f * dir.y + 1.0;
+        C3 a{sm.a[0], sm.a[1], sm.a[2]}, b{sm.b[0], sm.b[1], sm.b[2]};
+        return cadd(cscale(a, (float)(1.0 - t)), cscale(b, t));
+    }
+    if (sm.kind == 5) {
+        float theta = std::acos(dir.y);
+        float phi = std::atan2(dir.x, -dir.z);
+        float u = 0.5f + phi / 6.283185307179586232;
+        float v = theta / 3.14159265358979323846;
+        return nearest(s.texs[sm.tex[0]], u, v);
+    }
+    if (sm.kind == 4) {
+        int face = 2;
+        float u = 0.f, v = 0.f;
+        if ((std::fabs(dir.x) >= std::fabs(dir.y)) && (std::fabs(dir.x) >= std::fabs(dir.z))) {
+            if (dir.x > 0.0f) { face = 1; u = (dir.z / dir.x + 1.0f) * 0.5f; v = 1.f - (dir.y / dir.x + 1.0f) * 0.5f; }
+            else if (dir.x < 0.0f) { face = 0; u = (dir.z / dir.x + 1.0f) * 0.5f; v = (dir.y / dir.x + 1.0f) * 0.5f; }
+        } else if ((std::fabs(dir.y) >= std::fabs(dir.x)) && (std::fabs(dir.y) >= std::fabs(dir.z))) {
+            if (dir.y > 0.0f) { face = 2; u = (dir.x / dir.y + 1.0f) * 0.5f; v = 1.f - (dir.z / dir.y + 1.0f) * 0.5f; }
+            else if (dir.y < 0.0f) { face = 3; u = 1. - (dir.x / dir.y + 1.0f) * 0.5f; v = 1.f - (dir.z / dir.y + 1.0f) * 0.5f; }
+        } else if ((std::fabs(dir.z) >= std::fabs(dir.x)) && (std::fabs(dir.z) >= std::fabs(dir.y))) {
+            if (dir.z > 0.0f) { face = 5; u = 1. - (dir.x / dir.z + 1.0f) * 0.5f; v = 1.f - (dir.y / dir.z + 1.0f) * 0.5f; }
+            else if (dir.z < 0.0f) { face = 4; u = 1. - (dir.x / dir.z + 1.0f) * 0.5f; v = (dir.y / dir.z + 1.0f) * 0.5f; }
+        }
+        return nearest(s.texs[sm.tex[face]], u, v);
+    }
+    return C3{};
+}
+// IMaterial::get_sample (material.cc:65-70): the channel's sampler at the
+// hit's texcoord, or its constant colour (0 when absent)
+static C3 channel(const Scene &s, const Mat &m, int c, const float *konst, double tu, double tv) {
+    if (m.smp[c] != 0xffffffffu) return sample(s, s.smps[m.smp[c]], mk(tu, tv, 0.0));
+    return C3{konst[0], konst[1], konst[2]};
+}
+
 static C3 environment(const Scene &s, V dir) {
+    if (s.env_type == 3 || s.env_type == 4) return sample(s, s.smps[s.env_smp], dir);  // cubemap, erp
     if (s.env_type == 1) {  // sampler_gradient.cc:11-17
         V dn = nrm(dir);
         float t = 0.5f * dn.y + 1.0;
@@ -581,7 +683,7 @@ static C3 eval(const Ctx &cx, size_t depth, C3 in_i, V o, V d, uint32_t key, uin
     V no, nd;
     bool cont = true;
     if (m.type == 3) {
-        f = C3{m.ke[0], m.ke[1], m.ke[2]};
+        f = channel(*cx.s, m, 2, m.ke, h.tu, h.tv);
         cont = false;
     } else if (m.type == 4) {
         // dielectric.cc:24-32 sets only the ray: the intensity keeps
@@ -593,16 +695,16 @@ static C3 eval(const Ctx &cx, size_t depth, C3 in_i, V o, V d, uint32_t key, uin
     } else if (m.type == 0) {  // lambert.cc:30-40
         no = h.point + h.normal * 1e-8;
         nd = nrm(diffuse(h.normal, r));
-        f = cscale(C3{m.kd[0], m.kd[1], m.kd[2]}, (float)dot(h.normal, nd));
+        f = cscale(channel(*cx.s, m, 0, m.kd, h.tu, h.tv), (float)dot(h.normal, nd));
     } else {  // phong.cc:35-55, blinnphong.cc:34-61
         no = h.point + h.normal * 1e-8;
         double s = m.refl;
         double k = r.next();
         if (k > s) {
-            f = C3{m.kd[0], m.kd[1], m.kd[2]};
+            f = channel(*cx.s, m, 0, m.kd, h.tu, h.tv);
             nd = diffuse(h.normal, r);
         } else {
-            f = C3{m.ks[0], m.ks[1], m.ks[2]};
+            f = channel(*cx.s, m, 1, m.ks, h.tu, h.tv);
             nd = lobe(h.normal, -d, m.expo, r);
             if (m.type == 2) nd = nrm(nd);
         }
@@ -636,6 +738,13 @@ struct or_scene_desc {
     const float *mat_param;      // 11 floats: kd[3], ks[3], ke[3], reflectance, exponent
     uint32_t env_type;
     const float *env;            // a[3], b[3]
+    uint32_t n_textures;
+    const uint32_t *tex_info;    // 4 per texture: offset (texels), width, height, 0
+    const float *texels;         // 4 floats per texel
+    uint32_t n_samplers;
+    const uint32_t *samplers;    // 14 words each (see Smp)
+    const uint32_t *mat_smp;     // 3 per material or NULL
+    uint32_t env_smp;
 };
 
 // Sampler hooks for distribution tests: kind 0 hemisphere(n, n), 1 diffuse(n),
@@ -718,6 +827,22 @@ void *or_build(const or_scene_desc *in) {
         s->env_a[k] = in->env ? in->env[k] : 0.f;
         s->env_b[k] = in->env ? in->env[3 + k] : 0.f;
     }
+    for (uint32_t t = 0; t < in->n_textures; ++t) {
+        Tex x;
+        x.w = in->tex_info[4 * t + 1];
+        x.h = in->tex_info[4 * t + 2];
+        x.px = in->texels + 4 * (size_t)in->tex_info[4 * t];
+        s->texs.push_back(x);
+    }
+    for (uint32_t k = 0; k < in->n_samplers; ++k) {
+        Smp sm;
+        std::memcpy(&sm, in->samplers + 14 * (size_t)k, sizeof sm);
+        s->smps.push_back(sm);
+    }
+    if (in->mat_smp)
+        for (size_t i = 0; i < s->mats.size(); ++i)
+            for (int c = 0; c < 3; ++c) s->mats[i].smp[c] = in->mat_smp[3 * i + c];
+    s->env_smp = in->env_smp;
     return s;
 }
 
@@ -959,10 +1084,12 @@ uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double 
                                 if (ob.mat != 0xffffffffu) {
                                     const Mat &m = s->mats[ob.mat];
                                     const float wf = (float)weight;  // operator*(ColorRGBf, float)
+                                    const C3 ke = channel(*s, m, 2, m.ke, hh.tu, hh.tv);
+                                    const float kv[3] = {ke.r, ke.g, ke.b};
                                     for (int c = 0; c < 3; ++c) {
-                                        const float e = cl(m.ke[c] * wf);
+                                        const float e = cl(kv[c] * wf);
                                         px[c] = cl(px[c] + e);
-                                        val[c] = m.ke[c];
+                                        val[c] = kv[c];
                                     }
                                 }
                             }

@@ -156,3 +156,73 @@ def uv_scene(path, camera="pinhole"):
               camera=CAMERAS[camera],
               env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
     return path
+
+
+def texture_files(d):
+    """Small texture files of each decoded format in directory d (odd sizes,
+    so bilinear wraps and nearest clamps are exercised)."""
+    import os
+    from tests import imgfiles as I
+    rng = np.random.default_rng(23)
+    yy, xx = np.mgrid[0:9, 0:13]
+    rgb = np.stack([(xx * 19) % 256, (yy * 27) % 256, ((xx + yy) % 2) * 200 + 30], -1).astype(np.uint8)
+    rgba = np.concatenate([rng.integers(0, 256, (7, 10, 3)), rng.integers(0, 256, (7, 10, 1))], -1).astype(np.uint8)
+    f = {"rgb_png": I.png(os.path.join(d, "tex_rgb.png"), rgb),
+         "rgba_png": I.png(os.path.join(d, "tex_rgba.png"), rgba, filters=(4, 3, 1)),
+         "tga": I.tga(os.path.join(d, "tex.tga"), rng.integers(0, 256, (6, 11, 3)).astype(np.uint8), rle=True),
+         "bmp": I.bmp(os.path.join(d, "tex.bmp"), rng.integers(0, 256, (5, 6, 3)).astype(np.uint8)),
+         "erp": I.png(os.path.join(d, "erp.png"), rng.integers(0, 256, (16, 32, 3)).astype(np.uint8))}
+    for k, face in enumerate(["posx", "negx", "posy", "negy", "posz", "negz"]):
+        img = np.zeros((8, 8, 3), np.uint8)
+        img[..., k % 3] = 60 + 30 * k
+        img[..., (k + 1) % 3] = (np.arange(8)[None, :] * 31) % 256
+        img[..., (k + 2) % 3] = (np.arange(8)[:, None] * 29) % 256
+        f[face] = I.ppm(os.path.join(d, f"{face}.ppm"), img) if k % 2 else I.tga(os.path.join(d, f"{face}.tga"), img)
+    return {k: os.path.basename(v) for k, v in f.items()}
+
+
+def texture_scene(path, camera="pinhole", variant="emit", env="erp"):
+    """uv_scene's geometry with textured materials. variant "emit": every
+    object emissive through a texture (nearest, bilinear, flips; texcoords
+    outside [0, 1] on the mesh), gradient or ERP sampler read at its texcoord; variant "lit": textured
+    lambert / phong / blinn-phong diffuse and specular under one textured
+    light; variant "env": no objects, the environment alone. env: "erp",
+    "cubemap" or "gradient"."""
+    import os
+    from tests import scn
+    d = os.path.dirname(str(path))
+    uv_scene(path, camera)  # writes the mesh file
+    f = texture_files(d)
+    cube = {k: f[k] for k in ("posx", "negx", "posy", "negy", "posz", "negz")}
+    tex = lambda key, **kw: dict(type="texture", source=f[key], **kw)
+    if variant == "emit":
+        mats = {"m_floor": dict(type="emissive", ke=tex("rgb_png", filtering="bilinear")),
+                "m_a": dict(type="emissive", ke=tex("tga", flip_x=True)),
+                "m_b": dict(type="emissive", ke=tex("rgba_png", filtering="bilinear", flip_y=True)),
+                "m_tri": dict(type="emissive", ke=dict(type="erp", source=f["erp"])),
+                "m_m": dict(type="emissive", ke=tex("bmp")),
+                "m_t2": dict(type="emissive", ke=dict(type="gradient", a=(0.9, 0.2, 0.1), b=(0.1, 0.3, 1.0)))}
+    else:
+        mats = {"m_floor": dict(type="lambert", kd=tex("rgb_png", filtering="bilinear")),
+                "m_a": dict(type="phong", kd=tex("bmp"), ks=tex("tga", flip_y=True), exponent=20.0),
+                "m_b": dict(type="emissive", ke=tex("rgba_png", flip_x=True)),
+                "m_tri": dict(type="blinn_phong", kd=tex("rgba_png", filtering="bilinear"), ks=(0.3, 0.3, 0.3),
+                              exponent=8.0),
+                "m_m": dict(type="lambert", kd=dict(type="cubemap", **cube)),
+                "m_t2": dict(type="lambert", kd=dict(type="gradient", a=(0.9, 0.2, 0.1), b=(0.1, 0.3, 1.0)))}
+    envd = {"erp": dict(type="erp", source=f["erp"]), "cubemap": dict(type="cubemap", **cube),
+            "gradient": ("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0))}[env]
+    geo = {"floor": dict(type="plane", normal=(0, 1, 0), distance=0.0, u_scale=0.7, v_scale=1.3),
+           "a": dict(type="sphere", position=(-1.2, 0.6, 0.0), radius=0.6, u_scale=1.5, v_scale=0.8),
+           "b": dict(type="sphere", position=(0.2, 0.5, 0.4), radius=0.5),
+           "tri": dict(type="triangle", v=[(-2.0, 0.0, 1.5), (2.0, 0.0, 1.5), (0.0, 2.2, 1.8)],
+                       tc=[(0.1, 0.9), (0.8, 0.2), (0.45, -0.3)]),
+           "m": dict(type="mesh", source="uvmesh.obj"),
+           "t2": dict(type="triangle", v=[(1.0, 0.05, -1.0), (2.6, 0.05, -0.6), (1.8, 1.2, -0.8)],
+                      tc=[(-0.7, 1.6), (2.3, -0.4), (1.2, 2.7)])}
+    objs = {"floor": ("floor", "m_floor"), "a": ("a", "m_a"), "b": ("b", "m_b"), "tri": ("tri", "m_tri"),
+            "m": ("m", "m_m"), "t2": ("t2", "m_t2")}
+    if variant == "env":
+        geo, objs, mats = {}, {}, {}
+    scn.write(path, geometry=geo, objects=objs, materials=mats, camera=CAMERAS[camera], env=envd)
+    return path

@@ -8,9 +8,15 @@ def write(path, geometry, objects, materials, camera=None, env=None):
     """geometry: {name: dict(type=..., **props)}; objects: {name: (geometry, material)};
     materials: {name: dict(type=..., kd=, ks=, ke=, reflectance=, exponent=)};
     camera: dict(position, target, fov, aperture, flength), or dict(type=ods|erp|cubemap,
-    position, orientation, ipd); env: None | ("color", c) | ("gradient", a, b)."""
+    position, orientation, ipd); env: None | ("color", c) | ("gradient", a, b) | a sampler dict.
+    A material channel (kd / ks / ke) is a colour or a sampler dict: dict(type=texture, source,
+    filtering, flip_x, flip_y), dict(type=erp, source), dict(type=cubemap, posx=.., negz=..),
+    dict(type=gradient, a, b)."""
     out = ["title = test"]
-    if env is not None:
+    if isinstance(env, dict):
+        out.append("environment = {\n\ttype = %s\n\tconfig = {\n%s\n\t}\n}"
+                   % (env["type"], "\n".join(_sampler_props(env, "\t\t"))))
+    elif env is not None:
         if env[0] == "color":
             out.append("environment = {\n\ttype = color\n\tconfig = {\n\t\tvalue = %s\n\t}\n}" % col3(env[1]))
         else:
@@ -64,7 +70,10 @@ def write(path, geometry, objects, materials, camera=None, env=None):
     for name, p in materials.items():
         m.append(f"\t{name} = {{\n\t\ttype = {p['type']}\n\t\tproperties = {{\n\t\t\tsamplers = {{")
         for key, sname in (("kd", "diffuse"), ("ks", "specular"), ("ke", "emissive")):
-            if key in p:
+            if key in p and isinstance(p[key], dict):
+                m.append(f"\t\t\t\t{sname} = {{\n\t\t\t\t\ttype = {p[key]['type']}\n"
+                         + "\n".join(_sampler_props(p[key], "\t\t\t\t\t")) + "\n\t\t\t\t}")
+            elif key in p:
                 m.append(f"\t\t\t\t{sname} = {{\n\t\t\t\t\ttype = color\n\t\t\t\t\tvalue = {col3(p[key])}\n\t\t\t\t}}")
         m.append("\t\t\t}")
         sc = [(k, p[k]) for k in ("reflectance", "exponent", "ior") if k in p]
@@ -78,6 +87,20 @@ def write(path, geometry, objects, materials, camera=None, env=None):
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
     return str(path)
+
+
+def _sampler_props(smp, ind):
+    """Property lines of a sampler dict (parseutil.cc:147-203, 472-500)."""
+    lines = []
+    for k, v in smp.items():
+        if k == "type":
+            continue
+        if k in ("a", "b", "value"):
+            v = col3(v)
+        elif isinstance(v, bool):
+            v = "true" if v else "false"
+        lines.append(f"{ind}{k} = {v}")
+    return lines
 
 
 def write_obj(path, tris):

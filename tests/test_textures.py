@@ -1,0 +1,223 @@
+"""Texture images and samplers on the host and in the oracle.
+
+The decoders are checked against the bytes the test wrote (byte / 255.f,
+nimg/img.cc:30-57) and, where the compiled reference is built, against the
+reference's own loader (stb_image). The oracle's samplers (texture nearest /
+bilinear with flips, gradient, ERP, cubemap) are checked bit for bit against
+the reference through the emission integrator (material samplers read at the
+hit's texcoord) and through camera rays that miss everything (environment
+samplers), and statistically through the pathtracer with textured lambert /
+phong / blinn-phong channels."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import xtracer_amd as xa
+from oracle import oracle as O
+from tests import imgfiles as I
+from tests.helpers import texture_files, texture_scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference checker (oracle/_ref) not built")
+
+
+def _pixels(seed, h, w, ch):
+    return np.random.default_rng(seed).integers(0, 256, (h, w, ch)).astype(np.uint8)
+
+
+FILES = {
+    "png_rgb": lambda d: (I.png(d / "a.png", _pixels(1, 9, 13, 3)), _pixels(1, 9, 13, 3)),
+    "png_rgba_filters": lambda d: (I.png(d / "b.png", _pixels(2, 11, 7, 4), filters=(3, 4, 0, 2, 1)),
+                                   _pixels(2, 11, 7, 4)),
+    "png_paeth_only": lambda d: (I.png(d / "c.png", _pixels(3, 5, 33, 3), filters=(4,)), _pixels(3, 5, 33, 3)),
+    "tga_raw_bottom_up": lambda d: (I.tga(d / "a.tga", _pixels(4, 6, 5, 3)), _pixels(4, 6, 5, 3)),
+    "tga_raw_top_down_rgba": lambda d: (I.tga(d / "b.tga", _pixels(5, 4, 9, 4), top_down=True), _pixels(5, 4, 9, 4)),
+    "tga_rle": lambda d: (I.tga(d / "c.tga", np.repeat(_pixels(6, 7, 3, 3), 50, axis=1), rle=True),
+                          np.repeat(_pixels(6, 7, 3, 3), 50, axis=1)),
+    "tga_rle_rgba_top_down": lambda d: (I.tga(d / "d.tga", np.repeat(_pixels(7, 3, 40, 4), 2, axis=0), rle=True,
+                                              top_down=True), np.repeat(_pixels(7, 3, 40, 4), 2, axis=0)),
+    "bmp24_padded": lambda d: (I.bmp(d / "a.bmp", _pixels(8, 5, 7, 3)), _pixels(8, 5, 7, 3)),
+    "bmp24_top_down": lambda d: (I.bmp(d / "b.bmp", _pixels(9, 3, 5, 3), top_down=True), _pixels(9, 3, 5, 3)),
+    "bmp32": lambda d: (I.bmp(d / "c.bmp", _pixels(10, 4, 6, 4)), _pixels(10, 4, 6, 4)),
+    "ppm": lambda d: (I.ppm(d / "a.ppm", _pixels(11, 8, 3, 3)), _pixels(11, 8, 3, 3)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FILES))
+def test_decoder_known_bytes(name, tmp_path):
+    path, px = FILES[name](tmp_path)
+    got = xa.read_image(path)
+    assert np.array_equal(got.view(np.uint32), I.expected(px).view(np.uint32))
+
+
+def test_bmp32_without_alpha_is_opaque(tmp_path):
+    px = _pixels(12, 3, 4, 4)
+    px[..., 3] = 0
+    got = xa.read_image(I.bmp(tmp_path / "z.bmp", px))
+    assert np.all(got[..., 3] == 1.0)
+    assert np.array_equal(got[..., :3], I.expected(px)[..., :3])
+
+
+@pytest.mark.parametrize("case", ["grey_png", "interlaced_png", "png16", "unknown"])
+def test_decoder_refuses(case, tmp_path):
+    if case == "grey_png":  # one channel: the reference refuses it as well (img.cc:60)
+        path = I.png(tmp_path / "g.png", _pixels(1, 4, 4, 1))
+    elif case == "interlaced_png":
+        path = I.png(tmp_path / "i.png", _pixels(1, 4, 4, 3), interlace=1)
+    elif case == "png16":
+        path = I.png(tmp_path / "s.png", _pixels(1, 4, 4, 3), bit_depth=16)
+    else:
+        path = tmp_path / "x.jpg"
+        path.write_bytes(b"\xff\xd8\xff\xe0not decoded")
+    with pytest.raises(xa.XtError):
+        xa.read_image(path)
+    with pytest.raises(xa.XtError):
+        xa.read_image(tmp_path / "missing.png")
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(FILES) + ["bmp32_zero_alpha", "grey_png"])
+def test_decoder_matches_reference_loader(name, tmp_path):
+    """xt_image_read == nimg::io::load::image (stb_image) on every decoded
+    variant, and both refuse a one-channel PNG."""
+    if name == "bmp32_zero_alpha":
+        px = _pixels(12, 3, 4, 4)
+        px[..., 3] = 0
+        path = I.bmp(tmp_path / "z.bmp", px)
+    elif name == "grey_png":
+        path = I.png(tmp_path / "g.png", _pixels(1, 4, 4, 1))
+        assert O.ref_load_image(path) is None
+        with pytest.raises(xa.XtError):
+            xa.read_image(path)
+        return
+    else:
+        path, _ = FILES[name](tmp_path)
+    ref = O.ref_load_image(path)
+    assert ref is not None
+    assert np.array_equal(xa.read_image(path).view(np.uint32), ref.view(np.uint32))
+
+
+def test_scene_sampler_tables(tmp_path):
+    """Texture tables hold the decoded files with flip_x / flip_y applied
+    (nimg/transform.cc); samplers and material slots follow the .scn."""
+    path = texture_scene(str(tmp_path / "t.scn"), "pinhole", "emit", "cubemap")
+    s = xa.Scene(path)
+    t = s.samplers()
+    f = texture_files(str(tmp_path))
+    decoded = {k: xa.read_image(tmp_path / v) for k, v in f.items()}
+    texs = [t["texels"][o:o + w * h].reshape(h, w, 4) for o, w, h, _ in t["tex_info"]]
+    smp = t["samplers"]
+    kinds = sorted(int(k) for k in smp[:, 0])
+    assert kinds == [2, 2, 2, 2, 3, 4, 5]  # four textures, gradient, cubemap (environment), erp
+
+    def tex_of(i):
+        return texs[int(smp[i, 2])]
+
+    found = set()
+    for i in range(len(smp)):
+        k, bil = int(smp[i, 0]), int(smp[i, 1])
+        if k == 2 and tex_of(i).shape[:2] == (6, 11):  # tex.tga, nearest, flip_x
+            assert bil == 0
+            assert np.array_equal(tex_of(i), decoded["tga"][:, ::-1])
+            found.add("tga")
+        elif k == 2 and tex_of(i).shape[:2] == (7, 10):  # rgba png, bilinear, flip_y
+            assert np.array_equal(tex_of(i), decoded["rgba_png"][::-1])
+            found.add("rgba")
+        elif k == 2 and tex_of(i).shape[:2] == (5, 6):
+            assert np.array_equal(tex_of(i), decoded["bmp"])
+            found.add("bmp")
+        elif k == 2:
+            assert np.array_equal(tex_of(i), decoded["rgb_png"])
+            found.add("rgb")
+        elif k == 4:  # faces left, right, top, bottom, front, back
+            faces = [texs[int(x)] for x in smp[i, 2:8].view(np.int32)]
+            for face, name in zip(faces, ["negx", "posx", "posy", "negy", "posz", "negz"]):
+                assert np.array_equal(face, decoded[name])
+            assert t["env_smp"] == i
+            found.add("cube")
+    assert found == {"tga", "rgba", "rgb", "bmp", "cube"}
+
+
+def test_missing_texture_fails_scene_load(tmp_path):
+    path = texture_scene(str(tmp_path / "t.scn"), "pinhole", "emit", "erp")
+    os.remove(tmp_path / "tex.tga")
+    with pytest.raises(xa.XtError):
+        xa.Scene(path)
+
+
+_CHILD = r"""
+import json, sys, os
+import numpy as np
+sys.path.insert(0, {root!r})
+import xtracer_amd as xa
+from oracle import oracle as O
+from tests.helpers import camera_dict_of, texture_scene
+path = texture_scene({path!r}, {cam!r}, {variant!r}, {env!r})
+s = xa.Scene(path)
+ref = O.RefScene(path).render({integ}, 64, 48, samples=4, threads=1)
+mine, _ = O.OracleScene(s).render({integ}, 64, 48, camera_dict_of({cam!r}), samples=4)
+bad = int((ref.view(np.uint32) != mine.view(np.uint32)).any(axis=2).sum())
+print(json.dumps({{"bad": bad, "lit": int((ref[..., :3] > 0).any(axis=2).sum())}}))
+"""
+
+
+def _run(tmp_path, **kw):
+    code = _CHILD.format(root=ROOT, path=str(tmp_path / "t.scn"), **kw)
+    return json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=300).decode().splitlines()[-1])
+
+
+@needs_ref
+@pytest.mark.parametrize("cam", ["pinhole", "erp", "cubemap"])
+def test_material_samplers_bit_exact(cam, tmp_path):
+    """Emission integrator over emissive textures (nearest with flip_x,
+    bilinear with and without flip_y, RGBA), a gradient and an ERP sampler
+    read at texcoords of planes, spheres, a triangle and a mesh: oracle ==
+    reference bit for bit."""
+    out = _run(tmp_path, cam=cam, variant="emit", env="erp", integ=xa.EMISSION)
+    assert out["bad"] == 0, out
+    assert out["lit"] > 1000
+
+
+@needs_ref
+@pytest.mark.parametrize("cam", ["pinhole", "erp"])
+@pytest.mark.parametrize("env", ["erp", "cubemap", "gradient"])
+def test_environment_samplers_bit_exact(cam, env, tmp_path):
+    """Pathtracer camera rays that hit nothing return the environment's
+    sample (pathtracer/integrator.cc:46): oracle == reference bit for bit,
+    over every direction of an ERP camera."""
+    out = _run(tmp_path, cam=cam, variant="env", env=env, integ=xa.PATHTRACER)
+    assert out["bad"] == 0, out
+    assert out["lit"] == 64 * 48
+
+
+_STAT = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import xtracer_amd as xa
+from oracle import oracle as O
+from tests.helpers import camera_dict_of, texture_scene
+path = texture_scene({path!r}, {cam!r}, "lit", {env!r})
+spp = 256
+ref = O.RefScene(path).render(3, 24, 16, samples=spp, rdepth=6, threads=1, seed=7)
+_, _, mean, m2 = O.OracleScene(xa.Scene(path)).render(3, 24, 16, camera_dict_of({cam!r}), samples=spp, rdepth=6,
+                                                       seed=3, stats=True)
+se = np.sqrt(2 * (m2 / (spp - 1)) / spp)
+inside = np.abs(ref[..., :3] - mean) <= 2.576 * se + 1e-5
+print(json.dumps({{"inside": float(inside.mean())}}))
+"""
+
+
+@needs_ref
+@pytest.mark.parametrize("cam,env", [("pinhole", "erp"), ("erp", "cubemap")])
+def test_textured_pathtracer_statistically(cam, env, tmp_path):
+    """Textured lambert / phong / blinn-phong diffuse and specular channels,
+    a cubemap-sampled diffuse, a textured light and an ERP / cubemap
+    environment: per-pixel means within the 99% MC bound."""
+    code = _STAT.format(root=ROOT, path=str(tmp_path / "t.scn"), cam=cam, env=env)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
+    assert out["inside"] >= 0.97, out

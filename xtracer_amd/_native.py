@@ -30,9 +30,9 @@ CAMERA_THIN_LENS, CAMERA_ODS, CAMERA_ERP, CAMERA_CUBEMAP = 0, 1, 2, 3
 # Exported symbols, in the order include/xtracer_amd.h declares them.
 SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
-    "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals", "xt_scene_get_texcoords",
+    "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals", "xt_scene_get_texcoords", "xt_scene_get_samplers",
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
-    "xt_image_write", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
+    "xt_image_write", "xt_image_read", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
     "xt_serve_get_info", "xt_serve_set_policy",
@@ -144,6 +144,7 @@ def lib():
         "xt_scene_get_object_params": (I, [P, U32, P]),
         "xt_scene_get_vnormals": (I, [P, P]),
         "xt_scene_get_texcoords": (I, [P, P, P]),
+        "xt_scene_get_samplers": (I, [P, P, P, P, P, P, P]),
         "xt_scene_get_kdtree": (I, [P, P, P, P]),
         "xt_scene_mesh_octree": (I, [P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64),
                                      ctypes.POINTER(U64)]),
@@ -151,6 +152,7 @@ def lib():
         "xt_camera_make": (I, [ctypes.POINTER(ctypes.c_double)] * 3 + [ctypes.c_float] * 3 +
                            [ctypes.POINTER(Camera)]),
         "xt_image_write": (I, [ctypes.c_char_p, P, U32, U32]),
+        "xt_image_read": (I, [ctypes.c_char_p, P, P, P]),
         "xt_dev_create": (I, [ctypes.POINTER(P), I, U64]),
         "xt_dev_destroy": (None, [P]),
         "xt_last_error": (ctypes.c_char_p, [P]),
@@ -268,6 +270,25 @@ class Scene:
             raise XtError(rc)
         return tc, sc
 
+    def samplers(self):
+        """Textures and table samplers: dict(tex_info (n, 4) uint32, texels (m, 4) float32,
+        samplers (k, 14) uint32 words, mat_smp (n_materials, 3), env_smp)."""
+        counts = np.zeros(3, np.uint32)
+        self._check_rc(lib().xt_scene_get_samplers(self._h, _ptr(counts), None, None, None, None, None))
+        nt, nf, ns = (int(c) for c in counts)
+        tex = np.zeros((nt, 4), np.uint32)
+        texels = np.zeros((nf // 4, 4), np.float32)
+        smp = np.zeros((ns, 14), np.uint32)
+        ms = np.zeros((self.info().n_materials, 3), np.uint32)
+        env = np.zeros(1, np.uint32)
+        self._check_rc(lib().xt_scene_get_samplers(self._h, _ptr(counts), _ptr(tex), _ptr(texels), _ptr(smp), _ptr(ms),
+                                                   _ptr(env)))
+        return {"tex_info": tex, "texels": texels, "samplers": smp, "mat_smp": ms, "env_smp": int(env[0])}
+
+    def _check_rc(self, rc):
+        if rc:
+            raise XtError(rc)
+
     def kdtree(self):
         i = self.info()
         nodes = np.zeros((i.n_kd_nodes, 2), np.uint32)
@@ -329,6 +350,22 @@ def write_image(path, rgba):
     rc = lib().xt_image_write(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0])
     if rc:
         raise XtError(rc, f"xt_image_write({path})")
+
+
+def read_image(path):
+    """(H, W, 4) float32 pixels of a texture image, as scenes load them
+    (xt_image_read). Host-only."""
+    import numpy as np
+    w = ctypes.c_uint32()
+    h = ctypes.c_uint32()
+    rc = lib().xt_image_read(os.fsencode(path), None, ctypes.byref(w), ctypes.byref(h))
+    if rc:
+        raise XtError(rc, f"xt_image_read({path})")
+    a = np.zeros((h.value, w.value, 4), np.float32)
+    rc = lib().xt_image_read(os.fsencode(path), a.ctypes.data, ctypes.byref(w), ctypes.byref(h))
+    if rc:
+        raise XtError(rc, f"xt_image_read({path})")
+    return a
 
 
 class Device:

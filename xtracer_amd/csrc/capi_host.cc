@@ -6,6 +6,7 @@
 
 #include "../../include/xtracer_amd.h"
 #include "capi_internal.h"
+#include "host/image_load.h"
 #include "host/image_io.h"
 
 using namespace xt;
@@ -159,6 +160,22 @@ int xt_scene_get_texcoords(const xt_scene *s, double *prim_tc, double *uv_scale)
     return XT_OK;
 }
 
+int xt_scene_get_samplers(const xt_scene *s, uint32_t counts[3], uint32_t *tex_info, float *texels, uint32_t *samplers,
+                          uint32_t *mat_smp, uint32_t *env_smp) {
+    if (!s || !s->built || !counts) return XT_E_INVALID;
+    static_assert(sizeof(XtSampler) == 14 * sizeof(uint32_t), "XtSampler is 14 words");
+    const FlatScene &f = s->flat;
+    counts[0] = (uint32_t)f.textures.size();
+    counts[1] = (uint32_t)f.texels.size();
+    counts[2] = (uint32_t)f.samplers.size();
+    if (tex_info && !f.textures.empty()) std::memcpy(tex_info, f.textures.data(), f.textures.size() * sizeof(XtTexture));
+    if (texels && !f.texels.empty()) std::memcpy(texels, f.texels.data(), f.texels.size() * sizeof(float));
+    if (samplers && !f.samplers.empty()) std::memcpy(samplers, f.samplers.data(), f.samplers.size() * sizeof(XtSampler));
+    if (mat_smp && !f.mat_smp.empty()) std::memcpy(mat_smp, f.mat_smp.data(), f.mat_smp.size() * sizeof(uint32_t));
+    if (env_smp) *env_smp = f.env.smp;
+    return XT_OK;
+}
+
 int xt_scene_get_kdtree(const xt_scene *s, uint32_t *nodes, uint32_t *refs, float bounds[6]) {
     if (!s || !s->built) return XT_E_INVALID;
     const KdTree &k = s->flat.kd;
@@ -266,6 +283,17 @@ int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t
     if (!path || !rgba || !width || !height) return XT_E_INVALID;
     const int rc = xt::write_image(path, rgba, width, height);
     return rc == 0 ? XT_OK : rc == 2 ? XT_E_INVALID : XT_E_IO;
+}
+
+int xt_image_read(const char *path, float *rgba, uint32_t *width, uint32_t *height) {
+    if (!path || !width || !height) return XT_E_INVALID;
+    xt::Image img;
+    const int rc = xt::load_image(path, img);
+    if (rc) return rc == 1 ? XT_E_IO : XT_E_INVALID;
+    *width = img.width;
+    *height = img.height;
+    if (rgba) std::memcpy(rgba, img.rgba.data(), img.rgba.size() * sizeof(float));
+    return XT_OK;
 }
 
 }  // extern "C"

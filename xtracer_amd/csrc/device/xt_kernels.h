@@ -14,6 +14,7 @@
 
 #include "../../../include/xtracer_amd.h"
 #include "xt_sample.h"
+#include "xt_texture.h"
 #include "xt_trace.h"
 
 #define XT_BLOCK 256
@@ -643,6 +644,72 @@ __device__ __forceinline__ void push_ray(Queue &out, uint32_t slot, f3 o, f3 d, 
     out.path[slot] = path;
 }
 
+#pragma clang fp contract(off)
+// Texture coordinates of a hit in FP64 (hitrecord.h:33), as the reference's
+// primitives set them: a triangle's corner texcoords weighted by its
+// barycentrics (triangle.cc:110-114), a sphere's asin mapping of the normal
+// (sphere.cc:53-54), a plane's projection on two tangents, scaled and
+// wrapped (plane.cc:43-57). o, d: the ray; h: its hit record.
+__device__ __forceinline__ void hit_texcoord(const SceneView &S, uint32_t prim, const d3 o, const d3 d, const float4 h,
+                                             double &tu, double &tv) {
+    const double PI = 3.14159265358979323846;  // nmath::PI
+    tu = 0.0;
+    tv = 0.0;
+    if (prim >= XT_PLANE_BASE) {
+        const XtPlane &pl = S.planes[prim - XT_PLANE_BASE];
+        const double us = S.obj_uv[2 * pl.object], vs = S.obj_uv[2 * pl.object + 1];
+        double t = plane64(o, d, pl);
+        if (!(t > 0.0)) t = (double)h.x;
+        const d3 pt = dadd(o, dscl(d, t));
+        const d3 v = {pl.point64[0], pl.point64[1], pl.point64[2]};
+        const d3 nn = dnorm(d3{pl.normal64[0], pl.normal64[1], pl.normal64[2]});
+        const d3 uvec = {nn.y, nn.z, -nn.x};
+        const d3 vvec = dcross(uvec, nn);
+        const d3 vp = dadd(v, pt);
+        tu = ddot(uvec, vp) * us;
+        tv = ddot(vvec, vp) * vs;
+        if (tu > 1.f) tu -= (float)(int)tu;
+        if (tv > 1.f) tv -= (float)(int)tv;
+        if (tu < -1.f) tu -= (float)(int)tu;
+        if (tv < -1.f) tv -= (float)(int)tv;
+        if (tu < 0.f) tu = 1.f + tu;
+        if (tv < 0.f) tv = 1.f + tv;
+    } else {
+        const XtPrim64 &p = S.prims64[prim];
+        if (XT_PRIM_TYPE(__float_as_uint(S.prims[3 * prim].w)) == XT_PRIM_TRIANGLE) {
+            double bc[3];
+            const double t = tri64(o, d, p, bc);
+            if (!(t > 0.0)) {  // the FP64 predicate disagrees: FP32 barycentrics
+                bc[1] = h.z;
+                bc[2] = h.w;
+                bc[0] = 1.0 - bc[1] - bc[2];
+            }
+            const double *tc = S.tc64 + 6 * (size_t)prim;
+            tu = tc[0] * bc[0] + tc[2] * bc[1] + tc[4] * bc[2];
+            tv = tc[1] * bc[0] + tc[3] * bc[1] + tc[5] * bc[2];
+        } else {
+            const uint32_t obj = __float_as_uint(S.prims[3 * prim + 1].w);
+            const double us = S.obj_uv[2 * obj], vs = S.obj_uv[2 * obj + 1];
+            const d3 c = {p.a[0], p.a[1], p.a[2]};
+            const double r = p.b[0];
+            const double b = 2 * ddot(dsub(o, c), d);
+            const d3 nc = {-c.x, -c.y, -c.z};
+            const double cq = ddot(c, c) + ddot(o, o) + 2 * ddot(nc, o) - r * r;
+            const double disc = (b * b - 4 * cq);
+            const double s = sqrt(disc > 0.0 ? disc : 0.0);
+            const double t1 = (-b + s) / 2.0, t2 = (-b - s) / 2.0;
+            double t = t1 < t2 ? t1 : t2;
+            if (!(disc > 0.0 && t > XT_EPS64)) t = (double)h.x;
+            const d3 pt = dadd(o, dscl(d, t));
+            const double sg = t1 * t2 > 0. ? 1. : -1;
+            const d3 nn = {(pt.x - c.x) / r * sg, (pt.y - c.y) / r * sg, (pt.z - c.z) / r * sg};
+            tu = asin(nn.x / (us != 0.0f ? us : 1.0f)) / PI + 0.5;
+            tv = asin(nn.y / (vs != 0.0f ? vs : 1.0f)) / PI + 0.5;
+        }
+    }
+}
+#pragma clang fp contract(on)
+
 // Pathtracer (src/xtcore/integrator/pathtracer/integrator.cc:21-47). Path
 // state: T (product so far) and F (factor of the last hit); L = T * X where
 // X is the value of the current query: miss -> env, emissive -> F * Le,
@@ -674,7 +741,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
             const uint32_t q = qword & 0xffffu, rdepth = qword >> 16;
             const uint32_t prim = __float_as_uint(h.y);
             if (prim == XT_NONE) {
-                const f3 L = cmul(T, env_sample(S.env, d));
+                const f3 L = cmul(T, env_sample(S, d));
                 rad[path] = make_float4(L.x, L.y, L.z, 0.f);
             } else {
                 f3 p, N, g;
@@ -682,8 +749,20 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                 surface(S, prim, o, d, h.x, h.z, h.w, p, N, g, obj);
                 const uint32_t mi = S.obj_material[obj];
                 const XtMaterial m = S.materials[mi == XT_NONE ? 0 : mi];
+                // channel colours: the material's constants, or its samplers at
+                // the hit's texture coordinates (material.cc:65-70)
+                f3 kd = mk(m.kd[0], m.kd[1], m.kd[2]), ks = mk(m.ks[0], m.ks[1], m.ks[2]);
+                f3 ke = mk(m.ke[0], m.ke[1], m.ke[2]);
+                const uint32_t *ms = S.mat_smp + 3 * (mi == XT_NONE ? 0 : mi);
+                if (ms[0] != XT_NONE || ms[1] != XT_NONE || ms[2] != XT_NONE) {
+                    double tu, tv;
+                    hit_texcoord(S, prim, d3{o.x, o.y, o.z}, d3{d.x, d.y, d.z}, h, tu, tv);
+                    if (ms[0] != XT_NONE) kd = sampler_eval(S, S.samplers[ms[0]], tu, tv, 0.0);
+                    if (ms[1] != XT_NONE) ks = sampler_eval(S, S.samplers[ms[1]], tu, tv, 0.0);
+                    if (ms[2] != XT_NONE) ke = sampler_eval(S, S.samplers[ms[2]], tu, tv, 0.0);
+                }
                 if (m.type == XT_MAT_EMISSIVE) {
-                    const f3 L = cmul(cmul(T, F), mk(m.ke[0], m.ke[1], m.ke[2]));
+                    const f3 L = cmul(cmul(T, F), ke);
                     rad[path] = make_float4(L.x, L.y, L.z, 0.f);
                 } else if (m.type == XT_MAT_DIELECTRIC) {
                     rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -692,14 +771,14 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                     f3 f, dir;
                     if (m.type == XT_MAT_LAMBERT) {
                         dir = norm3(smp_diffuse(N, r));
-                        f = cscale(mk(m.kd[0], m.kd[1], m.kd[2]), dot3(N, dir));
+                        f = cscale(kd, dot3(N, dir));
                     } else {
                         const float k = r.next();
                         if (k > m.reflectance) {
-                            f = mk(m.kd[0], m.kd[1], m.kd[2]);
+                            f = kd;
                             dir = smp_diffuse(N, r);
                         } else {
-                            f = mk(m.ks[0], m.ks[1], m.ks[2]);
+                            f = ks;
                             dir = smp_lobe(N, scl(d, -1.f), m.exponent, r);
                             if (m.type == XT_MAT_BLINNPHONG) dir = norm3(dir);
                         }
@@ -889,17 +968,13 @@ __device__ __forceinline__ float4 normal_update(float4 a, bool hit, d3 n) {
     return make_float4(fmaxf((float)acc.x, 0.f), fmaxf((float)acc.y, 0.f), fmaxf((float)acc.z, 0.f), 1.f);
 }
 
-// Texture coordinates of the closest hit in FP64 (hitrecord.h:33), as the
-// reference's primitives set them: a triangle's corner texcoords weighted by
-// its barycentrics (triangle.cc:110-114), a sphere's asin mapping of the
-// normal (sphere.cc:53-54), a plane's projection on two tangents, scaled
-// and wrapped (plane.cc:43-57).
+// The uv integrator's shading: the hit's texture coordinates (hit_texcoord)
+// into the path's ray64 slot.
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_uv(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, double *ray64,
                                                        float4 *rad) {
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
-    const double PI = 3.14159265358979323846;  // nmath::PI
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t path = in.path[i];
         const float4 h = hit[i];
@@ -909,60 +984,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_uv(const SceneView S,
             continue;
         }
         double *q = ray64 + 6 * (size_t)path;
-        const d3 o = {q[0], q[1], q[2]}, d = {q[3], q[4], q[5]};
-        double tu = 0.0, tv = 0.0;
-        if (prim >= XT_PLANE_BASE) {
-            const XtPlane &pl = S.planes[prim - XT_PLANE_BASE];
-            const double us = S.obj_uv[2 * pl.object], vs = S.obj_uv[2 * pl.object + 1];
-            double t = plane64(o, d, pl);
-            if (!(t > 0.0)) t = (double)h.x;
-            const d3 pt = dadd(o, dscl(d, t));
-            const d3 v = {pl.point64[0], pl.point64[1], pl.point64[2]};
-            const d3 nn = dnorm(d3{pl.normal64[0], pl.normal64[1], pl.normal64[2]});
-            const d3 uvec = {nn.y, nn.z, -nn.x};
-            const d3 vvec = dcross(uvec, nn);
-            const d3 vp = dadd(v, pt);
-            tu = ddot(uvec, vp) * us;
-            tv = ddot(vvec, vp) * vs;
-            if (tu > 1.f) tu -= (float)(int)tu;
-            if (tv > 1.f) tv -= (float)(int)tv;
-            if (tu < -1.f) tu -= (float)(int)tu;
-            if (tv < -1.f) tv -= (float)(int)tv;
-            if (tu < 0.f) tu = 1.f + tu;
-            if (tv < 0.f) tv = 1.f + tv;
-        } else {
-            const XtPrim64 &p = S.prims64[prim];
-            if (XT_PRIM_TYPE(__float_as_uint(S.prims[3 * prim].w)) == XT_PRIM_TRIANGLE) {
-                double bc[3];
-                const double t = tri64(o, d, p, bc);
-                if (!(t > 0.0)) {  // the FP64 predicate disagrees: FP32 barycentrics
-                    bc[1] = h.z;
-                    bc[2] = h.w;
-                    bc[0] = 1.0 - bc[1] - bc[2];
-                }
-                const double *tc = S.tc64 + 6 * (size_t)prim;
-                tu = tc[0] * bc[0] + tc[2] * bc[1] + tc[4] * bc[2];
-                tv = tc[1] * bc[0] + tc[3] * bc[1] + tc[5] * bc[2];
-            } else {
-                const uint32_t obj = __float_as_uint(S.prims[3 * prim + 1].w);
-                const double us = S.obj_uv[2 * obj], vs = S.obj_uv[2 * obj + 1];
-                const d3 c = {p.a[0], p.a[1], p.a[2]};
-                const double r = p.b[0];
-                const double b = 2 * ddot(dsub(o, c), d);
-                const d3 nc = {-c.x, -c.y, -c.z};
-                const double cq = ddot(c, c) + ddot(o, o) + 2 * ddot(nc, o) - r * r;
-                const double disc = (b * b - 4 * cq);
-                const double s = sqrt(disc > 0.0 ? disc : 0.0);
-                const double t1 = (-b + s) / 2.0, t2 = (-b - s) / 2.0;
-                double t = t1 < t2 ? t1 : t2;
-                if (!(disc > 0.0 && t > XT_EPS64)) t = (double)h.x;
-                const d3 pt = dadd(o, dscl(d, t));
-                const double sg = t1 * t2 > 0. ? 1. : -1;
-                const d3 nn = {(pt.x - c.x) / r * sg, (pt.y - c.y) / r * sg, (pt.z - c.z) / r * sg};
-                tu = asin(nn.x / (us != 0.0f ? us : 1.0f)) / PI + 0.5;
-                tv = asin(nn.y / (vs != 0.0f ? vs : 1.0f)) / PI + 0.5;
-            }
-        }
+        double tu, tv;
+        hit_texcoord(S, prim, d3{q[0], q[1], q[2]}, d3{q[3], q[4], q[5]}, h, tu, tv);
         q[0] = tu;
         q[1] = tv;
         rad[path] = make_float4(0.f, 1.f, 0.f, 0.f);
@@ -994,7 +1017,7 @@ __device__ __forceinline__ float4 uv_update(float4 a, bool hit, double tu, doubl
 // object's emissive colour; rad.w marks a hit on an object with a material.
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_emission(const SceneView S, const Queue in,
                                                              const uint32_t *in_count, const float4 *hit,
-                                                             float4 *rad) {
+                                                             const double *ray64, float4 *rad) {
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -1008,6 +1031,14 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_emission(const SceneV
             if (mi != XT_NONE) {
                 const XtMaterial &m = S.materials[mi];
                 v = make_float4(m.ke[0], m.ke[1], m.ke[2], 1.f);
+                const uint32_t sm = S.mat_smp[3 * mi + 2];
+                if (sm != XT_NONE) {  // a sampled emissive channel, at the hit's texture coordinates
+                    const double *q = ray64 + 6 * (size_t)path;
+                    double tu, tv;
+                    hit_texcoord(S, prim, d3{q[0], q[1], q[2]}, d3{q[3], q[4], q[5]}, hit[i], tu, tv);
+                    const f3 c = sampler_eval(S, S.samplers[sm], tu, tv, 0.0);
+                    v = make_float4(c.x, c.y, c.z, 1.f);
+                }
             }
         }
         rad[path] = v;

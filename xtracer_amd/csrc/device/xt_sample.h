@@ -84,17 +84,4 @@ __device__ __forceinline__ f3 cscale(f3 a, float s) {
     return mk(fmaxf(a.x * s, 0.f), fmaxf(a.y * s, 0.f), fmaxf(a.z * s, 0.f));
 }
 
-// Gradient / colour environment (src/xtcore/sampler_gradient.cc:11-17,
-// sampler_col.cc:23-27); no environment is black.
-__device__ __forceinline__ f3 env_sample(const XtEnv &env, f3 dir) {
-    if (env.type == XT_ENV_GRADIENT) {
-        const f3 dn = norm3(dir);
-        const float t = 0.5f * dn.y + 1.0f;
-        const float s = 1.0f - t;
-        const f3 a = cscale(mk(env.a[0], env.a[1], env.a[2]), s);
-        const f3 b = cscale(mk(env.b[0], env.b[1], env.b[2]), t);
-        return mk(fmaxf(a.x + b.x, 0.f), fmaxf(a.y + b.y, 0.f), fmaxf(a.z + b.z, 0.f));
-    }
-    if (env.type == XT_ENV_COLOR) return mk(env.a[0], env.a[1], env.a[2]);
-    return mk(0.f, 0.f, 0.f);
-}
+

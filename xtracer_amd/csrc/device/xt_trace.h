@@ -24,6 +24,10 @@ struct SceneView {
     const float *vnormals;  // 9 per prim or null
     const double *tc64;     // 6 per prim: texcoords (u, v) of a triangle's corners
     const double *obj_uv;   // 2 per object: u_scale, v_scale
+    const float4 *texels;   // all textures' texels
+    const XtTexture *textures;
+    const XtSampler *samplers;
+    const uint32_t *mat_smp;  // 3 per material: sampler of diffuse / specular / emissive, or XT_NONE (colour)
     XtEnv env;
     float kb_lo[3], kb_hi[3];
     uint32_t n_prims, n_planes, n_oct;

@@ -99,6 +99,20 @@ Box64 tri_box(const dvec3 v[3]) {
 
 int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::string *err) {
     out = FlatScene();
+    for (const Image &im : sc.textures) {
+        out.textures.push_back(XtTexture{(uint32_t)(out.texels.size() / 4), im.width, im.height, 0u});
+        out.texels.insert(out.texels.end(), im.rgba.begin(), im.rgba.end());
+    }
+    auto add_sampler = [&](const SamplerDesc &sd) {
+        XtSampler x{};
+        x.kind = sd.kind;
+        x.bilinear = sd.bilinear;
+        for (int k = 0; k < 6; ++k) x.tex[k] = sd.tex[k];
+        std::memcpy(x.a, sd.a, sizeof x.a);
+        std::memcpy(x.b, sd.b, sizeof x.b);
+        out.samplers.push_back(x);
+        return (uint32_t)out.samplers.size() - 1;
+    };
     // materials, in handle order
     std::unordered_map<uint64_t, uint32_t> mat_index;
     for (auto &kv : sc.materials) {
@@ -116,12 +130,21 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
         x.exponent = m.exponent;
         mat_index[kv.first] = (uint32_t)out.materials.size();
         out.materials.push_back(x);
+        // a colour channel stays in kd / ks / ke; other samplers go to the table
+        for (int c = 0; c < 3; ++c)
+            out.mat_smp.push_back(m.smp[c].kind == SMP_NONE || m.smp[c].kind == SMP_COLOR ? XT_NO_MATERIAL
+                                                                                       : add_sampler(m.smp[c]));
         out.material_names.push_back(m.name);
         out.material_supported.push_back(m.unsupported.empty() ? 1 : 0);
     }
-    out.env.type = sc.env.type == ENV_GRADIENT ? XT_ENV_GRADIENT : sc.env.type == ENV_COLOR ? XT_ENV_COLOR : XT_ENV_NONE;
+    out.env.type = sc.env.type == ENV_GRADIENT ? XT_ENV_GRADIENT
+                   : sc.env.type == ENV_COLOR  ? XT_ENV_COLOR
+                   : sc.env.type == ENV_CUBEMAP ? XT_ENV_CUBEMAP
+                   : sc.env.type == ENV_ERP     ? XT_ENV_ERP
+                                                : XT_ENV_NONE;
     std::memcpy(out.env.a, sc.env.a, sizeof out.env.a);
     std::memcpy(out.env.b, sc.env.b, sizeof out.env.b);
+    out.env.smp = (sc.env.type == ENV_CUBEMAP || sc.env.type == ENV_ERP) ? add_sampler(sc.env.smp) : 0u;
     out.env_supported = sc.env.type != ENV_UNSUPPORTED;
 
     // per-geometry pre-ordered triangle lists, built once per mesh geometry

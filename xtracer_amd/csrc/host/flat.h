@@ -36,6 +36,10 @@ struct FlatScene {
     std::vector<uint8_t> material_supported;  // 0 when a sampler cannot be evaluated on the GPU
     XtEnv env{};
     bool env_supported = true;
+    std::vector<float> texels;          // 4 floats per texel, all textures
+    std::vector<XtTexture> textures;
+    std::vector<XtSampler> samplers;    // non-constant material channels and the environment
+    std::vector<uint32_t> mat_smp;      // 3 per material (diffuse, specular, emissive): sampler or XT_NO_MATERIAL
 
     // acceleration structures over the soup
     KdTree kd;

@@ -1,0 +1,220 @@
+#include "image_load.h"
+
+#include <zlib.h>
+
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iterator>
+
+namespace xt {
+
+namespace {
+
+uint32_t be32(const uint8_t *p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+uint32_t le32(const uint8_t *p) { return (uint32_t)p[3] << 24 | (uint32_t)p[2] << 16 | (uint32_t)p[1] << 8 | p[0]; }
+uint16_t le16(const uint8_t *p) { return (uint16_t)(p[1] << 8 | p[0]); }
+
+int fail(std::string *err, const std::string &msg, int code) {
+    if (err) *err = msg;
+    return code;
+}
+
+// channels 3 or 4, bytes row-major top-down -> byte / 255.f (img.cc:30-57)
+void to_float(const std::vector<uint8_t> &px, uint32_t w, uint32_t h, int ch, Image &out) {
+    out.width = w;
+    out.height = h;
+    out.rgba.resize((size_t)w * h * 4);
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        for (int c = 0; c < 3; ++c) out.rgba[4 * i + c] = px[ch * i + c] / 255.f;
+        out.rgba[4 * i + 3] = ch == 4 ? px[ch * i + 3] / 255.f : 1.f;
+    }
+}
+
+int paeth(int a, int b, int c) {
+    const int p = a + b - c, pa = p > a ? p - a : a - p, pb = p > b ? p - b : b - p, pc = p > c ? p - c : c - p;
+    return (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+}
+
+int load_png(const std::vector<uint8_t> &f, Image &out, std::string *err) {
+    size_t pos = 8;
+    uint32_t w = 0, h = 0;
+    int ch = 0;
+    std::vector<uint8_t> idat;
+    while (pos + 12 <= f.size()) {
+        const uint32_t n = be32(&f[pos]);
+        if (pos + 12 + (size_t)n > f.size()) return fail(err, "truncated PNG chunk", 2);
+        const uint8_t *type = &f[pos + 4], *data = &f[pos + 8];
+        if (!std::memcmp(type, "IHDR", 4)) {
+            if (n < 13) return fail(err, "bad PNG header", 2);
+            w = be32(data);
+            h = be32(data + 4);
+            const int depth = data[8], ctype = data[9];
+            if (depth != 8 || (ctype != 2 && ctype != 6) || data[10] || data[11] || data[12])
+                return fail(err, "PNG variant not decoded (8-bit RGB / RGBA, not interlaced)", 2);
+            ch = ctype == 2 ? 3 : 4;
+        } else if (!std::memcmp(type, "IDAT", 4)) {
+            idat.insert(idat.end(), data, data + n);
+        } else if (!std::memcmp(type, "IEND", 4)) {
+            break;
+        }
+        pos += 12 + (size_t)n;
+    }
+    if (!ch || !w || !h || w > (1u << 15) || h > (1u << 15)) return fail(err, "bad PNG", 2);
+    const size_t stride = (size_t)w * ch;
+    std::vector<uint8_t> raw((stride + 1) * h);
+    uLongf got = (uLongf)raw.size();
+    if (uncompress(raw.data(), &got, idat.data(), (uLong)idat.size()) != Z_OK || got != raw.size())
+        return fail(err, "PNG data does not inflate", 2);
+    std::vector<uint8_t> px(stride * h);
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint8_t filter = raw[y * (stride + 1)];
+        const uint8_t *in = &raw[y * (stride + 1) + 1];
+        uint8_t *row = &px[y * stride];
+        const uint8_t *up = y ? &px[(y - 1) * stride] : nullptr;
+        for (size_t i = 0; i < stride; ++i) {
+            const int a = i >= (size_t)ch ? row[i - ch] : 0, b = up ? up[i] : 0;
+            const int c = (up && i >= (size_t)ch) ? up[i - ch] : 0;
+            int pred;
+            switch (filter) {
+                case 0: pred = 0; break;
+                case 1: pred = a; break;
+                case 2: pred = b; break;
+                case 3: pred = (a + b) >> 1; break;
+                case 4: pred = paeth(a, b, c); break;
+                default: return fail(err, "bad PNG filter", 2);
+            }
+            row[i] = (uint8_t)(in[i] + pred);
+        }
+    }
+    to_float(px, w, h, ch, out);
+    return 0;
+}
+
+int load_tga(const std::vector<uint8_t> &f, Image &out, std::string *err) {
+    if (f.size() < 18) return fail(err, "truncated TGA", 2);
+    const int idlen = f[0], cmap = f[1], type = f[2], bits = f[16], desc = f[17];
+    const uint32_t w = le16(&f[12]), h = le16(&f[14]);
+    if (cmap || (type != 2 && type != 10) || (bits != 24 && bits != 32) || !w || !h)
+        return fail(err, "TGA variant not decoded (true colour, 24 / 32 bit)", 2);
+    const int ch = bits / 8;
+    const size_t n = (size_t)w * h;
+    std::vector<uint8_t> file_px(n * ch);
+    size_t pos = 18 + idlen, k = 0;
+    auto pixel = [&](size_t at) {  // BGR(A) -> RGB(A)
+        file_px[at * ch] = f[pos + 2];
+        file_px[at * ch + 1] = f[pos + 1];
+        file_px[at * ch + 2] = f[pos];
+        if (ch == 4) file_px[at * ch + 3] = f[pos + 3];
+    };
+    while (k < n) {
+        if (type == 2) {
+            if (pos + ch > f.size()) return fail(err, "truncated TGA", 2);
+            pixel(k++);
+            pos += ch;
+            continue;
+        }
+        if (pos >= f.size()) return fail(err, "truncated TGA", 2);
+        const int head = f[pos++], count = (head & 127) + 1;
+        if (head & 128) {
+            if (pos + ch > f.size()) return fail(err, "truncated TGA", 2);
+            for (int r = 0; r < count && k < n; ++r) pixel(k++);
+            pos += ch;
+        } else {
+            for (int r = 0; r < count && k < n; ++r) {
+                if (pos + ch > f.size()) return fail(err, "truncated TGA", 2);
+                pixel(k++);
+                pos += ch;
+            }
+        }
+    }
+    // rows are stored bottom-up unless the descriptor's top-origin bit is set
+    const bool top = (desc & 32) != 0;
+    std::vector<uint8_t> px(n * ch);
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint32_t src = top ? y : h - 1 - y;
+        std::memcpy(&px[(size_t)y * w * ch], &file_px[(size_t)src * w * ch], (size_t)w * ch);
+    }
+    to_float(px, w, h, ch, out);
+    return 0;
+}
+
+int load_bmp(const std::vector<uint8_t> &f, Image &out, std::string *err) {
+    if (f.size() < 54) return fail(err, "truncated BMP", 2);
+    const uint32_t off = le32(&f[10]), hsz = le32(&f[14]);
+    const int32_t wi = (int32_t)le32(&f[18]), hi = (int32_t)le32(&f[22]);
+    const int bits = le16(&f[28]);
+    const uint32_t comp = le32(&f[30]);
+    if (hsz < 40 || comp != 0 || (bits != 24 && bits != 32) || wi <= 0 || hi == 0)
+        return fail(err, "BMP variant not decoded (uncompressed 24 / 32 bit)", 2);
+    const uint32_t w = (uint32_t)wi, h = (uint32_t)(hi < 0 ? -hi : hi);
+    const int bpp = bits / 8, ch = bits == 32 ? 4 : 3;
+    const size_t row = ((size_t)w * bpp + 3) & ~(size_t)3;
+    if ((size_t)off + row * h > f.size()) return fail(err, "truncated BMP", 2);
+    std::vector<uint8_t> px((size_t)w * h * ch);
+    bool any_alpha = false;
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint32_t src = hi < 0 ? y : h - 1 - y;  // bottom-up unless the height is negative
+        const uint8_t *r = &f[off + src * row];
+        for (uint32_t x = 0; x < w; ++x) {
+            uint8_t *d = &px[((size_t)y * w + x) * ch];
+            d[0] = r[x * bpp + 2];
+            d[1] = r[x * bpp + 1];
+            d[2] = r[x * bpp];
+            if (ch == 4) {
+                d[3] = r[x * bpp + 3];
+                any_alpha = any_alpha || d[3] != 0;
+            }
+        }
+    }
+    // a 32-bit file whose alpha bytes are all zero is opaque
+    if (ch == 4 && !any_alpha)
+        for (size_t i = 0; i < (size_t)w * h; ++i) px[4 * i + 3] = 255;
+    to_float(px, w, h, ch, out);
+    return 0;
+}
+
+int load_ppm(const std::vector<uint8_t> &f, Image &out, std::string *err) {
+    size_t pos = 2;
+    long v[3];
+    for (int k = 0; k < 3; ++k) {
+        while (pos < f.size() && (std::isspace(f[pos]) || f[pos] == '#')) {
+            if (f[pos] == '#')
+                while (pos < f.size() && f[pos] != '\n') ++pos;
+            else
+                ++pos;
+        }
+        v[k] = 0;
+        if (pos >= f.size() || !std::isdigit(f[pos])) return fail(err, "bad PPM header", 2);
+        while (pos < f.size() && std::isdigit(f[pos])) v[k] = v[k] * 10 + (f[pos++] - '0');
+    }
+    ++pos;  // the single whitespace after maxval
+    if (v[2] != 255 || v[0] <= 0 || v[1] <= 0 || v[0] > 32768 || v[1] > 32768)
+        return fail(err, "PPM variant not decoded (maxval 255)", 2);
+    const uint32_t w = (uint32_t)v[0], h = (uint32_t)v[1];
+    if (pos + (size_t)w * h * 3 > f.size()) return fail(err, "truncated PPM", 2);
+    std::vector<uint8_t> px(f.begin() + pos, f.begin() + pos + (size_t)w * h * 3);
+    to_float(px, w, h, 3, out);
+    return 0;
+}
+
+}  // namespace
+
+int load_image(const std::string &path, Image &out, std::string *err) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in.good()) return fail(err, "cannot open " + path, 1);
+    std::vector<uint8_t> f((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    static const uint8_t png_sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    int rc;
+    if (f.size() >= 8 && !std::memcmp(f.data(), png_sig, 8)) rc = load_png(f, out, err);
+    else if (f.size() >= 2 && f[0] == 'B' && f[1] == 'M') rc = load_bmp(f, out, err);
+    else if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') rc = load_ppm(f, out, err);
+    else if (path.size() > 4 && (path.substr(path.size() - 4) == ".tga" || path.substr(path.size() - 4) == ".TGA"))
+        rc = load_tga(f, out, err);
+    else rc = fail(err, "image format not decoded: " + path, 2);
+    if (rc && err && err->find(path) == std::string::npos) *err += " (" + path + ")";
+    return rc;
+}
+
+}  // namespace xt

@@ -220,6 +220,66 @@ int load_geometry(Scene &sc, ncf::Node *p) {
     return 0;
 }
 
+// Loads texture `file` (relative to the scene's directory) into sc.textures,
+// flipped as the texture sampler's flip_x / flip_y ask (parseutil.cc:472-500,
+// nimg/transform.cc); returns its index or -1 with sc.error set.
+int32_t load_texture(Scene &sc, const std::string &file, bool flip_x, bool flip_y) {
+    Image im;
+    std::string err;
+    if (load_image(dir_of(sc.source) + file, im, &err)) {
+        sc.error = "texture " + file + ": " + err;
+        return -1;
+    }
+    const uint32_t w = im.width, h = im.height;
+    if (flip_x)
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w / 2; ++x)
+                for (int c = 0; c < 4; ++c)
+                    std::swap(im.rgba[4 * ((size_t)y * w + x) + c], im.rgba[4 * ((size_t)y * w + w - 1 - x) + c]);
+    if (flip_y)
+        for (uint32_t y = 0; y < h / 2; ++y)
+            for (uint32_t x = 0; x < 4 * w; ++x)
+                std::swap(im.rgba[(size_t)y * 4 * w + x], im.rgba[(size_t)(h - 1 - y) * 4 * w + x]);
+    sc.textures.push_back(std::move(im));
+    return (int32_t)sc.textures.size() - 1;
+}
+
+// deserialize_{rgba,gradient,texture,cubemap,erp} (parseutil.cc:147-203,
+// 472-500) into a SamplerDesc; `cfg` holds the sampler's properties.
+int parse_sampler(Scene &sc, const std::string &type, ncf::Node *cfg, SamplerDesc &out) {
+    out = SamplerDesc{};
+    if (type == "color") {
+        out.kind = SMP_COLOR;
+        col3_of(cfg, "value", out.a);
+    } else if (type == "gradient") {
+        out.kind = SMP_GRADIENT;
+        col3_of(cfg, "a", out.a);
+        col3_of(cfg, "b", out.b);
+    } else if (type == "texture") {
+        out.kind = SMP_TEXTURE;
+        const char *src = cfg->property("source"), *flt = cfg->property("filtering");
+        out.bilinear = flt && std::string(flt) == "bilinear" ? 1u : 0u;
+        out.tex[0] = load_texture(sc, src ? src : "", parse_bool(cfg->property("flip_x")),
+                                  parse_bool(cfg->property("flip_y")));
+        if (out.tex[0] < 0) return 1;
+    } else if (type == "erp") {
+        out.kind = SMP_ERP;
+        const char *src = cfg->property("source");
+        out.tex[0] = load_texture(sc, src ? src : "", false, false);
+        if (out.tex[0] < 0) return 1;
+    } else if (type == "cubemap") {
+        out.kind = SMP_CUBEMAP;
+        // posx right, posy top, posz front, negx left, negy bottom, negz back
+        static const char *names[6] = {"negx", "posx", "posy", "negy", "posz", "negz"};
+        for (int f = 0; f < 6; ++f) {
+            const char *src = cfg->property(names[f]);
+            out.tex[f] = load_texture(sc, src ? src : "", false, false);
+            if (out.tex[f] < 0) return 1;
+        }
+    }
+    return 0;
+}
+
 int load_material(Scene &sc, ncf::Node *p) {
     const char *type_c = p->property("type");
     std::string type = type_c ? type_c : "";
@@ -244,14 +304,14 @@ int load_material(Scene &sc, ncf::Node *p) {
         const char *st = e->property("type");
         std::string stype = st ? st : "";
         float *dst = nm == "diffuse" ? m.diffuse : nm == "specular" ? m.specular : nm == "emissive" ? m.emissive : nullptr;
+        const int slot = nm == "diffuse" ? 0 : nm == "specular" ? 1 : nm == "emissive" ? 2 : -1;
         if (nm == "emissive") m.is_emissive = true;
         if (stype == "color") {
             float c[3] = {0, 0, 0};
             col3_of(e, "value", c);
             if (dst) std::memcpy(dst, c, sizeof c);
-        } else if (dst) {
-            m.unsupported.push_back(nm + ":" + stype);
         }
+        if (slot >= 0 && parse_sampler(sc, stype, e, m.smp[slot])) return 1;
     }
     for (size_t i = 0; i < scalars->count_properties(); ++i) {
         const std::string &nm = scalars->property_name_at(i);
@@ -322,7 +382,8 @@ int load_scn(Scene &sc, const std::string &path, const std::vector<std::string> 
         sc.env.type = ENV_COLOR;
         col3_of(cfg, "value", sc.env.a);
     } else if (etype == "cubemap" || etype == "erp") {
-        sc.env.type = ENV_UNSUPPORTED;
+        sc.env.type = etype == "cubemap" ? ENV_CUBEMAP : ENV_ERP;
+        if (parse_sampler(sc, etype, cfg, sc.env.smp)) return 1;
     }
 
     const char *sections[] = {"camera", "geometry", "material", "texture", "object"};

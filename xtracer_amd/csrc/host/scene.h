@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include "image_load.h"
+
 #include "xt_vec.h"
 
 namespace xt {
@@ -55,6 +57,18 @@ struct Geometry {
 
 enum MatType : uint32_t { MAT_LAMBERT = 0, MAT_PHONG = 1, MAT_BLINNPHONG = 2, MAT_EMISSIVE = 3, MAT_DIELECTRIC = 4 };
 
+// A sampler of a material channel or of the environment
+// (src/xtcore/sampler_{col,tex,gradient,cubemap,erp}.cc). Textures index
+// Scene::textures; a cubemap's faces are in CUBEMAP_FACE order: left, right,
+// top, bottom, front, back (sampler_cubemap.h:11-19).
+enum SamplerKind : uint32_t { SMP_NONE = 0, SMP_COLOR = 1, SMP_TEXTURE = 2, SMP_GRADIENT = 3, SMP_CUBEMAP = 4, SMP_ERP = 5 };
+struct SamplerDesc {
+    SamplerKind kind = SMP_NONE;
+    float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};  // colour value, or gradient ends
+    int32_t tex[6] = {-1, -1, -1, -1, -1, -1};
+    uint32_t bilinear = 0;                      // texture filtering (parseutil.cc:489-495)
+};
+
 // Constant-colour material parameters. A sampler that is absent reads as
 // (0,0,0) and a scalar as 0 (src/xtcore/material.cc:58-70).
 struct Material {
@@ -64,6 +78,7 @@ struct Material {
     float diffuse[3] = {0, 0, 0}, specular[3] = {0, 0, 0}, emissive[3] = {0, 0, 0};
     float reflectance = 0, exponent = 0, ior = 0;
     bool is_emissive = false;           // an "emissive" sampler or scalar exists
+    SamplerDesc smp[3];                 // diffuse, specular, emissive
     std::vector<std::string> unsupported;  // samplers whose type the GPU path does not evaluate
 };
 
@@ -73,10 +88,11 @@ struct Object {
     uint64_t geometry = 0, material = 0;  // name handles
 };
 
-enum EnvType : uint32_t { ENV_NONE = 0, ENV_GRADIENT = 1, ENV_COLOR = 2, ENV_UNSUPPORTED = 3 };
+enum EnvType : uint32_t { ENV_NONE = 0, ENV_GRADIENT = 1, ENV_COLOR = 2, ENV_UNSUPPORTED = 3, ENV_CUBEMAP = 4, ENV_ERP = 5 };
 struct Environment {
     EnvType type = ENV_NONE;
     float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    SamplerDesc smp;  // cubemap / erp
 };
 
 struct Scene {
@@ -87,6 +103,7 @@ struct Scene {
     std::map<uint64_t, Material> materials;
     std::map<uint64_t, Object> objects;  // iteration order == scan order
     std::map<std::string, uint64_t> names;
+    std::vector<Image> textures;  // texture images the samplers read
     std::string error;
 
     uint64_t intern(const std::string &name);  // first-seen order handle

@@ -262,8 +262,14 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     XtMaterial *mats = nullptr;
     float *vn = nullptr;
     double *tc64 = nullptr, *obj_uv = nullptr;
+    float *texels = nullptr;
+    XtTexture *texs = nullptr;
+    XtSampler *smps = nullptr;
+    uint32_t *msmp = nullptr;
     std::vector<XtMaterial> mv = f.materials;
     if (mv.empty()) mv.push_back(XtMaterial{});
+    std::vector<uint32_t> ms = f.mat_smp;
+    ms.resize(3 * mv.size(), XT_NONE);  // the stand-in material samples nothing
     int rc = upload(d, &prims, f.prims, t, b);
     rc = rc ? rc : upload(d, &p64, f.prims64, t, b);
     rc = rc ? rc : upload(d, &planes, f.planes, t, b);
@@ -280,6 +286,10 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     }
     rc = rc ? rc : upload(d, &tc64, f.tc64, t, b);
     rc = rc ? rc : upload(d, &obj_uv, f.obj_uv_scale, t, b);
+    rc = rc ? rc : upload(d, &texels, f.texels, t, b);
+    rc = rc ? rc : upload(d, &texs, f.textures, t, b);
+    rc = rc ? rc : upload(d, &smps, f.samplers, t, b);
+    rc = rc ? rc : upload(d, &msmp, ms, t, b);
     if (!rc && !f.vnormals.empty()) rc = upload(d, &vn, f.vnormals, t, b);
     if (rc) {
         for (void *p : ds.allocs) hipFree(p);
@@ -301,6 +311,10 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     v.vnormals = vn;
     v.tc64 = tc64;
     v.obj_uv = obj_uv;
+    v.texels = (const float4 *)texels;
+    v.textures = texs;
+    v.samplers = smps;
+    v.mat_smp = msmp;
     v.env = f.env;
     for (int a = 0; a < 3; ++a) {
         v.kb_lo[a] = f.kd.bounds.lo[a];

@@ -62,6 +62,8 @@ struct XtMaterial {
     float exponent;
 };
 
+#define XT_ENV_CUBEMAP 3u
+#define XT_ENV_ERP 4u
 #define XT_ENV_NONE 0u
 #define XT_ENV_GRADIENT 1u
 #define XT_ENV_COLOR 2u
@@ -70,7 +72,26 @@ struct XtEnv {
     uint32_t type;
     float a[3];
     float b[3];
-    float pad;
+    uint32_t smp;  // cubemap / erp: index into the sampler table
+};
+
+// Samplers of material channels and the environment
+// (src/xtcore/sampler_{col,tex,gradient,cubemap,erp}.cc).
+#define XT_SMP_COLOR 1u
+#define XT_SMP_TEXTURE 2u
+#define XT_SMP_GRADIENT 3u
+#define XT_SMP_CUBEMAP 4u
+#define XT_SMP_ERP 5u
+struct XtSampler {
+    uint32_t kind;
+    uint32_t bilinear;  // texture filtering
+    int32_t tex[6];     // texture; erp; cubemap faces left, right, top, bottom, front, back
+    float a[3], b[3];   // colour, gradient ends
+};
+// A texture image: width * height float4 texels (byte / 255, row 0 at the
+// top) starting at `offset` in the texel array.
+struct XtTexture {
+    uint32_t offset, width, height, pad;
 };
 
 // kd-tree node, 8 bytes.

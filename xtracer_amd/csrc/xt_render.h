@@ -16,5 +16,12 @@ struct PassGroup {
 int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &lane,
                     hipStream_t st);
 int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace);
+
+// Integrators whose shading reads the camera ray in FP64 (the ray64 slot):
+// depth's distance, normal, uv and emission's texture coordinates.
+inline bool xt_needs_ray64(uint32_t integrator) {
+    return integrator == XT_INTEGRATOR_DEPTH || integrator == XT_INTEGRATOR_NORMAL ||
+           integrator == XT_INTEGRATOR_UV || integrator == XT_INTEGRATOR_EMISSION;
+}
 int xt_ensure_host_counters(xt_dev *d);
 

@@ -150,7 +150,8 @@ int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t
             hipLaunchKernelGGL(k_shade_uv, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64, w.rad);
             break;
         case XT_INTEGRATOR_EMISSION:
-            hipLaunchKernelGGL(k_shade_emission, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.rad);
+            hipLaunchKernelGGL(k_shade_emission, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64,
+                               w.rad);
             break;
         case XT_INTEGRATOR_DEPTH:
             hipLaunchKernelGGL(k_shade_depth, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, w.ray64,
@@ -222,8 +223,7 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
             const hipStream_t st = stream_of(k);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, G.lane.cnt,
                                (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
-            const bool fp64_rays = (G.integrator == XT_INTEGRATOR_DEPTH || G.integrator == XT_INTEGRATOR_NORMAL ||
-                                    G.integrator == XT_INTEGRATOR_UV) && r == 0;
+            const bool fp64_rays = xt_needs_ray64(G.integrator) && r == 0;
             rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
                                  time_trace && !n_st, st);
             if (rc) return rc;
@@ -264,8 +264,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         d->err = "pass too large";
         return XT_E_INVALID;
     }
-    const bool need64 = j.integrator == XT_INTEGRATOR_DEPTH || j.integrator == XT_INTEGRATOR_NORMAL ||
-                        j.integrator == XT_INTEGRATOR_UV;
+    const bool need64 = xt_needs_ray64(j.integrator);
     rc = xt_dev_ensure_wave(d, cap, need64);
     if (rc) return rc;
     rc = xt_dev_ensure_accum(d, (size_t)j.W * j.H);

@@ -361,8 +361,7 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
         groups[g].accel = grps[g].accel;
         groups[g].rounds = rounds;
         groups[g].ds = &d->scenes.at(grps[g].scene);
-        need64 = need64 || grps[g].integrator == XT_INTEGRATOR_DEPTH || grps[g].integrator == XT_INTEGRATOR_NORMAL ||
-                 grps[g].integrator == XT_INTEGRATOR_UV;
+        need64 = need64 || xt_needs_ray64(grps[g].integrator);
     }
     if (total >= (1ull << 31)) {
         d->err = "pass too large";
