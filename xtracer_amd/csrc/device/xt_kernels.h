@@ -718,9 +718,61 @@ __device__ __forceinline__ void hit_texcoord(const SceneView &S, uint32_t prim, 
 // leaves its hit_result's intensity at ColorRGBf's default 0, so the value
 // of the query is 0 whatever the refracted ray returns: the path ends with
 // L = 0 (the reference traces on and multiplies by 0).
+// Material channels of a sampled scene's pathtracer queue, ahead of
+// k_shade_pt: for queue slot i, chan[3i .. 3i+2] = the hit material's kd, ks
+// and ke (its samplers read at the hit's FP64 texture coordinates, or its
+// constant colours), or for a miss chan[3i] = the environment along the ray.
+// The samplers' FP64 code more than doubles a kernel's registers, so it runs
+// here and k_shade_pt keeps the register budget of an unsampled scene.
+static __global__ __launch_bounds__(XT_BLOCK) void k_channels(const SceneView S, const Queue in,
+                                                       const uint32_t *in_count, const float4 *hit, float4 *chan) {
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 h = hit[i];
+        const float4 a = in.r0[i], b = in.r1[i];
+        const uint32_t prim = __float_as_uint(h.y);
+        if (prim == XT_NONE) {
+            const f3 e = env_sample(S, mk(b.x, b.y, b.z));
+            chan[3 * (size_t)i] = make_float4(e.x, e.y, e.z, 0.f);
+            continue;
+        }
+        const uint32_t obj = prim >= XT_PLANE_BASE ? S.planes[prim - XT_PLANE_BASE].object
+                                                   : __float_as_uint(S.prims[3 * prim + 1].w);
+        const uint32_t mi = S.obj_material[obj];
+        const XtMaterial &m = S.materials[mi == XT_NONE ? 0 : mi];
+        f3 kd = mk(m.kd[0], m.kd[1], m.kd[2]), ks = mk(m.ks[0], m.ks[1], m.ks[2]);
+        f3 ke = mk(m.ke[0], m.ke[1], m.ke[2]);
+        const uint32_t *ms = S.mat_smp + 3 * (mi == XT_NONE ? 0 : mi);
+        const uint32_t s0 = ms[0], s1 = ms[1], s2 = ms[2];
+        if (s0 != XT_NONE || s1 != XT_NONE || s2 != XT_NONE) {
+            double tu, tv;
+            hit_texcoord(S, prim, d3{a.x, a.y, a.z}, d3{b.x, b.y, b.z}, h, tu, tv);
+            // one copy of the sampler code, run once per sampled channel
+#pragma unroll 1
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t si = c == 0 ? s0 : (c == 1 ? s1 : s2);
+                if (si == XT_NONE) continue;
+                const f3 v = sampler_eval(S, si, tu, tv, 0.0);
+                kd = c == 0 ? v : kd;
+                ks = c == 1 ? v : ks;
+                ke = c == 2 ? v : ke;
+            }
+        }
+        chan[3 * (size_t)i] = make_float4(kd.x, kd.y, kd.z, 0.f);
+        chan[3 * (size_t)i + 1] = make_float4(ks.x, ks.y, ks.z, 0.f);
+        chan[3 * (size_t)i + 2] = make_float4(ke.x, ke.y, ke.z, 0.f);
+    }
+}
+
+// SAMPLED: the material colours and the environment come from k_channels'
+// output `chan` (slot-indexed) instead of the material and environment
+// records.
+template <bool SAMPLED>
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, Queue out,
-                                                       uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad) {
+                                                       uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad,
+                                                       const float4 *chan) {
     __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -741,7 +793,14 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
             const uint32_t q = qword & 0xffffu, rdepth = qword >> 16;
             const uint32_t prim = __float_as_uint(h.y);
             if (prim == XT_NONE) {
-                const f3 L = cmul(T, env_sample(S, d));
+                f3 e;
+                if (SAMPLED) {
+                    const float4 c = chan[3 * (size_t)i];
+                    e = mk(c.x, c.y, c.z);
+                } else {
+                    e = env_constant(S.env, d);
+                }
+                const f3 L = cmul(T, e);
                 rad[path] = make_float4(L.x, L.y, L.z, 0.f);
             } else {
                 f3 p, N, g;
@@ -753,13 +812,11 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
                 // the hit's texture coordinates (material.cc:65-70)
                 f3 kd = mk(m.kd[0], m.kd[1], m.kd[2]), ks = mk(m.ks[0], m.ks[1], m.ks[2]);
                 f3 ke = mk(m.ke[0], m.ke[1], m.ke[2]);
-                const uint32_t *ms = S.mat_smp + 3 * (mi == XT_NONE ? 0 : mi);
-                if (ms[0] != XT_NONE || ms[1] != XT_NONE || ms[2] != XT_NONE) {
-                    double tu, tv;
-                    hit_texcoord(S, prim, d3{o.x, o.y, o.z}, d3{d.x, d.y, d.z}, h, tu, tv);
-                    if (ms[0] != XT_NONE) kd = sampler_eval(S, S.samplers[ms[0]], tu, tv, 0.0);
-                    if (ms[1] != XT_NONE) ks = sampler_eval(S, S.samplers[ms[1]], tu, tv, 0.0);
-                    if (ms[2] != XT_NONE) ke = sampler_eval(S, S.samplers[ms[2]], tu, tv, 0.0);
+                if (SAMPLED) {
+                    const float4 c0 = chan[3 * (size_t)i], c1 = chan[3 * (size_t)i + 1], c2 = chan[3 * (size_t)i + 2];
+                    kd = mk(c0.x, c0.y, c0.z);
+                    ks = mk(c1.x, c1.y, c1.z);
+                    ke = mk(c2.x, c2.y, c2.z);
                 }
                 if (m.type == XT_MAT_EMISSIVE) {
                     const f3 L = cmul(cmul(T, F), ke);
@@ -1036,7 +1093,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_emission(const SceneV
                     const double *q = ray64 + 6 * (size_t)path;
                     double tu, tv;
                     hit_texcoord(S, prim, d3{q[0], q[1], q[2]}, d3{q[3], q[4], q[5]}, hit[i], tu, tv);
-                    const f3 c = sampler_eval(S, S.samplers[sm], tu, tv, 0.0);
+                    const f3 c = sampler_eval(S, sm, tu, tv, 0.0);
                     v = make_float4(c.x, c.y, c.z, 1.f);
                 }
             }

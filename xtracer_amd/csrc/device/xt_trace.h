@@ -31,6 +31,7 @@ struct SceneView {
     XtEnv env;
     float kb_lo[3], kb_hi[3];
     uint32_t n_prims, n_planes, n_oct;
+    uint32_t sampled;  // 1 when a material channel or the environment has a sampler
 };
 
 struct RayF {

@@ -213,6 +213,8 @@ int load_image(const std::string &path, Image &out, std::string *err) {
     else if (path.size() > 4 && (path.substr(path.size() - 4) == ".tga" || path.substr(path.size() - 4) == ".TGA"))
         rc = load_tga(f, out, err);
     else rc = fail(err, "image format not decoded: " + path, 2);
+    // the device's texel addressing takes sides below 2^16
+    if (!rc && (out.width > 65535 || out.height > 65535)) rc = fail(err, "image larger than 65535 pixels a side", 2);
     if (rc && err && err->find(path) == std::string::npos) *err += " (" + path + ")";
     return rc;
 }

@@ -18,7 +18,8 @@ struct Image {
 };
 
 // 0: loaded; 1: the file cannot be read; 2: a format or variant this loader
-// does not decode (or one the reference rejects: 1- and 2-channel images).
+// does not decode (or one the reference rejects: 1- and 2-channel images), or
+// an image with a side above 65535 pixels.
 int load_image(const std::string &path, Image &out, std::string *err = nullptr);
 
 }  // namespace xt

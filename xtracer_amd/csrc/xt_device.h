@@ -39,6 +39,7 @@ struct WaveBuffers {
     uint32_t *flags = nullptr;
     Counters *cnt = nullptr;  // XT_MAX_LANES counter blocks
     double *ray64 = nullptr;
+    float4 *chan = nullptr;  // 3 per slot: sampled material colours (k_channels)
     std::vector<void *> allocs;
     uint64_t bytes = 0;
 };
@@ -83,7 +84,7 @@ static inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
     return (unsigned)g;
 }
 
-int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64);
+int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64, bool need_chan = false);
 int xt_dev_ensure_accum(xt_dev *d, size_t npix);
 void xt_dev_free_wave(xt_dev *d);
 void xt_dev_free_accum(xt_dev *d);

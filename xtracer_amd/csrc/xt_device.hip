@@ -78,8 +78,10 @@ void xt_dev_free_accum(xt_dev *d) {
     d->accum = AccumBuffers();
 }
 
-int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64) {
-    if (d->wave.cap >= cap && (!need64 || d->wave.ray64)) return XT_OK;
+int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64, bool need_chan) {
+    if (d->wave.cap >= cap && (!need64 || d->wave.ray64) && (!need_chan || d->wave.chan)) return XT_OK;
+    need64 = need64 || d->wave.ray64;  // keep what the buffers had
+    need_chan = need_chan || d->wave.chan;
     cap = std::max(cap, d->wave.cap);
     xt_dev_free_wave(d);
     WaveBuffers &w = d->wave;
@@ -100,6 +102,7 @@ int xt_dev_ensure_wave(xt_dev *d, size_t cap, bool need64) {
     rc = rc ? rc : dev_alloc(d, &w.flags, cap, t, b);
     rc = rc ? rc : dev_alloc(d, &w.cnt, XT_MAX_LANES, t, b);
     if (!rc && need64) rc = dev_alloc(d, &w.ray64, 6 * cap, t, b);
+    if (!rc && need_chan) rc = dev_alloc(d, &w.chan, 3 * cap, t, b);
     if (rc) {
         xt_dev_free_wave(d);
         return rc;
@@ -323,6 +326,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     v.n_prims = (uint32_t)f.prims.size();
     v.n_planes = (uint32_t)f.planes.size();
     v.n_oct = (uint32_t)f.oct_nodes.size();
+    v.sampled = f.samplers.empty() ? 0u : 1u;
     if (f.kd.nodes.empty()) {  // empty soup: a root leaf with no refs
         XtKdNode leaf;
         leaf.word = 3u;

@@ -19,6 +19,11 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace);
 
 // Integrators whose shading reads the camera ray in FP64 (the ray64 slot):
 // depth's distance, normal, uv and emission's texture coordinates.
+// The pathtracer over a scene with samplers shades from k_channels' output.
+inline bool xt_needs_chan(uint32_t integrator, const SceneView &v) {
+    return integrator == XT_INTEGRATOR_PATHTRACER && v.sampled;
+}
+
 inline bool xt_needs_ray64(uint32_t integrator) {
     return integrator == XT_INTEGRATOR_DEPTH || integrator == XT_INTEGRATOR_NORMAL ||
            integrator == XT_INTEGRATOR_UV || integrator == XT_INTEGRATOR_EMISSION;

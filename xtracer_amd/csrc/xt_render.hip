@@ -165,8 +165,22 @@ int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t
                 hipLaunchKernelGGL(k_shade_ao2, dim3(g), dim3(XT_BLOCK), 0, st, in, n_in, hit, w.rad);
             break;
         default:
-            hipLaunchKernelGGL(k_shade_pt, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out, n_out,
-                               w.ps0, w.ps1, w.rad);
+            // the sampler code more than doubles the kernel's registers:
+            // scenes without samplers take the variant that leaves it out
+            if (ds.view.sampled) {
+                if (!w.chan) {
+                    d->err = "sampled scene without channel buffers";
+                    return XT_E_INVALID;
+                }
+                float4 *chan = w.chan + 3 * L.off;
+                hipLaunchKernelGGL(k_channels, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, chan);
+                XT_LAUNCHED();
+                hipLaunchKernelGGL(k_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out,
+                                   n_out, w.ps0, w.ps1, w.rad, chan);
+            } else {
+                hipLaunchKernelGGL(k_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out,
+                                   n_out, w.ps0, w.ps1, w.rad, (const float4 *)nullptr);
+            }
     }
     XT_LAUNCHED();
     return XT_OK;
@@ -265,7 +279,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         return XT_E_INVALID;
     }
     const bool need64 = xt_needs_ray64(j.integrator);
-    rc = xt_dev_ensure_wave(d, cap, need64);
+    rc = xt_dev_ensure_wave(d, cap, need64, xt_needs_chan(j.integrator, ds->view));
     if (rc) return rc;
     rc = xt_dev_ensure_accum(d, (size_t)j.W * j.H);
     if (rc) return rc;

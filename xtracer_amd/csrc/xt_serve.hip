@@ -134,7 +134,7 @@ int serve_init(xt_dev *d) {
     if (rc) return rc;
     // wavefront buffers at the pass size up front, so that job buffers
     // admitted later cannot crowd them out of the budget
-    return xt_dev_ensure_wave(d, d->tuning.wave_paths, true);
+    return xt_dev_ensure_wave(d, d->tuning.wave_paths, true, true);
 }
 
 // Takes a pooled block of at least `need` bytes, or allocates one within the
@@ -346,7 +346,7 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
     std::vector<Segment> placed(segs);
     std::vector<PassGroup> groups(grps.size());
     size_t total = 0;
-    bool need64 = false;
+    bool need64 = false, need_chan = false;
     for (size_t g = 0; g < grps.size(); ++g) {
         groups[g].lane.off = total;
         uint32_t rounds = 0;
@@ -362,12 +362,13 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
         groups[g].rounds = rounds;
         groups[g].ds = &d->scenes.at(grps[g].scene);
         need64 = need64 || xt_needs_ray64(grps[g].integrator);
+        need_chan = need_chan || xt_needs_chan(grps[g].integrator, groups[g].ds->view);
     }
     if (total >= (1ull << 31)) {
         d->err = "pass too large";
         return XT_E_INVALID;
     }
-    int rc = xt_dev_ensure_wave(d, total, need64);
+    int rc = xt_dev_ensure_wave(d, total, need64, need_chan);
     if (rc) return rc;
     WaveBuffers &w = d->wave;
     for (size_t g = 0; g < grps.size(); ++g) groups[g].lane.cnt = w.cnt + g;
