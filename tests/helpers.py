@@ -226,3 +226,70 @@ def texture_scene(path, camera="pinhole", variant="emit", env="erp"):
         geo, objs, mats = {}, {}, {}
     scn.write(path, geometry=geo, objects=objs, materials=mats, camera=CAMERAS[camera], env=envd)
     return path
+
+
+def external_scene(path, camera="pinhole"):
+    """An object imported from an OBJ file with MTL materials (prefix
+    "ext_"): shapes split by `o` / `g`, `usemtl` switching inside a group
+    (the shape takes its first face's material; a group whose earlier faces
+    tinyobj drops), quads, vt / vn indices,
+    colour, emissive, textured (map_Kd, map_Ke, a backslash path) and
+    repeated material names; beside it a procedural icosahedron (scaled and
+    moved) and a gen(plane) of resolution 3."""
+    import os
+    from tests import imgfiles as I
+    from tests import scn
+    d = os.path.dirname(str(path))
+    os.makedirs(os.path.join(d, "maps"), exist_ok=True)
+    rng = np.random.default_rng(29)
+    I.png(os.path.join(d, "maps", "wood.png"), rng.integers(0, 256, (6, 9, 3)).astype(np.uint8))
+    I.tga(os.path.join(d, "maps", "glow.tga"), rng.integers(0, 256, (5, 4, 3)).astype(np.uint8))
+    with open(os.path.join(d, "model.mtl"), "w") as f:
+        f.write("# materials\nnewmtl red\nKd 0.8 0.1 0.1\nKs 0.2 0.2 0.2\nNs 12\n\n"
+                "newmtl shiny\nKd 0.2 0.3 0.6\nKs 0.5 0.5 0.4\nNs 40\nNi 1.5\n"
+                "newmtl lamp\nKd 0.1 0.1 0.1\nKe 3 2.5 2\n"
+                "newmtl wood\nKd 1 1 1\nmap_Kd maps\\wood.png\nKs 0.1 0.1 0.1\n"
+                "newmtl glow\nmap_Ke maps/glow.tga\n"
+                "newmtl red\nKd 0 1 0\n")
+    lines = ["mtllib model.mtl"]
+    k = 0
+    for gx in range(4):
+        for gz in range(3):
+            x0, z0 = -2.0 + gx * 1.1, -0.5 + gz * 1.0
+            y = 0.3 * ((gx + gz) % 3)
+            lines += [f"v {x0:.3f} {y:.3f} {z0:.3f}", f"v {x0 + 0.9:.3f} {y + 0.1:.3f} {z0:.3f}",
+                      f"v {x0 + 0.9:.3f} {y + 0.5:.3f} {z0 + 0.8:.3f}", f"v {x0:.3f} {y + 0.4:.3f} {z0 + 0.8:.3f}"]
+            k += 4
+    lines += ["vt 0 0", "vt 1 0", "vt 1 1", "vt 0 1", "vt 0.25 0.75", "vt 2.5 -1.5", "vn 0 1 0", "vn 0.1 0.9 0.1"]
+    quads = [[1 + 4 * q + c for c in range(4)] for q in range(12)]
+    def face(q, uv=True, n=False):
+        if uv and n:
+            return "f " + " ".join(f"{v}/{1 + c % 6}/{1 + c % 2}" for c, v in enumerate(q))
+        if uv:
+            return "f " + " ".join(f"{v}/{1 + (c + 2) % 6}" for c, v in enumerate(q))
+        return "f " + " ".join(str(v) for v in q)
+    lines += ["o floor", "usemtl red", face(quads[0]), face(quads[1], n=True),
+              "usemtl wood", face(quads[2]), face(quads[3], uv=True, n=True),
+              "g lamps", "usemtl lamp", face(quads[4], uv=False),
+              "g mixed", "usemtl glow", face(quads[5]), face(quads[6]), "usemtl shiny", face(quads[7]),
+              "g dropped", "usemtl red", face(quads[8]), "usemtl shiny",
+              "o tail", "usemtl wood", face(quads[9]), "f -4/-1 -3/-2 -2/-3", face(quads[11], n=True)]
+    with open(os.path.join(d, "model.obj"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    text = open(scn.write(path, geometry={
+        "ground": dict(type="plane", normal=(0, 1, 0), distance=-0.2),
+        "ball": dict(type="sphere", position=(1.6, 0.5, -1.2), radius=0.4)},
+        objects={"ground": ("ground", "grey"), "ball": ("ball", "grey")},
+        materials={"grey": dict(type="lambert", kd=(0.5, 0.5, 0.5))},
+        camera=CAMERAS[camera], env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))).read()
+    text = text.replace("geometry = {\n", "geometry = {\n\tico = {\n\t\ttype = mesh\n\t\tsource = gen(icosahedron)\n"
+                        "\t\tmodifiers = {\n\t\t\tscale = vec3(0.4,0.4,0.4)\n\t\t\ttranslation = vec3(-1.4,1.2,-1.0)\n"
+                        "\t\t}\n\t}\n\tgrid = {\n\t\ttype = mesh\n\t\tsource = gen(plane)\n\t\tresolution = 3\n"
+                        "\t\tmodifiers = {\n\t\t\tscale = vec3(1.5,1,1.5)\n\t\t\ttranslation = vec3(0.3,1.6,1.2)\n"
+                        "\t\t}\n\t}\n", 1)
+    text = text.replace("object = {\n", "object = {\n\tthing = {\n\t\tsource = model.obj\n\t\tprefix = ext_\n\t}\n"
+                        "\tico = {\n\t\tgeometry = ico\n\t\tmaterial = grey\n\t}\n"
+                        "\tgrid = {\n\t\tgeometry = grid\n\t\tmaterial = grey\n\t}\n", 1)
+    with open(path, "w") as f:
+        f.write(text)
+    return str(path)

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <unordered_map>
+#include <algorithm>
 #include <fstream>
 #include <sstream>
 
@@ -65,6 +67,8 @@ void col3_of(const ncf::Node *node, const char *name, float out[3]) {
             out[2] = c;
         }
     }
+    // the ColorRGBf constructor keeps channels at or above 0 (color.cc:6-10)
+    for (int k = 0; k < 3; ++k) out[k] = out[k] < 0.f ? 0.f : out[k];
 }
 
 std::string dir_of(const std::string &path) {
@@ -159,6 +163,56 @@ int apply_modifiers(Scene &sc, MeshData &m, ncf::Node *mods) {
     return 0;
 }
 
+// gen(icosahedron) (lib/nmesh/icosahedron.cc:10-70): the twelve vertices
+// (+-1, +-k, 0) and their cyclic permutations, k the golden ratio rounded to
+// float, normalised in FP64 and stored as float; the twenty faces of the
+// icosahedron in the generator's order. The generator leaves the faces'
+// normal and texcoord indices unset; here they are absent.
+void gen_icosahedron(MeshData &m) {
+    const double k = (float)((1.f + std::sqrt(5.0)) / 2.f);
+    const double c[12][3] = {{-1, k, 0}, {1, k, 0}, {-1, -k, 0}, {1, -k, 0}, {0, -1, k}, {0, 1, k},
+                             {0, -1, -k}, {0, 1, -k}, {k, 0, -1}, {k, 0, 1}, {-k, 0, -1}, {-k, 0, 1}};
+    for (const auto &v : c) {
+        const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int a = 0; a < 3; ++a) m.v.push_back((float)(v[a] / len));
+    }
+    static const int32_t faces[60] = {0, 11, 5, 0, 5, 1,  0, 1, 7,   0, 7, 10, 0, 10, 11, 1, 5, 9,  5, 11, 4,
+                                      11, 10, 2, 10, 7, 6, 7, 1, 8,  3, 9, 4,  3, 4, 2,   3, 2, 6,  3, 6, 8,
+                                      3, 8, 9,  4, 9, 5,  2, 4, 11,  6, 2, 10, 8, 6, 7,   9, 8, 1};
+    m.iv.assign(faces, faces + 60);
+    m.in.assign(60, -1);
+    m.it.assign(60, -1);
+}
+
+// gen(plane) (lib/nmesh/plane.cc:11-60): a unit square in y = 0 centred on
+// the origin, (res + 1)^2 vertices stepped by 1/res in FP64, normal +y,
+// texcoords x + 0.5, z + 0.5; two triangles per cell (v0, v1, v2), (v0, v2,
+// v3) with v1 the next row.
+void gen_plane(MeshData &m, size_t res) {
+    double px = -0.5, pz = -0.5;
+    for (size_t i = 0; i < res + 1; ++i) {
+        for (size_t j = 0; j < res + 1; ++j) {
+            m.v.insert(m.v.end(), {(float)px, 0.f, (float)pz});
+            m.n.insert(m.n.end(), {0.f, 1.f, 0.f});
+            m.uv.insert(m.uv.end(), {(float)(px + 0.5), (float)(pz + 0.5)});
+            px += 1.0 / res;
+        }
+        px = -0.5;
+        pz += 1.0 / res;
+    }
+    for (size_t i = 0; i < res; ++i) {
+        for (size_t j = 0; j < res; ++j) {
+            const int32_t v0 = (int32_t)(j + i * (res + 1)), v1 = v0 + (int32_t)res + 1, v2 = v0 + (int32_t)res + 2,
+                          v3 = v0 + 1;
+            for (int32_t v : {v0, v1, v2, v0, v2, v3}) {
+                m.iv.push_back(v);
+                m.in.push_back(v);
+                m.it.push_back(v);
+            }
+        }
+    }
+}
+
 int load_geometry(Scene &sc, ncf::Node *p) {
     const char *type_c = p->property("type");
     std::string type = type_c ? type_c : "";
@@ -200,13 +254,21 @@ int load_geometry(Scene &sc, ncf::Node *p) {
         std::string f = src ? src : "";
         char gen[200] = {0};
         if (std::sscanf(f.c_str(), "gen(%199[a-zA-Z])", gen) > 0) {
-            sc.error = "procedural mesh generators are not supported: " + f;
-            return 1;
-        }
-        std::string err;
-        if (read_obj(dir_of(sc.source) + f, g.mesh, &err)) {
-            sc.error = "failed to load mesh " + f + ": " + err;
-            return 1;
+            // procedural meshes (parseutil.cc:295-314); an unknown generator
+            // leaves the mesh empty
+            if (!std::strcmp(gen, "icosahedron")) {
+                gen_icosahedron(g.mesh);
+            } else if (!std::strcmp(gen, "plane")) {
+                const char *r = p->property("resolution");
+                long res = r ? std::strtol(r, nullptr, 0) : 0;
+                gen_plane(g.mesh, res < 1 ? 1 : (size_t)res);
+            }
+        } else {
+            std::string err;
+            if (read_obj(dir_of(sc.source) + f, g.mesh, &err)) {
+                sc.error = "failed to load mesh " + f + ": " + err;
+                return 1;
+            }
         }
         if (p->has_group("modifiers"))
             if (apply_modifiers(sc, g.mesh, p->group("modifiers"))) return 1;
@@ -223,11 +285,11 @@ int load_geometry(Scene &sc, ncf::Node *p) {
 // Loads texture `file` (relative to the scene's directory) into sc.textures,
 // flipped as the texture sampler's flip_x / flip_y ask (parseutil.cc:472-500,
 // nimg/transform.cc); returns its index or -1 with sc.error set.
-int32_t load_texture(Scene &sc, const std::string &file, bool flip_x, bool flip_y) {
+int32_t load_texture_at(Scene &sc, const std::string &path, bool flip_x, bool flip_y) {
     Image im;
     std::string err;
-    if (load_image(dir_of(sc.source) + file, im, &err)) {
-        sc.error = "texture " + file + ": " + err;
+    if (load_image(path, im, &err)) {
+        sc.error = "texture " + path + ": " + err;
         return -1;
     }
     const uint32_t w = im.width, h = im.height;
@@ -242,6 +304,10 @@ int32_t load_texture(Scene &sc, const std::string &file, bool flip_x, bool flip_
                 std::swap(im.rgba[(size_t)y * 4 * w + x], im.rgba[(size_t)(h - 1 - y) * 4 * w + x]);
     sc.textures.push_back(std::move(im));
     return (int32_t)sc.textures.size() - 1;
+}
+
+int32_t load_texture(Scene &sc, const std::string &file, bool flip_x, bool flip_y) {
+    return load_texture_at(sc, dir_of(sc.source) + file, flip_x, flip_y);
 }
 
 // deserialize_{rgba,gradient,texture,cubemap,erp} (parseutil.cc:147-203,
@@ -325,11 +391,119 @@ int load_material(Scene &sc, ncf::Node *p) {
     return 0;
 }
 
-int load_object(Scene &sc, ncf::Node *p) {
-    if (p->has_property("source")) {
-        sc.error = "external object sources are not supported: " + p->name();
-        return 1;
+// A material channel of an OBJ's MTL material (create_sampler,
+// parseutil.cc:566-586): its texture, relative to the OBJ's directory with
+// '\\' read as '/', mirrored left to right; or its colour.
+int mtl_channel(Scene &sc, const std::string &dir, const std::string &tex, const float col[3], float dst[3],
+                SamplerDesc &smp) {
+    smp = SamplerDesc{};
+    if (!tex.empty()) {
+        std::string path = dir + tex;
+        std::replace(path.begin(), path.end(), '\\', '/');
+        smp.kind = SMP_TEXTURE;
+        smp.tex[0] = load_texture_at(sc, path, true, false);
+        for (int k = 0; k < 3; ++k) dst[k] = 0.f;
+        return smp.tex[0] < 0 ? 1 : 0;
     }
+    smp.kind = SMP_COLOR;
+    for (int k = 0; k < 3; ++k) smp.a[k] = dst[k] = col[k] < 0.f ? 0.f : col[k];
+    return 0;
+}
+
+// One shape of an OBJ file as a mesh: only the attributes its triangles use,
+// in first-use order, with the indices remapped (the values are unchanged).
+void shape_mesh(const ObjFile &o, const ObjShape &sh, MeshData &m) {
+    auto remap = [](const std::vector<int32_t> &idx, const std::vector<float> &src, int width,
+                    std::vector<float> &dst_vals, std::vector<int32_t> &dst_idx) {
+        std::unordered_map<int32_t, int32_t> seen;
+        const int32_t count = (int32_t)(src.size() / width);
+        dst_idx.reserve(idx.size());
+        for (int32_t i : idx) {
+            if (i < 0 || i >= count) {
+                dst_idx.push_back(-1);
+                continue;
+            }
+            auto it = seen.find(i);
+            if (it == seen.end()) {
+                it = seen.emplace(i, (int32_t)(dst_vals.size() / width)).first;
+                dst_vals.insert(dst_vals.end(), src.begin() + (size_t)i * width, src.begin() + (size_t)(i + 1) * width);
+            }
+            dst_idx.push_back(it->second);
+        }
+    };
+    remap(sh.iv, o.v, 3, m.v, m.iv);
+    remap(sh.in, o.n, 3, m.n, m.in);
+    remap(sh.it, o.uv, 2, m.uv, m.it);
+}
+
+// An object imported from an OBJ file with its MTL materials
+// (create_object(Scene*, filepath, prefix), parseutil.cc:588-682). Each MTL
+// material becomes `prefix + name`: emissive when it has an emission colour
+// above 0 or an emission texture, Blinn-Phong otherwise (diffuse, specular,
+// ior = Ni, exponent = Ns); a repeated name keeps its first material. Each
+// shape k (from 1) becomes geometry and object `k + prefix + shape name`
+// with the material of its first face, taken, as the reference does, by
+// the face's material index among the distinct names. A file that cannot be
+// read adds nothing (the reference logs a warning and goes on).
+int load_external_object(Scene &sc, ncf::Node *p) {
+    const char *src = p->property("source"), *pre = p->property("prefix");
+    const std::string path = dir_of(sc.source) + (src ? src : "");
+    const std::string prefix = pre ? pre : "";
+    const std::string dir = dir_of(path);
+    ObjFile o;
+    std::string err;
+    if (read_obj_file(path, dir, o, &err)) return 0;
+    std::vector<uint64_t> matids;
+    for (const ObjMaterial &om : o.materials) {
+        const std::string name = prefix + om.name;
+        const uint64_t id = sc.intern(name);
+        if (std::find(matids.begin(), matids.end(), id) != matids.end()) continue;
+        matids.push_back(id);
+        Material m;
+        m.name = name;
+        m.handle = id;
+        const bool emissive = om.ke[0] > 0.f || om.ke[1] > 0.f || om.ke[2] > 0.f || !om.map_ke.empty();
+        if (emissive) {
+            m.type = MAT_EMISSIVE;
+            m.is_emissive = true;
+            if (mtl_channel(sc, dir, om.map_ke, om.ke, m.emissive, m.smp[2])) return 1;
+        } else {
+            m.type = MAT_BLINNPHONG;
+            if (mtl_channel(sc, dir, om.map_kd, om.kd, m.diffuse, m.smp[0])) return 1;
+            if (mtl_channel(sc, dir, om.map_ks, om.ks, m.specular, m.smp[1])) return 1;
+            m.ior = om.ior;
+            m.exponent = om.shininess;
+        }
+        sc.materials[id] = m;
+    }
+    int count = 0;
+    for (const ObjShape &sh : o.shapes) {
+        ++count;
+        const std::string name = std::to_string(count) + prefix + sh.name;
+        const uint64_t id = sc.intern(name);
+        const int mi = sh.mat.empty() ? -1 : sh.mat[0];
+        if (mi < 0 || (size_t)mi >= matids.size()) {
+            sc.error = "shape " + name + " of " + path + " has no material";
+            return 1;
+        }
+        Geometry g;
+        g.name = name;
+        g.handle = id;
+        g.type = GEOM_MESH;
+        shape_mesh(o, sh, g.mesh);
+        sc.geometries[id] = std::move(g);
+        Object ob;
+        ob.name = name;
+        ob.handle = id;
+        ob.geometry = id;
+        ob.material = matids[mi];
+        sc.objects[id] = ob;
+    }
+    return 0;
+}
+
+int load_object(Scene &sc, ncf::Node *p) {
+    if (p->has_property("source")) return load_external_object(sc, p);
     Object o;
     o.name = p->name();
     o.handle = sc.intern(p->name());
@@ -406,7 +580,8 @@ int load_scn(Scene &sc, const std::string &path, const std::vector<std::string> 
 }
 
 namespace {
-// Parses "v", "v/vt", "v//vn" or "v/vt/vn" into 0-based indices.
+// Parses "v", "v/vt", "v//vn" or "v/vt/vn" into 0-based indices (tinyobj's
+// parseTriple / fixIndex: negative indices count from the end, 0 reads 0).
 bool parse_corner(const char *s, int nv, int nt, int nn, int out[3]) {
     int vals[3] = {0, 0, 0};
     bool has[3] = {false, false, false};
@@ -431,41 +606,134 @@ bool parse_corner(const char *s, int nv, int nt, int nn, int out[3]) {
             out[i] = -1;
             continue;
         }
-        int x = vals[i];
-        out[i] = x > 0 ? x - 1 : counts[i] + x;
+        const int x = vals[i];
+        out[i] = x > 0 ? x - 1 : (x == 0 ? 0 : counts[i] + x);
     }
     return has[0];
 }
+
+bool is_space(char c) { return c == ' ' || c == '\t'; }
+
+// The line without its trailing newline, and the offset of its first token.
+const char *line_token(std::string &line) {
+    while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
+    const char *t = line.c_str();
+    t += std::strspn(t, " \t");
+    return t;
+}
+
+// First whitespace-delimited word after `t` (sscanf "%s").
+std::string first_word(const char *t) {
+    t += std::strspn(t, " \t\r\n");
+    return std::string(t, std::strcspn(t, " \t\r\n"));
+}
+
+void read3(const char *t, float out[3]) {
+    out[0] = out[1] = out[2] = 0.f;
+    std::sscanf(t, "%f %f %f", &out[0], &out[1], &out[2]);
+}
+
+// LoadMtl (tinyobjloader 1.0): `newmtl` starts a material and stores the
+// previous one when it has a name; the last one is stored at the end in any
+// case, so an unreadable file still yields one unnamed default material.
+// The first material of a name is the one `usemtl` finds.
+void read_mtl(const std::string &path, std::vector<ObjMaterial> &mats, std::map<std::string, int> &index) {
+    std::ifstream in(path);
+    ObjMaterial m;
+    std::string line;
+    auto flush = [&]() {
+        index.insert({m.name, (int)mats.size()});
+        mats.push_back(m);
+    };
+    while (in.good() && std::getline(in, line)) {
+        while (!line.empty() && (line.back() == ' ' || line.back() == '\t')) line.pop_back();
+        const char *t = line_token(line);
+        if (!*t || *t == '#') continue;
+        if (!std::strncmp(t, "newmtl", 6) && is_space(t[6])) {
+            if (!m.name.empty()) flush();
+            m = ObjMaterial();
+            m.name = first_word(t + 7);
+        } else if (t[0] == 'K' && t[1] == 'd' && is_space(t[2])) {
+            read3(t + 2, m.kd);
+        } else if (t[0] == 'K' && t[1] == 's' && is_space(t[2])) {
+            read3(t + 2, m.ks);
+        } else if (t[0] == 'K' && t[1] == 'e' && is_space(t[2])) {
+            read3(t + 2, m.ke);
+        } else if (t[0] == 'N' && t[1] == 'i' && is_space(t[2])) {
+            m.ior = (float)std::strtod(t + 2, nullptr);
+        } else if (t[0] == 'N' && t[1] == 's' && is_space(t[2])) {
+            m.shininess = (float)std::strtod(t + 2, nullptr);
+        } else if (!std::strncmp(t, "map_Kd", 6) && is_space(t[6])) {
+            m.map_kd = t + 7;
+        } else if (!std::strncmp(t, "map_Ks", 6) && is_space(t[6])) {
+            m.map_ks = t + 7;
+        } else if (!std::strncmp(t, "map_Ke", 6) && is_space(t[6])) {
+            m.map_ke = t + 7;
+        }
+    }
+    flush();
+}
 }  // namespace
 
-int read_obj(const std::string &path, MeshData &m, std::string *err) {
+int read_obj_file(const std::string &path, const std::string &mtl_dir, ObjFile &o, std::string *err) {
     std::ifstream in(path);
     if (!in.good()) {
         if (err) *err = "cannot open " + path;
         return 1;
     }
+    std::map<std::string, int> mat_index;
+    std::vector<std::vector<int>> group;  // faces not yet in a shape: 3 ints per corner
+    std::string name;
+    int material = -1;
+    ObjShape shape;
+    // exportFaceGroupToShape: the pending faces, fan-triangulated, join the
+    // shape with the current material; false when there were none
+    auto flush_faces = [&]() -> bool {
+        if (group.empty()) return false;
+        for (const std::vector<int> &f : group) {
+            const size_t corners = f.size() / 3;
+            for (size_t k = 2; k < corners; ++k) {
+                const size_t idx[3] = {0, k - 1, k};
+                for (size_t c : idx) {
+                    shape.iv.push_back(f[3 * c]);
+                    shape.it.push_back(f[3 * c + 1]);
+                    shape.in.push_back(f[3 * c + 2]);
+                }
+                shape.mat.push_back(material);
+            }
+        }
+        shape.name = name;
+        return true;
+    };
+    // a `g` / `o` line or the end: the shape is kept only when faces were
+    // pending (faces an earlier `usemtl` moved into it are dropped otherwise)
+    auto end_shape = [&]() {
+        if (flush_faces()) o.shapes.push_back(shape);
+        shape = ObjShape();
+        group.clear();
+    };
     std::string line;
     std::vector<int> face;
     while (std::getline(in, line)) {
-        const char *s = line.c_str();
-        while (*s == ' ' || *s == '\t') ++s;
-        if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
+        const char *s = line_token(line);
+        if (!*s || *s == '#') continue;
+        if (s[0] == 'v' && is_space(s[1])) {
             float x = 0, y = 0, z = 0;
             std::sscanf(s + 2, "%f %f %f", &x, &y, &z);
-            m.v.insert(m.v.end(), {x, y, z});
-        } else if (s[0] == 'v' && s[1] == 'n') {
+            o.v.insert(o.v.end(), {x, y, z});
+        } else if (s[0] == 'v' && s[1] == 'n' && is_space(s[2])) {
             float x = 0, y = 0, z = 0;
             std::sscanf(s + 3, "%f %f %f", &x, &y, &z);
-            m.n.insert(m.n.end(), {x, y, z});
-        } else if (s[0] == 'v' && s[1] == 't') {
+            o.n.insert(o.n.end(), {x, y, z});
+        } else if (s[0] == 'v' && s[1] == 't' && is_space(s[2])) {
             float u = 0, v = 0;
             std::sscanf(s + 3, "%f %f", &u, &v);
-            m.uv.insert(m.uv.end(), {u, v});
-        } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
+            o.uv.insert(o.uv.end(), {u, v});
+        } else if (s[0] == 'f' && is_space(s[1])) {
             std::istringstream ss(s + 2);
             std::string tok;
             face.clear();
-            const int nv = (int)m.v.size() / 3, nt = (int)m.uv.size() / 2, nn = (int)m.n.size() / 3;
+            const int nv = (int)o.v.size() / 3, nt = (int)o.uv.size() / 2, nn = (int)o.n.size() / 3;
             while (ss >> tok) {
                 int c[3];
                 if (!parse_corner(tok.c_str(), nv, nt, nn, c)) {
@@ -474,16 +742,47 @@ int read_obj(const std::string &path, MeshData &m, std::string *err) {
                 }
                 face.insert(face.end(), {c[0], c[1], c[2]});
             }
-            const size_t corners = face.size() / 3;
-            for (size_t k = 2; k < corners; ++k) {  // fan triangulation, as tinyobj does
-                const size_t idx[3] = {0, k - 1, k};
-                for (size_t j : idx) {
-                    m.iv.push_back(face[3 * j]);
-                    m.it.push_back(face[3 * j + 1]);
-                    m.in.push_back(face[3 * j + 2]);
-                }
+            // two corners make no triangle (but count as a pending face);
+            // fewer have no meaning
+            if (face.size() < 6) {
+                if (err) *err = "face with fewer than two corners: " + line;
+                return 1;
             }
+            group.push_back(face);
+        } else if (!std::strncmp(s, "usemtl", 6) && is_space(s[6])) {
+            auto it = mat_index.find(first_word(s + 7));
+            const int id = it == mat_index.end() ? -1 : it->second;
+            if (id != material) {
+                flush_faces();
+                group.clear();
+                material = id;
+            }
+        } else if (!std::strncmp(s, "mtllib", 6) && is_space(s[6])) {
+            read_mtl(mtl_dir + first_word(s + 7), o.materials, mat_index);
+        } else if (s[0] == 'g' && is_space(s[1])) {
+            end_shape();
+            std::istringstream ss(s + 2);
+            std::string w;
+            name = (ss >> w) ? w : "";
+        } else if (s[0] == 'o' && is_space(s[1])) {
+            end_shape();
+            name = first_word(s + 2);
         }
+    }
+    end_shape();
+    return 0;
+}
+
+int read_obj(const std::string &path, MeshData &m, std::string *err) {
+    ObjFile o;
+    if (read_obj_file(path, "", o, err)) return 1;
+    m.v = std::move(o.v);
+    m.n = std::move(o.n);
+    m.uv = std::move(o.uv);
+    for (const ObjShape &sh : o.shapes) {
+        m.iv.insert(m.iv.end(), sh.iv.begin(), sh.iv.end());
+        m.it.insert(m.it.end(), sh.it.begin(), sh.it.end());
+        m.in.insert(m.in.end(), sh.in.begin(), sh.in.end());
     }
     return 0;
 }

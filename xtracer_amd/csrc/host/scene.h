@@ -114,8 +114,31 @@ struct Scene {
 // after parsing (parseutil.cc:715-738). Returns a LoadStatus.
 int load_scn(Scene &scene, const std::string &path, const std::vector<std::string> &mods);
 
-// Minimal Wavefront OBJ reader (positions, normals, texcoords, fan-triangulated
-// faces, 1-based and negative indices). Returns 0 on success.
+// Wavefront OBJ as nmesh::io::import::obj reads it through tinyobjloader
+// 1.0 (lib/nmesh/obj.cc:14-131): attributes, shapes split at `g` / `o`,
+// fan-triangulated faces with the `usemtl` material of each, and the MTL
+// materials of `mtllib` files.
+struct ObjShape {
+    std::string name;
+    std::vector<int32_t> iv, in, it;  // 3 per triangle, -1 when absent
+    std::vector<int32_t> mat;         // per triangle, -1: none
+};
+struct ObjMaterial {
+    std::string name;
+    float kd[3] = {0, 0, 0}, ks[3] = {0, 0, 0}, ke[3] = {0, 0, 0};
+    float shininess = 1.f, ior = 1.f;
+    std::string map_kd, map_ks, map_ke;
+};
+struct ObjFile {
+    std::vector<float> v, n, uv;
+    std::vector<ObjShape> shapes;
+    std::vector<ObjMaterial> materials;
+};
+// Reads `path`; `mtl_dir` prefixes mtllib names. Returns 0 on success.
+int read_obj_file(const std::string &path, const std::string &mtl_dir, ObjFile &out, std::string *err);
+
+// All shapes of an OBJ file as one triangle soup (a `mesh` geometry,
+// Mesh::build_octree(object_t&), mesh.cc:75-113). Returns 0 on success.
 int read_obj(const std::string &path, MeshData &out, std::string *err);
 
 }  // namespace xt
