@@ -186,7 +186,7 @@ int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t
  * at the top. Replaces nimg::io::load::image (lib/nimg/img.cc:17-66) for the
  * files it reads as 8-bit RGB / RGBA: PNG (not interlaced), TGA (raw or
  * run-length, 24 / 32 bit), BMP (uncompressed 24 / 32 bit), PPM (P6, maxval
- * 255). With rgba NULL only *width / *height are set. XT_E_IO when the file
+ * 255), Radiance HDR (converted to 8 bits as stb_image does). With rgba NULL only *width / *height are set. XT_E_IO when the file
  * cannot be read, XT_E_INVALID for a format or variant not decoded. */
 int xt_image_read(const char *path, float *rgba, uint32_t *width, uint32_t *height);
 

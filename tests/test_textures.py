@@ -101,6 +101,40 @@ def test_decoder_matches_reference_loader(name, tmp_path):
     assert np.array_equal(xa.read_image(path).view(np.uint32), ref.view(np.uint32))
 
 
+def _flat_hdr(path, w, h, seed):
+    """A Radiance file with flat (not run-length) scanlines at a width where
+    stb expects run-length ones: it reads them flat from the first pixel on."""
+    rgbe = np.random.default_rng(seed).integers(0, 256, (h * w, 4)).astype(np.uint8)
+    rgbe[0, 0] = 200  # not the 2 2 header
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n" % (h, w) + rgbe.tobytes())
+    return path
+
+
+@needs_ref
+@pytest.mark.parametrize("w,h", [(5, 3), (16, 9), (40, 7), (33000, 2)])
+@pytest.mark.parametrize("writer", ["ours", "reference", "flat"])
+def test_hdr_matches_reference_loader(w, h, writer, tmp_path):
+    """Radiance HDR read the way stbi_load does (RGBE to float, then
+    powf(v, 1 / 2.2) * 255 + 0.5 truncated to 8 bits): files from our writer
+    (run-length scanlines for widths 8..32767), the reference's writer and
+    flat scanlines, equal to the reference's loader bit for bit."""
+    rng = np.random.default_rng(w + h)
+    img = (rng.random((h, w, 4)) ** 3 * 8).astype(np.float32)
+    img[..., 3] = 1
+    img[0, 0, :3] = 0
+    path = tmp_path / "t.hdr"
+    if writer == "ours":
+        xa.write_image(path, img)
+    elif writer == "reference":
+        O.ref_save_image("hdr", path, img)
+    else:
+        _flat_hdr(path, w, h, w)
+    ref = O.ref_load_image(path)
+    assert ref is not None
+    assert np.array_equal(xa.read_image(path).view(np.uint32), ref.view(np.uint32))
+
+
 def test_scene_sampler_tables(tmp_path):
     """Texture tables hold the decoded files with flip_x / flip_y applied
     (nimg/transform.cc); samplers and material slots follow the .scn."""

@@ -3,6 +3,8 @@
 #include <zlib.h>
 
 #include <cctype>
+#include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -199,6 +201,105 @@ int load_ppm(const std::vector<uint8_t> &f, Image &out, std::string *err) {
     return 0;
 }
 
+// Radiance RGBE read as stbi_load returns it (stb_image 2.x): the header's
+// FORMAT=32-bit_rle_rgbe and "-Y h +X w" layout; scanlines flat for widths
+// outside [8, 32767], else run-length per component (a scanline without the
+// 2 2 header turns the rest of the image into flat pixels, from the second
+// pixel on, as stb does); RGBE -> float by ldexp(1, e - 136), then to 8 bits
+// by powf(v, 1 / 2.2f) * 255 + 0.5, clamped to [0, 255] and truncated.
+// Bytes past the end read as 0.
+int load_hdr(const std::vector<uint8_t> &f, Image &out, std::string *err) {
+    size_t pos = 0;
+    auto get8 = [&]() -> uint8_t { return pos < f.size() ? f[pos++] : 0; };
+    auto token = [&]() {
+        std::string t;
+        while (pos < f.size()) {
+            const char c = (char)f[pos++];
+            if (c == '\n') break;
+            t.push_back(c);
+        }
+        return t;
+    };
+    const std::string sig = token();
+    if (sig != "#?RADIANCE" && sig != "#?RGBE") return fail(err, "not a Radiance HDR file", 2);
+    bool valid = false;
+    for (;;) {
+        const std::string t = token();
+        if (t.empty()) break;
+        if (t == "FORMAT=32-bit_rle_rgbe") valid = true;
+    }
+    if (!valid) return fail(err, "HDR format not decoded", 2);
+    const std::string res = token();
+    if (res.compare(0, 3, "-Y ")) return fail(err, "HDR layout not decoded", 2);
+    char *e = nullptr;
+    const long h = std::strtol(res.c_str() + 3, &e, 10);
+    while (*e == ' ') ++e;
+    if (std::strncmp(e, "+X ", 3)) return fail(err, "HDR layout not decoded", 2);
+    const long w = std::strtol(e + 3, nullptr, 10);
+    if (w <= 0 || h <= 0 || w > 65535 || h > 65535) return fail(err, "bad HDR size", 2);
+    std::vector<float> hdr((size_t)w * h * 3);
+    auto convert = [](float *o, const uint8_t *in) {
+        if (in[3]) {
+            const float f1 = std::ldexp(1.0f, in[3] - (int)(128 + 8));
+            o[0] = in[0] * f1;
+            o[1] = in[1] * f1;
+            o[2] = in[2] * f1;
+        } else {
+            o[0] = o[1] = o[2] = 0.f;
+        }
+    };
+    auto flat_from = [&](size_t first) {
+        for (size_t p = first; p < (size_t)w * h; ++p) {
+            uint8_t rgbe[4];
+            for (uint8_t &b : rgbe) b = get8();
+            convert(&hdr[3 * p], rgbe);
+        }
+    };
+    if (w < 8 || w >= 32768) {
+        flat_from(0);
+    } else {
+        std::vector<uint8_t> line((size_t)w * 4);
+        for (long j = 0; j < h; ++j) {
+            const int c1 = get8(), c2 = get8();
+            int len = get8();
+            if (c1 != 2 || c2 != 2 || (len & 0x80)) {
+                const uint8_t rgbe[4] = {(uint8_t)c1, (uint8_t)c2, (uint8_t)len, get8()};
+                convert(&hdr[0], rgbe);
+                flat_from(1);
+                break;
+            }
+            len = (len << 8) | get8();
+            if (len != w) return fail(err, "bad HDR scanline length", 2);
+            for (int k = 0; k < 4; ++k) {
+                long i = 0;
+                while (i < w) {
+                    int count = get8();
+                    if (count > 128) {
+                        const uint8_t value = get8();
+                        count -= 128;
+                        if (count > w - i) return fail(err, "bad HDR run", 2);
+                        for (int z = 0; z < count; ++z) line[4 * i++ + k] = value;
+                    } else {
+                        if (count > w - i) return fail(err, "bad HDR run", 2);
+                        for (int z = 0; z < count; ++z) line[4 * i++ + k] = get8();
+                    }
+                }
+            }
+            for (long i = 0; i < w; ++i) convert(&hdr[3 * ((size_t)j * w + i)], &line[4 * i]);
+        }
+    }
+    std::vector<uint8_t> px(hdr.size());
+    const float gamma = 1.0f / 2.2f;
+    for (size_t i = 0; i < hdr.size(); ++i) {
+        float z = std::pow(hdr[i] * 1.0f, gamma) * 255 + 0.5f;
+        if (z < 0) z = 0;
+        if (z > 255) z = 255;
+        px[i] = (uint8_t)(int)z;
+    }
+    to_float(px, (uint32_t)w, (uint32_t)h, 3, out);
+    return 0;
+}
+
 }  // namespace
 
 int load_image(const std::string &path, Image &out, std::string *err) {
@@ -210,6 +311,7 @@ int load_image(const std::string &path, Image &out, std::string *err) {
     if (f.size() >= 8 && !std::memcmp(f.data(), png_sig, 8)) rc = load_png(f, out, err);
     else if (f.size() >= 2 && f[0] == 'B' && f[1] == 'M') rc = load_bmp(f, out, err);
     else if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') rc = load_ppm(f, out, err);
+    else if (f.size() >= 2 && f[0] == '#' && f[1] == '?') rc = load_hdr(f, out, err);
     else if (path.size() > 4 && (path.substr(path.size() - 4) == ".tga" || path.substr(path.size() - 4) == ".TGA"))
         rc = load_tga(f, out, err);
     else rc = fail(err, "image format not decoded: " + path, 2);

@@ -4,7 +4,8 @@
 // 3 channels), top row first. These decoders are written from the format
 // specifications for the files stb_image reads that way: PNG (8-bit RGB /
 // RGBA, not interlaced), TGA (true colour, raw or run-length, 24 / 32 bit),
-// BMP (uncompressed 24 / 32 bit) and binary PPM (P6, maxval 255).
+// BMP (uncompressed 24 / 32 bit), binary PPM (P6, maxval 255) and Radiance
+// HDR, which stb_image converts to 8 bits.
 #pragma once
 #include <cstdint>
 #include <string>
