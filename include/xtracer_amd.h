@@ -272,6 +272,17 @@ typedef struct xt_job_stats {
 
 /* Plain one-job-per-launch render. rgba_out: width*height*4 floats, host. */
 int xt_render(xt_dev *d, const xt_job *job, float *rgba_out, xt_job_stats *stats);
+
+/* xt_render, then the render's ray graph written to ply_path as ASCII PLY:
+ * for a depth job one two-vertex path per sample, camera origin to hit point
+ * ((0, 0, 0) on a miss, the hit record's default), coloured with the pixel
+ * after that sample, in the
+ * reference's tile / pixel / sample order; other integrators record no paths.
+ * Replaces render::assemble(raygraph_t&) + raygraph::write (context.cc:54-61,
+ * util/raygraph.cc:8-78) as the GUI's export calls them (action.cc:65-85).
+ * The samples are held on the host: about 56 B per sample. XT_E_IO when the
+ * file cannot be written. */
+int xt_render_raygraph(xt_dev *d, const xt_job *job, float *rgba_out, const char *ply_path);
 /* Same with a device-resident output buffer. */
 int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_stats *stats);
 

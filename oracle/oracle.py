@@ -219,6 +219,7 @@ def ref():
         L.xr_save_image.argtypes = [ctypes.c_int, ctypes.c_char_p, P, ctypes.c_int, ctypes.c_int]
         L.xr_camera_rays.argtypes = [P, ctypes.c_uint64, P, ctypes.c_float, ctypes.c_float, P, P]
         L.xr_load_image.argtypes = [ctypes.c_char_p, P, P, P]
+        L.xr_write_raygraph.argtypes = [P, ctypes.c_char_p]
         _ref = L
     return _ref
 
@@ -294,6 +295,11 @@ class RefScene:
             raise RuntimeError("xr_render failed")
         return img
 
+
+    def write_raygraph(self, path):
+        """PLY of the last render's ray graph, written by the reference."""
+        if ref().xr_write_raygraph(self._h, str(path).encode()):
+            raise RuntimeError("reference raygraph::write failed")
     def __del__(self):
         try:
             if self._h:

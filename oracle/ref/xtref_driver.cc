@@ -21,6 +21,7 @@
 #include <xtcore/parseutil.h>
 #include <xtcore/strpool.h>
 #include <xtcore/log.h>
+#include <xtcore/util/raygraph.h>
 #include <xtcore/integrator/wireframe/integrator.h>
 #include <nimg/img.h>
 #include <nimg/pixmap.h>
@@ -256,6 +257,16 @@ int xr_render_tiles(void *h, int integrator, int width, int height, int samples,
     if (queries) *queries = q;
     delete in;
     return 0;
+}
+
+// The ray graph of the last xr_render through the reference's own
+// render::assemble(raygraph_t&) and raygraph::write (context.cc:54-61,
+// util/raygraph.cc:8-78), as the GUI exports it (action.cc:65-85).
+int xr_write_raygraph(void *h, const char *path) {
+    render::context_t *ctx = (render::context_t *)h;
+    raygraph::raygraph_t rg;
+    render::assemble(rg, *ctx);
+    return raygraph::write(path, rg);
 }
 
 // Primary rays of the scene's first camera (the one xr_render uses) at n

@@ -8,6 +8,8 @@ synthetic scenes (regenerated identically anywhere by xtracer_amd.scenes):
   ref_mc_<scene>.npz      high-spp ao / pathtracer framebuffers (glibc rand())
   ref_images.npz          a framebuffer and the bytes of the reference's
                           png / bmp / tga / hdr / exr files of it (nimg::io::save)
+  ref_raygraph_spheres.npz  a depth render (33x33, GRID aa 2) and the bytes of
+                          the reference's ray-graph PLY of it (raygraph::write)
 Every scene is loaded in a fresh process: the reference's string pool is
 process-global, and its handle order decides the scan order.
 
@@ -97,7 +99,27 @@ def make_images():
     print("ok images")
 
 
+RAYGRAPH = r"""
+import sys
+import numpy as np
+sys.path.insert(0, {root!r})
+from oracle import oracle as O
+from xtracer_amd import scenes
+path = scenes.SCENES["spheres"][0]()
+r = O.RefScene(path)
+img = r.render(1, 33, 33, samples=1, aa=2, threads=1)
+r.write_raygraph({tmp!r})
+np.savez_compressed({out!r} + "/ref_raygraph_spheres.npz", image=img, ply=np.fromfile({tmp!r}, np.uint8),
+                    width=33, height=33, samples=1, aa=2)
+print("ok raygraph")
+"""
+
+
 def main():
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        code = RAYGRAPH.format(root=ROOT, tmp=os.path.join(d, "rg.ply"), out=HERE)
+        print(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().strip())
     make_images()
     for name, (kind, kw) in SCENES.items():
         code = CHILD.format(root=ROOT, name=name, kind=kind, kw=json.dumps(kw), out=HERE)

@@ -188,3 +188,30 @@ def test_dielectric_same_stream(device, kind, tmp_path):
     assert close.mean() >= 0.98, f"only {close.mean():.4f} of pixels agree"
     assert abs(st.queries - nq) <= 0.01 * nq
     device.release(sid)
+
+
+@pytest.mark.gpu
+def test_raygraph_equals_reference_file(device, tmp_path):
+    """The ray graph of a depth render (33x33, GRID aa 2: partial 32-pixel
+    tiles, four samples per pixel) is byte-identical to the PLY the
+    reference's raygraph::write produced for the same render
+    (tests/golden/ref_raygraph_spheres.npz), and the image is too."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_raygraph_spheres.npz"))
+    s = xa.Scene(scenes.SCENES["spheres"][0]())
+    sid = device.upload(s)
+    job = xa.make_job(sid, xa.DEPTH, int(z["width"]), int(z["height"]), s.camera(), samples=int(z["samples"]),
+                      aa=int(z["aa"]))
+    img = device.render_raygraph(job, tmp_path / "rg.ply")
+    assert np.array_equal(img.view(np.uint32), z["image"].view(np.uint32))
+    got = np.fromfile(tmp_path / "rg.ply", np.uint8)
+    ref = z["ply"]
+    if not np.array_equal(got, ref):
+        a, b = got.tobytes().decode().splitlines(), ref.tobytes().decode().splitlines()
+        diff = [(k, x, y) for k, (x, y) in enumerate(zip(a, b)) if x != y][:5]
+        raise AssertionError(f"{len(a)} vs {len(b)} lines; first differences {diff}")
+    # other integrators record no paths: a header with empty elements
+    device.render_raygraph(xa.make_job(sid, xa.STENCIL, 8, 8, s.camera()), tmp_path / "st.ply")
+    text = (tmp_path / "st.ply").read_text()
+    assert "element vertex 0\n" in text and "element edge 0\n" in text and text.endswith("end_header\n")
+    device.release(sid)

@@ -32,7 +32,7 @@ SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
     "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals", "xt_scene_get_texcoords", "xt_scene_get_samplers",
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
-    "xt_image_write", "xt_image_read", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
+    "xt_image_write", "xt_image_read", "xt_render_raygraph", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
     "xt_serve_get_info", "xt_serve_set_policy",
@@ -153,6 +153,7 @@ def lib():
                            [ctypes.POINTER(Camera)]),
         "xt_image_write": (I, [ctypes.c_char_p, P, U32, U32]),
         "xt_image_read": (I, [ctypes.c_char_p, P, P, P]),
+        "xt_render_raygraph": (I, [P, ctypes.POINTER(Job), P, ctypes.c_char_p]),
         "xt_dev_create": (I, [ctypes.POINTER(P), I, U64]),
         "xt_dev_destroy": (None, [P]),
         "xt_last_error": (ctypes.c_char_p, [P]),
@@ -429,6 +430,12 @@ class Device:
         self._check(lib().xt_render(self._d, ctypes.byref(job), _ptr(img), ctypes.byref(st)))
         return img, st
 
+    def render_raygraph(self, job, ply_path):
+        """render() without stats, and the render's ray graph written to
+        `ply_path` (xt_render_raygraph); returns the image."""
+        img = np.zeros((job.height, job.width, 4), np.float32)
+        self._check(lib().xt_render_raygraph(self._d, ctypes.byref(job), _ptr(img), os.fsencode(ply_path)))
+        return img
     def render_device(self, job, out_ptr):
         st = JobStats()
         self._check(lib().xt_render_device(self._d, ctypes.byref(job), ctypes.c_void_p(out_ptr), ctypes.byref(st)))

@@ -920,7 +920,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_ao2(const Queue in, c
 // reference's predicate for the primitive found, rounded to float.
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_depth(const SceneView S, const Queue in,
                                                           const uint32_t *in_count, const float4 *hit,
-                                                          const double *ray64, float4 *rad) {
+                                                          double *ray64, float4 *rad) {
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -947,6 +947,12 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_depth(const SceneView
         const double dist = dlen(dsub(o, pt));
         const float depth = (float)(1. / log(dist));
         rad[path] = make_float4(depth, 1.f, 0.f, 0.f);  // y = 1 marks a hit
+        // the hit point replaces the direction in the path's FP64 ray, for
+        // the ray graph's sample (the camera ray is not read after this)
+        double *w = ray64 + 6 * (size_t)path;
+        w[3] = pt.x;
+        w[4] = pt.y;
+        w[5] = pt.z;
     }
 }
 

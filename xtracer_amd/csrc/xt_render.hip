@@ -5,7 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <functional>
+#include <new>
+#include <vector>
 
 #include "xt_device.h"
 #include "xt_render.h"
@@ -266,7 +270,10 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
     return join();
 }
 
-static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats *stats) {
+// `after_pass(j, base, spp_pass)`, when given, runs on the host after each
+// pass's accumulation has been queued.
+static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats *stats,
+                       const std::function<int(const JobDev &, uint32_t, uint32_t)> *after_pass = nullptr) {
     JobDev j;
     const DevScene *ds = nullptr;
     int rc = xt_job_check(d, job, j, &ds);
@@ -318,6 +325,10 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(j.NT, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
                            spp_pass, 0u, j.NT, w.rad, need64 ? w.ray64 : nullptr, a.accf, a.acc64);
         XT_LAUNCHED();
+        if (after_pass) {
+            rc = (*after_pass)(j, base, spp_pass);
+            if (rc) return rc;
+        }
     }
     hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, a.accf,
                        a.acc64, (float4 *)d_out);
@@ -361,5 +372,109 @@ extern "C" int xt_render(xt_dev *d, const xt_job *job, float *rgba_out, xt_job_s
     rc = render_core(d, job, (float *)d->accum.out, stats);
     if (rc) return rc;
     XT_TRY(hipMemcpy(rgba_out, d->accum.out, npix * sizeof(float4), hipMemcpyDeviceToHost));
+    return XT_OK;
+}
+
+// Ray graph of a depth render (integrator/depth/integrator.cc:43-51): per
+// sample a two-vertex path from the camera ray's origin (colour 0) to its hit
+// point ((0, 0, 0) on a miss) coloured with the pixel's value after that
+// sample. Samples are gathered per pass from the FP64 rays (origin, and the
+// hit point k_shade_depth leaves in the direction slots) and the per-sample
+// depth, then ordered as the reference's tiles hold them: 32-pixel tiles in
+// row-major order (segment_framebuffer, tile.cc:67-93, the context's default
+// tile size), pixels x-major within a tile (aa.cc:55-56), samples in FIFO
+// order; raygraph::write (util/raygraph.cc:8-78) writes them as ASCII PLY,
+// numbers as an ostream prints them (%g).
+extern "C" int xt_render_raygraph(xt_dev *d, const xt_job *job, float *rgba_out, const char *ply_path) {
+    if (!d || !job || !rgba_out || !ply_path) return XT_E_INVALID;
+    const bool depth = job->integrator == XT_INTEGRATOR_DEPTH;
+    const size_t npix = (size_t)job->width * job->height;
+    std::vector<double> org, pnt;
+    std::vector<float> val;
+    std::vector<uint8_t> hits;
+    uint32_t total = 0, W = job->width, H = job->height;
+    double weight = 0;
+    std::function<int(const JobDev &, uint32_t, uint32_t)> grab = [&](const JobDev &j, uint32_t base,
+                                                                         uint32_t spp_pass) -> int {
+        if (org.empty()) {
+            total = j.total;
+            weight = j.weight;
+            const size_t n = npix * total;
+            org.resize(3 * n);
+            pnt.resize(3 * n);
+            val.resize(n);
+            hits.resize(n);
+        }
+        const size_t cap = (size_t)spp_pass * j.NT;
+        std::vector<double> r64(6 * cap);
+        std::vector<float4> rad(cap);
+        XT_TRY(hipStreamSynchronize(d->stream));
+        XT_TRY(hipMemcpy(r64.data(), d->wave.ray64, r64.size() * sizeof(double), hipMemcpyDeviceToHost));
+        XT_TRY(hipMemcpy(rad.data(), d->wave.rad, cap * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cap; ++i) {
+            const uint32_t s = base + (uint32_t)(i / j.NT), q = (uint32_t)(i % j.NT);
+            const uint32_t tile = q >> 6, within = q & 63u;
+            const uint32_t x = (tile % j.tiles_x) * 8u + (within & 7u), y = (tile / j.tiles_x) * 8u + (within >> 3);
+            if (x >= j.W || y >= j.H || s >= j.total) continue;
+            const size_t k = ((size_t)y * j.W + x) * total + s;
+            const bool hit = rad[i].y != 0.f;
+            for (int c = 0; c < 3; ++c) {
+                org[3 * k + c] = r64[6 * i + c];
+                pnt[3 * k + c] = hit ? r64[6 * i + 3 + c] : 0.0;
+            }
+            val[k] = rad[i].x;
+            hits[k] = hit;
+        }
+        return XT_OK;
+    };
+    int rc = xt_dev_ensure_accum(d, npix);
+    if (rc) return rc;
+    try {
+        rc = render_core(d, job, (float *)d->accum.out, nullptr, depth ? &grab : nullptr);
+    } catch (const std::bad_alloc &) {
+        d->err = "ray graph too large for host memory";
+        return XT_E_INVALID;
+    }
+    if (rc) return rc;
+    XT_TRY(hipMemcpy(rgba_out, d->accum.out, npix * sizeof(float4), hipMemcpyDeviceToHost));
+    FILE *f = std::fopen(ply_path, "w");
+    if (!f) {
+        d->err = std::string("cannot write ") + ply_path;
+        return XT_E_IO;
+    }
+    const size_t n = depth ? npix * total : 0;
+    std::fprintf(f, "ply\nformat ascii 1.0\ncomment object: A single line\nelement vertex %zu\n", 2 * n);
+    std::fprintf(f, "property float x\nproperty float y\nproperty float z\nproperty float red\n"
+                    "property float green\nproperty float blue\nelement edge %zu\nproperty int vertex1\n"
+                    "property int vertex2\nend_header\n", n);
+    const uint32_t TS = 32;
+    auto each = [&](const std::function<void(size_t)> &fn) {
+        for (uint32_t ty = 0; ty < (H + TS - 1) / TS; ++ty)
+            for (uint32_t tx = 0; tx < (W + TS - 1) / TS; ++tx)
+                for (uint32_t x = tx * TS; x < std::min(W, tx * TS + TS); ++x)
+                    for (uint32_t y = ty * TS; y < std::min(H, ty * TS + TS); ++y) fn((size_t)y * W + x);
+    };
+    if (depth) {
+        each([&](size_t pix) {
+            float r = 0.f;  // the tile starts at zero
+            for (uint32_t s = 0; s < total; ++s) {
+                const size_t k = pix * total + s;
+                // depth = color.r() + depth * weight, then ColorRGBAf's clamp at 0
+                if (hits[k]) r = std::max((float)((double)r + (double)val[k] * weight), 0.f);
+                std::fprintf(f, "%g %g %g 0 0 0\n", org[3 * k], org[3 * k + 1], org[3 * k + 2]);
+                std::fprintf(f, "%g %g %g %g %g %g\n", pnt[3 * k], pnt[3 * k + 1], pnt[3 * k + 2], (double)r,
+                             (double)r, (double)r);
+            }
+        });
+        size_t at = 0;
+        each([&](size_t) {
+            for (uint32_t s = 0; s < total; ++s, at += 2) std::fprintf(f, "%zu %zu\n", at, at + 1);
+        });
+    }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) {
+        d->err = std::string("cannot write ") + ply_path;
+        return XT_E_IO;
+    }
     return XT_OK;
 }
