@@ -293,3 +293,41 @@ def external_scene(path, camera="pinhole"):
     with open(path, "w") as f:
         f.write(text)
     return str(path)
+
+
+def extrude_scene(path, camera="pinhole"):
+    """A mesh around the origin (a subdivided octahedron from an OBJ file)
+    with the `extrude` modifier: every vertex pushed to 10 * its direction *
+    the red channel of a cubemap (texture_files' faces) in that direction;
+    moved first by the rotation / scale / translation modifiers."""
+    import os
+    from tests import scn
+    d = os.path.dirname(str(path))
+    f = texture_files(d)
+    verts = [np.array(v, float) for v in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+    tris = [(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)]
+    for _ in range(2):  # two midpoint subdivisions, the new vertices pushed out to the unit sphere
+        out = []
+        for a, b, c in tris:
+            m = []
+            for p, q in ((a, b), (b, c), (c, a)):
+                v = verts[p] + verts[q]
+                verts.append(v / np.linalg.norm(v))
+                m.append(len(verts) - 1)
+            out += [(a, m[0], m[2]), (m[0], b, m[1]), (m[2], m[1], c), (m[0], m[1], m[2])]
+        tris = out
+    scn.write_obj(os.path.join(d, "blob.obj"), [[verts[i] for i in t] for t in tris])
+    scn.write(path, geometry={"blob": dict(type="mesh", source="blob.obj"),
+                              "floor": dict(type="plane", normal=(0, 1, 0), distance=-1.5)},
+              objects={"blob": ("blob", "grey"), "floor": ("floor", "grey")},
+              materials={"grey": dict(type="lambert", kd=(0.6, 0.6, 0.6))},
+              camera=dict(position=(0.5, 2.0, -9.0), target=(0.0, 0.3, 0.0), fov=50.0),
+              env=("gradient", (1.0, 1.0, 1.0), (0.5, 0.7, 1.0)))
+    faces = "\n".join(f"\t\t\t\t{k} = {f[k]}" for k in ("posx", "negx", "posy", "negy", "posz", "negz"))
+    text = open(path).read().replace(
+        "\t\tsource = blob.obj\n",
+        "\t\tsource = blob.obj\n\t\tmodifiers = {\n\t\t\trotation = vec3(10,20,5)\n\t\t\tscale = vec3(0.9,1.1,1)\n"
+        "\t\t\textrude = {\n" + faces + "\n\t\t\t}\n\t\t}\n", 1)
+    with open(path, "w") as fh:
+        fh.write(text)
+    return str(path)

@@ -138,3 +138,48 @@ def test_external_scene_matches_reference(tmp_path):
         assert bad == 0, (integ, out["bad"])
         assert lit > 100, (integ, out["bad"])
     assert out["inside"] >= 0.97, out
+
+
+def test_extrude_radii(tmp_path):
+    """extrude (extrude.cc:9-27) puts every vertex at 10 * (red channel of
+    the cubemap in its direction) from the origin."""
+    from tests.helpers import extrude_scene
+    s = xa.Scene(extrude_scene(str(tmp_path / "x.scn")))
+    v = s.arrays()["prims64"][:, :9].reshape(-1, 3)
+    r = np.linalg.norm(v, axis=1)
+    assert r.max() <= 10.0 + 1e-5 and len(np.unique(np.round(r, 4))) > 5
+
+
+_EXTRUDE = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import xtracer_amd as xa
+from oracle import oracle as O
+from xtracer_amd.scenes import camera_dict
+path = {path!r}
+s = xa.Scene(path)
+r = O.RefScene(path)
+orc = O.OracleScene(s)
+rng = np.random.default_rng(5)
+o = rng.uniform(-6, 6, (40000, 3))
+d = rng.normal(size=(40000, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+t1, ob1, _, p1, n1 = orc.trace(o, d)
+t2, ob2, p2, n2 = r.trace(o, d)
+res = {{"trace": bool(np.array_equal(t1, t2) and np.array_equal(ob1, ob2)), "hits": int((ob1 >= 0).sum())}}
+cam = camera_dict(position=(0.5, 2.0, -9.0), target=(0.0, 0.3, 0.0), fov=50.0)
+res["bad"] = [int((r.render(k, 64, 48, samples=4, threads=1).view(np.uint32)
+                   != orc.render(k, 64, 48, cam, samples=4)[0].view(np.uint32)).any(axis=2).sum()) for k in (0, 1, 4)]
+print(json.dumps(res))
+"""
+
+
+@needs_ref
+def test_extrude_matches_reference(tmp_path):
+    """A rotated, scaled mesh extruded by a cubemap: hits and stencil /
+    depth / normal images equal the compiled reference's bit for bit."""
+    from tests.helpers import extrude_scene
+    path = extrude_scene(str(tmp_path / "x.scn"))
+    code = _EXTRUDE.format(root=ROOT, path=path)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
+    assert out["trace"] and out["hits"] > 10000 and out["bad"] == [0, 0, 0], out

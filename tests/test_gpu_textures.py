@@ -84,6 +84,27 @@ def test_textured_pathtracer_same_stream(device, cam, env, tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("accel", [xa.ACCEL_KDTREE, xa.ACCEL_OCTREE])
+def test_extruded_mesh(device, accel, tmp_path):
+    """A mesh pushed out by the extrude modifier's cubemap: stencil, depth
+    and normal images bit-exact against the oracle."""
+    from tests.helpers import extrude_scene
+    from xtracer_amd.scenes import camera_dict
+    path = extrude_scene(str(tmp_path / "x.scn"))
+    s = xa.Scene(path)
+    sid = device.upload(s)
+    cam = camera_dict(position=(0.5, 2.0, -9.0), target=(0.0, 0.3, 0.0), fov=50.0)
+    try:
+        for integrator in (xa.STENCIL, xa.DEPTH, xa.NORMAL):
+            img, st = device.render(xa.make_job(sid, integrator, 96, 72, s.camera(), samples=4, seed=5, accel=accel))
+            ref, nq = OracleScene(s).render(integrator, 96, 72, cam, samples=4, seed=5)
+            assert st.queries == nq
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), integrator
+    finally:
+        device.release(sid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("accel", [xa.ACCEL_KDTREE, xa.ACCEL_OCTREE])
 def test_external_objects(device, accel, tmp_path):
     """An OBJ object with MTL materials (an emissive shape textured through
     map_Ke, mirrored left to right; colour, emissive and Blinn-Phong shapes)

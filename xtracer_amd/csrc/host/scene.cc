@@ -128,6 +128,63 @@ dmat3 euler(const dvec3 &e) {
 
 // Mesh modifiers in the order parseutil.cc:332-353 applies them
 // (lib/nmesh/transform.cc:8-54, lib/nmesh/invnormals.cc).
+int parse_sampler(Scene &sc, const std::string &type, ncf::Node *cfg, SamplerDesc &out);
+
+// Red channel of a cubemap sampler at direction (x, y, z): Cubemap::sample
+// (sampler_cubemap.cc:14-41) over nimg::sample::nearest (sample.cc:8-17) and
+// Pixmap::pixel_ro's edge clamp, in the reference's float / double mix.
+float cubemap_red(const Scene &sc, const SamplerDesc &cm, double x, double y, double z) {
+    const double l = std::sqrt(x * x + y * y + z * z);
+    if (l != 0) {
+        x /= l;
+        y /= l;
+        z /= l;
+    }
+    int face = 2;
+    float u = 0.f, v = 0.f;
+    if (std::fabs(x) >= std::fabs(y) && std::fabs(x) >= std::fabs(z)) {
+        if (x > 0.0f) {
+            face = 1;
+            u = (z / x + 1.0f) * 0.5f;
+            v = 1.f - (y / x + 1.0f) * 0.5f;
+        } else if (x < 0.0f) {
+            face = 0;
+            u = (z / x + 1.0f) * 0.5f;
+            v = (y / x + 1.0f) * 0.5f;
+        }
+    } else if (std::fabs(y) >= std::fabs(x) && std::fabs(y) >= std::fabs(z)) {
+        if (y > 0.0f) {
+            face = 2;
+            u = (x / y + 1.0f) * 0.5f;
+            v = 1.f - (z / y + 1.0f) * 0.5f;
+        } else if (y < 0.0f) {
+            face = 3;
+            u = 1. - (x / y + 1.0f) * 0.5f;
+            v = 1.f - (z / y + 1.0f) * 0.5f;
+        }
+    } else if (std::fabs(z) >= std::fabs(x) && std::fabs(z) >= std::fabs(y)) {
+        if (z > 0.0f) {
+            face = 5;
+            u = 1. - (x / z + 1.0f) * 0.5f;
+            v = 1.f - (y / z + 1.0f) * 0.5f;
+        } else if (z < 0.0f) {
+            face = 4;
+            u = 1. - (x / z + 1.0f) * 0.5f;
+            v = (y / z + 1.0f) * 0.5f;
+        }
+    }
+    const Image &im = sc.textures[cm.tex[face]];
+    double du = u, dv = v;
+    du = du < 0 ? 0 : (du > 1 ? du - std::floor(du) : du);
+    dv = dv < 0 ? 0 : (dv > 1 ? dv - std::floor(dv) : dv);
+    const double fx = du * (float)im.width, fy = dv * (float)im.height;
+    size_t px = fx >= 0.0 && fx < 18446744073709551616.0 ? (size_t)fx : (size_t)1 << 63;
+    size_t py = fy >= 0.0 && fy < 18446744073709551616.0 ? (size_t)fy : (size_t)1 << 63;
+    px = px >= im.width ? im.width - 1 : px;
+    py = py >= im.height ? im.height - 1 : py;
+    return im.rgba[4 * (py * im.width + px)];
+}
+
 int apply_modifiers(Scene &sc, MeshData &m, ncf::Node *mods) {
     dvec3 rot = vec3_of(mods, "rotation", dvec3(0, 0, 0));
     dvec3 scl = vec3_of(mods, "scale", dvec3(1, 1, 1));
@@ -156,9 +213,23 @@ int apply_modifiers(Scene &sc, MeshData &m, ncf::Node *mods) {
     }
     if (mods->has_property("flip_normals") && parse_bool(mods->property("flip_normals")))
         for (auto &x : m.n) x = -x;
+    // extrude (extrude.cc:9-27): each vertex moves to 10 * its direction from
+    // the origin * the red channel of the cubemap in that direction; the
+    // cubemap's faces are read for this alone
     if (mods->has_group("extrude")) {
-        sc.error = "the extrude mesh modifier is not supported";
-        return 1;
+        const size_t keep = sc.textures.size();
+        SamplerDesc cm;
+        if (parse_sampler(sc, "cubemap", mods->group("extrude"), cm)) return 1;
+        for (size_t i = 0; i + 2 < m.v.size(); i += 3) {
+            const double x = m.v[i], y = m.v[i + 1], z = m.v[i + 2];
+            const double l = std::sqrt(x * x + y * y + z * z);
+            const double dx = l != 0 ? x / l : x, dy = l != 0 ? y / l : y, dz = l != 0 ? z / l : z;
+            const double c = cubemap_red(sc, cm, x, y, z);
+            m.v[i] = (float)(10. * dx * c);
+            m.v[i + 1] = (float)(10. * dy * c);
+            m.v[i + 2] = (float)(10. * dz * c);
+        }
+        sc.textures.resize(keep);
     }
     return 0;
 }
