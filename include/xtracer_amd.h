@@ -184,7 +184,8 @@ int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t
 /* Reads a texture image as the scene loader does for `texture`, `cubemap`
  * and `erp` samplers: RGBA floats (byte / 255, alpha 1 for RGB files), row 0
  * at the top. Replaces nimg::io::load::image (lib/nimg/img.cc:17-66) for the
- * files it reads as 8-bit RGB / RGBA: PNG (not interlaced), TGA (raw or
+ * files it reads as 8-bit RGB / RGBA: PNG (RGB, RGBA, palette; any depth;
+ * Adam7; tRNS), TGA (raw or
  * run-length, 24 / 32 bit), BMP (uncompressed 24 / 32 bit), PPM (P6, maxval
  * 255), Radiance HDR (converted to 8 bits as stb_image does). With rgba NULL only *width / *height are set. XT_E_IO when the file
  * cannot be read, XT_E_INVALID for a format or variant not decoded. */

@@ -14,26 +14,60 @@ def _paeth(a, b, c):
     return np.where((pa <= pb) & (pa <= pc), a, np.where(pb <= pc, b, c))
 
 
-def png(path, px, filters=(0, 1, 2, 3, 4), bit_depth=8, interlace=0):
-    """8-bit RGB (C=3), RGBA (C=4) or grey (C=1); row y uses filters[y % len(filters)]."""
-    h, w, ch = px.shape
-    ctype = {1: 0, 3: 2, 4: 6}[ch]
+def _filtered(rows, bpp, filters, start=0):
+    """Filter bytes rows (list of uint8 arrays) with filters[(start + y) % len]."""
     raw = bytearray()
-    prev = np.zeros(w * ch, np.int64)
-    for y in range(h):
-        row = px[y].reshape(-1).astype(np.int64)
-        a = np.concatenate([np.zeros(ch, np.int64), row[:-ch]])
-        c = np.concatenate([np.zeros(ch, np.int64), prev[:-ch]])
-        f = filters[y % len(filters)]
+    prev = np.zeros(len(rows[0]), np.int64) if rows else None
+    for y, r in enumerate(rows):
+        row = r.astype(np.int64)
+        a = np.concatenate([np.zeros(bpp, np.int64), row[:-bpp]])
+        c = np.concatenate([np.zeros(bpp, np.int64), prev[:-bpp]])
+        f = filters[(start + y) % len(filters)]
         pred = [np.zeros_like(row), a, prev, (a + prev) >> 1, _paeth(a, prev, c)][f]
         raw.append(f)
         raw += ((row - pred) % 256).astype(np.uint8).tobytes()
         prev = row
+    return raw
+
+
+def _pack(samples, depth):
+    """One row of samples (uint values) as bytes at `depth` bits each."""
+    if depth == 16:
+        return np.asarray(samples, ">u2").view(np.uint8)
+    if depth == 8:
+        return np.asarray(samples, np.uint8)
+    bits = np.unpackbits(np.asarray(samples, np.uint8)[:, None], axis=1)[:, 8 - depth:].reshape(-1)
+    return np.packbits(bits)
+
+
+def png(path, px, filters=(0, 1, 2, 3, 4), bit_depth=8, interlace=0, palette=None, trns=None):
+    """px: (H, W, C) samples. C = 3 RGB, 4 RGBA, 1 grey, or palette indices
+    with `palette` ((n, 3) uint8); bit_depth 1/2/4 (palette), 8 or 16
+    (samples up to 65535); Adam7 when interlace=1; `trns`: raw tRNS bytes."""
+    h, w, ch = px.shape
+    ctype = 3 if palette is not None else {1: 0, 3: 2, 4: 6}[ch]
+    bpp = max(1, ch * bit_depth // 8)
+
+    def rows_of(img):
+        return [_pack(img[y].reshape(-1), bit_depth) for y in range(img.shape[0])]
+    if not interlace:
+        raw = _filtered(rows_of(px), bpp, filters)
+    else:
+        raw = bytearray()
+        for p, (xo, yo, xs, ys) in enumerate(zip((0, 4, 0, 2, 0, 1, 0), (0, 0, 4, 0, 2, 0, 1), (8, 8, 4, 4, 2, 2, 1),
+                                                  (8, 8, 8, 4, 4, 2, 2))):
+            sub = px[yo::ys, xo::xs]
+            if sub.shape[0] and sub.shape[1]:
+                raw += _filtered(rows_of(sub), bpp, filters, start=p)
 
     def chunk(t, d):
         return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
 
     data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, bit_depth, ctype, 0, 0, interlace))
+    if palette is not None:
+        data += chunk(b"PLTE", np.asarray(palette, np.uint8).tobytes())
+    if trns is not None:
+        data += chunk(b"tRNS", bytes(trns))
     data += chunk(b"IDAT", zlib.compress(bytes(raw), 9)) + chunk(b"IEND", b"")
     with open(path, "wb") as f:
         f.write(data)

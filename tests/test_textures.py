@@ -29,6 +29,45 @@ def _pixels(seed, h, w, ch):
     return np.random.default_rng(seed).integers(0, 256, (h, w, ch)).astype(np.uint8)
 
 
+def _palette_png(path, depth, w, h, trns, adam7):
+    """Palette PNG at `depth` bits; returns the file and the expanded pixels."""
+    rng = np.random.default_rng(depth + w)
+    n = min(1 << depth, 40)
+    pal = rng.integers(0, 256, (n, 3)).astype(np.uint8)
+    idx = rng.integers(0, n, (h, w, 1)).astype(np.uint8)
+    alpha = rng.integers(0, 256, n - 3).astype(np.uint8) if trns else None  # the last entries stay opaque
+    I.png(path, idx, bit_depth=depth, interlace=int(adam7), palette=pal, trns=alpha, filters=(1, 4, 2, 0, 3))
+    out = pal[idx[..., 0]]
+    if trns:
+        a = np.full(n, 255, np.uint8)
+        a[:n - 3] = alpha
+        out = np.concatenate([out, a[idx]], axis=-1)
+    return path, out
+
+
+def _deep_png(path, ch, w, h, adam7):
+    """16-bit PNG: stb keeps each sample's high byte."""
+    px = np.random.default_rng(ch + w).integers(0, 65536, (h, w, ch)).astype(np.uint16)
+    I.png(path, px, bit_depth=16, interlace=int(adam7), filters=(4, 1, 3))
+    return path, (px >> 8).astype(np.uint8)
+
+
+def _keyed_png(path, depth, w, h):
+    """RGB PNG with a tRNS key colour: alpha 0 where a pixel equals it."""
+    rng = np.random.default_rng(depth + 3 * w)
+    top = 256 if depth == 8 else 65536
+    px = rng.integers(0, 4, (h, w, 3)) * (top // 4)
+    key = px[1, 2].copy()
+    trns = np.asarray(key, ">u2").tobytes()
+    if depth == 8:  # stb keeps the key's low byte: a high byte makes no difference
+        trns = bytes(b | 0x80 if k % 2 == 0 else b for k, b in enumerate(trns))
+    I.png(path, px.astype(np.uint16 if depth == 16 else np.uint8), bit_depth=depth, interlace=int(depth == 8),
+          trns=trns)
+    out = (px >> 8 if depth == 16 else px).astype(np.uint8)
+    a = np.where((px == key).all(axis=-1), 0, 255).astype(np.uint8)[..., None]
+    return path, np.concatenate([out, a], axis=-1)
+
+
 FILES = {
     "png_rgb": lambda d: (I.png(d / "a.png", _pixels(1, 9, 13, 3)), _pixels(1, 9, 13, 3)),
     "png_rgba_filters": lambda d: (I.png(d / "b.png", _pixels(2, 11, 7, 4), filters=(3, 4, 0, 2, 1)),
@@ -44,7 +83,20 @@ FILES = {
     "bmp24_top_down": lambda d: (I.bmp(d / "b.bmp", _pixels(9, 3, 5, 3), top_down=True), _pixels(9, 3, 5, 3)),
     "bmp32": lambda d: (I.bmp(d / "c.bmp", _pixels(10, 4, 6, 4)), _pixels(10, 4, 6, 4)),
     "ppm": lambda d: (I.ppm(d / "a.ppm", _pixels(11, 8, 3, 3)), _pixels(11, 8, 3, 3)),
+    "png_palette8": lambda d: _palette_png(d / "p8.png", 8, 13, 9, False, False),
+    "png_palette2_trns": lambda d: _palette_png(d / "p2.png", 2, 11, 7, True, False),
+    "png_palette4_adam7": lambda d: _palette_png(d / "p4.png", 4, 19, 13, True, True),
+    "png_rgb16": lambda d: _deep_png(d / "d16.png", 3, 9, 6, False),
+    "png_rgba16_adam7": lambda d: _deep_png(d / "d16i.png", 4, 17, 11, True),
+    "png_rgb_adam7_key": lambda d: _keyed_png(d / "k8.png", 8, 23, 10),
+    "png_rgb16_key": lambda d: _keyed_png(d / "k16.png", 16, 7, 5),
 }
+
+
+# Variants the reference's stb_image 2.14 decodes against the PNG
+# specification (channels shifted, wrong palette indices); the decoder
+# follows the specification for them, which Pillow confirms where present.
+STB_214_DIFFERS = {"png_palette2_trns", "png_palette4_adam7", "png_rgb16_key", "png_rgb_adam7_key"}
 
 
 @pytest.mark.parametrize("name", sorted(FILES))
@@ -52,6 +104,21 @@ def test_decoder_known_bytes(name, tmp_path):
     path, px = FILES[name](tmp_path)
     got = xa.read_image(path)
     assert np.array_equal(got.view(np.uint32), I.expected(px).view(np.uint32))
+
+
+@pytest.mark.parametrize("name", sorted(n for n in FILES if n.startswith("png")))
+def test_png_files_decode_alike_in_pillow(name, tmp_path):
+    """The test's PNG files are well-formed: an independent decoder reads
+    the same colours (and palette alpha)."""
+    Image = pytest.importorskip("PIL.Image")
+    path, px = FILES[name](tmp_path)
+    exp = np.round(I.expected(px) * 255).astype(np.uint8)
+    im = Image.open(path)
+    im.load()
+    got = np.asarray(im.convert("RGBA"))
+    assert np.array_equal(got[..., :3], exp[..., :3])
+    if im.mode == "P":
+        assert np.array_equal(got[..., 3], exp[..., 3])
 
 
 def test_bmp32_without_alpha_is_opaque(tmp_path):
@@ -62,14 +129,10 @@ def test_bmp32_without_alpha_is_opaque(tmp_path):
     assert np.array_equal(got[..., :3], I.expected(px)[..., :3])
 
 
-@pytest.mark.parametrize("case", ["grey_png", "interlaced_png", "png16", "unknown"])
+@pytest.mark.parametrize("case", ["grey_png", "unknown"])
 def test_decoder_refuses(case, tmp_path):
     if case == "grey_png":  # one channel: the reference refuses it as well (img.cc:60)
         path = I.png(tmp_path / "g.png", _pixels(1, 4, 4, 1))
-    elif case == "interlaced_png":
-        path = I.png(tmp_path / "i.png", _pixels(1, 4, 4, 3), interlace=1)
-    elif case == "png16":
-        path = I.png(tmp_path / "s.png", _pixels(1, 4, 4, 3), bit_depth=16)
     else:
         path = tmp_path / "x.jpg"
         path.write_bytes(b"\xff\xd8\xff\xe0not decoded")
@@ -80,10 +143,11 @@ def test_decoder_refuses(case, tmp_path):
 
 
 @needs_ref
-@pytest.mark.parametrize("name", sorted(FILES) + ["bmp32_zero_alpha", "grey_png"])
+@pytest.mark.parametrize("name", sorted(set(FILES) - STB_214_DIFFERS) + ["bmp32_zero_alpha", "grey_png"])
 def test_decoder_matches_reference_loader(name, tmp_path):
     """xt_image_read == nimg::io::load::image (stb_image) on every decoded
-    variant, and both refuse a one-channel PNG."""
+    variant but those stb 2.14 gets wrong, and both refuse a one-channel
+    PNG."""
     if name == "bmp32_zero_alpha":
         px = _pixels(12, 3, 4, 4)
         px[..., 3] = 0
@@ -133,6 +197,16 @@ def test_hdr_matches_reference_loader(w, h, writer, tmp_path):
     ref = O.ref_load_image(path)
     assert ref is not None
     assert np.array_equal(xa.read_image(path).view(np.uint32), ref.view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(STB_214_DIFFERS))
+def test_stb_214_differs_from_the_specification(name, tmp_path):
+    """The reference's loader does not return the file's pixels for these
+    variants; the decoder does (test_decoder_known_bytes)."""
+    path, px = FILES[name](tmp_path)
+    ref = O.ref_load_image(path)
+    assert ref is not None and not np.array_equal(ref, I.expected(px))
 
 
 def test_scene_sampler_tables(tmp_path):

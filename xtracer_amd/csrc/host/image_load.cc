@@ -2,6 +2,7 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
@@ -39,11 +40,45 @@ int paeth(int a, int b, int c) {
     return (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
 }
 
+// Undoes the row filters of one (sub)image of w x h pixels: `in` holds h rows
+// of a filter byte and `stride` bytes; `bpp` is the filter's byte distance.
+bool unfilter(const uint8_t *in, size_t stride, uint32_t h, size_t bpp, std::vector<uint8_t> &rows) {
+    rows.assign(stride * h, 0);
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint8_t filter = in[y * (stride + 1)];
+        const uint8_t *src = in + y * (stride + 1) + 1;
+        uint8_t *row = &rows[y * stride];
+        const uint8_t *up = y ? &rows[(y - 1) * stride] : nullptr;
+        for (size_t i = 0; i < stride; ++i) {
+            const int a = i >= bpp ? row[i - bpp] : 0, b = up ? up[i] : 0;
+            const int c = (up && i >= bpp) ? up[i - bpp] : 0;
+            int pred;
+            switch (filter) {
+                case 0: pred = 0; break;
+                case 1: pred = a; break;
+                case 2: pred = b; break;
+                case 3: pred = (a + b) >> 1; break;
+                case 4: pred = paeth(a, b, c); break;
+                default: return false;
+            }
+            row[i] = (uint8_t)(src[i] + pred);
+        }
+    }
+    return true;
+}
+
+// PNG as stbi_load(..., 0) returns it, for the images the reference keeps
+// (3 or 4 channels): RGB, RGBA and palette images at any bit depth (16-bit
+// samples keep their high byte; palette indices of 1, 2 or 4 bits), plain or
+// Adam7-interlaced. A tRNS chunk adds alpha: per palette entry, or 0 where an
+// RGB pixel equals the key colour (at 16 bits for 16-bit files, else the
+// key's low bytes). Grey
+// images come out with 1 or 2 channels, which the reference refuses.
 int load_png(const std::vector<uint8_t> &f, Image &out, std::string *err) {
     size_t pos = 8;
     uint32_t w = 0, h = 0;
-    int ch = 0;
-    std::vector<uint8_t> idat;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<uint8_t> idat, pal, trns;
     while (pos + 12 <= f.size()) {
         const uint32_t n = be32(&f[pos]);
         if (pos + 12 + (size_t)n > f.size()) return fail(err, "truncated PNG chunk", 2);
@@ -52,10 +87,14 @@ int load_png(const std::vector<uint8_t> &f, Image &out, std::string *err) {
             if (n < 13) return fail(err, "bad PNG header", 2);
             w = be32(data);
             h = be32(data + 4);
-            const int depth = data[8], ctype = data[9];
-            if (depth != 8 || (ctype != 2 && ctype != 6) || data[10] || data[11] || data[12])
-                return fail(err, "PNG variant not decoded (8-bit RGB / RGBA, not interlaced)", 2);
-            ch = ctype == 2 ? 3 : 4;
+            depth = data[8];
+            ctype = data[9];
+            interlace = data[12];
+            if (data[10] || data[11] || interlace > 1) return fail(err, "bad PNG header", 2);
+        } else if (!std::memcmp(type, "PLTE", 4)) {
+            pal.assign(data, data + n);
+        } else if (!std::memcmp(type, "tRNS", 4)) {
+            trns.assign(data, data + n);
         } else if (!std::memcmp(type, "IDAT", 4)) {
             idat.insert(idat.end(), data, data + n);
         } else if (!std::memcmp(type, "IEND", 4)) {
@@ -63,34 +102,76 @@ int load_png(const std::vector<uint8_t> &f, Image &out, std::string *err) {
         }
         pos += 12 + (size_t)n;
     }
-    if (!ch || !w || !h || w > (1u << 15) || h > (1u << 15)) return fail(err, "bad PNG", 2);
-    const size_t stride = (size_t)w * ch;
-    std::vector<uint8_t> raw((stride + 1) * h);
+    if (!w || !h || w > (1u << 15) || h > (1u << 15)) return fail(err, "bad PNG size", 2);
+    int file_ch;  // samples per pixel in the file
+    switch (ctype) {
+        case 2: file_ch = 3; break;
+        case 6: file_ch = 4; break;
+        case 3: file_ch = 1; break;
+        default: return fail(err, "grey PNG: 1 or 2 channels, which the reference refuses", 2);
+    }
+    const bool ok_depth = ctype == 3 ? (depth == 1 || depth == 2 || depth == 4 || depth == 8)
+                                     : (depth == 8 || depth == 16);
+    if (!ok_depth) return fail(err, "bad PNG bit depth", 2);
+    if (ctype == 3 && (pal.empty() || pal.size() % 3)) return fail(err, "PNG palette missing", 2);
+    const int out_ch = ctype == 6 ? 4 : (trns.empty() ? 3 : 4);
+    // the passes of the image: one, or Adam7's seven
+    static const int xo[7] = {0, 4, 0, 2, 0, 1, 0}, yo[7] = {0, 0, 4, 0, 2, 0, 1};
+    static const int xs[7] = {8, 8, 4, 4, 2, 2, 1}, ys[7] = {8, 8, 8, 4, 4, 2, 2};
+    const int passes = interlace ? 7 : 1;
+    size_t raw_len = 0;
+    uint32_t pw[7], ph[7];
+    size_t pstride[7];
+    for (int p = 0; p < passes; ++p) {
+        pw[p] = interlace ? (w - xo[p] + xs[p] - 1) / xs[p] : w;
+        ph[p] = interlace ? (h - yo[p] + ys[p] - 1) / ys[p] : h;
+        if (interlace && (w <= (uint32_t)xo[p] || h <= (uint32_t)yo[p])) pw[p] = ph[p] = 0;
+        pstride[p] = ((size_t)pw[p] * file_ch * depth + 7) / 8;
+        if (pw[p] && ph[p]) raw_len += (pstride[p] + 1) * ph[p];
+    }
+    std::vector<uint8_t> raw(raw_len);
     uLongf got = (uLongf)raw.size();
     if (uncompress(raw.data(), &got, idat.data(), (uLong)idat.size()) != Z_OK || got != raw.size())
         return fail(err, "PNG data does not inflate", 2);
-    std::vector<uint8_t> px(stride * h);
-    for (uint32_t y = 0; y < h; ++y) {
-        const uint8_t filter = raw[y * (stride + 1)];
-        const uint8_t *in = &raw[y * (stride + 1) + 1];
-        uint8_t *row = &px[y * stride];
-        const uint8_t *up = y ? &px[(y - 1) * stride] : nullptr;
-        for (size_t i = 0; i < stride; ++i) {
-            const int a = i >= (size_t)ch ? row[i - ch] : 0, b = up ? up[i] : 0;
-            const int c = (up && i >= (size_t)ch) ? up[i - ch] : 0;
-            int pred;
-            switch (filter) {
-                case 0: pred = 0; break;
-                case 1: pred = a; break;
-                case 2: pred = b; break;
-                case 3: pred = (a + b) >> 1; break;
-                case 4: pred = paeth(a, b, c); break;
-                default: return fail(err, "bad PNG filter", 2);
+    std::vector<uint8_t> px((size_t)w * h * out_ch);
+    std::vector<uint8_t> rows;
+    const size_t bpp = std::max<size_t>(1, (size_t)file_ch * depth / 8);
+    size_t at = 0;
+    for (int p = 0; p < passes; ++p) {
+        if (!pw[p] || !ph[p]) continue;
+        if (!unfilter(&raw[at], pstride[p], ph[p], bpp, rows)) return fail(err, "bad PNG filter", 2);
+        at += (pstride[p] + 1) * ph[p];
+        for (uint32_t j = 0; j < ph[p]; ++j) {
+            const uint8_t *row = &rows[j * pstride[p]];
+            for (uint32_t i = 0; i < pw[p]; ++i) {
+                const uint32_t x = interlace ? i * xs[p] + xo[p] : i, y = interlace ? j * ys[p] + yo[p] : j;
+                uint8_t *o = &px[((size_t)y * w + x) * out_ch];
+                if (ctype == 3) {
+                    const size_t bit = (size_t)i * depth;
+                    const int idx = depth == 8 ? row[i] : (row[bit / 8] >> (8 - depth - bit % 8)) & ((1 << depth) - 1);
+                    if ((size_t)idx * 3 + 2 >= pal.size()) return fail(err, "PNG palette index out of range", 2);
+                    o[0] = pal[3 * idx];
+                    o[1] = pal[3 * idx + 1];
+                    o[2] = pal[3 * idx + 2];
+                    if (out_ch == 4) o[3] = (size_t)idx < trns.size() ? trns[idx] : 255;
+                    continue;
+                }
+                bool key = out_ch == 4 && ctype == 2 && trns.size() >= 6;
+                for (int c = 0; c < file_ch; ++c) {
+                    if (depth == 16) {
+                        const uint8_t *sv = &row[((size_t)i * file_ch + c) * 2];
+                        o[c] = sv[0];  // the high byte: stb's 16 -> 8 bit reduction
+                        key = key && sv[0] == trns[2 * c] && sv[1] == trns[2 * c + 1];
+                    } else {
+                        o[c] = row[(size_t)i * file_ch + c];
+                        key = key && o[c] == trns[2 * c + 1];  // stb keeps the key's low byte
+                    }
+                }
+                if (out_ch == 4 && ctype == 2) o[3] = key ? 0 : 255;
             }
-            row[i] = (uint8_t)(in[i] + pred);
         }
     }
-    to_float(px, w, h, ch, out);
+    to_float(px, w, h, out_ch, out);
     return 0;
 }
 
