@@ -21,8 +21,9 @@ using namespace xtamd;
 
 static int usage() {
     std::fprintf(stderr,
-                 "usage: xtrender scene.scn [-renderer stencil|depth|ao|pathtracer|normal|emission|wireframe] [-res WxH] [-samples N]\n"
-                 "       [-rdepth N] [-aa N] [-cam NAME] [-mod a.b.c:v] [-accel kdtree|octree] [-seed N]\n"
+                 "usage: xtrender scene.scn [-renderer stencil|depth|ao|pathtracer|normal|uv|emission|wireframe] [-res WxH]\n"
+                 "       [-samples N] [-rdepth N] [-aa N] [-tile_size N] [-threads N] [-cam NAME] [-mod a.b.c:v]\n"
+                 "       [-accel kdtree|octree] [-seed N]\n"
                  "       [-outdir DIR] [-out FILE(.png|.bmp|.tga|.hdr|.exr|.pfm|.ppm|.raw)]\n");
     return 1;
 }
@@ -45,6 +46,8 @@ int main(int argc, char **argv) {
         } else if (a == "-samples") ctx.params.samples = std::strtoul(v, nullptr, 10);
         else if (a == "-rdepth") ctx.params.rdepth = std::strtoul(v, nullptr, 10);
         else if (a == "-aa") ctx.params.aa = std::strtoul(v, nullptr, 10);
+        else if (a == "-tile_size") ctx.params.tile_size = std::strtoul(v, nullptr, 10);
+        else if (a == "-threads") ctx.params.threads = std::strtoul(v, nullptr, 10);  // CPU threads: no effect here
         else if (a == "-cam") ctx.params.camera = v;
         else if (a == "-mod") mods.push_back(v);
         else if (a == "-accel") ctx.params.accel = std::strcmp(v, "octree") ? XT_ACCEL_KDTREE : XT_ACCEL_OCTREE;
