@@ -199,6 +199,69 @@ def test_hdr_matches_reference_loader(w, h, writer, tmp_path):
     assert np.array_equal(xa.read_image(path).view(np.uint32), ref.view(np.uint32))
 
 
+def _jpeg(path, w, h, seed, **kw):
+    """A JPEG written by Pillow from a smooth pattern plus noise."""
+    Image = pytest.importorskip("PIL.Image")
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = np.stack([(xx * 5) % 256, (yy * 7) % 256, ((xx + yy) * 3) % 256], -1) // 2
+    px = (base + np.random.default_rng(seed).integers(0, 60, (h, w, 3))).astype(np.uint8)
+    Image.fromarray(px).save(path, **kw)
+    return path
+
+
+JPEGS = {
+    "444_q50": dict(w=53, h=37, quality=50, subsampling=0),
+    "422_q95": dict(w=53, h=37, quality=95, subsampling=1),
+    "420_q75": dict(w=53, h=37, quality=75, subsampling=2),
+    "420_odd": dict(w=17, h=9, quality=85, subsampling=2),
+    "420_one_pixel": dict(w=1, h=1, quality=85, subsampling=2),
+    "420_narrow": dict(w=3, h=40, quality=60, subsampling=2),
+    "420_optimized": dict(w=64, h=48, quality=80, subsampling=2, optimize=True),
+    "420_restarts": dict(w=70, h=50, quality=80, subsampling=2, restart_marker_blocks=3),
+    "444_restart_rows": dict(w=40, h=33, quality=90, subsampling=0, restart_marker_rows=1),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(JPEGS))
+def test_jpeg_matches_reference_loader(name, tmp_path):
+    """Baseline JPEG (4:4:4, 4:2:2, 4:2:0 chroma, odd and one-pixel sizes,
+    optimised Huffman tables, restart intervals) equals the reference's
+    stb_image decode bit for bit: integer IDCT, chroma upsampling and
+    fixed-point YCbCr -> RGB."""
+    kw = dict(JPEGS[name])
+    path = _jpeg(tmp_path / "t.jpg", kw.pop("w"), kw.pop("h"), 7, **kw)
+    ref = O.ref_load_image(path)
+    assert ref is not None
+    assert np.array_equal(xa.read_image(path).view(np.uint32), ref.view(np.uint32))
+
+
+def test_jpeg_close_to_pillow(tmp_path):
+    """Independent of the reference: the decode is within a few levels of
+    Pillow's (another IDCT and upsampler)."""
+    Image = pytest.importorskip("PIL.Image")
+    path = _jpeg(tmp_path / "t.jpg", 53, 37, 3, quality=90, subsampling=0)
+    mine = np.round(xa.read_image(path)[..., :3] * 255)
+    pil = np.asarray(Image.open(path).convert("RGB"), np.float64)
+    assert np.abs(mine - pil).max() <= 4
+
+
+@pytest.mark.parametrize("kind", ["grey", "progressive"])
+def test_jpeg_refused(kind, tmp_path):
+    """Grey JPEGs have one channel, which the reference refuses too;
+    progressive JPEGs are not decoded (the reference's stb reads them)."""
+    Image = pytest.importorskip("PIL.Image")
+    path = tmp_path / "t.jpg"
+    if kind == "grey":
+        Image.fromarray(np.arange(64 * 32, dtype=np.uint16).reshape(32, 64).astype(np.uint8)).save(path)
+    else:
+        _jpeg(path, 40, 30, 1, progressive=True)
+    with pytest.raises(xa.XtError):
+        xa.read_image(path)
+    if kind == "grey" and O.ref_available():
+        assert O.ref_load_image(path) is None
+
+
 @needs_ref
 @pytest.mark.parametrize("name", sorted(STB_214_DIFFERS))
 def test_stb_214_differs_from_the_specification(name, tmp_path):

@@ -381,6 +381,449 @@ int load_hdr(const std::vector<uint8_t> &f, Image &out, std::string *err) {
     return 0;
 }
 
+// Baseline JPEG (SOF0 / SOF1, Huffman, 8-bit, one scan) as stb_image 2.14
+// decodes it for stbi_load: coefficients times the quantiser kept as 16-bit
+// values, the integer IDCT of the IJG "islow" method with stb's 12-bit
+// constants and roundings, chroma upsampled by stb's row filters (triangle
+// 2x horizontal / vertical / both, replication otherwise), YCbCr -> RGB in
+// stb's reduced-precision fixed point. Three components give RGB; one gives
+// grey, which the reference refuses. Progressive and arithmetic-coded files
+// are not decoded.
+namespace jpg {
+
+const uint8_t dezigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,
+                              12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6,  7,  14, 21, 28,
+                              35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
+                              58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+
+struct Huff {
+    // canonical code lengths -> (code, length) -> symbol
+    std::vector<uint16_t> code;
+    std::vector<uint8_t> len, sym;
+    bool ok = false;
+};
+
+struct Comp {
+    int id = 0, h = 1, v = 1, tq = 0, td = 0, ta = 0;
+    int x = 0, y = 0, w2 = 0, h2 = 0, dc = 0;
+    std::vector<uint8_t> plane;
+};
+
+struct Bits {
+    const std::vector<uint8_t> &f;
+    size_t pos;
+    uint32_t acc = 0;
+    int n = 0;
+    bool marker = false;  // a marker was met: zeros are fed from here on
+    void reset(size_t p) {
+        pos = p;
+        acc = 0;
+        n = 0;
+        marker = false;
+    }
+    int fill() {
+        while (n <= 24) {
+            int b = 0;
+            if (!marker && pos < f.size()) {
+                b = f[pos];
+                if (b == 0xff) {
+                    const int c = pos + 1 < f.size() ? f[pos + 1] : 0;
+                    if (c == 0) {
+                        pos += 2;
+                    } else {
+                        marker = true;
+                        b = 0;
+                    }
+                } else {
+                    ++pos;
+                }
+            }
+            acc |= (uint32_t)b << (24 - n);
+            n += 8;
+        }
+        return 0;
+    }
+    int bit() {
+        if (n < 1) fill();
+        const int r = (int)(acc >> 31);
+        acc <<= 1;
+        --n;
+        return r;
+    }
+    int get(int k) {
+        int r = 0;
+        for (int i = 0; i < k; ++i) r = (r << 1) | bit();
+        return r;
+    }
+};
+
+// the symbol of the next Huffman code, or -1
+int decode(Bits &b, const Huff &h) {
+    int code = 0;
+    size_t k = 0;
+    for (int l = 1; l <= 16; ++l) {
+        code = (code << 1) | b.bit();
+        while (k < h.len.size() && h.len[k] == l) {
+            if (h.code[k] == code) return h.sym[k];
+            ++k;
+        }
+    }
+    return -1;
+}
+
+// JPEG's receive + extend: `k` bits as a signed magnitude category value
+int extend(Bits &b, int k) {
+    if (!k) return 0;
+    const int v = b.get(k);
+    return v < (1 << (k - 1)) ? v - (1 << k) + 1 : v;
+}
+
+inline uint8_t clamp8(int x) { return (unsigned)x > 255 ? (x < 0 ? 0 : 255) : (uint8_t)x; }
+inline int f2f(double x) { return (int)(x * 4096 + 0.5); }
+
+// one pass of the islow 1-D IDCT over s0..s7; results in x0..x3, t0..t3
+struct Idct1 {
+    int t0, t1, t2, t3, x0, x1, x2, x3;
+    Idct1(int s0, int s1, int s2, int s3, int s4, int s5, int s6, int s7) {
+        int p1, p2, p3, p4, p5;
+        p2 = s2;
+        p3 = s6;
+        p1 = (p2 + p3) * f2f(0.5411961f);
+        t2 = p1 + p3 * f2f(-1.847759065f);
+        t3 = p1 + p2 * f2f(0.765366865f);
+        p2 = s0;
+        p3 = s4;
+        t0 = (p2 + p3) * 4096;
+        t1 = (p2 - p3) * 4096;
+        x0 = t0 + t3;
+        x3 = t0 - t3;
+        x1 = t1 + t2;
+        x2 = t1 - t2;
+        t0 = s7;
+        t1 = s5;
+        t2 = s3;
+        t3 = s1;
+        p3 = t0 + t2;
+        p4 = t1 + t3;
+        p1 = t0 + t3;
+        p2 = t1 + t2;
+        p5 = (p3 + p4) * f2f(1.175875602f);
+        t0 = t0 * f2f(0.298631336f);
+        t1 = t1 * f2f(2.053119869f);
+        t2 = t2 * f2f(3.072711026f);
+        t3 = t3 * f2f(1.501321110f);
+        p1 = p5 + p1 * f2f(-0.899976223f);
+        p2 = p5 + p2 * f2f(-2.562915447f);
+        p3 = p3 * f2f(-1.961570560f);
+        p4 = p4 * f2f(-0.390180644f);
+        t3 += p1 + p4;
+        t2 += p2 + p3;
+        t1 += p2 + p4;
+        t0 += p1 + p3;
+    }
+};
+
+void idct(uint8_t *out, int stride, const int16_t d[64]) {
+    int val[64];
+    for (int i = 0; i < 8; ++i) {  // columns, 2 extra bits kept
+        const int16_t *c = d + i;
+        int *v = val + i;
+        if (!c[8] && !c[16] && !c[24] && !c[32] && !c[40] && !c[48] && !c[56]) {
+            const int dc = c[0] * 4;
+            for (int k = 0; k < 8; ++k) v[8 * k] = dc;
+            continue;
+        }
+        Idct1 q(c[0], c[8], c[16], c[24], c[32], c[40], c[48], c[56]);
+        q.x0 += 512, q.x1 += 512, q.x2 += 512, q.x3 += 512;
+        v[0] = (q.x0 + q.t3) >> 10;
+        v[56] = (q.x0 - q.t3) >> 10;
+        v[8] = (q.x1 + q.t2) >> 10;
+        v[48] = (q.x1 - q.t2) >> 10;
+        v[16] = (q.x2 + q.t1) >> 10;
+        v[40] = (q.x2 - q.t1) >> 10;
+        v[24] = (q.x3 + q.t0) >> 10;
+        v[32] = (q.x3 - q.t0) >> 10;
+    }
+    for (int i = 0; i < 8; ++i) {  // rows: remove 1 << 17, round, add 128
+        const int *v = val + 8 * i;
+        uint8_t *o = out + i * stride;
+        Idct1 q(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+        const int bias = 65536 + (128 << 17);
+        q.x0 += bias, q.x1 += bias, q.x2 += bias, q.x3 += bias;
+        o[0] = clamp8((q.x0 + q.t3) >> 17);
+        o[7] = clamp8((q.x0 - q.t3) >> 17);
+        o[1] = clamp8((q.x1 + q.t2) >> 17);
+        o[6] = clamp8((q.x1 - q.t2) >> 17);
+        o[2] = clamp8((q.x2 + q.t1) >> 17);
+        o[5] = clamp8((q.x2 - q.t1) >> 17);
+        o[3] = clamp8((q.x3 + q.t0) >> 17);
+        o[4] = clamp8((q.x3 - q.t0) >> 17);
+    }
+}
+
+// stb's row upsamplers: `near` is the nearer source row, `far` the other
+void up_v2(uint8_t *o, const uint8_t *nr, const uint8_t *fr, int w) {
+    for (int i = 0; i < w; ++i) o[i] = (uint8_t)((3 * nr[i] + fr[i] + 2) >> 2);
+}
+void up_h2(uint8_t *o, const uint8_t *in, int w) {
+    if (w == 1) {
+        o[0] = o[1] = in[0];
+        return;
+    }
+    o[0] = in[0];
+    o[1] = (uint8_t)((in[0] * 3 + in[1] + 2) >> 2);
+    int i;
+    for (i = 1; i < w - 1; ++i) {
+        const int n = 3 * in[i] + 2;
+        o[2 * i] = (uint8_t)((n + in[i - 1]) >> 2);
+        o[2 * i + 1] = (uint8_t)((n + in[i + 1]) >> 2);
+    }
+    o[2 * i] = (uint8_t)((in[w - 2] * 3 + in[w - 1] + 2) >> 2);
+    o[2 * i + 1] = in[w - 1];
+}
+void up_hv2(uint8_t *o, const uint8_t *nr, const uint8_t *fr, int w) {
+    if (w == 1) {
+        o[0] = o[1] = (uint8_t)((3 * nr[0] + fr[0] + 2) >> 2);
+        return;
+    }
+    int t1 = 3 * nr[0] + fr[0], t0;
+    o[0] = (uint8_t)((t1 + 2) >> 2);
+    for (int i = 1; i < w; ++i) {
+        t0 = t1;
+        t1 = 3 * nr[i] + fr[i];
+        o[2 * i - 1] = (uint8_t)((3 * t0 + t1 + 8) >> 4);
+        o[2 * i] = (uint8_t)((3 * t1 + t0 + 8) >> 4);
+    }
+    o[2 * w - 1] = (uint8_t)((t1 + 2) >> 2);
+}
+
+}  // namespace jpg
+
+int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
+    using namespace jpg;
+    int16_t q[4][64] = {};
+    Huff hd[4], ha[4];
+    std::vector<Comp> comps;
+    uint32_t W = 0, H = 0;
+    int restart = 0, hmax = 1, vmax = 1;
+    size_t pos = 2;
+    auto u16 = [&](size_t p) { return p + 1 < f.size() ? (int)(f[p] << 8 | f[p + 1]) : 0; };
+    bool frame = false;
+    for (;;) {
+        while (pos < f.size() && f[pos] != 0xff) ++pos;
+        while (pos < f.size() && f[pos] == 0xff) ++pos;
+        if (pos >= f.size()) return fail(err, "JPEG without image data", 2);
+        const int m = f[pos++];
+        if (m == 0xd8 || (m >= 0xd0 && m <= 0xd7) || m == 0x01) continue;
+        if (m == 0xd9) return fail(err, "JPEG without scan", 2);
+        const int len = u16(pos);
+        if (len < 2 || pos + len > f.size()) return fail(err, "truncated JPEG", 2);
+        const uint8_t *d = &f[pos + 2];
+        const int n = len - 2;
+        if (m == 0xdb) {  // quantisation tables, stored in natural order
+            for (int k = 0; k < n;) {
+                const int pq = d[k] >> 4, tq = d[k] & 15;
+                ++k;
+                if (tq > 3 || k + 64 * (pq + 1) > n) return fail(err, "bad JPEG DQT", 2);
+                for (int i = 0; i < 64; ++i, k += pq + 1)
+                    q[tq][dezigzag[i]] = (int16_t)(pq ? (d[k] << 8 | d[k + 1]) : d[k]);
+            }
+        } else if (m == 0xc4) {  // Huffman tables
+            for (int k = 0; k < n;) {
+                const int tc = d[k] >> 4, th = d[k] & 15;
+                if (tc > 1 || th > 3 || k + 17 > n) return fail(err, "bad JPEG DHT", 2);
+                Huff h;
+                int code = 0, total = 0;
+                for (int l = 1; l <= 16; ++l) total += d[k + l];
+                if (k + 17 + total > n) return fail(err, "bad JPEG DHT", 2);
+                int at = k + 17;
+                for (int l = 1; l <= 16; ++l) {
+                    for (int i = 0; i < d[k + l]; ++i) {
+                        h.code.push_back((uint16_t)code++);
+                        h.len.push_back((uint8_t)l);
+                        h.sym.push_back(d[at++]);
+                    }
+                    code <<= 1;
+                }
+                h.ok = true;
+                (tc ? ha : hd)[th] = h;
+                k = at;
+            }
+        } else if (m == 0xdd) {
+            restart = u16(pos + 2);
+        } else if (m == 0xc0 || m == 0xc1) {
+            if (n < 6 || d[0] != 8) return fail(err, "JPEG precision not decoded", 2);
+            H = (uint32_t)(d[1] << 8 | d[2]);
+            W = (uint32_t)(d[3] << 8 | d[4]);
+            const int nc = d[5];
+            if (!W || !H || W > 65535 || H > 65535 || (nc != 1 && nc != 3) || n < 6 + 3 * nc)
+                return fail(err, "JPEG frame not decoded", 2);
+            for (int c = 0; c < nc; ++c) {
+                Comp k;
+                k.id = d[6 + 3 * c];
+                k.h = d[7 + 3 * c] >> 4;
+                k.v = d[7 + 3 * c] & 15;
+                k.tq = d[8 + 3 * c];
+                if (k.h < 1 || k.h > 4 || k.v < 1 || k.v > 4 || k.tq > 3) return fail(err, "bad JPEG frame", 2);
+                hmax = std::max(hmax, k.h);
+                vmax = std::max(vmax, k.v);
+                comps.push_back(k);
+            }
+            frame = true;
+        } else if (m >= 0xc2 && m <= 0xcf && m != 0xc4 && m != 0xc8 && m != 0xcc) {
+            return fail(err, "progressive / arithmetic / lossless JPEG not decoded", 2);
+        } else if (m == 0xda) {
+            if (!frame) return fail(err, "JPEG scan before frame", 2);
+            pos += len;
+            const int ns = d[0];
+            if (ns != (int)comps.size()) return fail(err, "JPEG with several scans not decoded", 2);
+            std::vector<int> order;
+            for (int s = 0; s < ns; ++s) {
+                const int id = d[1 + 2 * s];
+                int c = -1;
+                for (size_t k = 0; k < comps.size(); ++k)
+                    if (comps[k].id == id) c = (int)k;
+                if (c < 0) return fail(err, "bad JPEG scan", 2);
+                comps[c].td = d[2 + 2 * s] >> 4;
+                comps[c].ta = d[2 + 2 * s] & 15;
+                if (comps[c].td > 3 || comps[c].ta > 3 || !hd[comps[c].td].ok || !ha[comps[c].ta].ok)
+                    return fail(err, "JPEG Huffman table missing", 2);
+                order.push_back(c);
+            }
+            // frame layout (MCU-padded planes)
+            const int mcuw = hmax * 8, mcuh = vmax * 8;
+            const int mx = ((int)W + mcuw - 1) / mcuw, my = ((int)H + mcuh - 1) / mcuh;
+            for (Comp &c : comps) {
+                c.x = ((int)W * c.h + hmax - 1) / hmax;
+                c.y = ((int)H * c.v + vmax - 1) / vmax;
+                c.w2 = mx * c.h * 8;
+                c.h2 = my * c.v * 8;
+                c.plane.assign((size_t)c.w2 * c.h2, 0);
+                c.dc = 0;
+            }
+            Bits b{f, pos};
+            int todo = restart ? restart : 1 << 30;
+            auto block = [&](Comp &c, int bx, int by) -> bool {
+                int16_t data[64] = {};
+                const int t = decode(b, hd[c.td]);
+                if (t < 0 || t > 16) return false;
+                const int diff = t ? extend(b, t) : 0;
+                c.dc += diff;
+                data[0] = (int16_t)(c.dc * q[c.tq][0]);
+                for (int k = 1; k < 64;) {
+                    const int rs = decode(b, ha[c.ta]);
+                    if (rs < 0) return false;
+                    const int r = rs >> 4, s = rs & 15;
+                    if (!s) {
+                        if (r != 15) break;  // end of block
+                        k += 16;
+                        continue;
+                    }
+                    k += r;
+                    if (k > 63) return false;
+                    const int z = dezigzag[k++];
+                    data[z] = (int16_t)(extend(b, s) * q[c.tq][z]);
+                }
+                idct(&c.plane[(size_t)by * 8 * c.w2 + (size_t)bx * 8], c.w2, data);
+                return true;
+            };
+            // after each restart interval: the next marker must be RSTn, which
+            // resets the bit reader and the DC predictions; any other marker
+            // ends the scan there, as in stb
+            auto restart_marker = [&]() -> bool {
+                if (--todo > 0) return true;
+                size_t p = b.pos;
+                while (p + 1 < f.size() && !(f[p] == 0xff && f[p + 1] != 0 && f[p + 1] != 0xff)) ++p;
+                if (p + 1 >= f.size() || f[p + 1] < 0xd0 || f[p + 1] > 0xd7) return false;
+                b.reset(p + 2);
+                for (Comp &c : comps) c.dc = 0;
+                todo = restart;
+                return true;
+            };
+            if (ns == 1) {
+                Comp &c = comps[order[0]];
+                const int bw = (c.x + 7) >> 3, bh = (c.y + 7) >> 3;
+                for (int by = 0; by < bh; ++by)
+                    for (int bx = 0; bx < bw; ++bx) {
+                        if (!block(c, bx, by)) return fail(err, "corrupt JPEG data", 2);
+                        if (!restart_marker()) by = bh, bx = bw;
+                    }
+            } else {
+                for (int j = 0; j < my; ++j)
+                    for (int i = 0; i < mx; ++i) {
+                        for (int c : order)
+                            for (int y = 0; y < comps[c].v; ++y)
+                                for (int x = 0; x < comps[c].h; ++x)
+                                    if (!block(comps[c], i * comps[c].h + x, j * comps[c].v + y))
+                                        return fail(err, "corrupt JPEG data", 2);
+                        if (!restart_marker()) j = my, i = mx;
+                    }
+            }
+            break;
+        }
+        pos += len;
+    }
+    if (comps.size() != 3) return fail(err, "grey JPEG: 1 channel, which the reference refuses", 2);
+    // upsample each component row by row as stb's load_jpeg_image does, then
+    // convert to RGB
+    struct Res {
+        int hs, vs, ystep, ypos, wlo;
+        size_t line0, line1;
+    };
+    Res r[3];
+    for (int k = 0; k < 3; ++k) {
+        r[k].hs = hmax / comps[k].h;
+        r[k].vs = vmax / comps[k].v;
+        r[k].ystep = r[k].vs >> 1;
+        r[k].wlo = ((int)W + r[k].hs - 1) / r[k].hs;
+        r[k].ypos = 0;
+        r[k].line0 = r[k].line1 = 0;
+    }
+    std::vector<uint8_t> px((size_t)W * H * 3), buf[3];
+    for (int k = 0; k < 3; ++k) buf[k].assign((size_t)W + 3, 0);
+    const uint8_t *row[3];
+    for (uint32_t j = 0; j < H; ++j) {
+        for (int k = 0; k < 3; ++k) {
+            Res &s = r[k];
+            const Comp &c = comps[k];
+            const bool bot = s.ystep >= (s.vs >> 1);
+            const uint8_t *nr = &c.plane[bot ? s.line1 : s.line0], *fr = &c.plane[bot ? s.line0 : s.line1];
+            uint8_t *o = buf[k].data();
+            if (s.hs == 1 && s.vs == 1) {
+                row[k] = nr;
+            } else {
+                if (s.hs == 1 && s.vs == 2) up_v2(o, nr, fr, s.wlo);
+                else if (s.hs == 2 && s.vs == 1) up_h2(o, nr, s.wlo);
+                else if (s.hs == 2 && s.vs == 2) up_hv2(o, nr, fr, s.wlo);
+                else
+                    for (int i = 0; i < s.wlo; ++i)
+                        for (int t = 0; t < s.hs; ++t) o[i * s.hs + t] = nr[i];
+                row[k] = o;
+            }
+            if (++s.ystep >= s.vs) {
+                s.ystep = 0;
+                s.line0 = s.line1;
+                if (++s.ypos < c.y) s.line1 += (size_t)c.w2;
+            }
+        }
+        uint8_t *o = &px[(size_t)j * W * 3];
+        for (uint32_t i = 0; i < W; ++i) {
+            // stb's reduced-precision YCbCr -> RGB (20-bit fixed point)
+            const int yf = (row[0][i] << 20) + (1 << 19);
+            const int cr = row[2][i] - 128, cb = row[1][i] - 128;
+            auto fx = [](float x) { return ((int)(x * 4096.0f + 0.5f)) << 8; };
+            int rr = yf + cr * fx(1.40200f);
+            int gg = yf + (cr * -fx(0.71414f)) + ((cb * -fx(0.34414f)) & (int)0xffff0000);
+            int bb = yf + cb * fx(1.77200f);
+            o[3 * i] = clamp8(rr >> 20);
+            o[3 * i + 1] = clamp8(gg >> 20);
+            o[3 * i + 2] = clamp8(bb >> 20);
+        }
+    }
+    to_float(px, W, H, 3, out);
+    return 0;
+}
+
 }  // namespace
 
 int load_image(const std::string &path, Image &out, std::string *err) {
@@ -393,6 +836,7 @@ int load_image(const std::string &path, Image &out, std::string *err) {
     else if (f.size() >= 2 && f[0] == 'B' && f[1] == 'M') rc = load_bmp(f, out, err);
     else if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') rc = load_ppm(f, out, err);
     else if (f.size() >= 2 && f[0] == '#' && f[1] == '?') rc = load_hdr(f, out, err);
+    else if (f.size() >= 3 && f[0] == 0xff && f[1] == 0xd8 && f[2] == 0xff) rc = load_jpeg(f, out, err);
     else if (path.size() > 4 && (path.substr(path.size() - 4) == ".tga" || path.substr(path.size() - 4) == ".TGA"))
         rc = load_tga(f, out, err);
     else rc = fail(err, "image format not decoded: " + path, 2);
