@@ -187,8 +187,9 @@ int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t
  * files it reads as 8-bit RGB / RGBA: PNG (RGB, RGBA, palette; any depth;
  * Adam7; tRNS), TGA (raw or
  * run-length, 24 / 32 bit), BMP (uncompressed 24 / 32 bit), PPM (P6, maxval
- * 255), Radiance HDR (converted to 8 bits as stb_image does), baseline JPEG
- * (stb_image's integer IDCT, chroma upsampling and YCbCr conversion). With rgba NULL only *width / *height are set. XT_E_IO when the file
+ * 255), Radiance HDR (converted to 8 bits as stb_image does), baseline and
+ * progressive JPEG (stb_image's integer IDCT, chroma upsampling and YCbCr
+ * conversion). With rgba NULL only *width / *height are set. XT_E_IO when the file
  * cannot be read, XT_E_INVALID for a format or variant not decoded. */
 int xt_image_read(const char *path, float *rgba, uint32_t *width, uint32_t *height);
 

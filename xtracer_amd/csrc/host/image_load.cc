@@ -381,14 +381,14 @@ int load_hdr(const std::vector<uint8_t> &f, Image &out, std::string *err) {
     return 0;
 }
 
-// Baseline JPEG (SOF0 / SOF1, Huffman, 8-bit, one scan) as stb_image 2.14
-// decodes it for stbi_load: coefficients times the quantiser kept as 16-bit
+// JPEG (baseline SOF0 / SOF1 in one scan, or progressive SOF2 over several;
+// Huffman, 8-bit) as stb_image 2.14 decodes it for stbi_load: coefficients times the quantiser kept as 16-bit
 // values, the integer IDCT of the IJG "islow" method with stb's 12-bit
 // constants and roundings, chroma upsampled by stb's row filters (triangle
 // 2x horizontal / vertical / both, replication otherwise), YCbCr -> RGB in
 // stb's reduced-precision fixed point. Three components give RGB; one gives
-// grey, which the reference refuses. Progressive and arithmetic-coded files
-// are not decoded.
+// grey, which the reference refuses. Arithmetic-coded and lossless files are
+// not decoded.
 namespace jpg {
 
 const uint8_t dezigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,
@@ -407,6 +407,7 @@ struct Comp {
     int id = 0, h = 1, v = 1, tq = 0, td = 0, ta = 0;
     int x = 0, y = 0, w2 = 0, h2 = 0, dc = 0;
     std::vector<uint8_t> plane;
+    std::vector<int16_t> coef;  // progressive: 64 per block, w2 / 8 blocks a row
 };
 
 struct Bits {
@@ -608,14 +609,17 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
     int restart = 0, hmax = 1, vmax = 1;
     size_t pos = 2;
     auto u16 = [&](size_t p) { return p + 1 < f.size() ? (int)(f[p] << 8 | f[p + 1]) : 0; };
-    bool frame = false;
+    bool frame = false, progressive = false, laid_out = false;
     for (;;) {
         while (pos < f.size() && f[pos] != 0xff) ++pos;
         while (pos < f.size() && f[pos] == 0xff) ++pos;
         if (pos >= f.size()) return fail(err, "JPEG without image data", 2);
         const int m = f[pos++];
         if (m == 0xd8 || (m >= 0xd0 && m <= 0xd7) || m == 0x01) continue;
-        if (m == 0xd9) return fail(err, "JPEG without scan", 2);
+        if (m == 0xd9) {
+            if (progressive && laid_out) break;  // the scans are all in
+            return fail(err, "JPEG without scan", 2);
+        }
         const int len = u16(pos);
         if (len < 2 || pos + len > f.size()) return fail(err, "truncated JPEG", 2);
         const uint8_t *d = &f[pos + 2];
@@ -651,7 +655,8 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
             }
         } else if (m == 0xdd) {
             restart = u16(pos + 2);
-        } else if (m == 0xc0 || m == 0xc1) {
+        } else if (m == 0xc0 || m == 0xc1 || m == 0xc2) {
+            progressive = m == 0xc2;
             if (n < 6 || d[0] != 8) return fail(err, "JPEG precision not decoded", 2);
             H = (uint32_t)(d[1] << 8 | d[2]);
             W = (uint32_t)(d[3] << 8 | d[4]);
@@ -670,13 +675,15 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
                 comps.push_back(k);
             }
             frame = true;
-        } else if (m >= 0xc2 && m <= 0xcf && m != 0xc4 && m != 0xc8 && m != 0xcc) {
-            return fail(err, "progressive / arithmetic / lossless JPEG not decoded", 2);
+        } else if (m >= 0xc3 && m <= 0xcf && m != 0xc4 && m != 0xc8 && m != 0xcc) {
+            return fail(err, "arithmetic / lossless JPEG not decoded", 2);
         } else if (m == 0xda) {
             if (!frame) return fail(err, "JPEG scan before frame", 2);
             pos += len;
             const int ns = d[0];
-            if (ns != (int)comps.size()) return fail(err, "JPEG with several scans not decoded", 2);
+            if (!progressive && ns != (int)comps.size()) return fail(err, "JPEG with several scans not decoded", 2);
+            if (ns < 1 || ns > (int)comps.size() || n < 4 + 2 * ns) return fail(err, "bad JPEG scan", 2);
+            const int ss = d[1 + 2 * ns], se = d[2 + 2 * ns], ah = d[3 + 2 * ns] >> 4, al = d[3 + 2 * ns] & 15;
             std::vector<int> order;
             for (int s = 0; s < ns; ++s) {
                 const int id = d[1 + 2 * s];
@@ -686,22 +693,108 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
                 if (c < 0) return fail(err, "bad JPEG scan", 2);
                 comps[c].td = d[2 + 2 * s] >> 4;
                 comps[c].ta = d[2 + 2 * s] & 15;
-                if (comps[c].td > 3 || comps[c].ta > 3 || !hd[comps[c].td].ok || !ha[comps[c].ta].ok)
+                if (comps[c].td > 3 || comps[c].ta > 3) return fail(err, "bad JPEG scan", 2);
+                const bool need_dc = !progressive || (ss == 0 && ah == 0), need_ac = !progressive || ss > 0;
+                if ((need_dc && !hd[comps[c].td].ok) || (need_ac && !ha[comps[c].ta].ok))
                     return fail(err, "JPEG Huffman table missing", 2);
                 order.push_back(c);
             }
             // frame layout (MCU-padded planes)
             const int mcuw = hmax * 8, mcuh = vmax * 8;
             const int mx = ((int)W + mcuw - 1) / mcuw, my = ((int)H + mcuh - 1) / mcuh;
-            for (Comp &c : comps) {
-                c.x = ((int)W * c.h + hmax - 1) / hmax;
-                c.y = ((int)H * c.v + vmax - 1) / vmax;
-                c.w2 = mx * c.h * 8;
-                c.h2 = my * c.v * 8;
-                c.plane.assign((size_t)c.w2 * c.h2, 0);
-                c.dc = 0;
-            }
+            if (!laid_out)
+                for (Comp &c : comps) {
+                    c.x = ((int)W * c.h + hmax - 1) / hmax;
+                    c.y = ((int)H * c.v + vmax - 1) / vmax;
+                    c.w2 = mx * c.h * 8;
+                    c.h2 = my * c.v * 8;
+                    c.plane.assign((size_t)c.w2 * c.h2, 0);
+                    if (progressive) c.coef.assign((size_t)c.w2 * c.h2, 0);
+                }
+            laid_out = true;
+            for (Comp &c : comps) c.dc = 0;
+            int eob_run = 0;
             Bits b{f, pos};
+            // progressive blocks (stb's decode_block_prog_dc / _ac): values
+            // kept unquantised in the coefficient store until the end
+            auto prog_block = [&](Comp &c, int bx, int by) -> bool {
+                int16_t *data = &c.coef[((size_t)by * (c.w2 / 8) + bx) * 64];
+                if (ss == 0) {
+                    if (se != 0) return false;
+                    if (ah == 0) {
+                        for (int k = 1; k < 64; ++k) data[k] = 0;
+                        const int t = decode(b, hd[c.td]);
+                        if (t < 0 || t > 16) return false;
+                        c.dc += t ? extend(b, t) : 0;
+                        data[0] = (int16_t)(c.dc * (1 << al));
+                    } else if (b.bit()) {
+                        data[0] = (int16_t)(data[0] + (1 << al));
+                    }
+                    return true;
+                }
+                if (ah == 0) {
+                    if (eob_run) {
+                        --eob_run;
+                        return true;
+                    }
+                    int k = ss;
+                    do {
+                        const int rs = decode(b, ha[c.ta]);
+                        if (rs < 0) return false;
+                        const int s = rs & 15, r = rs >> 4;
+                        if (s == 0) {
+                            if (r < 15) {
+                                eob_run = (1 << r) + (r ? b.get(r) : 0) - 1;
+                                break;
+                            }
+                            k += 16;
+                        } else {
+                            k += r;
+                            if (k > 63) return false;
+                            data[dezigzag[k++]] = (int16_t)(extend(b, s) * (1 << al));
+                        }
+                    } while (k <= se);
+                    return true;
+                }
+                const int16_t bit = (int16_t)(1 << al);
+                auto refine = [&](int16_t *p) {
+                    if (b.bit() && (*p & bit) == 0) *p = (int16_t)(*p > 0 ? *p + bit : *p - bit);
+                };
+                if (eob_run) {
+                    --eob_run;
+                    for (int k = ss; k <= se; ++k)
+                        if (data[dezigzag[k]] != 0) refine(&data[dezigzag[k]]);
+                    return true;
+                }
+                int k = ss;
+                do {
+                    const int rs = decode(b, ha[c.ta]);
+                    if (rs < 0) return false;
+                    int s = rs & 15, r = rs >> 4;
+                    if (s == 0) {
+                        if (r < 15) {
+                            eob_run = (1 << r) - 1 + (r ? b.get(r) : 0);
+                            r = 64;
+                        }
+                    } else {
+                        if (s != 1) return false;
+                        s = b.bit() ? bit : -bit;
+                    }
+                    while (k <= se) {
+                        int16_t *p = &data[dezigzag[k++]];
+                        if (*p != 0) {
+                            refine(p);
+                        } else {
+                            if (r == 0) {
+                                *p = (int16_t)s;
+                                break;
+                            }
+                            --r;
+                        }
+                    }
+                } while (k <= se);
+                return true;
+            };
             int todo = restart ? restart : 1 << 30;
             auto block = [&](Comp &c, int bx, int by) -> bool {
                 int16_t data[64] = {};
@@ -737,6 +830,7 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
                 if (p + 1 >= f.size() || f[p + 1] < 0xd0 || f[p + 1] > 0xd7) return false;
                 b.reset(p + 2);
                 for (Comp &c : comps) c.dc = 0;
+                eob_run = 0;
                 todo = restart;
                 return true;
             };
@@ -745,7 +839,8 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
                 const int bw = (c.x + 7) >> 3, bh = (c.y + 7) >> 3;
                 for (int by = 0; by < bh; ++by)
                     for (int bx = 0; bx < bw; ++bx) {
-                        if (!block(c, bx, by)) return fail(err, "corrupt JPEG data", 2);
+                        if (!(progressive ? prog_block(c, bx, by) : block(c, bx, by)))
+                            return fail(err, "corrupt JPEG data", 2);
                         if (!restart_marker()) by = bh, bx = bw;
                     }
             } else {
@@ -754,16 +849,27 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
                         for (int c : order)
                             for (int y = 0; y < comps[c].v; ++y)
                                 for (int x = 0; x < comps[c].h; ++x)
-                                    if (!block(comps[c], i * comps[c].h + x, j * comps[c].v + y))
+                                    if (!(progressive ? prog_block(comps[c], i * comps[c].h + x, j * comps[c].v + y)
+                                                      : block(comps[c], i * comps[c].h + x, j * comps[c].v + y)))
                                         return fail(err, "corrupt JPEG data", 2);
                         if (!restart_marker()) j = my, i = mx;
                     }
             }
-            break;
+            if (!progressive) break;
+            pos = b.pos;  // the next marker follows the scan's data
+            continue;
         }
         pos += len;
     }
     if (comps.size() != 3) return fail(err, "grey JPEG: 1 channel, which the reference refuses", 2);
+    if (progressive)  // stb's jpeg_finish: dequantise (16-bit products) and transform every block
+        for (Comp &c : comps)
+            for (int by = 0; by < (c.y + 7) >> 3; ++by)
+                for (int bx = 0; bx < (c.x + 7) >> 3; ++bx) {
+                    int16_t *data = &c.coef[((size_t)by * (c.w2 / 8) + bx) * 64];
+                    for (int k = 0; k < 64; ++k) data[k] = (int16_t)(data[k] * q[c.tq][k]);
+                    idct(&c.plane[(size_t)by * 8 * c.w2 + (size_t)bx * 8], c.w2, data);
+                }
     // upsample each component row by row as stb's load_jpeg_image does, then
     // convert to RGB
     struct Res {

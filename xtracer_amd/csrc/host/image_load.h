@@ -5,8 +5,8 @@
 // specifications for the files stb_image reads that way: PNG (8-bit RGB /
 // RGBA, not interlaced), TGA (true colour, raw or run-length, 24 / 32 bit),
 // BMP (uncompressed 24 / 32 bit), binary PPM (P6, maxval 255), Radiance HDR,
-// which stb_image converts to 8 bits, and baseline JPEG decoded with
-// stb_image's integer arithmetic.
+// which stb_image converts to 8 bits, and baseline / progressive JPEG decoded
+// with stb_image's integer arithmetic.
 #pragma once
 #include <cstdint>
 #include <string>
