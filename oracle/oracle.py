@@ -57,6 +57,45 @@ def camera_rays(cam, width, height, xy):
     return o, d
 
 
+def thin_lens_rays(cam, width, height, xy, jitter):
+    """The oracle's thin-lens rays at sample coordinates xy (n x 2 float32)
+    with the lens jitters given (n x 2 doubles, the values prng_c returned)."""
+    xy = np.ascontiguousarray(xy, np.float32)
+    jitter = np.ascontiguousarray(jitter, np.float64)
+    o = np.zeros((len(xy), 3))
+    d = np.zeros((len(xy), 3))
+    lib().or_thin_lens_rays(ctypes.byref(_job(0, width, height, cam)), len(xy), _p(xy), _p(jitter), _p(o), _p(d))
+    return o, d
+
+
+def lens_jitter(cam, keys):
+    """The lens jitters (n x 2) the oracle draws for sample keys `keys`."""
+    keys = np.ascontiguousarray(keys, np.uint32)
+    out = np.zeros((len(keys), 2))
+    lib().or_lens_jitter(ctypes.byref(_job(0, 1, 1, cam)), len(keys), _p(keys), _p(out))
+    return out
+
+
+def aa_random(width, level, pixels, seed=1, job_key=0, draws=None):
+    """RANDOM anti-aliasing of `pixels` (n x 2 uint32 x, y) at `level`: the
+    float sample coordinates (n, level, 2) and the uniform draws (n, level, 2)
+    they come from; with `draws` given those are used instead of the
+    oracle's own."""
+    pixels = np.ascontiguousarray(pixels, np.uint32)
+    n = len(pixels)
+    coords = np.zeros((n, level, 2), np.float32)
+    out = np.zeros((n, level, 2))
+    cam = {"position": (0, 0, 0)}
+    jb = _job(0, width, 1, cam, aa=level, aa_mode=1, seed=seed, job_key=job_key)
+    dr = None if draws is None else np.ascontiguousarray(draws, np.float64)
+    lib().or_aa_random(ctypes.byref(jb), n, _p(pixels), _p(dr), _p(out), _p(coords))
+    return coords, out
+
+
+def path_key(seed, job_key, pixel, sample):
+    return int(lib().or_path_key(seed, job_key, pixel, sample))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -71,6 +110,11 @@ def lib():
         L.or_render.restype = ctypes.c_uint64
         L.or_render.argtypes = [P, ctypes.POINTER(OrJob), P, P, P]
         L.or_camera_rays.argtypes = [ctypes.POINTER(OrJob), ctypes.c_uint64, P, P, P]
+        L.or_thin_lens_rays.argtypes = [ctypes.POINTER(OrJob), ctypes.c_uint64, P, P, P, P]
+        L.or_lens_jitter.argtypes = [ctypes.POINTER(OrJob), ctypes.c_uint64, P, P]
+        L.or_aa_random.argtypes = [ctypes.POINTER(OrJob), ctypes.c_uint64, P, P, P, P]
+        L.or_path_key.restype = ctypes.c_uint32
+        L.or_path_key.argtypes = [ctypes.c_uint32] * 4
         L.or_mesh_octree.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), P]
         L.or_sample.restype = ctypes.c_uint64
         L.or_sample.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, P]
@@ -215,9 +259,11 @@ def ref():
         L.xr_trace.argtypes = [P, ctypes.c_uint64, P, P, P, P, P]
         L.xr_octree_trace.argtypes = [ctypes.c_uint64, P, ctypes.c_uint64, P, P, P, P]
         L.xr_render.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                ctypes.c_int, ctypes.c_int, ctypes.c_uint, P]
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint, P]
         L.xr_save_image.argtypes = [ctypes.c_int, ctypes.c_char_p, P, ctypes.c_int, ctypes.c_int]
         L.xr_camera_rays.argtypes = [P, ctypes.c_uint64, P, ctypes.c_float, ctypes.c_float, P, P]
+        L.xr_thin_lens_rays.argtypes = [P, ctypes.c_uint64, P, ctypes.c_float, ctypes.c_float, ctypes.c_uint, P, P, P]
+        L.xr_aa_random.argtypes = [ctypes.c_uint64, P, ctypes.c_uint, ctypes.c_uint, P, P, P]
         L.xr_load_image.argtypes = [ctypes.c_char_p, P, P, P]
         L.xr_write_raygraph.argtypes = [P, ctypes.c_char_p]
         _ref = L
@@ -288,9 +334,21 @@ class RefScene:
             raise RuntimeError("scene has no camera")
         return o, d
 
-    def render(self, integrator, width, height, samples=1, aa=1, rdepth=100, threads=1, seed=1):
+    def thin_lens_rays(self, width, height, xy, seed=1):
+        """Thin-lens primary rays of the scene's first camera at xy (n x 2
+        float32), ray i under srand(seed + i), with the two lens jitters each
+        ray drew (n x 2): (origins, directions, jitters)."""
+        xy = np.ascontiguousarray(xy, np.float32)
+        n = len(xy)
+        o, d, j = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 2))
+        if ref().xr_thin_lens_rays(self._h, n, _p(xy), width, height, seed, _p(o), _p(d), _p(j)):
+            raise RuntimeError("scene has no thin-lens camera")
+        return o, d, j
+
+    def render(self, integrator, width, height, samples=1, aa=1, rdepth=100, threads=1, seed=1, aa_mode=0):
         img = np.zeros((height, width, 4), np.float32)
-        rc = ref().xr_render(self._h, integrator, width, height, samples, aa, rdepth, threads, seed, _p(img))
+        rc = ref().xr_render(self._h, integrator, width, height, samples, aa, aa_mode, rdepth, threads, seed,
+                             _p(img))
         if rc:
             raise RuntimeError("xr_render failed")
         return img
@@ -306,6 +364,19 @@ class RefScene:
                 ref().xr_free(self._h)
         except Exception:
             pass
+
+
+def ref_aa_random(pixels, level, seed=1):
+    """antialiasing::gen_samples_random of the real reference for each pixel
+    (n x 2, pixel i under srand(seed + i)): the sample coordinates (n, level,
+    2, doubles), weights (n, level) and the prng_c(0, 1) draws (n, level, 2)
+    they came from."""
+    pixels = np.ascontiguousarray(pixels, np.float32)
+    n = len(pixels)
+    coords, weights, draws = np.zeros((n, level, 2)), np.zeros((n, level)), np.zeros((n, level, 2))
+    if ref().xr_aa_random(n, _p(pixels), level, seed, _p(coords), _p(weights), _p(draws)):
+        raise RuntimeError("xr_aa_random failed")
+    return coords, weights, draws
 
 
 def ref_octree_trace(tris, origins, directions):

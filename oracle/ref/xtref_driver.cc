@@ -22,6 +22,9 @@
 #include <xtcore/strpool.h>
 #include <xtcore/log.h>
 #include <xtcore/util/raygraph.h>
+#include <xtcore/aa.h>
+#include <xtcore/camera/perspective.h>
+#include <nmath/prng.h>
 #include <xtcore/integrator/wireframe/integrator.h>
 #include <nimg/img.h>
 #include <nimg/pixmap.h>
@@ -172,8 +175,9 @@ void xr_octree_trace(uint64_t n_tris, const double *tris, uint64_t n, const doub
 
 // Renders with a reference integrator (0 stencil, 1 depth, 2 ao, 3 pathtracer,
 // 4 normal, 5 uv, 6 emission, 7 wireframe) and assembles the framebuffer into rgba (W*H*4 floats).
-int xr_render(void *h, int integrator, int width, int height, int samples, int aa, int rdepth, int threads,
-              unsigned seed, float *rgba) {
+// aa_mode: 0 GRID, 1 RANDOM (params_t::sample_distribution).
+int xr_render(void *h, int integrator, int width, int height, int samples, int aa, int aa_mode, int rdepth,
+              int threads, unsigned seed, float *rgba) {
     render::context_t *ctx = (render::context_t *)h;
     render::IIntegrator *in = nullptr;
     switch (integrator) {
@@ -194,6 +198,8 @@ int xr_render(void *h, int integrator, int width, int height, int samples, int a
     p.aa = aa;
     p.rdepth = rdepth;
     p.threads = threads;
+    p.sample_distribution = aa_mode == 1 ? antialiasing::SAMPLE_DISTRIBUTION_RANDOM
+                                         : antialiasing::SAMPLE_DISTRIBUTION_GRID;
     p.camera = ctx->scene.m_cameras.empty() ? 0 : ctx->scene.m_cameras.begin()->first;
     ctx->params = p;
     ctx->init();
@@ -284,6 +290,69 @@ int xr_camera_rays(void *h, uint64_t n, const float *xy, float width, float heig
         d[3 * i] = r.direction.x;
         d[3 * i + 1] = r.direction.y;
         d[3 * i + 2] = r.direction.z;
+    }
+    return 0;
+}
+
+// Thin-lens primary rays of the scene's first camera with the lens draws
+// recorded. glibc rand() is one process-global sequence, so after
+// srand(seed + i) the two prng_c(-a/2, a/2) draws get_primary_ray makes
+// (perspective.cc:77-78) are reproduced exactly by seeding again and calling
+// the same prng_c here. Every translation unit holds its own copy of
+// prng_c's "initialized" flag; both copies are touched first so that neither
+// re-seeds from time(NULL) later. jitter receives 2 doubles per ray.
+int xr_thin_lens_rays(void *h, uint64_t n, const float *xy, float width, float height, unsigned seed, double *o,
+                      double *d, double *jitter) {
+    render::context_t *ctx = (render::context_t *)h;
+    if (ctx->scene.m_cameras.empty()) return 1;
+    asset::ICamera *cam = ctx->scene.get_camera(ctx->scene.m_cameras.begin()->first);
+    camera::Perspective *p = dynamic_cast<camera::Perspective *>(cam);
+    if (!p) return 1;
+    (void)p->get_primary_ray(0.f, 0.f, width, height);
+    (void)nmath::prng_c(0.0, 1.0);
+    const nmath::scalar_t half = p->aperture / 2.f;  // perspective.cc:76
+    for (uint64_t i = 0; i < n; ++i) {
+        srand(seed + (unsigned)i);
+        Ray r = cam->get_primary_ray(xy[2 * i], xy[2 * i + 1], width, height);
+        srand(seed + (unsigned)i);
+        jitter[2 * i] = nmath::prng_c(-half, half);
+        jitter[2 * i + 1] = nmath::prng_c(-half, half);
+        o[3 * i] = r.origin.x;
+        o[3 * i + 1] = r.origin.y;
+        o[3 * i + 2] = r.origin.z;
+        d[3 * i] = r.direction.x;
+        d[3 * i + 1] = r.direction.y;
+        d[3 * i + 2] = r.direction.z;
+    }
+    return 0;
+}
+
+// antialiasing::gen_samples_random (aa.cc:34-48) for n pixels at `level`,
+// with its draws recorded the same way: per pixel, srand(seed + i), the
+// reference's sample set, then srand(seed + i) again and the same number of
+// prng_c(0, 1) draws. coords and draws receive 2 * level doubles per pixel,
+// weights level doubles per pixel, in the order the set pops them.
+int xr_aa_random(uint64_t n, const float *pixels, unsigned level, unsigned seed, double *coords, double *weights,
+                 double *draws) {
+    {
+        antialiasing::sample_set_t warm;
+        antialiasing::gen_samples_random(warm, nmath::Vector2f(0, 0), 1);
+    }
+    (void)nmath::prng_c(0.0, 1.0);
+    for (uint64_t i = 0; i < n; ++i) {
+        antialiasing::sample_set_t set;
+        srand(seed + (unsigned)i);
+        antialiasing::gen_samples_random(set, nmath::Vector2f(pixels[2 * i], pixels[2 * i + 1]), level);
+        if (set.count() != level) return 1;
+        for (unsigned k = 0; k < level; ++k) {
+            antialiasing::sample_rgba_t s;
+            set.pop(s);
+            coords[2 * (i * level + k)] = s.coords.x;
+            coords[2 * (i * level + k) + 1] = s.coords.y;
+            weights[i * level + k] = s.weight;
+        }
+        srand(seed + (unsigned)i);
+        for (unsigned k = 0; k < 2 * level; ++k) draws[2 * i * level + k] = nmath::prng_c(0.0, 1.0);
     }
     return 0;
 }

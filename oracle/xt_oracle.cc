@@ -622,19 +622,30 @@ static void primary_cubemap(const Cam &c, float x, float y, float width, float h
     d = nrm(rotated(m, v));
 }
 
-// perspective.cc:38-98
-static void primary_persp(const Cam &c, float x, float y, float w, float h, uint32_t key, V &o, V &d) {
+// The thin lens's two jitters, prng_c(-aperture/2, aperture/2) each
+// (perspective.cc:76-78, prng.inl:82: uniform over the closed range), drawn
+// from query stream 0 of the sample: the first moves x, the second y.
+static void lens_jitter(const Cam &c, uint32_t key, double &jx, double &jy) {
+    const double half = c.aperture / 2.f;
+    const uint32_t st = stream_of(key, 0);
+    jx = unit(draw(st, 0)) * ((half - -half)) + -half;
+    jy = unit(draw(st, 1)) * ((half - -half)) + -half;
+}
+
+// perspective.cc:38-98 with the lens jitters given (used only when
+// flength > 0).
+static void primary_persp_at(const Cam &c, float x, float y, float w, float h, double jx, double jy, V &o, V &d) {
     double aspect = (double)w / (double)h;
     o = c.pos;
     V dir = mk((2.0 * (double)x / (double)w) - 1.0, ((2.0 * (double)y / (double)h) - 1.0) / aspect,
                1.0 / std::tan(c.fov * 0.017453292519943 / 2.0));
     V org = o;
     if (c.flength > 0) {
+        // the lens sample lies on the image plane (z = dir.z), the ray aims at
+        // the focal point flength further along the unjittered direction
         V fo = dir;
-        double half = c.aperture / 2.f;
-        uint32_t st = stream_of(key, 0);
-        fo.x += unit(draw(st, 0)) * ((half - -half)) + -half;
-        fo.y += unit(draw(st, 1)) * ((half - -half)) + -half;
+        fo.x += jx;
+        fo.y += jy;
         V fp = dir + (double)c.flength * nrm(dir);
         dir = fp - fo;
         org = fo;
@@ -645,6 +656,29 @@ static void primary_persp(const Cam &c, float x, float y, float w, float h, uint
     };
     d = nrm(xf(dir));
     if (c.flength > 0) o = xf(org) + c.pos;
+}
+
+static void primary_persp(const Cam &c, float x, float y, float w, float h, uint32_t key, V &o, V &d) {
+    double jx = 0.0, jy = 0.0;
+    if (c.flength > 0) lens_jitter(c, key, jx, jy);
+    primary_persp_at(c, x, y, w, h, jx, jy, o, d);
+}
+
+// antialiasing::gen_samples_random (aa.cc:34-48): sub-sample `a` of pixel
+// (x, y) sits at pixel + (float(prng_c(0, 1)), float(prng_c(0, 1))), the
+// Vector2f sum in double, handed to get_primary_ray as floats. The two draws
+// come from stream 0 of a key of their own per (pixel, a).
+static void aa_random_draws(uint32_t seed, uint32_t job_key, uint32_t pix, uint32_t a, double &ux, double &uy) {
+    const uint32_t k2 = path_key(seed, job_key, pix, 0xfff00000u + a);
+    const uint32_t st = stream_of(k2, 0);
+    ux = unit(draw(st, 0));
+    uy = unit(draw(st, 1));
+}
+
+static void aa_random_at(uint32_t x, uint32_t y, double ux, double uy, float &cx, float &cy) {
+    const float rx = (float)ux, ry = (float)uy;
+    cx = (float)((double)x + (double)rx);
+    cy = (float)((double)y + (double)ry);
 }
 
 // ICamera::get_primary_ray of the camera's kind.
@@ -945,6 +979,63 @@ void or_camera_rays(const or_job *jb, uint64_t n, const float *xy, double *o, do
     }
 }
 
+// Thin-lens primary rays with the lens jitters given (2 doubles per ray, the
+// values prng_c returned): the camera arithmetic alone, for comparison with
+// the reference's rays under recorded draws.
+void or_thin_lens_rays(const or_job *jb, uint64_t n, const float *xy, const double *jitter, double *o, double *d) {
+    const Cam cam = cam_of(jb);
+    for (uint64_t i = 0; i < n; ++i) {
+        V a, b;
+        primary_persp_at(cam, xy[2 * i], xy[2 * i + 1], (float)jb->width, (float)jb->height, jitter[2 * i],
+                         jitter[2 * i + 1], a, b);
+        o[3 * i] = a.x;
+        o[3 * i + 1] = a.y;
+        o[3 * i + 2] = a.z;
+        d[3 * i] = b.x;
+        d[3 * i + 1] = b.y;
+        d[3 * i + 2] = b.z;
+    }
+}
+
+// The lens jitters the oracle draws for sample keys `keys` (2 per key).
+void or_lens_jitter(const or_job *jb, uint64_t n, const uint32_t *keys, double *jitter) {
+    const Cam cam = cam_of(jb);
+    for (uint64_t i = 0; i < n; ++i) lens_jitter(cam, keys[i], jitter[2 * i], jitter[2 * i + 1]);
+}
+
+// RANDOM anti-aliasing: with draws NULL, the oracle's own draws for the
+// `level` sub-samples of each pixel (pixel index y * width + x) are written
+// to draws_out (2 per sub-sample, may be NULL); with draws given, those are
+// used. coords receives the float sample coordinates (2 per sub-sample).
+void or_aa_random(const or_job *jb, uint64_t n, const uint32_t *pixels, const double *draws, double *draws_out,
+                  float *coords) {
+    const uint32_t L = jb->aa;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t x = pixels[2 * i], y = pixels[2 * i + 1];
+        for (uint32_t a = 0; a < L; ++a) {
+            const uint64_t k = i * L + a;
+            double ux, uy;
+            if (draws) {
+                ux = draws[2 * k];
+                uy = draws[2 * k + 1];
+            } else {
+                aa_random_draws(jb->seed, jb->job_key, y * jb->width + x, a, ux, uy);
+            }
+            if (draws_out) {
+                draws_out[2 * k] = ux;
+                draws_out[2 * k + 1] = uy;
+            }
+            aa_random_at(x, y, ux, uy, coords[2 * k], coords[2 * k + 1]);
+        }
+    }
+}
+
+// The sample key of (pixel, sample) the device and the oracle derive their
+// random streams from.
+uint32_t or_path_key(uint32_t seed, uint32_t job_key, uint32_t pixel, uint32_t sample) {
+    return path_key(seed, job_key, pixel, sample);
+}
+
 uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double *m2) {
     Scene *s = (Scene *)h;
     const Cam cam = cam_of(jb);
@@ -979,11 +1070,9 @@ uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double 
                 for (uint32_t a = 0; a < n_aa; ++a) {
                     float cxp, cyp;
                     if (jb->aa_mode == 1) {
-                        uint32_t k2 = path_key(jb->seed, jb->job_key, pix, 0xfff00000u + a);
-                        uint32_t st = stream_of(k2, 0);
-                        float rx = (float)unit(draw(st, 0)), ry = (float)unit(draw(st, 1));
-                        cxp = (float)((double)x + (double)rx);
-                        cyp = (float)((double)y + (double)ry);
+                        double ux, uy;
+                        aa_random_draws(jb->seed, jb->job_key, pix, a, ux, uy);
+                        aa_random_at(x, y, ux, uy, cxp, cyp);
                     } else if (L <= 1) {
                         cxp = (float)((double)x + 0.5);
                         cyp = (float)((double)y + 0.5);

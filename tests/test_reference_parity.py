@@ -68,6 +68,19 @@ def _synthetic(kind, **kw):
     return scenes.SCENES[kind][0](**kw), scenes.camera_dict(**scenes.SCENES[kind][1])
 
 
+def _pinhole_copy(path):
+    """The scene with its thin lens closed (aperture and flength 0), written
+    beside it so that relative mesh sources still resolve."""
+    import re
+    text = open(path).read()
+    text = re.sub(r"(\n\s*aperture = )[^\n]*", r"\g<1>0", text)
+    text = re.sub(r"(\n\s*flength = )[^\n]*", r"\g<1>0", text)
+    out = path[:-4] + "_pinhole.scn"
+    with open(out, "w") as f:
+        f.write(text)
+    return out
+
+
 _RENDER_CHILD = r"""
 import json, sys
 import numpy as np
@@ -99,8 +112,11 @@ def test_oracle_deterministic_renders_bit_exact(kind, kw, integrator, tmp_path):
     else:
         path, cam = _synthetic(kind, **kw)
     if cam["flength"] > 0:
+        # a thin lens draws rand() per primary ray: its rays are pinned bit for
+        # bit under recorded draws (test_lens_aa_parity.py) and its renders
+        # statistically below; the deterministic images use the pinhole
+        path = _pinhole_copy(path)
         cam = dict(cam, flength=0.0, aperture=0.0)
-        pytest.skip("thin-lens scene: primary rays use rand()")
     code = _RENDER_CHILD.format(root=ROOT, path=path, cam=json.dumps(cam), integ=integrator, w=48, h=32, spp=4)
     out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=300).decode().splitlines()[-1])
     assert out["bad"] == 0, out
@@ -139,10 +155,13 @@ import xtracer_amd as xa
 from oracle import oracle as O
 s = xa.Scene({path!r})
 cam = json.loads({cam!r})
-ref = O.RefScene({path!r}).render({integ}, {w}, {h}, samples={spp}, rdepth={rd}, threads=1, seed=7)
-_, _, mean, m2 = O.OracleScene(s).render({integ}, {w}, {h}, cam, samples={spp}, rdepth={rd}, seed=3, stats=True)
-var = m2 / ({spp} - 1)
-se = np.sqrt(2 * var / {spp})
+ref = O.RefScene({path!r}).render({integ}, {w}, {h}, samples={spp}, rdepth={rd}, threads=1, seed=7, aa={aa},
+                                  aa_mode={aa_mode})
+_, _, mean, m2 = O.OracleScene(s).render({integ}, {w}, {h}, cam, samples={spp}, rdepth={rd}, seed=3, stats=True,
+                                         aa={aa}, aa_mode={aa_mode})
+n = {spp} * max(1, {aa})
+var = m2 / (n - 1)
+se = np.sqrt(2 * var / n)
 inside = np.abs(ref[..., :3] - mean) <= 2.576 * se + 1e-5
 print(json.dumps({{"inside": float(inside.mean()), "mean_ref": float(ref[..., :3].mean()), "mean_orc": float(mean.mean())}}))
 """
@@ -156,7 +175,35 @@ def test_oracle_estimators_match_reference_statistically(integrator, kind, kw, r
     streams; per-pixel means must agree within the 99% MC bound."""
     path, cam = _synthetic(kind, **kw)
     code = _STAT_CHILD.format(root=ROOT, path=path, cam=json.dumps(cam), integ=integrator, w=24, h=16, spp=256,
-                              rd=rdepth)
+                              rd=rdepth, aa=1, aa_mode=0)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
+    assert out["inside"] >= 0.97, out
+
+
+@needs_ref
+@pytest.mark.parametrize("integrator,rdepth", [(1, 1), (2, 100), (3, 6)])
+def test_thin_lens_renders_match_reference_statistically(integrator, rdepth):
+    """The pathtracer benchmark's thin lens (aperture 0.05, flength 2.5) on
+    the displaced icosphere: depth, ao and pathtracer per-pixel
+    means of the oracle within the 99% MC bound of the reference's, which
+    draws its lens jitters from rand()."""
+    path, cam = _synthetic("icosphere", subdiv=3)
+    assert cam["flength"] > 0 and cam["aperture"] > 0
+    code = _STAT_CHILD.format(root=ROOT, path=path, cam=json.dumps(cam), integ=integrator, w=24, h=16, spp=256,
+                              rd=rdepth, aa=1, aa_mode=0)
+    out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
+    assert out["inside"] >= 0.97, out
+
+
+@needs_ref
+@pytest.mark.parametrize("integrator,kind,kw", [(1, "spheres", {}), (3, "terrain", {"n": 21})])
+def test_random_aa_renders_match_reference_statistically(integrator, kind, kw):
+    """RANDOM anti-aliasing (aa.cc:34-48) at level 256 with one sample per
+    position, so that every sample of a pixel is independent: depth and
+    pathtracer per-pixel means within the 99% MC bound of the reference's."""
+    path, cam = _synthetic(kind, **kw)
+    code = _STAT_CHILD.format(root=ROOT, path=path, cam=json.dumps(cam), integ=integrator, w=24, h=16, spp=1,
+                              rd=6, aa=256, aa_mode=1)
     out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
     assert out["inside"] >= 0.97, out
 
@@ -185,7 +232,7 @@ def test_dielectric_pathtracer_statistically(kind, tmp_path):
     from tests.helpers import camera_dict_of, camera_scene
     path = camera_scene(str(tmp_path / "cam.scn"), kind)
     code = _STAT_CHILD.format(root=ROOT, path=path, cam=json.dumps(camera_dict_of(kind)), integ=3, w=24, h=16,
-                              spp=256, rd=6)
+                              spp=256, rd=6, aa=1, aa_mode=0)
     out = json.loads(subprocess.check_output([sys.executable, "-c", code], timeout=600).decode().splitlines()[-1])
     assert out["inside"] >= 0.97, out
 
