@@ -101,3 +101,25 @@ def test_include(tmp_path):
                  "material = {\n\tm = {\n\t\ttype = lambert\n\t}\n}\n")
     s = xa.Scene(str(p))
     assert s.objects()[0]["geom_type"] == 1 and s.object_params(0)[3] == 1.0
+
+
+def test_mesh_face_indices_out_of_range(tmp_path):
+    """A mesh OBJ whose faces index past the vertex, texcoord or normal
+    arrays loads and builds: such an index reads as absent (vertex (0, 0, 0),
+    default texcoord, no vertex normal), as for imported objects."""
+    (tmp_path / "m.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0.5 0.5\nvn 0 0 1\n"
+                                    "f 1 2 3\nf 1 2 9999\nf 1/7/9 2/1/1 3//-5\nf -9 2 3\n")
+    (tmp_path / "m.scn").write_text(
+        "geometry = {\n\tg = {\n\t\ttype = mesh\n\t\tsource = m.obj\n\t}\n}\n"
+        "object = {\n\to = {\n\t\tgeometry = g\n\t\tmaterial = m\n\t}\n}\n"
+        "material = {\n" + scenes._material("m", "lambert", kd=(0.5, 0.5, 0.5)) + "\n}\n")
+    s = xa.Scene(str(tmp_path / "m.scn"))
+    arr = s.arrays()
+    assert len(arr["prims"]) == 4
+    order = np.argsort(arr["prim_local"])
+    v = arr["prims64"][order].reshape(4, 3, 3)
+    assert np.array_equal(v[0], [[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    assert np.array_equal(v[1][2], [0, 0, 0])
+    assert np.array_equal(v[3][0], [0, 0, 0])
+    tc, _ = s.texcoords()
+    assert np.array_equal(tc[order][2], [0, 0, 0.5, 0.5, 0, 0])

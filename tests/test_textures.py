@@ -251,6 +251,59 @@ def test_jpeg_close_to_pillow(tmp_path):
     assert np.abs(mine - pil).max() <= 4
 
 
+def _segments(data, marker):
+    """Offsets of the segments with `marker` (their 0xFF byte)."""
+    out, k = [], 2
+    while k + 4 <= len(data) and data[k] == 0xFF:
+        m, ln = data[k + 1], (data[k + 2] << 8) | data[k + 3]
+        if m == marker:
+            out.append(k)
+        if m == 0xDA:  # entropy-coded data follows: skip to the next marker
+            k += 2 + ln
+            while k + 1 < len(data) and not (data[k] == 0xFF and data[k + 1] not in (0x00,) and
+                                             not 0xD0 <= data[k + 1] <= 0xD7):
+                k += 1
+            continue
+        k += 2 + ln
+    return out
+
+
+@pytest.mark.parametrize("case", ["refinement_se", "spectral_reversed", "approx_bits", "baseline_ss",
+                                  "second_frame"])
+def test_jpeg_corrupt_headers_refused(case, tmp_path):
+    """Scan headers outside the ranges a decoder can index (spectral end
+    past 63, start after end, approximation bits above 13; a baseline scan
+    with a spectral start or approximation bits) and a second frame header
+    are refused, as stb_image refuses them ("bad SOS")."""
+    prog = case != "baseline_ss"
+    path = _jpeg(tmp_path / "t.jpg", 40, 30, 5, quality=80, subsampling=2, progressive=prog)
+    data = bytearray(open(path, "rb").read())
+    sos = _segments(data, 0xDA)
+    if case == "second_frame":
+        sof = _segments(data, 0xC2)[0]
+        ln = (data[sof + 2] << 8) | data[sof + 3]
+        data[sos[1]:sos[1]] = data[sof:sof + 2 + ln]
+    else:
+        def tail(k):  # offset of (Ss, Se, Ah/Al) in the scan header at k
+            return k + 5 + 2 * data[k + 4]
+        if case == "refinement_se":
+            k = next(k for k in sos if data[tail(k)] > 0 and data[tail(k) + 2] >> 4)
+            data[tail(k) + 1] = 0xFF
+        elif case == "spectral_reversed":
+            k = next(k for k in sos if data[tail(k)] > 0)
+            data[tail(k)], data[tail(k) + 1] = 10, 5
+        elif case == "approx_bits":
+            data[tail(sos[0]) + 2] = 0xE0
+        else:
+            data[tail(sos[0])] = 5
+    bad = tmp_path / "bad.jpg"
+    bad.write_bytes(bytes(data))
+    with pytest.raises(xa.XtError):
+        xa.read_image(str(bad))
+    if O.ref_available() and case != "second_frame":
+        assert O.ref_load_image(str(bad)) is None
+
+
 def test_grey_jpeg_refused(tmp_path):
     """Grey JPEGs have one channel, which the reference refuses too."""
     Image = pytest.importorskip("PIL.Image")

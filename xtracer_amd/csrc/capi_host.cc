@@ -2,6 +2,7 @@
 // FP64 camera set-up. Needs no GPU.
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <new>
 
 #include "../../include/xtracer_amd.h"
@@ -288,7 +289,12 @@ int xt_image_write(const char *path, const float *rgba, uint32_t width, uint32_t
 int xt_image_read(const char *path, float *rgba, uint32_t *width, uint32_t *height) {
     if (!path || !width || !height) return XT_E_INVALID;
     xt::Image img;
-    const int rc = xt::load_image(path, img);
+    int rc;
+    try {
+        rc = xt::load_image(path, img);
+    } catch (const std::exception &) {  // e.g. a header declaring more pixels than memory holds
+        return XT_E_INVALID;
+    }
     if (rc) return rc == 1 ? XT_E_IO : XT_E_INVALID;
     *width = img.width;
     *height = img.height;

@@ -656,6 +656,9 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
         } else if (m == 0xdd) {
             restart = u16(pos + 2);
         } else if (m == 0xc0 || m == 0xc1 || m == 0xc2) {
+            // one frame per file: a second header would add components
+            // without coefficient stores after the first scan's layout
+            if (frame) return fail(err, "bad JPEG: second frame header", 2);
             progressive = m == 0xc2;
             if (n < 6 || d[0] != 8) return fail(err, "JPEG precision not decoded", 2);
             H = (uint32_t)(d[1] << 8 | d[2]);
@@ -684,6 +687,14 @@ int load_jpeg(const std::vector<uint8_t> &f, Image &out, std::string *err) {
             if (!progressive && ns != (int)comps.size()) return fail(err, "JPEG with several scans not decoded", 2);
             if (ns < 1 || ns > (int)comps.size() || n < 4 + 2 * ns) return fail(err, "bad JPEG scan", 2);
             const int ss = d[1 + 2 * ns], se = d[2 + 2 * ns], ah = d[3 + 2 * ns] >> 4, al = d[3 + 2 * ns] & 15;
+            // the spectral range indexes the 64-entry zigzag table and the
+            // approximation bits shift 16-bit coefficients (stb rejects the
+            // same headers as "bad SOS")
+            if (progressive) {
+                if (ss > 63 || se > 63 || ss > se || ah > 13 || al > 13) return fail(err, "bad JPEG SOS", 2);
+            } else if (ss != 0 || ah != 0 || al != 0) {
+                return fail(err, "bad JPEG SOS", 2);
+            }
             std::vector<int> order;
             for (int s = 0; s < ns; ++s) {
                 const int id = d[1 + 2 * s];

@@ -855,6 +855,16 @@ int read_obj(const std::string &path, MeshData &m, std::string *err) {
         m.it.insert(m.it.end(), sh.it.begin(), sh.it.end());
         m.in.insert(m.in.end(), sh.in.begin(), sh.in.end());
     }
+    // an index past its array (or before it, from a negative index) reads
+    // as absent, as the imported-object path reads it: vertex (0, 0, 0),
+    // Triangle()'s default texcoord, no vertex normal
+    auto bound = [](std::vector<int> &idx, size_t count) {
+        for (int &i : idx)
+            if (i < 0 || (size_t)i >= count) i = -1;
+    };
+    bound(m.iv, m.v.size() / 3);
+    bound(m.it, m.uv.size() / 2);
+    bound(m.in, m.n.size() / 3);
     return 0;
 }
 

@@ -16,7 +16,6 @@ cached under a directory keyed by their parameters.
 import hashlib
 import json
 import os
-import sys
 
 import numpy as np
 
@@ -60,32 +59,19 @@ def _mark(path):
         f.write("ok\n")
 
 
-def _fmt_verts(chunk):
-    return "".join("v %.9g %.9g %.9g\n" % tuple(v) for v in chunk)
+_CHUNK = 1 << 20
 
 
-def _fmt_faces(chunk):
-    return "".join("f %d %d %d\n" % tuple(t) for t in chunk)
-
-
-def _gpu_initialised():
-    torch = sys.modules.get("torch")
-    try:
-        return bool(torch is not None and torch.cuda.is_initialized())
-    except Exception:
-        return True
-
-
-def _format(fn, rows):
-    """Formats rows with fn, in forked worker processes for big meshes (only
-    while this process has not initialised the GPU runtime)."""
-    if len(rows) < (1 << 20) or _gpu_initialised():
-        return fn(rows)
-    import multiprocessing as mp
-    workers = max(1, min(16, os.cpu_count() or 1))
-    chunks = np.array_split(rows, workers * 4)
-    with mp.get_context("fork").Pool(workers) as pool:
-        return "".join(pool.map(fn, chunks))
+def _fmt_rows(pattern, rows):
+    """Formats rows of 3 numbers with one %-pattern per row, a million rows
+    per string operation. Everything runs in this process: a scene may be
+    generated after the GPU runtime has been initialised (by torch, by the
+    native library or by a profiler's preload), and forking then is unsafe."""
+    out = []
+    for k in range(0, len(rows), _CHUNK):
+        part = rows[k:k + _CHUNK]
+        out.append((pattern * len(part)) % tuple(part.ravel().tolist()))
+    return "".join(out)
 
 
 def write_obj(path, verts, faces, normals=None):
@@ -95,13 +81,13 @@ def write_obj(path, verts, faces, normals=None):
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         f.write("# xtracer_amd synthetic mesh\n")
-        f.write(_format(_fmt_verts, verts.astype(np.float64)))
+        f.write(_fmt_rows("v %.9g %.9g %.9g\n", verts.astype(np.float64)))
         if normals is not None:
             normals = np.asarray(normals, np.float32).astype(np.float64)
             f.write("".join("vn %.9g %.9g %.9g\n" % tuple(n) for n in normals))
             f.write("".join("f %d//%d %d//%d %d//%d\n" % (a, a, b, b, c, c) for a, b, c in (faces + 1)))
         else:
-            f.write(_format(_fmt_faces, faces + 1))
+            f.write(_fmt_rows("f %d %d %d\n", faces + 1))
     os.replace(tmp, path)
 
 
