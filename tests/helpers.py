@@ -331,3 +331,87 @@ def extrude_scene(path, camera="pinhole"):
     with open(path, "w") as fh:
         fh.write(text)
     return str(path)
+
+
+def record_normals(scene, arr, o, d, t, prim, u, v):
+    """Normals that the reference's hit record would carry for the device hits
+    (prim, t, barycentrics u / v of corners 1 / 2): triangles
+    Sum n_i * bc_i with vertex normals (geometric normal when that sum is zero,
+    src/xtcore/math/triangle.cc:115-117), spheres (p - c) / r
+    (sphere.cc:52), planes their normal as given (plane.cc:41). NaN rows for
+    misses."""
+    n = len(prim)
+    out = np.full((n, 3), np.nan)
+    p64 = arr["prims64"]
+    prim = np.asarray(prim, np.uint64)
+    soup = prim < len(p64)
+    idx = prim[soup].astype(np.int64)
+    q = p64[idx]
+    is_tri = (arr["prims"]["tf"][idx] & 15) == 0
+    g = np.cross(q[:, 3:6] - q[:, 0:3], q[:, 6:9] - q[:, 0:3])
+    g /= np.maximum(np.linalg.norm(g, axis=1), 1e-300)[:, None]
+    res = g.copy()
+    vn = scene.vnormals()
+    if vn is not None:
+        bu, bv = u[soup].astype(np.float64), v[soup].astype(np.float64)
+        w = np.stack([1.0 - bu - bv, bu, bv], 1)
+        vv = vn[idx].astype(np.float64).reshape(-1, 3, 3)
+        s = (vv * w[:, :, None]).sum(1)
+        use = (np.abs(vv).sum((1, 2)) > 0) & (np.abs(s).sum(1) > 0)
+        res[use] = s[use]
+    sph = ~is_tri
+    if sph.any():
+        pt = o[soup][sph].astype(np.float64) + d[soup][sph].astype(np.float64) * t[soup][sph, None].astype(np.float64)
+        res[sph] = (pt - q[sph, 0:3]) / q[sph, 3:4]
+    out[np.flatnonzero(soup)] = res
+    pl = (prim >= xa.HIT_PLANE_BASE) & (prim != xa.HIT_NONE)
+    if pl.any():
+        out[pl] = arr["planes"]["normal64"][(prim[pl] - xa.HIT_PLANE_BASE).astype(np.int64)]
+    return out
+
+
+def bary(p, a, b, c):
+    v0, v1, v2 = b - a, c - a, p - a
+    d00 = (v0 * v0).sum(1)
+    d01 = (v0 * v1).sum(1)
+    d11 = (v1 * v1).sum(1)
+    d20 = (v2 * v0).sum(1)
+    d21 = (v2 * v1).sum(1)
+    den = d00 * d11 - d01 * d01
+    v = (d11 * d20 - d01 * d21) / den
+    w = (d00 * d21 - d01 * d20) / den
+    return np.stack([1 - v - w, v, w], 1)
+
+
+# the FP32 test defers every candidate whose smallest barycentric lies within
+# 2e-5 of an edge; disagreements with the FP64 reference must lie in that band
+EDGE_BAND = 2e-5
+
+
+def plane_point(arr, prim, o, d):
+    """FP64 point where each ray meets the plane of soup triangle `prim`
+    (the ray's origin for non-triangles)."""
+    out = o.astype(np.float64).copy()
+    p64 = arr["prims64"]
+    ok = prim < len(p64)
+    if ok.any():
+        q = p64[prim[ok].astype(np.int64)]
+        nrm = np.cross(q[:, 3:6] - q[:, 0:3], q[:, 6:9] - q[:, 0:3])
+        oo, dd = o[ok].astype(np.float64), d[ok].astype(np.float64)
+        den = (nrm * dd).sum(1)
+        tt = ((nrm * (q[:, 0:3] - oo)).sum(1)) / np.where(den != 0, den, 1e-300)
+        out[np.flatnonzero(ok)] = oo + dd * tt[:, None]
+    return out
+
+
+def edge_band(arr, prim, point, tol=EDGE_BAND):
+    """True where `point` lies within `tol` (barycentric) of an edge of soup prim `prim`."""
+    out = np.zeros(len(prim), bool)
+    p64 = arr["prims64"]
+    ok = prim < len(p64)
+    if ok.any():
+        q = p64[prim[ok].astype(np.int64)]
+        tri = (arr["prims"]["tf"][prim[ok].astype(np.int64)] & 15) == 0
+        bc = bary(point[ok], q[:, 0:3], q[:, 3:6], q[:, 6:9])
+        out[np.flatnonzero(ok)] = tri & (np.abs(bc).min(1) < tol)
+    return out

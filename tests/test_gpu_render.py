@@ -45,6 +45,43 @@ def test_bit_exact(device, kind, kw, samples, integrator, accel, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("integrator", [xa.STENCIL, xa.DEPTH])
+@pytest.mark.parametrize("kind,kw", [("spheres", {}), ("icosphere", {"subdiv": 4})])
+@pytest.mark.parametrize("aa", [1, 3])
+def test_aa_random_bit_exact(device, kind, kw, integrator, aa):
+    """RANDOM anti-aliasing (src/xtcore/aa.cc:34-48): the device places each
+    pixel's sub-samples with the oracle's draws, so stencil and depth stay bit
+    for bit the oracle's; the icosphere's camera is a thin lens."""
+    s, sid, cam = _setup(device, kind, **kw)
+    W, H, spp = 80, 56, 2
+    job = xa.make_job(sid, integrator, W, H, s.camera(), samples=spp, aa=aa, aa_mode=xa.AA_RANDOM, seed=9)
+    img, st = device.render(job)
+    ref, nq = OracleScene(s).render(integrator, W, H, cam, samples=spp, aa=aa, aa_mode=xa.AA_RANDOM, seed=9)
+    assert st.queries == nq == W * H * spp * aa
+    diff = np.flatnonzero((img.view(np.uint32) != ref.view(np.uint32)).any(axis=2))
+    assert len(diff) == 0, f"{len(diff)} pixels differ"
+    # the sub-sample positions matter: the GRID image differs
+    grid, _ = OracleScene(s).render(integrator, W, H, cam, samples=spp, aa=aa, seed=9)
+    assert (aa == 1 and integrator == xa.STENCIL) or not np.array_equal(grid, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", [xa.AO, xa.PATHTRACER])
+def test_aa_random_same_stream(device, integrator):
+    """RANDOM anti-aliasing with the stochastic integrators: same paths as the
+    oracle except where an FP32 decision differs."""
+    s, sid, cam = _setup(device, "spheres")
+    W, H, spp = 48, 32, 4
+    job = xa.make_job(sid, integrator, W, H, s.camera(), samples=spp, aa=2, aa_mode=xa.AA_RANDOM, rdepth=5,
+                      seed=13, accum=xa.ACCUM_REFERENCE)
+    img, st = device.render(job)
+    ref, nq = OracleScene(s).render(integrator, W, H, cam, samples=spp, aa=2, aa_mode=xa.AA_RANDOM, rdepth=5, seed=13)
+    close = np.isclose(img[..., :3], ref[..., :3], rtol=1e-3, atol=1e-4).all(axis=2)
+    assert close.mean() >= 0.995, f"only {close.mean():.4f} of pixels agree"
+    assert abs(st.queries - nq) <= 0.005 * nq
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("integrator,kind,kw", [(xa.AO, "spheres", {}), (xa.PATHTRACER, "icosphere", {"subdiv": 4}),
                                                 (xa.PATHTRACER, "spheres", {})])
 def test_same_stream(device, integrator, kind, kw):

@@ -5,35 +5,10 @@ import pytest
 
 import xtracer_amd as xa
 from oracle.oracle import OracleScene
-from tests.helpers import camera_rays, mixed_rays, scene_path, soup_to_object_local
+from tests.helpers import (bary as _bary, camera_rays, edge_band as _edge_band, mixed_rays, plane_point as _plane_point,
+                           record_normals, scene_path, soup_to_object_local)
 
 SCENES = [("spheres", {}), ("icosphere", {"subdiv": 4}), ("strands", {"strands": 400}), ("terrain", {"n": 41})]
-
-
-def _bary(p, a, b, c):
-    v0, v1, v2 = b - a, c - a, p - a
-    d00 = (v0 * v0).sum(1)
-    d01 = (v0 * v1).sum(1)
-    d11 = (v1 * v1).sum(1)
-    d20 = (v2 * v0).sum(1)
-    d21 = (v2 * v1).sum(1)
-    den = d00 * d11 - d01 * d01
-    v = (d11 * d20 - d01 * d21) / den
-    w = (d00 * d21 - d01 * d20) / den
-    return np.stack([1 - v - w, v, w], 1)
-
-
-def _edge_band(arr, prim, point, tol=1e-3):
-    """True where `point` lies within `tol` (barycentric) of an edge of soup prim `prim`."""
-    out = np.zeros(len(prim), bool)
-    p64 = arr["prims64"]
-    ok = prim < len(p64)
-    if ok.any():
-        q = p64[prim[ok].astype(np.int64)]
-        tri = (arr["prims"]["tf"][prim[ok].astype(np.int64)] & 15) == 0
-        bc = _bary(point[ok], q[:, 0:3], q[:, 3:6], q[:, 6:9])
-        out[np.flatnonzero(ok)] = tri & (np.abs(bc).min(1) < tol)
-    return out
 
 
 @pytest.mark.gpu
@@ -56,8 +31,7 @@ def test_hit_parity(device, kind, kw, accel):
     mism = np.flatnonzero(~same)
     # every disagreement must sit in the edge band of one of the two primitives
     if len(mism):
-        gpu_pt = o[mism].astype(np.float64) + d[mism].astype(np.float64) * np.where(np.isfinite(t[mism]), t[mism], 0)[:, None]
-        band = _edge_band(arr, prim[mism], gpu_pt)
+        band = _edge_band(arr, prim[mism], _plane_point(arr, prim[mism], o[mism], d[mism]))
         # the oracle's primitive as a soup index
         first = {k: ob["first_prim"] for k, ob in enumerate(s.objects())}
         inv = {}
@@ -71,6 +45,19 @@ def test_hit_parity(device, kind, kw, accel):
     rel = np.abs(t[hit].astype(np.float64) - tr[hit]) / np.maximum(tr[hit], 1e-30)
     assert rel.max() <= 1e-5, f"t relative error {rel.max():.3g}"
     assert np.isinf(t[same & (obr < 0)]).all()
+    # barycentrics of agreed triangle hits against the oracle's FP64 hit point
+    soup = hit & (prim < len(arr["prims64"]))
+    tri = np.zeros(n, bool)
+    tri[soup] = (arr["prims"]["tf"][prim[soup].astype(np.int64)] & 15) == 0
+    if tri.any():
+        q = arr["prims64"][prim[tri].astype(np.int64)]
+        bc = _bary(ptr[tri], q[:, 0:3], q[:, 3:6], q[:, 6:9])
+        err = np.maximum(np.abs(u[tri] - bc[:, 1]), np.abs(v[tri] - bc[:, 2]))
+        assert err.max() <= 1e-5, f"barycentric error {err.max():.3g}"
+    # normals that the hit records give, within 1e-5 relative
+    nd = record_normals(s, arr, o, d, t, prim, u, v)
+    dn = np.linalg.norm(nd[hit] - nr[hit], axis=1) / np.maximum(np.linalg.norm(nr[hit], axis=1), 1e-30)
+    assert dn.max() <= 1e-5, f"normal relative error {dn.max():.3g}"
 
 
 @pytest.mark.gpu

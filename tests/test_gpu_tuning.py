@@ -9,7 +9,8 @@ from oracle.oracle import OracleScene
 from xtracer_amd import scenes
 from tests.helpers import mixed_rays, scene_path
 
-DEFAULT = {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO}
+DEFAULT = {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO,
+           "trace_segments": 1, "ray_order": 1, "kd_layout": 1, "kd_leaf": 1}
 
 
 @pytest.fixture
@@ -21,8 +22,9 @@ def tuned(device):
 @pytest.mark.gpu
 def test_tuning_roundtrip_and_rejects(tuned):
     t = tuned.set_tuning(**DEFAULT)
-    assert t == {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": 0}
-    for bad in ({"kd_stack": 10}, {"trace_block": 64}, {"wave_paths": 100}, {"trace_blocks_per_cu": 65}):
+    assert t == dict(DEFAULT, trace_blocks_per_cu=0)
+    for bad in ({"kd_stack": 10}, {"trace_block": 64}, {"wave_paths": 100}, {"trace_blocks_per_cu": 65},
+                {"trace_segments": 4}, {"ray_order": 3}, {"kd_layout": 3}, {"kd_leaf": 3}):
         with pytest.raises(xa.XtError):
             tuned.set_tuning(**bad)
     assert tuned.tuning() == t
@@ -60,5 +62,35 @@ def test_render_bit_exact_small_shapes(tuned, integrator):
     tuned.release(sid)
     assert st.passes == spp  # 4096 paths per pass: one sample per pass
     ref, nq = OracleScene(s).render(integrator, W, H, cam, samples=spp, seed=5)
+    assert st.queries == nq
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+LAYOUTS = [{"trace_segments": 8}, {"ray_order": 2}, {"kd_layout": 2}, {"kd_leaf": 2},
+           {"trace_segments": 8, "ray_order": 2, "kd_layout": 2, "kd_leaf": 2}]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("knobs", LAYOUTS)
+def test_layout_knobs_invariant(tuned, knobs):
+    """Per-XCD queue segments, tile-major camera rays, the 128-byte-line node
+    order and leaves read through the reference list give the default's hits
+    and the oracle's bit-exact depth image."""
+    s = xa.Scene(scene_path("icosphere", subdiv=5))
+    tuned.set_tuning(**DEFAULT)
+    sid = tuned.upload(s)
+    o, d = mixed_rays(None, 1 << 16, seed=9)
+    base = tuned.trace(sid, o, d, xa.ACCEL_KDTREE)
+    tuned.set_tuning(**knobs)
+    sid2 = tuned.upload(s)  # kd_layout applies to scenes uploaded afterwards
+    got = tuned.trace(sid2, o, d, xa.ACCEL_KDTREE)
+    assert np.array_equal(base[1], got[1])
+    np.testing.assert_allclose(got[0], base[0], rtol=1e-6)
+    cam = scenes.camera_dict(**scenes.SCENES["icosphere"][1])
+    W, H, spp = 88, 56, 3
+    img, st = tuned.render(xa.make_job(sid2, xa.DEPTH, W, H, s.camera(), samples=spp, seed=4))
+    tuned.release(sid)
+    tuned.release(sid2)
+    ref, nq = OracleScene(s).render(xa.DEPTH, W, H, cam, samples=spp, seed=4)
     assert st.queries == nq
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

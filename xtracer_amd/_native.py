@@ -70,7 +70,8 @@ class Camera(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_uint32) for n in ("kd_stack", "trace_block", "wave_paths", "trace_blocks_per_cu")]
+    _fields_ = [(n, ctypes.c_uint32) for n in ("kd_stack", "trace_block", "wave_paths", "trace_blocks_per_cu",
+                                                "trace_segments", "ray_order", "kd_layout", "kd_leaf")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -394,7 +395,8 @@ class Device:
 
     def set_tuning(self, **knobs):
         """Sets launch-shape knobs (kd_stack, trace_block, wave_paths,
-        trace_blocks_per_cu); unnamed knobs keep their value."""
+        trace_blocks_per_cu, trace_segments, ray_order, kd_layout, kd_leaf); unnamed
+        knobs keep their value."""
         t = Tuning(**{k: int(v) for k, v in knobs.items()})
         self._check(lib().xt_dev_set_tuning(self._d, ctypes.byref(t)))
         return self.tuning()

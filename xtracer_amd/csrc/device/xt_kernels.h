@@ -31,8 +31,10 @@ enum {
     CNT_RECHECK = 4, // rays listed for the FP64 re-check of their deferred candidates
     CNT_WORDS = 8
 };
+#define XT_TRACE_SEGMENTS 8  // XCDs of an MI355X: one trace-queue segment each
 struct Counters {
     uint32_t n[CNT_WORDS];
+    uint32_t seg[XT_TRACE_SEGMENTS];  // work cursors of the trace queue's segments (reset with CNT_CURSOR)
     unsigned long long queries, retests, steps_exceeded, trace_fail;
 };
 
@@ -103,6 +105,7 @@ __device__ __forceinline__ bool slot_pixel(const JobDev &j, uint32_t q, uint32_t
 static __global__ void k_reset(Counters *c, uint32_t mask) {
     const uint32_t i = threadIdx.x;
     if (i < CNT_WORDS && (mask >> i) & 1u) c->n[i] = 0;
+    if (i < XT_TRACE_SEGMENTS && (mask >> CNT_CURSOR) & 1u) c->seg[i] = 0;
     if (i == 0 && (mask >> 31) & 1u) {
         c->queries = 0;
         c->retests = 0;
@@ -116,6 +119,7 @@ static __global__ void k_reset_lanes(Counters *c, uint32_t mask) {
     Counters *b = c + blockIdx.x;
     const uint32_t i = threadIdx.x;
     if (i < CNT_WORDS && (mask >> i) & 1u) b->n[i] = 0;
+    if (i < XT_TRACE_SEGMENTS && (mask >> CNT_CURSOR) & 1u) b->seg[i] = 0;
     if (i == 0 && (mask >> 31) & 1u) {
         b->queries = 0;
         b->retests = 0;
@@ -329,23 +333,39 @@ __device__ __forceinline__ void aa_coords(const JobDev &j, uint32_t px, uint32_t
 }
 #pragma clang fp contract(on)
 
-// Slots [0, spp_pass * qn): slot i renders sample pass_base + i / qn of the
-// pixel at tile position q0 + i % qn (a whole pass has q0 = 0, qn = NT) and
-// writes queue entry i; its path id is path_base + i (path state, radiance
-// and FP64 rays are indexed by path id). Slots without a sample (pixels past
+// Slots [0, spp_pass * qn): path path_base + sl * qn + qi renders sample
+// pass_base + sl of the pixel at tile position q0 + qi (a whole pass has
+// q0 = 0, qn = NT); path state, radiance and FP64 rays are indexed by path id.
+// The queue entry that carries it follows the ray order below. Slots without a sample (pixels past
 // the image edge, samples past the job's count) get a dead ray, which the
 // trace turns into a miss whose radiance is never accumulated. The path
 // state word ps1.w holds the query count (low 16 bits) and the job's rdepth
 // (high 16 bits).
 static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
                                                      uint32_t q0, uint32_t qn, uint32_t path_base, Queue out,
-                                                     float4 *ps0, float4 *ps1, float4 *rad, double *ray64) {
+                                                     float4 *ps0, float4 *ps1, float4 *rad, double *ray64,
+                                                     uint32_t tile_major) {
     const uint32_t n = spp_pass * qn;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t qword = 1u | (j.rdepth << 16);
+    // tile_major (with whole tiles): queue entry i holds sample sl of slot w
+    // of tile t, i = (t * spp_pass + sl) * 64 + w, so that a stretch of the
+    // queue covers one part of the image; otherwise the pass's samples follow
+    // each other over the whole image. The path id (and so the radiance slot
+    // and the accumulation order) is sl * qn + q - q0 either way.
+    const bool tiles = tile_major && (qn & 63u) == 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t path = path_base + i;
-        const uint32_t sl = i / qn, q = q0 + (i - sl * qn);
+        uint32_t sl, qi;
+        if (tiles) {
+            const uint32_t t = i / (64u * spp_pass), r = i - t * 64u * spp_pass;
+            sl = r >> 6;
+            qi = t * 64u + (r & 63u);
+        } else {
+            sl = i / qn;
+            qi = i - sl * qn;
+        }
+        const uint32_t path = path_base + sl * qn + qi;
+        const uint32_t q = q0 + qi;
         const uint32_t s = pass_base + sl;
         uint32_t x = 0, y = 0;
         const bool live = slot_pixel(j, q, x, y) && s < j.total;
@@ -395,23 +415,51 @@ __device__ __forceinline__ void seed_best(const RayF &r, Best &b) {
     }
 }
 
+// First ray of segment `s` of `nseg` over a queue of n rays; segments start
+// on multiples of a wave's chunk, segment nseg starts at n.
+__device__ __forceinline__ uint32_t seg_begin(uint32_t n, uint32_t s, uint32_t nseg) {
+    if (s >= nseg) return n;
+    constexpr uint32_t G = 64u * XT_TRACE_CHUNK;
+    return (uint32_t)((uint64_t)n * s / nseg) / G * G;
+}
+
 // Persistent trace: each wave takes XT_TRACE_CHUNK x 64 consecutive rays at
 // a time from a device cursor (one atomic per chunk) until the queue is
 // drained, so waves that drew short traversals pick up more work instead of
-// idling.
+// idling. With nseg = 8 the queue is cut into 8 contiguous segments with a
+// cursor each; workgroups are dispatched to the 8 XCDs round-robin, so the
+// waves of XCD (block % 8) drain that segment first and then help with the
+// others. Rays next to each other in a queue come from neighbouring pixels,
+// so each XCD's L2 then holds the nodes and leaves of one part of the image
+// rather than of all of it.
 template <int ACCEL, int STACK, int BS>
 static __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(const SceneView S, const Queue in, const uint32_t *in_count,
                                                      float4 *hit, uint32_t *retest_list, uint4 *recheck_list,
-                                                     Counters *cnt) {
+                                                     Counters *cnt, uint32_t nseg) {
     __shared__ uint2 kstk[ACCEL == XT_ACCEL_KDTREE ? (STACK + 1) * BS : 1];
     __shared__ uint32_t ostk[ACCEL == XT_ACCEL_OCTREE ? XT_OCT_STACK * BS : 1];
     const uint32_t n = *in_count;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->queries, (unsigned long long)n);
     const uint32_t lane = lane_id();
     uint32_t chunk = 0, part = XT_TRACE_CHUNK;
+    uint32_t seg = blockIdx.x % nseg, tried = 0;
     for (;;) {
         if (part == XT_TRACE_CHUNK) {  // next chunk of XT_TRACE_CHUNK x 64 rays for this wave
-            if (lane == 0) chunk = atomicAdd(&cnt->n[CNT_CURSOR], 64u * XT_TRACE_CHUNK);
+            if (lane == 0) {
+                chunk = n;
+                // every segment is tried at most once after it ran dry, so
+                // the loop ends
+                while (tried < nseg) {
+                    const uint32_t lo = seg_begin(n, seg, nseg), hi = seg_begin(n, seg + 1, nseg);
+                    const uint32_t c = atomicAdd(&cnt->seg[seg], 64u * XT_TRACE_CHUNK);
+                    if (c < hi - lo) {
+                        chunk = lo + c;
+                        break;
+                    }
+                    ++tried;
+                    seg = seg + 1 == nseg ? 0 : seg + 1;
+                }
+            }
             chunk = __shfl(chunk, 0);
             part = 0;
         }

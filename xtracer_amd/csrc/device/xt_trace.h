@@ -32,6 +32,7 @@ struct SceneView {
     float kb_lo[3], kb_hi[3];
     uint32_t n_prims, n_planes, n_oct;
     uint32_t sampled;  // 1 when a material channel or the environment has a sampler
+    uint32_t kd_indexed;  // 1: the FP32 walk reads leaf primitives from `prims` through `kd_refs`
 };
 
 struct RayF {
@@ -91,6 +92,10 @@ struct Tester32 {
     // them, which would cost extra memory round trips)
     __device__ __forceinline__ void test_leaf_range(uint32_t k, uint32_t n) {
         if (n == 0) return;
+        if (S.kd_indexed) {
+            test_leaf_indexed(k, n);
+            return;
+        }
         float4 a = S.kd_leaf[3 * k], b = S.kd_leaf[3 * k + 1], c = S.kd_leaf[3 * k + 2];
         for (uint32_t j = 1; j <= n; ++j) {
             float4 na = a, nb = b, nc = c;
@@ -105,6 +110,28 @@ struct Tester32 {
             a = na;
             b = nb;
             c = nc;
+        }
+    }
+    // the same through the reference list: one soup record per reference,
+    // the next reference's record loaded while this one is tested
+    __device__ __forceinline__ void test_leaf_indexed(uint32_t k, uint32_t n) {
+        uint32_t p = S.kd_refs[k];
+        float4 a = S.prims[3 * p], b = S.prims[3 * p + 1], c = S.prims[3 * p + 2];
+        for (uint32_t j = 1; j <= n; ++j) {
+            float4 na = a, nb = b, nc = c;
+            uint32_t np = p;
+            if (j < n) {
+                np = S.kd_refs[k + j];
+                na = S.prims[3 * np];
+                nb = S.prims[3 * np + 1];
+                nc = S.prims[3 * np + 2];
+            }
+            asm volatile("" ::"v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
+            test_rec(p, a, b, c);
+            a = na;
+            b = nb;
+            c = nc;
+            p = np;
         }
     }
 };

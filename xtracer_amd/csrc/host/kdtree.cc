@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <deque>
 #include <cmath>
 #include <thread>
 
@@ -281,6 +282,41 @@ void kdtree_build(const std::vector<Box32> &pb, const double *tri_verts, const u
     out.bounds = root;
     out.depth = G.depth;
     out.leaves = G.leaves;
+}
+
+std::vector<XtKdNode> kd_line_layout(const std::vector<XtKdNode> &nodes) {
+    if (nodes.size() < 3 || (nodes[0].word & 3u) == 3u) return nodes;
+    struct Pair {
+        uint32_t old_first;  // first node of the pair in `nodes`
+        uint32_t parent;     // the parent's index in the output
+    };
+    std::vector<XtKdNode> out;
+    out.reserve(nodes.size() + 1);
+    out.push_back(nodes[0]);
+    out.push_back(XtKdNode{});
+    out[1].word = 3u;  // padding: an empty leaf
+    out[1].count = 0;
+    std::deque<Pair> roots;
+    roots.push_back(Pair{nodes[0].word >> 2, 0});
+    while (!roots.empty()) {
+        std::deque<Pair> line;
+        line.push_back(roots.front());
+        roots.pop_front();
+        do {
+            const Pair p = line.front();
+            line.pop_front();
+            const uint32_t at = (uint32_t)out.size();
+            out[p.parent].word = (at << 2) | (out[p.parent].word & 3u);
+            for (uint32_t k = 0; k < 2; ++k) {
+                const XtKdNode nd = nodes[p.old_first + k];
+                out.push_back(nd);
+                if ((nd.word & 3u) != 3u) line.push_back(Pair{nd.word >> 2, at + k});
+            }
+        } while (!line.empty() && out.size() % 16 != 0);
+        // the line's frontier: each pair roots a line of its own, next
+        for (auto it = line.rbegin(); it != line.rend(); ++it) roots.push_front(*it);
+    }
+    return out;
 }
 
 }  // namespace xt

@@ -41,4 +41,12 @@ struct KdTree {
 void kdtree_build(const std::vector<Box32> &prim_boxes, const double *tri_verts, const uint8_t *is_tri,
                   const KdParams &p, KdTree &out);
 
+// The same tree with its child pairs regrouped so that a descent stays in one
+// 128-byte line (16 nodes, 8 child pairs) for about three levels at a time:
+// each line is filled breadth-first from one pair down, and the pairs left at
+// its frontier start the next lines, depth-first. The root stays node 0 and
+// node 1 is padding, so pairs start on even nodes and lines on multiples of
+// 16. Leaves keep their reference ranges.
+std::vector<XtKdNode> kd_line_layout(const std::vector<XtKdNode> &nodes);
+
 }  // namespace xt

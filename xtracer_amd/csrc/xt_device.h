@@ -72,7 +72,7 @@ struct xt_dev {
     int trace_launches = 0;
     std::vector<int> timed_pairs;
     ServeState *serve = nullptr;
-    xt_tuning tuning{12, 256, 16u << 20, 0};
+    xt_tuning tuning{12, 256, 16u << 20, 0, 1, 1, 1, 1};
     std::string err;
 };
 

@@ -212,6 +212,34 @@ int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t) {
             n.trace_blocks_per_cu = t->trace_blocks_per_cu;
         }
     }
+    if (t->trace_segments) {
+        if (t->trace_segments != 1 && t->trace_segments != XT_TRACE_SEGMENTS) {
+            d->err = "trace_segments must be 1 or 8";
+            return XT_E_INVALID;
+        }
+        n.trace_segments = t->trace_segments;
+    }
+    if (t->ray_order) {
+        if (t->ray_order > 2) {
+            d->err = "ray_order must be 1 or 2";
+            return XT_E_INVALID;
+        }
+        n.ray_order = t->ray_order;
+    }
+    if (t->kd_leaf) {
+        if (t->kd_leaf > 2) {
+            d->err = "kd_leaf must be 1 or 2";
+            return XT_E_INVALID;
+        }
+        n.kd_leaf = t->kd_leaf;
+    }
+    if (t->kd_layout) {
+        if (t->kd_layout > 2) {
+            d->err = "kd_layout must be 1 or 2";
+            return XT_E_INVALID;
+        }
+        n.kd_layout = t->kd_layout;
+    }
     d->tuning = n;
     return XT_OK;
 }
@@ -276,7 +304,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     int rc = upload(d, &prims, f.prims, t, b);
     rc = rc ? rc : upload(d, &p64, f.prims64, t, b);
     rc = rc ? rc : upload(d, &planes, f.planes, t, b);
-    rc = rc ? rc : upload(d, &kd, f.kd.nodes, t, b);
+    rc = rc ? rc : upload(d, &kd, d->tuning.kd_layout == 2 ? xt::kd_line_layout(f.kd.nodes) : f.kd.nodes, t, b);
     rc = rc ? rc : upload(d, &kdr, f.kd.refs, t, b);
     rc = rc ? rc : upload(d, &kdl, leaf, t, b);
     rc = rc ? rc : upload(d, &oct, f.oct_nodes, t, b);
@@ -362,7 +390,7 @@ int xt_scene_release(xt_dev *d, uint32_t scene_id) {
 // blocks the CU keeps resident, from the occupancy calculator.
 template <int STACK, int BS>
 static void launch_kd_trace_sb(unsigned per_cu, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
-                               const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt) {
+                               const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt, uint32_t nseg) {
     static int resident = 0;
     if (per_cu == 0) {
         if (resident == 0) {
@@ -378,24 +406,27 @@ static void launch_kd_trace_sb(unsigned per_cu, int n_cu, size_t cap, hipStream_
     const size_t need = (cap + BS - 1) / BS;
     const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>(need, (size_t)n_cu * per_cu));
     hipLaunchKernelGGL((k_trace<XT_ACCEL_KDTREE, STACK, BS>), dim3(g), dim3(BS), 0, st, v, q, count, hit, retest, recheck,
-                       cnt);
+                       cnt, nseg);
 }
 
 template <int STACK>
 static void launch_kd_trace_s(unsigned bs, unsigned per_cu, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
-                              const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt) {
-    if (bs == 128) launch_kd_trace_sb<STACK, 128>(per_cu, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt);
-    else launch_kd_trace_sb<STACK, 256>(per_cu, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt);
+                              const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt, uint32_t nseg) {
+    if (bs == 128) launch_kd_trace_sb<STACK, 128>(per_cu, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt, nseg);
+    else launch_kd_trace_sb<STACK, 256>(per_cu, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt, nseg);
 }
 
-static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream_t st, const SceneView &v,
+static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream_t st, const SceneView &view,
                             const Queue &q, const uint32_t *count, float4 *hit, uint32_t *retest, uint4 *recheck, Counters *cnt) {
+    const uint32_t nseg = tn.trace_segments == XT_TRACE_SEGMENTS ? XT_TRACE_SEGMENTS : 1u;
+    SceneView v = view;
+    v.kd_indexed = tn.kd_leaf == 2;
     const unsigned bs = tn.trace_block, pc = tn.trace_blocks_per_cu;
     switch (tn.kd_stack) {
-        case 8: launch_kd_trace_s<8>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
-        case 12: launch_kd_trace_s<12>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
-        case 24: launch_kd_trace_s<24>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
-        default: launch_kd_trace_s<16>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt); break;
+        case 8: launch_kd_trace_s<8>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt, nseg); break;
+        case 12: launch_kd_trace_s<12>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt, nseg); break;
+        case 24: launch_kd_trace_s<24>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt, nseg); break;
+        default: launch_kd_trace_s<16>(bs, pc, n_cu, cap, st, v, q, count, hit, retest, recheck, cnt, nseg); break;
     }
 }
 
@@ -415,7 +446,8 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const Wave
         // 16 resident 64-lane blocks per CU (LDS: 20 KB each)
         const unsigned g = (unsigned)std::min<size_t>(L.cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
         hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, st,
-                           ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
+                           ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt,
+                           tn.trace_segments == XT_TRACE_SEGMENTS ? XT_TRACE_SEGMENTS : 1u);
     } else {
         launch_kd_trace(tn, d->n_cu, L.cap, st, ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
     }
