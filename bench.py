@@ -13,10 +13,15 @@ dominant kernel (the trace kernel); cpu_baseline times the reference renderer
 (or the oracle port when the reference checker is not built) on the host.
 """
 import argparse
+import csv
 import ctypes
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -26,11 +31,9 @@ HBM_PEAK_GBS = 8000.0
 TRACE_BYTES_PER_QUERY = 48      # read ray (2 x float4) + write hit (float4)
 PIPE_BYTES_PER_QUERY = 184      # ray/hit/path-state streams of trace + shade + compaction
 PIPE_BYTES_PER_SAMPLE = 152     # camera ray + state write, FP64 accumulate read + write
-# HBM bytes the trace kernel moves per query on this workload, measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
-# (profiles/hbm_traffic_pathtracer_icosphere_1080p16.md); a measured figure,
-# re-taken whenever the kernel or the tree changes.
-TRACE_TRAFFIC_PER_QUERY = 327
+TRACE_KERNEL = "k_trace"
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+COPY_PROBE_MIB = 1024
 
 
 def log(msg):
@@ -49,9 +52,10 @@ def parse():
     ap.add_argument("--rdepth", type=int, default=8)
     ap.add_argument("--accel", choices=["kdtree", "octree"], default="kdtree")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per wavefront pass (0: auto)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=1,
-                    help="CPU baseline threads (the reference's glibc rand() lock makes more threads slower)")
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="budget of the CPU baseline samples (half on all threads, half on one)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: the host's cores, at most 16, the GPU's share of the host)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="search the launch shapes first (default: the library's defaults, which are the shapes "
@@ -64,18 +68,99 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed steps, copy MIB MiB device-to-device once (a kernel of known traffic "
                          "that scales hardware byte counters)")
+    ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
+                    help="auto: before touching the GPU, run two 2-frame copies of this workload under rocprofv3 "
+                         "counter collection (FETCH_SIZE; WRITE_SIZE + TCC_HIT/TCC_MISS) and report the trace "
+                         "kernel's measured traffic per query and L2 hit rate")
     return ap.parse_args()
+
+
+def _counter_rows(out_dir):
+    """(kernel name, counter, value) rows of a rocprofv3 --pmc csv run."""
+    rows = []
+    for f in glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                rows.append((r.get("Kernel_Name", ""), r.get("Counter_Name", ""), float(r.get("Counter_Value", 0))))
+    return rows
+
+
+def traffic_probe(args):
+    """Runs a 2-frame copy of this benchmark (same scene and frame, plus a
+    1 GiB device copy) twice under rocprofv3 counter collection, one counter
+    group per run, and returns the trace kernel's HBM-side traffic per query
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for the copy of
+    known size) and its L2 hit rate. Must run before this process touches the
+    GPU: the children are separate processes."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return {"error": "rocprofv3 not found"}
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "1", "--no-cpu-baseline",
+             "--serve-jobs", "0", "--traffic", "off", "--copy-probe", str(COPY_PROBE_MIB),
+             "--subdiv", str(args.subdiv), "--width", str(args.width), "--height", str(args.height),
+             "--spp", str(args.spp), "--rdepth", str(args.rdepth), "--accel", args.accel]
+    if args.tuning:
+        child += ["--tuning", args.tuning]
+    passes = (["FETCH_SIZE"], ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"])
+    sums, copy, queries = {}, {}, []
+    with tempfile.TemporaryDirectory(prefix="xt_pmc_") as tmp:
+        for k, counters in enumerate(passes):
+            out = os.path.join(tmp, f"pass{k}")
+            cmd = ["timeout", "-s", "KILL", "180", exe, "--pmc", *counters, "--output-format", "csv", "-d", out,
+                   "-o", "run", "--", *child]
+            log(f"traffic probe pass {k}: {' '.join(counters)}")
+            r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True)
+            lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+            if not lines:
+                return {"error": f"counter run {k} ended with status {r.returncode} and no result line",
+                        "stderr_tail": r.stderr[-400:]}
+            queries.append(json.loads(lines[-1])["extra"]["queries_per_frame"] * 2)
+            for name, ctr, val in _counter_rows(out):
+                if name.split("<")[0].split("(")[0].strip().endswith(TRACE_KERNEL):
+                    sums[ctr] = sums.get(ctr, 0.0) + val
+                elif COPY_KERNEL in name:
+                    copy[ctr] = max(copy.get(ctr, 0.0), val)
+    need = ("FETCH_SIZE", "WRITE_SIZE")
+    if any(c not in sums or not copy.get(c) for c in need):
+        return {"error": f"counters missing: trace {sorted(sums)}, copy {sorted(copy)}"}
+    kib = COPY_PROBE_MIB * 1024.0
+    f_fetch, f_write = kib / copy["FETCH_SIZE"], kib / copy["WRITE_SIZE"]
+    fetch_b = sums["FETCH_SIZE"] * f_fetch * 1024.0 / queries[0]
+    write_b = sums["WRITE_SIZE"] * f_write * 1024.0 / queries[1]
+    hit, miss = sums.get("TCC_HIT_sum"), sums.get("TCC_MISS_sum")
+    return {"bytes_per_query": round(fetch_b + write_b, 1), "fetch_bytes_per_query": round(fetch_b, 1),
+            "write_bytes_per_query": round(write_b, 1), "fetch_scale": round(f_fetch, 4),
+            "write_scale": round(f_write, 4),
+            "l2_hit_rate": round(hit / (hit + miss), 4) if hit is not None and miss and hit + miss > 0 else None,
+            "method": "rocprofv3 --pmc over 2 frames of this workload in child processes; FETCH_SIZE and "
+                      "WRITE_SIZE summed over the k_trace dispatches, each scaled by its reading for a "
+                      f"{COPY_PROBE_MIB} MiB device copy, divided by the frames' queries; "
+                      "L2 hit rate = TCC_HIT / (TCC_HIT + TCC_MISS) over the same dispatches"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scene_path, args, threads):
     """Reference renderer (oracle/_ref) on a spread subset of the frame's
-    32x32 tiles; falls back to the oracle port. A query's cost does not depend
-    on how many samples a pixel takes, so the sample runs at 1 spp on more
-    tiles (raising spp only once every tile is in). Returns the cpu_baseline
-    object."""
+    32x32 tiles, on `threads` host threads and on 1 thread; falls back to the
+    oracle port. A query's cost does not depend on how many samples a pixel
+    takes, so the sample runs at 1 spp on more tiles (raising spp only once
+    every tile is in). Each run is sized to half of --cpu-seconds. Returns the
+    cpu_baseline object (value: the run on `threads` threads)."""
     from oracle import oracle as O
     W, H = args.width, args.height
     tiles = ((W + 31) // 32) * ((H + 31) // 32)
+    budget = 0.5 * args.cpu_seconds
+    model = _cpu_model()
     if O.ref_available():
         L = O.ref()
         L.xr_render_tiles.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7 + [ctypes.c_uint, ctypes.c_int,
@@ -84,29 +169,41 @@ def cpu_baseline(scene_path, args, threads):
         r = O.RefScene(scene_path)
         load_s = time.perf_counter() - t0
         log(f"cpu baseline: reference scene loaded in {load_s:.1f}s")
-        sec, q = ctypes.c_double(), ctypes.c_uint64()
-        # calibrate on 4 tiles per thread spread over the frame at 1 spp
-        stride = max(1, tiles // (4 * threads))
-        L.xr_render_tiles(r._h, 3, W, H, 1, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec), ctypes.byref(q))
-        n_cal = (tiles + stride - 1) // stride
-        log(f"cpu baseline: calibration {n_cal} tiles at 1 spp on {threads} threads in {sec.value:.2f}s")
-        spp = 1
-        if sec.value < 0.5 * args.cpu_seconds:
-            per_tile = max(sec.value / n_cal, 1e-6)
-            want = args.cpu_seconds * threads / per_tile  # tile-samples that fill the budget
-            if want >= tiles:
-                stride, spp = 1, int(max(1, min(args.spp, want // tiles)))
-            else:
-                stride = max(1, int(tiles // max(want, 1)))
-            L.xr_render_tiles(r._h, 3, W, H, spp, 1, args.rdepth, threads, 1, stride, ctypes.byref(sec),
-                              ctypes.byref(q))
-            log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles at {spp} spp in {sec.value:.2f}s")
+
+        def run(nt):
+            sec, q = ctypes.c_double(), ctypes.c_uint64()
+            # calibrate on one tile per thread spread over the frame at 1 spp
+            # (tiles near the mesh's symmetry planes cost seconds each)
+            stride = max(1, tiles // nt)
+            L.xr_render_tiles(r._h, 3, W, H, 1, 1, args.rdepth, nt, 1, stride, ctypes.byref(sec), ctypes.byref(q))
+            n_cal = (tiles + stride - 1) // stride
+            log(f"cpu baseline: calibration {n_cal} tiles at 1 spp on {nt} threads in {sec.value:.2f}s")
+            spp = 1
+            if sec.value < 0.5 * budget:
+                per_tile = max(sec.value / n_cal, 1e-6)
+                want = budget / per_tile  # tile-samples that fill the budget at this thread count
+                if want >= tiles:
+                    stride, spp = 1, int(max(1, min(args.spp, want // tiles)))
+                else:
+                    stride = max(1, int(tiles // max(want, 1)))
+                L.xr_render_tiles(r._h, 3, W, H, spp, 1, args.rdepth, nt, 1, stride, ctypes.byref(sec),
+                                  ctypes.byref(q))
+                log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles at {spp} spp on {nt} threads in "
+                    f"{sec.value:.2f}s")
+            return q.value, sec.value, stride, spp
+
+        qa, sa, stride, spp = run(threads)
+        q1, s1, stride1, spp1 = run(1) if threads > 1 else (qa, sa, stride, spp)
         n_tiles = (tiles + stride - 1) // stride
-        return {"value": q.value / sec.value / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
-                "sample": f"reference pathtracer (compiled -O3 from its sources, OpenMP dynamic tiles, {threads} "
-                          f"thread(s)) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) of the same frame at "
-                          f"{spp} spp, rdepth {args.rdepth}; {q.value} Scene::intersection calls in "
-                          f"{sec.value:.2f}s; scene load + octree build {load_s:.1f}s not timed"}
+        return {"value": qa / sa / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
+                "cpu_model": model, "host_threads": os.cpu_count(),
+                "single_thread": {"value": q1 / s1 / 1e6, "cores": 1, "queries": q1, "seconds": round(s1, 2),
+                                  "tiles": (tiles + stride1 - 1) // stride1, "spp": spp1},
+                "sample": f"reference pathtracer (compiled -O3 from its sources, its OpenMP dynamic tile loop on "
+                          f"{threads} thread(s)) on every {stride}th 32x32 tile ({n_tiles} of {tiles}) of the same "
+                          f"frame at {spp} spp, rdepth {args.rdepth}; {qa} Scene::intersection calls in "
+                          f"{sa:.2f}s; scene load + octree build {load_s:.1f}s not timed; the same on 1 thread "
+                          f"in single_thread"}
     import xtracer_amd as xa
     from xtracer_amd import scenes
     s = xa.Scene(scene_path, build=True)
@@ -116,8 +213,9 @@ def cpu_baseline(scene_path, args, threads):
     t0 = time.perf_counter()
     _, nq = orc.render(3, W, H, cam, samples=1, rdepth=args.rdepth, threads=threads, tile_stride=stride)
     dt = time.perf_counter() - t0
-    return {"value": nq / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle port on every {stride}th tile at 1 spp; {nq} queries in {dt:.2f}s"}
+    return {"value": nq / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu_model": model,
+            "sample": f"FP64 oracle port on every {stride}th tile at 1 spp on {threads} threads; {nq} queries in "
+                      f"{dt:.2f}s"}
 
 
 def _pct(xs, q):
@@ -250,6 +348,11 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would render its own "
               "tiles or jobs with a replicated scene); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # counter runs first: they are child processes, and this one has not
+    # touched the GPU yet
+    probe = traffic_probe(args) if args.traffic == "auto" else None
+    if probe:
+        log(f"traffic probe: {probe}")
     import torch
 
     import xtracer_amd as xa
@@ -335,9 +438,11 @@ def main():
                    "prims": info.n_prims, "accel": args.accel},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": int(TRACE_TRAFFIC_PER_QUERY * queries / max(1, launches)) if launches else None,
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per k_trace dispatch "
-                                       "(copy-calibrated), per query, times queries per launch",
+                     "traffic": (int(probe["bytes_per_query"] * queries / max(1, launches))
+                                 if probe and "bytes_per_query" in probe and launches else None),
+                     "traffic_per_query": probe.get("bytes_per_query") if probe else None,
+                     "l2_hit_rate": probe.get("l2_hit_rate") if probe else None,
+                     "traffic_probe": probe,
                      "kernel": "k_trace (kd-tree)", "bytes_per_query": TRACE_BYTES_PER_QUERY,
                      "avg_launch_ms": round(trace_ms / max(1, launches), 4), "launches": launches},
         "roofline_pipeline": {"achieved": round(pipe, 2), "frac": round(pipe / HBM_PEAK_GBS, 5),
