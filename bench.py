@@ -52,8 +52,6 @@ def parse():
     ap.add_argument("--rdepth", type=int, default=8)
     ap.add_argument("--accel", choices=["kdtree", "octree"], default="kdtree")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per wavefront pass (0: auto)")
-    ap.add_argument("--cpu-seconds", type=float, default=24.0,
-                    help="budget of the CPU baseline samples (half on all threads, half on one)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU baseline threads (0: the host's cores, at most 16, the GPU's share of the host)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -61,8 +59,10 @@ def parse():
                     help="search the launch shapes first (default: the library's defaults, which are the shapes "
                          "this search picked for this workload on MI355X)")
     ap.add_argument("--tuning", default="", help="launch shape as JSON (xt_tuning fields); skips the autotuner")
-    ap.add_argument("--serve-jobs", type=int, default=64, help="jobs of the mixed batch (0: skip serving)")
-    ap.add_argument("--serve-requests", type=int, default=48, help="Poisson requests per arrival rate")
+    ap.add_argument("--serve-jobs", type=int, default=256, help="jobs of the mixed batch (0: skip serving)")
+    ap.add_argument("--serve-requests", type=int, default=512, help="Poisson requests per arrival rate")
+    ap.add_argument("--serve-checked", type=int, default=4,
+                    help="served requests per arrival rate whose image is compared with the oracle's")
     ap.add_argument("--serve-rates", default="0.5,0.8,0.95", help="arrival rates as fractions of capacity")
     ap.add_argument("--strands", type=int, default=50000, help="strands of the hair scene in the serving mix")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -153,13 +153,11 @@ def cpu_baseline(scene_path, args, threads):
     """Reference renderer (oracle/_ref) on a spread subset of the frame's
     32x32 tiles, on `threads` host threads and on 1 thread; falls back to the
     oracle port. A query's cost does not depend on how many samples a pixel
-    takes, so the sample runs at 1 spp on more tiles (raising spp only once
-    every tile is in). Each run is sized to half of --cpu-seconds. Returns the
+    takes, so the sample runs at 1 spp, 4 tiles per thread. Returns the
     cpu_baseline object (value: the run on `threads` threads)."""
     from oracle import oracle as O
     W, H = args.width, args.height
     tiles = ((W + 31) // 32) * ((H + 31) // 32)
-    budget = 0.5 * args.cpu_seconds
     model = _cpu_model()
     if O.ref_available():
         L = O.ref()
@@ -171,26 +169,14 @@ def cpu_baseline(scene_path, args, threads):
         log(f"cpu baseline: reference scene loaded in {load_s:.1f}s")
 
         def run(nt):
+            # 4 tiles per thread spread evenly over the frame at 1 spp; a tile's
+            # cost ranges from milliseconds (sky) to seconds (tiles over the
+            # mesh's symmetry planes), so a sample is not extrapolated
             sec, q = ctypes.c_double(), ctypes.c_uint64()
-            # calibrate on one tile per thread spread over the frame at 1 spp
-            # (tiles near the mesh's symmetry planes cost seconds each)
-            stride = max(1, tiles // nt)
+            stride = max(1, tiles // (4 * nt))
             L.xr_render_tiles(r._h, 3, W, H, 1, 1, args.rdepth, nt, 1, stride, ctypes.byref(sec), ctypes.byref(q))
-            n_cal = (tiles + stride - 1) // stride
-            log(f"cpu baseline: calibration {n_cal} tiles at 1 spp on {nt} threads in {sec.value:.2f}s")
-            spp = 1
-            if sec.value < 0.5 * budget:
-                per_tile = max(sec.value / n_cal, 1e-6)
-                want = budget / per_tile  # tile-samples that fill the budget at this thread count
-                if want >= tiles:
-                    stride, spp = 1, int(max(1, min(args.spp, want // tiles)))
-                else:
-                    stride = max(1, int(tiles // max(want, 1)))
-                L.xr_render_tiles(r._h, 3, W, H, spp, 1, args.rdepth, nt, 1, stride, ctypes.byref(sec),
-                                  ctypes.byref(q))
-                log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles at {spp} spp on {nt} threads in "
-                    f"{sec.value:.2f}s")
-            return q.value, sec.value, stride, spp
+            log(f"cpu baseline: {(tiles + stride - 1) // stride} tiles at 1 spp on {nt} threads in {sec.value:.2f}s")
+            return q.value, sec.value, stride, 1
 
         qa, sa, stride, spp = run(threads)
         q1, s1, stride1, spp1 = run(1) if threads > 1 else (qa, sa, stride, spp)
@@ -223,24 +209,29 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
-def _orbit_camera(xa, cam, rng):
+def _orbit_pose(cam, rng):
     """A random pose on the sphere through the scene camera's position around
-    its target (seeded), same lens."""
+    its target (seeded), same lens, rounded to float like the scene loader."""
     import numpy as np
     p, t = np.asarray(cam["position"], float), np.asarray(cam["target"], float)
     r = float(np.linalg.norm(p - t))
     az, el = rng.uniform(0, 2 * np.pi), rng.uniform(0.05, 0.6)
     pos = t + r * np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
-    return xa.make_camera(tuple(pos), tuple(t), (0.0, 1.0, 0.0), cam.get("fov", 45.0), cam.get("aperture", 0.0),
-                          cam.get("flength", 0.0))
+    f = lambda v: tuple(float(np.float32(c)) for c in v)  # noqa: E731
+    return {"position": f(pos), "target": f(t), "fov": float(np.float32(cam.get("fov", 45.0))),
+            "aperture": float(np.float32(cam.get("aperture", 0.0))),
+            "flength": float(np.float32(cam.get("flength", 0.0)))}
 
 
-def serving(dev, xa, scenes, args, ico_sid, ico_scene):
+def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
     """Mixed-job batch (served vs one job per launch) and Poisson arrivals
     with p50/p99 latency. Scenes: 64 spheres (ao), displaced icosphere,
-    strands and terrain (pathtracer)."""
+    strands and terrain (pathtracer); resolutions from {128^2 .. 1024^2} and
+    1..64 spp. Per-request image error against the FP64 oracle (same seeded
+    paths) on a sample of the small spheres / terrain requests."""
     import numpy as np
     import torch
+    from oracle.oracle import OracleScene
     rng = np.random.default_rng(2024)
     pool = []
     t0 = time.perf_counter()
@@ -257,19 +248,22 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene):
     def draw(n, key0, orbit):
         jobs = []
         for k in range(n):
-            kind, sc, sid, integ, rdepth, camd = pool[int(rng.integers(len(pool)))]
+            e = pool[int(rng.integers(len(pool)))]
+            kind, sc, sid, integ, rdepth, camd = e
             res = int(rng.choice([128, 256, 512, 1024]))
             spp = int(rng.choice([1, 4, 16, 64]))
-            cam = _orbit_camera(xa, camd, rng) if orbit else sc.camera()
-            jobs.append(xa.make_job(sid, integ, res, res, cam, samples=spp, rdepth=rdepth, seed=7,
-                                    job_key=key0 + k))
+            pose = _orbit_pose(camd, rng) if orbit else dict(camd)
+            cam = xa.make_camera(pose["position"], pose["target"], (0.0, 1.0, 0.0), pose.get("fov", 45.0),
+                                 pose.get("aperture", 0.0), pose.get("flength", 0.0)) if orbit else sc.camera()
+            jobs.append((e, pose, xa.make_job(sid, integ, res, res, cam, samples=spp, rdepth=rdepth, seed=7,
+                                              job_key=key0 + k)))
         return jobs
 
     def plain(jobs, arrivals=None):
         out = torch.empty((1024, 1024, 4), dtype=torch.float32, device="cuda")
         lat, q = [], 0
         t0 = time.perf_counter()
-        for k, j in enumerate(jobs):
+        for k, (_, _, j) in enumerate(jobs):
             if arrivals is not None:
                 wait = t0 + arrivals[k] - time.perf_counter()
                 if wait > 0:
@@ -277,52 +271,79 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene):
             st = dev.render_device(j, out.data_ptr())
             q += st.queries
             lat.append((time.perf_counter() - t0 - (arrivals[k] if arrivals is not None else 0.0)) * 1e3)
-        return time.perf_counter() - t0, q, lat
+        return time.perf_counter() - t0, q, lat, {}
 
-    def served(jobs, arrivals=None):
+    def served(jobs, arrivals=None, keep=()):
         info0 = dev.serve_info()
         t0 = time.perf_counter()
-        pending, lat, k = {}, [], 0
+        pending, lat, k, imgs = {}, [], 0, {}
         while k < len(jobs) or pending:
             now = time.perf_counter() - t0
             while k < len(jobs) and (arrivals is None or arrivals[k] <= now):
-                j = jobs[k]
+                j = jobs[k][2]
                 late = 0.0 if arrivals is None else (time.perf_counter() - t0 - arrivals[k]) * 1e3
-                pending[dev.submit(j)] = (j.width, j.height, late)
+                pending[dev.submit(j)] = (k, j.width, j.height, late)
                 k += 1
             if not pending:
                 time.sleep(max(0.0, t0 + arrivals[k] - time.perf_counter()))
                 continue
             dev.serve_step()
             for t in list(pending):
-                w, h, late = pending[t]
-                done, _, st = dev.poll(t, w, h)
+                i, w, h, late = pending[t]
+                done, img, st = dev.poll(t, w, h)
                 if done:
                     lat.append(late + st.latency_ms)
+                    if i in keep:
+                        imgs[i] = img
                     del pending[t]
         wall = time.perf_counter() - t0
         info1 = dev.serve_info()
-        return wall, info1["queries"] - info0["queries"], lat
+        return wall, info1["queries"] - info0["queries"], lat, imgs
 
-    def summary(wall, q, lat):
+    def summary(wall, q, lat, _imgs):
         return {"wall_s": round(wall, 3), "Mrays_s": round(q / wall / 1e6, 1),
                 "p50_ms": round(_pct(lat, 50), 1), "p99_ms": round(_pct(lat, 99), 1)}
 
-    policies = ("fair", "srpt")
-    res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1), "policies": list(policies)}
+    def request_errors(jobs, imgs):
+        """Relative L1 difference of served images from the oracle's render of
+        the same job (same seeded paths), and the share of pixels within 1e-3."""
+        out = []
+        for i, img in sorted(imgs.items()):
+            e, pose, j = jobs[i]
+            ref, _ = OracleScene(e[1]).render(j.integrator, j.width, j.height, scenes.camera_dict(**pose),
+                                              samples=j.samples, rdepth=j.rdepth, seed=j.seed, job_key=j.job_key)
+            a, b = img[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+            rel = float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30))
+            agree = float(np.isclose(a, b, rtol=1e-3, atol=1e-4).all(axis=2).mean())
+            out.append({"scene": e[0], "res": j.width, "spp": j.samples, "rel_l1": round(rel, 6),
+                        "pixels_within_1e-3": round(agree, 5)})
+        return out
+
+    def pick_checked(jobs, n):
+        small = [i for i, (e, _, j) in enumerate(jobs)
+                 if e[0] in ("spheres", "terrain") and j.width <= 256 and j.samples <= 16]
+        return set(small[:n])
+
+    policies = ("fcfs", "srpt", "hybrid")
+    res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1), "engine": dev.serve_config(),
+           "policies": list(policies)}
     jobs = draw(args.serve_jobs, 10000, False)
-    samples = sum(j.width * j.height * j.samples for j in jobs)
-    served(jobs[:4])  # warm the server's buffers
-    pw, pq, plat = plain(jobs)
-    res["batch"] = {"jobs": len(jobs), "samples": samples, "plain": summary(pw, pq, plat)}
+    samples = sum(j.width * j.height * j.samples for _, _, j in jobs)
+    served(jobs[:4])  # warm the server's buffers and graph
+    pw, pq, plat, _ = plain(jobs)
+    res["batch"] = {"jobs": len(jobs), "samples": samples, "plain": summary(pw, pq, plat, None)}
     walls = {}
     for pol in policies:
         dev.set_serve_policy(pol)
-        sw, sq, slat = served(jobs)
+        sw, sq, slat, _ = served(jobs)
         walls[pol] = sw
-        res["batch"]["served_" + pol] = summary(sw, sq, slat)
+        res["batch"]["served_" + pol] = summary(sw, sq, slat, None)
         log(f"serving batch ({pol}): plain {pw:.2f}s served {sw:.2f}s")
     res["batch"]["served_over_plain"] = {pol: round(pw / walls[pol], 3) for pol in policies}
+    # the served engine can at best run the mix at the single-frame rate:
+    # served/plain <= frame Mrays/s / plain Mrays/s on this mix
+    plain_mrays = pq / pw / 1e6
+    res["batch"]["served_over_plain_bound"] = round(frame_mrays / plain_mrays, 3) if plain_mrays > 0 else None
     mu = len(jobs) / min(walls.values())
     res["capacity_jobs_s"] = round(mu, 2)
     res["poisson"] = []
@@ -330,14 +351,21 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene):
         lam = rate * mu
         jobs = draw(args.serve_requests, 20000 + int(rate * 1000) * 100, True)
         arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs)))
+        keep = pick_checked(jobs, args.serve_checked)
         entry = {"load": rate, "lambda_jobs_s": round(lam, 2), "requests": len(jobs)}
-        for pol in policies:
+        imgs = {}
+        for pol in ("srpt", "hybrid"):
             dev.set_serve_policy(pol)
-            entry["served_" + pol] = summary(*served(jobs, arrivals))
+            wall, q, lat, got = served(jobs, arrivals, keep)
+            entry["served_" + pol] = summary(wall, q, lat, None)
+            imgs = imgs or got
         entry["plain"] = summary(*plain(jobs, arrivals))
+        t0 = time.perf_counter()
+        entry["request_error"] = request_errors(jobs, imgs)
+        entry["request_error_s"] = round(time.perf_counter() - t0, 1)
         res["poisson"].append(entry)
         log(f"poisson load {rate}: p99 plain {entry['plain']['p99_ms']} ms, "
-            + ", ".join(f"{pol} {entry['served_' + pol]['p99_ms']} ms" for pol in policies))
+            + ", ".join(f"{pol} {entry['served_' + pol]['p99_ms']} ms" for pol in ("srpt", "hybrid")))
     dev.set_serve_policy("srpt")
     return res
 
@@ -417,7 +445,7 @@ def main():
     pipe_bytes = queries * PIPE_BYTES_PER_QUERY + samples * PIPE_BYTES_PER_SAMPLE
     pipe = pipe_bytes / (wall) / 1e9
     threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
-    serve = serving(dev, xa, scenes, args, sid, scene) if args.serve_jobs > 0 else None
+    serve = serving(dev, xa, scenes, args, sid, scene, value) if args.serve_jobs > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(path, args, threads)
     line = {
         "metric": "Mrays/s at 1080p 16spp (pathtracer, plain one-job-per-launch)",

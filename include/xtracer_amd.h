@@ -300,15 +300,17 @@ int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_sta
 /* Serving: continuous batching of many jobs on one device, in place of the
  * reference's one blocking IIntegrator::render() per context.
  * xt_submit queues the job and returns a ticket; it is admitted (FIFO) once
- * its buffers (80 B per pixel) fit the HBM budget, and refused with
- * XT_E_BUDGET when they never could. xt_serve_step runs one wavefront pass
- * shared by all admitted jobs (each advances by a fair share of samples) and
- * reports the jobs not yet finished. xt_poll sets *done once the job's image
+ * its buffers (80 B per pixel in whole passes, 40 on the regeneration engine
+ * below) fit the HBM budget, and refused with
+ * XT_E_BUDGET when they never could. xt_serve_step runs the configured
+ * query rounds of the regeneration engine and one wavefront pass shared by
+ * the other admitted jobs, and reports the jobs not yet finished. xt_poll
+ * sets *done once the job's image
  * is ready; with rgba_out (width*height*4 floats) it then copies the image
  * and forgets the ticket, with rgba_out NULL it only reports (the image is
  * kept for a later call); stats->latency_ms = completion - submission and
- * stats->queue_ms = admission - submission. A served image is bit-identical
- * to the same job rendered with xt_render. xt_drain steps until every
+ * stats->queue_ms = admission - submission. A job served in whole passes is
+ * bit-identical to the same job rendered with xt_render. xt_drain steps until every
  * admitted and waiting job has finished. */
 int xt_submit(xt_dev *d, const xt_job *job, uint64_t *ticket);
 int xt_poll(xt_dev *d, uint64_t ticket, int *done, float *rgba_out, xt_job_stats *stats);
@@ -326,12 +328,39 @@ typedef struct xt_serve_info {
 } xt_serve_info;
 int xt_serve_get_info(const xt_dev *d, xt_serve_info *info);
 
-/* Pass scheduling: XT_SERVE_SRPT (default) fills each pass with the jobs
- * that have the least work left; XT_SERVE_FAIR lets the oldest job take half
- * of each pass and shares the rest equally. Images do not depend on it. */
+/* Scheduling. XT_SERVE_SRPT (default): the jobs with the least work left go
+ * first. XT_SERVE_FAIR: whole passes give the oldest job half of each pass and
+ * share the rest equally; the regeneration engine gives every job an equal
+ * share of each round's free path slots. XT_SERVE_FCFS: admission order.
+ * XT_SERVE_HYBRID: jobs with less work left than half the path slots first
+ * (shortest first), then the others in admission order. Images do not depend
+ * on it. */
 #define XT_SERVE_FAIR 0
 #define XT_SERVE_SRPT 1
+#define XT_SERVE_FCFS 2
+#define XT_SERVE_HYBRID 3
 int xt_serve_set_policy(xt_dev *d, int policy);
+
+/* Engines. ao and pathtracer jobs with the default accumulation run on the
+ * path-regeneration engine: a pool of path slots shared by all of them, in
+ * which every query round refills the slots of finished paths with camera
+ * rays of waiting samples (in priority order), so jobs join and leave between
+ * rounds; each round is one fixed launch sequence, replayed from a hipGraph.
+ * Their radiance is summed per pixel in 64-bit fixed point (2^-32), which makes
+ * the image independent of the order in which paths end: a served image
+ * differs from the same job's xt_render image by the rounding of the two sums
+ * (at most a few units in the last place of the float result). Every other
+ * job runs in whole passes (bit-identical to xt_render). In
+ * xt_serve_configure a 0 field keeps its value. */
+typedef struct xt_serve_config {
+    uint32_t regen;           /* 2: the regeneration engine as above; 1: every job in whole passes (2) */
+    uint32_t graph;           /* 2: rounds replayed from a hipGraph; 1: launched kernel by kernel (2) */
+    uint32_t rounds_per_step; /* query rounds per xt_serve_step on the regeneration engine, even (4) */
+    uint32_t paths;           /* path slots of the regeneration engine, 4096..2^28 (wave_paths); set while
+                                 it holds no job */
+} xt_serve_config;
+int xt_serve_configure(xt_dev *d, const xt_serve_config *c);
+int xt_serve_get_config(xt_dev *d, xt_serve_config *c);
 
 #ifdef __cplusplus
 }

@@ -1,17 +1,37 @@
-"""Continuous-batching server: served images equal one-job renders bit for
-bit, jobs join at pass boundaries, admission follows the HBM budget."""
+"""Continuous-batching server.
+
+Jobs served in whole passes (the deterministic integrators, and every job
+with the regeneration engine off) equal one-job renders bit for bit. ao and
+pathtracer jobs on the path-regeneration engine sum each pixel's samples in
+64-bit fixed point, so their images do not depend on the schedule (graph or
+launch by launch, policy, mix, path slots) and differ from the one-job
+render's compensated FP64 sum only by the rounding of the two sums. Jobs join
+between rounds; admission follows the HBM budget."""
 import numpy as np
 import pytest
 
 import xtracer_amd as xa
+from oracle.oracle import OracleScene
+from xtracer_amd import scenes
 from tests.helpers import scene_path
+
+# served (fixed-point sum) against xt_render (Neumaier FP64 sum): both are
+# exact to far below float resolution, so they agree to a few float ulps
+REGEN_RTOL, REGEN_ATOL = 1e-6, 1e-9
 
 
 @pytest.fixture
 def small_passes(device):
+    """Passes and the regeneration engine's slot pool of 64 Ki paths, so that
+    larger jobs take many passes / rounds."""
+    device.drain()
     device.set_tuning(wave_paths=1 << 16)
+    device.serve_configure(regen=True, graph=True, rounds_per_step=4, paths=1 << 16)
     yield device
+    device.drain()
     device.set_tuning(wave_paths=16 << 20)
+    device.set_serve_policy("srpt")
+    device.serve_configure(regen=True, graph=True, rounds_per_step=4, paths=16 << 20)
 
 
 def _scenes(device):
@@ -32,31 +52,110 @@ def _drain_all(device, tickets):
     return got
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("policy", ["srpt", "fair"])
-def test_served_equals_alone(small_passes, policy):
-    dev = small_passes
-    dev.set_serve_policy(policy)
-    sc = _scenes(dev)
+def _regen(job):
+    return job.integrator in (xa.AO, xa.PATHTRACER) and job.accum == xa.ACCUM_DEFAULT
+
+
+def _same(img, ref, regen):
+    if not regen:
+        return np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    return np.allclose(img, ref, rtol=REGEN_RTOL, atol=REGEN_ATOL)
+
+
+MIX = [("spheres", xa.STENCIL, 64, 48, 4), ("spheres", xa.DEPTH, 80, 40, 2),
+       ("icosphere", xa.AO, 72, 56, 8), ("icosphere", xa.PATHTRACER, 96, 64, 16),
+       ("spheres", xa.PATHTRACER, 40, 40, 5), ("icosphere", xa.PATHTRACER, 33, 17, 3),
+       ("icosphere", xa.PATHTRACER, 320, 240, 3),  # more tile positions than a pass: sliced samples
+       ("icosphere", xa.NORMAL, 64, 64, 2), ("spheres", xa.EMISSION, 48, 32, 4),
+       ("icosphere", xa.WIREFRAME, 40, 24, 3), ("spheres", xa.AO, 56, 40, 6)]
+
+
+def _mix_jobs(sc):
     jobs = []
-    for k, (kind, integ, w, h, spp) in enumerate([
-            ("spheres", xa.STENCIL, 64, 48, 4), ("spheres", xa.DEPTH, 80, 40, 2),
-            ("icosphere", xa.AO, 72, 56, 8), ("icosphere", xa.PATHTRACER, 96, 64, 16),
-            ("spheres", xa.PATHTRACER, 40, 40, 5), ("icosphere", xa.PATHTRACER, 33, 17, 3),
-            ("icosphere", xa.PATHTRACER, 320, 240, 3),  # more tile positions than a pass: sliced samples
-            ("icosphere", xa.NORMAL, 64, 64, 2), ("spheres", xa.EMISSION, 48, 32, 4),
-            ("icosphere", xa.WIREFRAME, 40, 24, 3)]):
+    for k, (kind, integ, w, h, spp) in enumerate(MIX):
         s, sid = sc[kind]
         jobs.append(xa.make_job(sid, integ, w, h, s.camera(), samples=spp, rdepth=5, seed=7 + k, job_key=k))
+    return jobs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", ["srpt", "fair", "fcfs", "hybrid"])
+@pytest.mark.parametrize("regen", [True, False])
+def test_served_equals_alone(small_passes, policy, regen):
+    dev = small_passes
+    dev.set_serve_policy(policy)
+    dev.serve_configure(regen=regen)
+    sc = _scenes(dev)
+    jobs = _mix_jobs(sc)
     tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
     got = _drain_all(dev, tickets)
     assert dev.serve_info()["jobs_active"] == 0
-    dev.set_serve_policy("srpt")
     for (t, (img, st)), j in zip(got.items(), jobs):
         ref, _ = dev.render(j)
-        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), f"job {t} differs from its one-job render"
+        assert _same(img, ref, regen and _regen(j)), f"job {t} ({j.integrator}) differs from its one-job render"
         assert st.latency_ms >= st.queue_ms >= 0.0
         assert st.samples == j.samples * j.width * j.height
+
+
+@pytest.mark.gpu
+def test_regen_images_independent_of_schedule(small_passes):
+    """Graph replay or kernel-by-kernel launches, every policy, other path-slot
+    counts and rounds per step, alone or in the mix: bit-identical images."""
+    dev = small_passes
+    sc = _scenes(dev)
+    jobs = [j for j in _mix_jobs(sc) if _regen(j)]
+    runs = []
+    for policy, graph, rounds in (("srpt", True, 4), ("fcfs", False, 2), ("hybrid", True, 8), ("fair", False, 6)):
+        dev.set_serve_policy(policy)
+        dev.serve_configure(graph=graph, rounds_per_step=rounds)
+        tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
+        got = _drain_all(dev, tickets)
+        runs.append([img for img, _ in got.values()])
+    for r in runs[1:]:
+        for a, b in zip(runs[0], r):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for k, j in enumerate(jobs):  # one job at a time
+        t = dev.submit(j)
+        alone = _drain_all(dev, {t: (j.width, j.height)})[t][0]
+        assert np.array_equal(alone.view(np.uint32), runs[0][k].view(np.uint32)), f"job {k} alone"
+
+
+@pytest.mark.gpu
+def test_regen_other_path_slot_count(device):
+    """A smaller slot pool (more regeneration per path) gives the same images."""
+    device.set_tuning(wave_paths=1 << 16)
+    try:
+        sc = _scenes(device)
+        jobs = [j for j in _mix_jobs(sc) if _regen(j)][:3]
+        imgs = []
+        for paths in (1 << 16, 1 << 13):
+            device.drain()
+            device.serve_configure(regen=True, paths=paths)
+            assert device.serve_config()["paths"] == paths
+            tickets = {device.submit(j): (j.width, j.height) for j in jobs}
+            imgs.append([img for img, _ in _drain_all(device, tickets).values()])
+        for a, b in zip(*imgs):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    finally:
+        device.serve_configure(paths=16 << 20)
+        device.set_tuning(wave_paths=16 << 20)
+
+
+@pytest.mark.gpu
+def test_regen_same_stream_as_oracle(small_passes):
+    """A served pathtracer job traces the oracle's paths (same counter-based
+    streams): pixels agree except where an FP32 decision differs."""
+    dev = small_passes
+    s = xa.Scene(scene_path("icosphere", subdiv=4))
+    sid = dev.upload(s)
+    cam = scenes.camera_dict(**scenes.SCENES["icosphere"][1])
+    W, H, spp = 64, 48, 8
+    job = xa.make_job(sid, xa.PATHTRACER, W, H, s.camera(), samples=spp, rdepth=6, seed=11)
+    t = dev.submit(job)
+    img = _drain_all(dev, {t: (W, H)})[t][0]
+    ref, _ = OracleScene(s).render(xa.PATHTRACER, W, H, cam, samples=spp, rdepth=6, seed=11)
+    close = np.isclose(img[..., :3], ref[..., :3], rtol=1e-3, atol=1e-4).all(axis=2)
+    assert close.mean() >= 0.98, f"only {close.mean():.4f} of pixels agree"
 
 
 @pytest.mark.gpu
@@ -83,15 +182,19 @@ def test_small_job_overtakes_large(small_passes):
     done_big, img_big, _ = dev.poll(tb, 256, 256)
     assert done_big
     ref, _ = dev.render(big)
-    assert np.array_equal(img_big.view(np.uint32), ref.view(np.uint32))
+    assert _same(img_big, ref, True)
+    ref_small, _ = dev.render(small)
+    assert _same(img, ref_small, True)
 
 
 @pytest.mark.gpu
-def test_budget_admission():
+@pytest.mark.parametrize("regen", [True, False])
+def test_budget_admission(regen):
     s = xa.Scene(scene_path("spheres"))
     dev = xa.Device(0, hbm_budget=512 << 20)
     try:
         dev.set_tuning(wave_paths=1 << 16)
+        dev.serve_configure(regen=regen)
         sid = dev.upload(s)
         with pytest.raises(xa.XtError) as e:
             dev.submit(xa.make_job(sid, xa.STENCIL, 4096, 4096, s.camera()))  # 1.3 GB of job buffers
@@ -112,8 +215,25 @@ def test_budget_admission():
         assert np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
         with pytest.raises(xa.XtError):
             dev.poll(t1, w, h)  # a finished ticket is forgotten
+        # ao jobs of 160 MB on the regeneration engine (40 B per pixel): the
+        # second waits likewise
+        j3 = xa.make_job(sid, xa.AO, w, h, s.camera(), samples=1, job_key=3)
+        j4 = xa.make_job(sid, xa.AO, w, h, s.camera(), samples=1, job_key=4)
+        t3, t4 = dev.submit(j3), dev.submit(j4)
+        dev.drain()
+        assert dev.poll(t3, w, h)[0] and dev.poll(t4, w, h)[0]
     finally:
         dev.close()
+
+
+@pytest.mark.gpu
+def test_serve_config_rejects(device):
+    with pytest.raises(xa.XtError):
+        device.serve_configure(rounds_per_step=3)
+    with pytest.raises(xa.XtError):
+        device.serve_configure(paths=100)
+    with pytest.raises(xa.XtError):
+        device.set_serve_policy(7)
 
 
 @pytest.mark.gpu
@@ -124,6 +244,11 @@ def test_scene_in_use(device):
     with pytest.raises(xa.XtError) as e:
         device.release(sid)
     assert e.value.code == 7
+    device.drain()
+    assert device.poll(t, 32, 32)[0]
+    t = device.submit(xa.make_job(sid, xa.AO, 32, 32, s.camera(), samples=2))
+    with pytest.raises(xa.XtError) as e:
+        device.release(sid)
     device.drain()
     assert device.poll(t, 32, 32)[0]
     device.release(sid)

@@ -1,0 +1,112 @@
+"""The batched multi-job workload and served rendering under Poisson
+arrivals, on the four resident scenes of the benchmark mix (64 spheres,
+displaced icosphere, strands, terrain) at sizes a test run finishes in
+seconds: 256 jobs with resolutions and sample counts drawn like the full mix
+(each scaled down), deterministic-integrator jobs checked bit for bit against
+the oracle, ao / pathtracer jobs against their one-job renders."""
+import numpy as np
+import pytest
+
+import xtracer_amd as xa
+from oracle.oracle import OracleScene
+from xtracer_amd import scenes
+from tests.helpers import scene_path
+
+REGEN_RTOL, REGEN_ATOL = 1e-6, 1e-9
+MIX_SCENES = [("spheres", {}, xa.AO, 1), ("icosphere", {"subdiv": 4}, xa.PATHTRACER, 8),
+              ("strands", {"strands": 1500}, xa.PATHTRACER, 6), ("terrain", {"n": 81}, xa.PATHTRACER, 8)]
+RES = (16, 32, 64, 128)   # the full mix draws 128^2 .. 1024^2
+SPP = (1, 2, 4, 8)        # the full mix draws 1 .. 64
+
+
+@pytest.fixture(scope="module")
+def mix(device):
+    out = []
+    for kind, kw, integ, rdepth in MIX_SCENES:
+        s = xa.Scene(scene_path(kind, **kw))
+        out.append((kind, s, device.upload(s), integ, rdepth, scenes.camera_dict(**scenes.SCENES[kind][1])))
+    device.set_tuning(wave_paths=1 << 18)
+    device.set_serve_policy("srpt")
+    yield out
+    device.set_tuning(wave_paths=16 << 20)
+    for e in out:
+        device.release(e[2])
+
+
+def _draw(mix, n, rng, key0, det_every=4):
+    """n jobs: scene uniform over the mix, resolution and spp uniform over RES
+    and SPP; every det_every-th job renders depth or stencil instead (small,
+    for the oracle)."""
+    jobs = []
+    for k in range(n):
+        e = mix[int(rng.integers(len(mix)))]
+        res, spp = int(rng.choice(RES)), int(rng.choice(SPP))
+        integ, rdepth = e[3], e[4]
+        if k % det_every == det_every - 1:
+            integ = xa.DEPTH if k % 2 else xa.STENCIL
+            res, spp = min(res, 32), min(spp, 2)
+        jobs.append((e, xa.make_job(e[2], integ, res, res, e[1].camera(), samples=spp, rdepth=rdepth, seed=5,
+                                    job_key=key0 + k)))
+    return jobs
+
+
+def _check(device, results, jobs):
+    for (e, j), (img, st) in zip(jobs, results):
+        if j.integrator in (xa.DEPTH, xa.STENCIL):
+            ref, nq = OracleScene(e[1]).render(j.integrator, j.width, j.height, e[5], samples=j.samples, seed=j.seed,
+                                               job_key=j.job_key)
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), f"{e[0]} job {j.job_key} vs oracle"
+        else:
+            ref, _ = device.render(j)
+            assert np.allclose(img, ref, rtol=REGEN_RTOL, atol=REGEN_ATOL), f"{e[0]} job {j.job_key} vs alone"
+        assert st.samples == j.samples * j.width * j.height
+        assert st.latency_ms >= st.queue_ms >= 0.0
+
+
+@pytest.mark.gpu
+def test_batch_256_jobs_four_scenes(device, mix):
+    rng = np.random.default_rng(2024)
+    jobs = _draw(mix, 256, rng, 1000)
+    q0 = device.serve_info()["queries"]
+    tickets = [device.submit(j) for _, j in jobs]
+    device.drain()
+    results = []
+    for t, (_, j) in zip(tickets, jobs):
+        done, img, st = device.poll(t, j.width, j.height)
+        assert done
+        results.append((img, st))
+    info = device.serve_info()
+    assert info["jobs_active"] == 0 and info["jobs_waiting"] == 0
+    assert info["queries"] > q0
+    _check(device, results, jobs)
+
+
+@pytest.mark.gpu
+def test_poisson_arrivals(device, mix):
+    """Requests arrive between server steps at exponential gaps (seeded) while
+    earlier ones are on the device; each result is the same as alone, latency
+    covers queueing, and a request never finishes before it arrived."""
+    rng = np.random.default_rng(77)
+    jobs = _draw(mix, 48, rng, 5000, det_every=6)
+    gaps = rng.exponential(0.6, len(jobs))  # in server steps
+    arrive = np.floor(np.cumsum(gaps)).astype(int)
+    pending, results, k, step, order = {}, {}, 0, 0, []
+    while k < len(jobs) or pending:
+        while k < len(jobs) and arrive[k] <= step:
+            pending[device.submit(jobs[k][1])] = k
+            k += 1
+        device.serve_step()
+        step += 1
+        for t in list(pending):
+            j = jobs[pending[t]][1]
+            done, img, st = device.poll(t, j.width, j.height)
+            if done:
+                results[pending[t]] = (img, st)
+                order.append(pending[t])
+                del pending[t]
+        assert step < 5000
+    _check(device, [results[i] for i in range(len(jobs))], jobs)
+    lat = np.array([results[i][1].latency_ms for i in range(len(jobs))])
+    assert np.percentile(lat, 50) <= np.percentile(lat, 99)
+    # jobs overlapped: some request finished while an earlier one was still running
+    assert order != sorted(order)

@@ -24,7 +24,8 @@ ACCUM_DEFAULT, ACCUM_REFERENCE, ACCUM_FP64 = 0, 1, 2
 HIT_NONE = 0xFFFFFFFF
 HIT_PLANE_BASE = 0xFFFF0000
 TUNING_AUTO = 0xFFFFFFFF  # xt_tuning.trace_blocks_per_cu: resident fill
-SERVE_FAIR, SERVE_SRPT = 0, 1
+SERVE_FAIR, SERVE_SRPT, SERVE_FCFS, SERVE_HYBRID = 0, 1, 2, 3
+SERVE_POLICIES = {"fair": SERVE_FAIR, "srpt": SERVE_SRPT, "fcfs": SERVE_FCFS, "hybrid": SERVE_HYBRID}
 CAMERA_THIN_LENS, CAMERA_ODS, CAMERA_ERP, CAMERA_CUBEMAP = 0, 1, 2, 3
 
 # Exported symbols, in the order include/xtracer_amd.h declares them.
@@ -35,7 +36,7 @@ SYMBOLS = [
     "xt_image_write", "xt_image_read", "xt_render_raygraph", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
-    "xt_serve_get_info", "xt_serve_set_policy",
+    "xt_serve_get_info", "xt_serve_set_policy", "xt_serve_configure", "xt_serve_get_config",
 ]
 
 
@@ -72,6 +73,13 @@ class Camera(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("kd_stack", "trace_block", "wave_paths", "trace_blocks_per_cu",
                                                 "trace_segments", "ray_order", "kd_layout", "kd_leaf")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("regen", "graph", "rounds_per_step", "paths")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -172,6 +180,8 @@ def lib():
         "xt_drain": (I, [P]),
         "xt_serve_get_info": (I, [P, ctypes.POINTER(ServeInfo)]),
         "xt_serve_set_policy": (I, [P, I]),
+        "xt_serve_configure": (I, [P, ctypes.POINTER(ServeConfig)]),
+        "xt_serve_get_config": (I, [P, ctypes.POINTER(ServeConfig)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -467,10 +477,24 @@ class Device:
         return True, img, st
 
     def set_serve_policy(self, policy):
-        """SERVE_FAIR or SERVE_SRPT (or "fair" / "srpt")."""
+        """SERVE_FAIR / SERVE_SRPT / SERVE_FCFS / SERVE_HYBRID (or their names)."""
         if isinstance(policy, str):
-            policy = {"fair": SERVE_FAIR, "srpt": SERVE_SRPT}[policy]
+            policy = SERVE_POLICIES[policy]
         self._check(lib().xt_serve_set_policy(self._d, int(policy)))
+
+    def serve_configure(self, regen=0, graph=0, rounds_per_step=0, paths=0):
+        """regen / graph: True or False (None keeps the value); rounds_per_step,
+        paths: 0 keeps the value. Returns the configuration in force."""
+        def flag(v):
+            return 0 if v in (0, None) and v is not False else (2 if v else 1)
+        c = ServeConfig(flag(regen), flag(graph), int(rounds_per_step), int(paths))
+        self._check(lib().xt_serve_configure(self._d, ctypes.byref(c)))
+        return self.serve_config()
+
+    def serve_config(self):
+        c = ServeConfig()
+        self._check(lib().xt_serve_get_config(self._d, ctypes.byref(c)))
+        return c.as_dict()
 
     def serve_info(self):
         info = ServeInfo()

@@ -816,11 +816,14 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_channels(const SceneView S,
 // SAMPLED: the material colours and the environment come from k_channels'
 // output `chan` (slot-indexed) instead of the material and environment
 // records.
+// With `retire` (the server's path regeneration), every path that ends here
+// is appended to that list.
 template <bool SAMPLED>
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, Queue out,
                                                        uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad,
-                                                       const float4 *chan) {
+                                                       const float4 *chan, uint32_t *retire = nullptr,
+                                                       uint32_t *retire_n = nullptr) {
     __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -903,6 +906,11 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
         }
         const uint32_t slot = block_append<XT_BLOCK>(cont, out_count, lds);
         if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
+        if (retire) {
+            const bool ended = i < n && !cont;
+            const uint32_t rs = block_append<XT_BLOCK>(ended, retire_n, lds);
+            if (ended) retire[rs] = path;
+        }
     }
 }
 

@@ -98,5 +98,17 @@ struct WaveLane {
 };
 int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &lane, const double *ray64,
                     uint32_t *flags, bool time_it, hipStream_t st);
+// The buffers one trace launch reads and writes.
+struct TraceBufs {
+    Queue q;
+    const uint32_t *count = nullptr;  // rays in q
+    float4 *hit = nullptr;
+    uint32_t *retest = nullptr;
+    uint4 *recheck = nullptr;
+    Counters *cnt = nullptr;
+    size_t cap = 0;  // queue capacity (sizes the grids)
+};
+int xt_launch_trace_bufs(xt_dev *d, const DevScene &ds, int accel, const TraceBufs &B, const double *ray64,
+                         uint32_t *flags, bool time_it, hipStream_t st);
 void xt_serve_shutdown(xt_dev *d);
 bool xt_serve_uses_scene(const xt_dev *d, uint32_t scene_id);

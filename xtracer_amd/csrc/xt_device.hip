@@ -430,26 +430,21 @@ static void launch_kd_trace(const xt_tuning &tn, int n_cu, size_t cap, hipStream
     }
 }
 
-// Launches trace + FP64 re-test over queue `qi` of a lane (count in
-// lane.cnt->n[qi]).
-int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &L, const double *ray64,
-                    uint32_t *flags, bool time_it, hipStream_t st) {
-    WaveBuffers &w = d->wave;
-    Queue q{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
-    float4 *hit = w.hit + L.off;
-    uint32_t *retest = w.retest + L.off;
-    uint4 *recheck = w.recheck + L.off;
+// Launches trace + FP64 re-check / re-test over queue `q` (count at `count`)
+// into explicit buffers.
+int xt_launch_trace_bufs(xt_dev *d, const DevScene &ds, int accel, const TraceBufs &B, const double *ray64,
+                         uint32_t *flags, bool time_it, hipStream_t st) {
     const bool timed = time_it && d->trace_launches < XT_TRACE_EVENTS - 1;
     if (timed) hipEventRecord(d->tev[d->trace_launches], st);
     const xt_tuning &tn = d->tuning;
     if (accel == XT_ACCEL_OCTREE) {
         // 16 resident 64-lane blocks per CU (LDS: 20 KB each)
-        const unsigned g = (unsigned)std::min<size_t>(L.cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
+        const unsigned g = (unsigned)std::min<size_t>(B.cap / XT_OCT_BLOCK + 1, (size_t)d->n_cu * 16);
         hipLaunchKernelGGL((k_trace<XT_ACCEL_OCTREE, 1, XT_OCT_BLOCK>), dim3(g), dim3(XT_OCT_BLOCK), 0, st,
-                           ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt,
+                           ds.view, B.q, B.count, B.hit, B.retest, B.recheck, B.cnt,
                            tn.trace_segments == XT_TRACE_SEGMENTS ? XT_TRACE_SEGMENTS : 1u);
     } else {
-        launch_kd_trace(tn, d->n_cu, L.cap, st, ds.view, q, &L.cnt->n[qi], hit, retest, recheck, L.cnt);
+        launch_kd_trace(tn, d->n_cu, B.cap, st, ds.view, B.q, B.count, B.hit, B.retest, B.recheck, B.cnt);
     }
     if (timed) {
         hipEventRecord(d->tev[d->trace_launches + 1], st);
@@ -457,18 +452,34 @@ int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const Wave
         d->trace_launches += 2;
     }
     XT_LAUNCHED();
-    hipLaunchKernelGGL(k_recheck, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 256)), dim3(XT_BLOCK), 0, st,
-                       ds.view, q, hit, recheck, retest, L.cnt, ray64, flags);
+    hipLaunchKernelGGL(k_recheck, dim3(grid_for(B.cap / 64 + 1, XT_BLOCK, 256)), dim3(XT_BLOCK), 0, st,
+                       ds.view, B.q, B.hit, B.recheck, B.retest, B.cnt, ray64, flags);
     XT_LAUNCHED();
     if (accel == XT_ACCEL_OCTREE) {
-        hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(L.cap / 64 + 1, XT_OCT_BLOCK, 2048)),
-                           dim3(XT_OCT_BLOCK), 0, st, ds.view, q, hit, retest, L.cnt, ray64, flags);
+        hipLaunchKernelGGL(k_retest<XT_ACCEL_OCTREE>, dim3(grid_for(B.cap / 64 + 1, XT_OCT_BLOCK, 2048)),
+                           dim3(XT_OCT_BLOCK), 0, st, ds.view, B.q, B.hit, B.retest, B.cnt, ray64, flags);
     } else {
-        hipLaunchKernelGGL(k_retest<XT_ACCEL_KDTREE>, dim3(grid_for(L.cap / 64 + 1, XT_BLOCK, 1024)), dim3(XT_BLOCK),
-                           0, st, ds.view, q, hit, retest, L.cnt, ray64, flags);
+        hipLaunchKernelGGL(k_retest<XT_ACCEL_KDTREE>, dim3(grid_for(B.cap / 64 + 1, XT_BLOCK, 1024)), dim3(XT_BLOCK),
+                           0, st, ds.view, B.q, B.hit, B.retest, B.cnt, ray64, flags);
     }
     XT_LAUNCHED();
     return XT_OK;
+}
+
+// Launches trace + FP64 re-test over queue `qi` of a lane (count in
+// lane.cnt->n[qi]).
+int xt_launch_trace(xt_dev *d, const DevScene &ds, int accel, int qi, const WaveLane &L, const double *ray64,
+                    uint32_t *flags, bool time_it, hipStream_t st) {
+    WaveBuffers &w = d->wave;
+    TraceBufs B;
+    B.q = Queue{w.q[qi].r0 + L.off, w.q[qi].r1 + L.off, w.q[qi].path + L.off};
+    B.count = &L.cnt->n[qi];
+    B.hit = w.hit + L.off;
+    B.retest = w.retest + L.off;
+    B.recheck = w.recheck + L.off;
+    B.cnt = L.cnt;
+    B.cap = L.cap;
+    return xt_launch_trace_bufs(d, ds, accel, B, ray64, flags, time_it, st);
 }
 
 extern "C" int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const void *ray0, const void *ray1,

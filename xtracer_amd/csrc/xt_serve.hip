@@ -32,6 +32,7 @@
 #include <tuple>
 #include <vector>
 
+#include "device/xt_regen.h"
 #include "xt_device.h"
 #include "xt_render.h"
 
@@ -63,6 +64,11 @@ constexpr uint64_t kJobBytesPerPixel = sizeof(float4) + 3 * sizeof(double2) + si
 
 uint64_t job_bytes(size_t npix) { return ((uint64_t)npix * kJobBytesPerPixel + 255) & ~255ull; }
 
+// regeneration engine: 3 fixed-point sums and the resolved image per pixel
+constexpr uint64_t kRegenJobBytesPerPixel = 3 * sizeof(long long) + sizeof(float4);
+
+uint64_t regen_job_bytes(size_t npix) { return ((uint64_t)npix * kRegenJobBytesPerPixel + 255) & ~255ull; }
+
 struct DevBlock {
     void *p = nullptr;
     uint64_t bytes = 0;
@@ -79,6 +85,10 @@ struct SJob {
     xt_job job;
     JobDev jd;
     int state = WAITING;
+    bool regen = false;    // runs on the path-regeneration engine
+    int entry = -1;        // its device job-table entry once started there
+    uint64_t paths_done = 0;
+    long long *acc = nullptr;
     uint32_t issued = 0;  // whole samples handed to passes
     uint32_t qc = 0;      // tile positions of sample `issued` handed out so far
     uint32_t passes = 0;
@@ -101,6 +111,38 @@ struct Segment {
 
 }  // namespace
 
+// Host side of the path-regeneration engine (see device/xt_regen.h).
+struct RegenLaneH {
+    bool used = false;
+    uint32_t scene = 0, integrator = 0, accel = 0;
+    void *blk = nullptr;
+    uint64_t bytes = 0;
+    float4 *hit = nullptr, *chan = nullptr;
+    uint32_t *retest = nullptr;
+    uint4 *recheck = nullptr;
+};
+
+struct RegenState {
+    bool ready = false;
+    uint32_t P = 0;
+    void *blk = nullptr;
+    uint64_t bytes = 0;
+    RegenView V{};
+    Queue *lq = nullptr;  // device copy of lq_host
+    Queue lq_host[2 * XT_REGEN_MAX_LANES] = {};
+    RegenLaneH lanes[XT_REGEN_MAX_LANES];
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+    xt_tuning captured{};
+    RegenJob *h_job = nullptr;             // pinned staging
+    unsigned long long *h_done = nullptr;  // pinned, per table entry
+    uint32_t *h_prio = nullptr;            // pinned: n_prio, cap_per_job, prio[]
+    Counters *h_cnt = nullptr;             // pinned, per lane
+    std::vector<int> free_entries;
+    unsigned long long queries_seen[XT_REGEN_MAX_LANES] = {}, retests_seen[XT_REGEN_MAX_LANES] = {};
+};
+
 struct ServeState {
     std::map<uint64_t, SJob> jobs;
     std::deque<uint64_t> waiting;
@@ -115,6 +157,8 @@ struct ServeState {
     double epoch_ms = 0;
     xt_serve_info info{};
     int policy = XT_SERVE_SRPT;
+    xt_serve_config cfg{2, 2, 4, 0};
+    RegenState R;
 };
 
 namespace {
@@ -186,20 +230,37 @@ int take_host_block(xt_dev *d, ServeState &S, size_t need, HostBlock &out) {
     return XT_OK;
 }
 
+bool regen_eligible(const ServeState &S, const JobDev &jd) {
+    return S.cfg.regen == 2 && jd.accum == XT_ACCUM_FP64 &&
+           (jd.integrator == XT_INTEGRATOR_AO || jd.integrator == XT_INTEGRATOR_PATHTRACER);
+}
+
 // Admits waiting jobs in FIFO order while their buffers fit.
 int admit(xt_dev *d, ServeState &S) {
     while (!S.waiting.empty()) {
         SJob &J = S.jobs[S.waiting.front()];
         bool ok = false;
-        int rc = take_dev_block(d, S, job_bytes(J.npix), J.blk, ok);
+        J.regen = regen_eligible(S, J.jd);
+        int rc = take_dev_block(d, S, J.regen ? regen_job_bytes(J.npix) : job_bytes(J.npix), J.blk, ok);
         if (rc) return rc;
         if (!ok) break;
         char *base = (char *)J.blk.p;
-        J.accf = (float4 *)base;
-        J.acc64 = (double2 *)(base + J.npix * sizeof(float4));
-        J.out = (float4 *)(base + J.npix * (sizeof(float4) + 3 * sizeof(double2)));
-        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J.npix * 16, 256, 8192)), dim3(256), 0, d->stream,
-                           (uint32_t *)J.blk.p, 0u, J.npix * 16);
+        if (J.regen) {
+            // 3 fixed-point sums, then the resolved image
+            J.acc = (long long *)base;
+            J.out = (float4 *)(base + J.npix * 3 * sizeof(long long));
+            J.accf = nullptr;
+            J.acc64 = nullptr;
+            J.entry = -1;
+            hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J.npix * 6, 256, 8192)), dim3(256), 0, d->stream,
+                               (uint32_t *)J.blk.p, 0u, J.npix * 6);
+        } else {
+            J.accf = (float4 *)base;
+            J.acc64 = (double2 *)(base + J.npix * sizeof(float4));
+            J.out = (float4 *)(base + J.npix * (sizeof(float4) + 3 * sizeof(double2)));
+            hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J.npix * 16, 256, 8192)), dim3(256), 0, d->stream,
+                               (uint32_t *)J.blk.p, 0u, J.npix * 16);
+        }
         XT_LAUNCHED();
         if (!S.free_events.empty()) {
             J.done_ev = S.free_events.back();
@@ -222,7 +283,7 @@ void plan_srpt(xt_dev *d, ServeState &S, std::vector<Segment> &segs) {
     std::vector<std::pair<uint64_t, uint64_t>> order;  // (remaining slots, ticket)
     for (uint64_t t : S.active) {
         const SJob &J = S.jobs[t];
-        if (J.issued < J.jd.total)
+        if (!J.regen && J.issued < J.jd.total)
             order.push_back({(uint64_t)(J.jd.total - J.issued) * J.jd.NT - J.qc, t});
     }
     std::stable_sort(order.begin(), order.end(),
@@ -269,7 +330,7 @@ void plan_pass(xt_dev *d, ServeState &S, std::vector<Segment> &segs) {
     uint64_t oldest = 0;
     for (uint64_t t : S.active) {
         const SJob &J = S.jobs[t];
-        if (J.issued < J.jd.total) {
+        if (!J.regen && J.issued < J.jd.total) {
             oldest = t;
             break;
         }
@@ -279,7 +340,7 @@ void plan_pass(xt_dev *d, ServeState &S, std::vector<Segment> &segs) {
     for (size_t k = 0; k < S.active.size(); ++k) {
         const uint64_t t = S.active[(S.rr + k) % S.active.size()];
         const SJob &J = S.jobs[t];
-        if (t != oldest && J.issued < J.jd.total) order.push_back(t);
+        if (t != oldest && !J.regen && J.issued < J.jd.total) order.push_back(t);
     }
     const int64_t C = std::max<int64_t>(64, d->tuning.wave_paths & ~63u);
     const int64_t share = std::max<int64_t>(64, (C / (int64_t)order.size()) & ~63ll);
@@ -434,11 +495,388 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
     return XT_OK;
 }
 
+
+// ------------------------------------------------- path-regeneration engine
+
+
+int regen_malloc(xt_dev *d, void **p, uint64_t bytes) {
+    if (d->used + bytes > d->budget) {
+        d->err = "regeneration engine buffers exceed the HBM budget";
+        return XT_E_BUDGET;
+    }
+    XT_TRY(hipMalloc(p, bytes));
+    d->used += bytes;
+    return XT_OK;
+}
+
+void regen_free(xt_dev *d, void *&p, uint64_t &bytes) {
+    if (p) {
+        (void)hipFree(p);
+        d->used -= std::min(d->used, bytes);
+    }
+    p = nullptr;
+    bytes = 0;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Shared buffers of the engine: path state, free ring, retire list, job
+// table, control block, segments, lane counters and lane queue table.
+int regen_init(xt_dev *d, ServeState &S) {
+    RegenState &R = S.R;
+    if (R.ready) return XT_OK;
+    const uint32_t P = S.cfg.paths ? S.cfg.paths : d->tuning.wave_paths;
+    const size_t sz[] = {16ull * P, 16ull * P, 16ull * P, 8ull * P, 4ull * P, 4ull * P,
+                         sizeof(RegenJob) * XT_REGEN_MAX_JOBS, 8ull * XT_REGEN_MAX_JOBS, sizeof(RegenCtl),
+                         sizeof(RegenSeg) * XT_REGEN_MAX_JOBS, sizeof(Counters) * XT_REGEN_MAX_LANES,
+                         sizeof(Queue) * 2 * XT_REGEN_MAX_LANES};
+    size_t off[sizeof(sz) / sizeof(sz[0])], total = 0;
+    for (size_t k = 0; k < sizeof(sz) / sizeof(sz[0]); ++k) {
+        off[k] = total;
+        total += align256(sz[k]);
+    }
+    int rc = regen_malloc(d, &R.blk, total);
+    if (rc) return rc;
+    R.bytes = total;
+    char *b = (char *)R.blk;
+    RegenView &V = R.V;
+    V.ps0 = (float4 *)(b + off[0]);
+    V.ps1 = (float4 *)(b + off[1]);
+    V.rad = (float4 *)(b + off[2]);
+    V.pinfo = (uint2 *)(b + off[3]);
+    V.ring = (uint32_t *)(b + off[4]);
+    V.retire = (uint32_t *)(b + off[5]);
+    V.jobs = (RegenJob *)(b + off[6]);
+    V.done = (unsigned long long *)(b + off[7]);
+    V.ctl = (RegenCtl *)(b + off[8]);
+    V.segs = (RegenSeg *)(b + off[9]);
+    V.cnt = (Counters *)(b + off[10]);
+    R.lq = (Queue *)(b + off[11]);
+    V.lq = R.lq;
+    XT_TRY(hipHostMalloc((void **)&R.h_job, sizeof(RegenJob) * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
+    XT_TRY(hipHostMalloc((void **)&R.h_done, 8ull * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
+    XT_TRY(hipHostMalloc((void **)&R.h_prio, 4ull * (XT_REGEN_MAX_JOBS + 2), hipHostMallocDefault));
+    XT_TRY(hipHostMalloc((void **)&R.h_cnt, sizeof(Counters) * XT_REGEN_MAX_LANES, hipHostMallocDefault));
+    std::memset(R.lq_host, 0, sizeof R.lq_host);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(align256(sz[8]) / 4, 256, 64)), dim3(256), 0, d->stream,
+                       (uint32_t *)V.ctl, 0u, align256(sz[8]) / 4);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2ull * XT_REGEN_MAX_JOBS, 256, 64)), dim3(256), 0, d->stream,
+                       (uint32_t *)V.done, 0u, 2ull * XT_REGEN_MAX_JOBS);
+    hipLaunchKernelGGL(k_regen_init, dim3(grid_for(P, 256, 4096)), dim3(256), 0, d->stream, V.ctl, V.ring, P);
+    hipLaunchKernelGGL(k_reset_lanes, dim3(XT_REGEN_MAX_LANES), dim3(64), 0, d->stream, V.cnt, 0x800000ffu);
+    XT_LAUNCHED();
+    XT_TRY(hipMemcpyAsync(R.lq, R.lq_host, sizeof R.lq_host, hipMemcpyHostToDevice, d->stream));
+    XT_TRY(hipStreamSynchronize(d->stream));
+    R.P = P;
+    R.free_entries.clear();
+    for (int e = XT_REGEN_MAX_JOBS - 1; e >= 0; --e) R.free_entries.push_back(e);
+    for (auto &q : R.queries_seen) q = 0;
+    for (auto &q : R.retests_seen) q = 0;
+    R.ready = true;
+    R.dirty = true;
+    return XT_OK;
+}
+
+void regen_drop_graph(RegenState &R) {
+    if (R.exec) (void)hipGraphExecDestroy(R.exec);
+    if (R.graph) (void)hipGraphDestroy(R.graph);
+    R.exec = nullptr;
+    R.graph = nullptr;
+    R.dirty = true;
+}
+
+void regen_shutdown(xt_dev *d, ServeState &S) {
+    RegenState &R = S.R;
+    regen_drop_graph(R);
+    for (RegenLaneH &L : R.lanes) regen_free(d, L.blk, L.bytes);
+    regen_free(d, R.blk, R.bytes);
+    if (R.h_job) (void)hipHostFree(R.h_job);
+    if (R.h_done) (void)hipHostFree(R.h_done);
+    if (R.h_prio) (void)hipHostFree(R.h_prio);
+    if (R.h_cnt) (void)hipHostFree(R.h_cnt);
+    R = RegenState();
+}
+
+// The lane of (scene, integrator, accel), created on first use (its queues,
+// hit and re-test buffers sized for every path slot). -1: all lanes busy.
+int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
+    RegenState &R = S.R;
+    lane = -1;
+    int idle = -1;
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+        const RegenLaneH &L = R.lanes[l];
+        if (L.used && L.scene == J.job.scene_id && L.integrator == J.jd.integrator && L.accel == J.job.accel) {
+            lane = l;
+            return XT_OK;
+        }
+        if (!L.used && idle < 0) idle = l;
+    }
+    if (idle < 0) return XT_OK;
+    RegenLaneH &L = R.lanes[idle];
+    const DevScene &ds = d->scenes.at(J.job.scene_id);
+    const bool chan = xt_needs_chan(J.jd.integrator, ds.view);
+    const uint64_t P = R.P;
+    const size_t qsz = align256(16 * P), psz = align256(4 * P);
+    const size_t total = 4 * qsz + 2 * psz + qsz /*hit*/ + psz /*retest*/ + qsz /*recheck*/ + (chan ? 3 * qsz : 0);
+    int rc = regen_malloc(d, &L.blk, total);
+    if (rc) return rc;
+    L.bytes = total;
+    char *b = (char *)L.blk;
+    for (int qi = 0; qi < 2; ++qi) {
+        Queue &q = R.lq_host[qi * XT_REGEN_MAX_LANES + idle];
+        q.r0 = (float4 *)b;
+        q.r1 = (float4 *)(b + qsz);
+        q.path = (uint32_t *)(b + 2 * qsz);
+        b += 2 * qsz + psz;
+    }
+    L.hit = (float4 *)b;
+    b += qsz;
+    L.retest = (uint32_t *)b;
+    b += psz;
+    L.recheck = (uint4 *)b;
+    b += qsz;
+    L.chan = chan ? (float4 *)b : nullptr;
+    L.used = true;
+    L.scene = J.job.scene_id;
+    L.integrator = J.jd.integrator;
+    L.accel = J.job.accel;
+    hipLaunchKernelGGL(k_reset_lanes, dim3(1), dim3(64), 0, d->stream, R.V.cnt + idle, 0x800000ffu);
+    XT_LAUNCHED();
+    XT_TRY(hipMemcpyAsync(R.lq, R.lq_host, sizeof R.lq_host, hipMemcpyHostToDevice, d->stream));
+    R.queries_seen[idle] = R.retests_seen[idle] = 0;
+    R.dirty = true;
+    lane = idle;
+    return XT_OK;
+}
+
+// One query round over queue parity qi: plan, new camera rays, trace and
+// shade per lane, retire.
+int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
+    RegenState &R = S.R;
+    RegenView &V = R.V;
+    const hipStream_t st = d->stream;
+    const unsigned gP = grid_for(R.P, XT_BLOCK, 8192);
+    hipLaunchKernelGGL(k_regen_plan, dim3(1), dim3(64), 0, st, V, qi);
+    hipLaunchKernelGGL(k_regen_gen, dim3(gP), dim3(XT_BLOCK), 0, st, V, qi);
+    hipLaunchKernelGGL(k_regen_round_start, dim3(1), dim3(1), 0, st, V.ctl);
+    XT_LAUNCHED();
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+        RegenLaneH &L = R.lanes[l];
+        if (!L.used) continue;
+        const DevScene &ds = d->scenes.at(L.scene);
+        Counters *cnt = V.cnt + l;
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, cnt,
+                           (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
+        const Queue in = R.lq_host[qi * XT_REGEN_MAX_LANES + l], out = R.lq_host[(qi ^ 1) * XT_REGEN_MAX_LANES + l];
+        TraceBufs B;
+        B.q = in;
+        B.count = &cnt->n[qi];
+        B.hit = L.hit;
+        B.retest = L.retest;
+        B.recheck = L.recheck;
+        B.cnt = cnt;
+        B.cap = R.P;
+        int rc = xt_launch_trace_bufs(d, ds, (int)L.accel, B, nullptr, nullptr, false, st);
+        if (rc) return rc;
+        const unsigned g = grid_for(R.P, XT_BLOCK, 8192);
+        if (L.integrator == XT_INTEGRATOR_AO) {
+            hipLaunchKernelGGL(k_regen_shade_ao, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, out,
+                               &cnt->n[qi ^ 1], V.ps0, V.ps1, V.rad, V.retire, &V.ctl->retire_n);
+        } else if (L.chan) {
+            hipLaunchKernelGGL(k_channels, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, L.chan);
+            XT_LAUNCHED();
+            hipLaunchKernelGGL(k_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, out,
+                               &cnt->n[qi ^ 1], V.ps0, V.ps1, V.rad, (const float4 *)L.chan, V.retire,
+                               &V.ctl->retire_n);
+        } else {
+            hipLaunchKernelGGL(k_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit,
+                               out, &cnt->n[qi ^ 1], V.ps0, V.ps1, V.rad, (const float4 *)nullptr, V.retire,
+                               &V.ctl->retire_n);
+        }
+        XT_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_regen_retire, dim3(gP), dim3(XT_BLOCK), 0, st, V);
+    XT_LAUNCHED();
+    return XT_OK;
+}
+
+// Captures rounds over parity 0 then 1 (queue parity is back at 0 after a
+// replay) as one hipGraph.
+int regen_capture(xt_dev *d, ServeState &S) {
+    RegenState &R = S.R;
+    regen_drop_graph(R);
+    XT_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    int rc = regen_round(d, S, 0);
+    if (!rc) rc = regen_round(d, S, 1);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(d->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        d->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return XT_E_DEVICE;
+    }
+    R.graph = g;
+    XT_TRY(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0));
+    R.captured = d->tuning;
+    R.dirty = false;
+    return XT_OK;
+}
+
+// Priority order of the started, unfinished jobs (table entries) by policy.
+void regen_priorities(ServeState &S, std::vector<uint32_t> &prio, uint32_t &cap) {
+    std::vector<std::pair<uint64_t, const SJob *>> js;  // (admission rank, job)
+    uint64_t rank = 0;
+    for (uint64_t t : S.active) {
+        const SJob &J = S.jobs[t];
+        ++rank;
+        if (J.regen && J.entry >= 0) js.push_back({rank, &J});
+    }
+    auto left = [](const SJob *J) { return (uint64_t)J->jd.total * J->jd.NT - J->paths_done; };
+    cap = 0;
+    const int pol = S.policy;
+    if (pol == XT_SERVE_SRPT) {
+        std::stable_sort(js.begin(), js.end(), [&](const auto &a, const auto &b) { return left(a.second) < left(b.second); });
+    } else if (pol == XT_SERVE_HYBRID) {
+        const uint64_t shortw = S.R.P / 2;
+        std::stable_sort(js.begin(), js.end(), [&](const auto &a, const auto &b) {
+            const bool sa = left(a.second) <= shortw, sb = left(b.second) <= shortw;
+            if (sa != sb) return sa;
+            if (sa) return left(a.second) < left(b.second);
+            return a.first < b.first;
+        });
+    } else if (pol == XT_SERVE_FAIR && !js.empty()) {
+        cap = std::max<uint32_t>(64, S.R.P / (uint32_t)js.size());
+        std::rotate(js.begin(), js.begin() + (S.rr % js.size()), js.end());
+        S.rr++;
+    }
+    prio.clear();
+    for (const auto &e : js) prio.push_back((uint32_t)e.second->entry);
+}
+
+// One step of the engine: start admitted jobs, set priorities, run the
+// configured rounds, then finish the jobs whose paths have all ended.
+int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
+    int rc = regen_init(d, S);
+    if (rc) return rc;
+    RegenState &R = S.R;
+    RegenView &V = R.V;
+    // start admitted jobs that have no table entry yet
+    for (uint64_t t : S.active) {
+        SJob &J = S.jobs[t];
+        if (!J.regen || J.entry >= 0 || R.free_entries.empty()) continue;
+        int lane = -1;
+        rc = regen_lane(d, S, J, lane);
+        if (rc) return rc;
+        if (lane < 0) continue;  // every lane serves another key: wait
+        J.entry = R.free_entries.back();
+        R.free_entries.pop_back();
+        RegenJob &h = R.h_job[J.entry];
+        std::memset(&h, 0, sizeof h);
+        h.jd = J.jd;
+        h.total = (unsigned long long)J.jd.total * J.jd.NT;
+        h.next = 0;
+        h.acc = J.acc;
+        h.lane = (uint32_t)lane;
+        R.h_done[J.entry] = 0;
+        XT_TRY(hipMemcpyAsync(V.jobs + J.entry, &h, sizeof h, hipMemcpyHostToDevice, d->stream));
+        XT_TRY(hipMemcpyAsync(V.done + J.entry, R.h_done + J.entry, 8, hipMemcpyHostToDevice, d->stream));
+        J.paths_done = 0;
+    }
+    std::vector<uint32_t> prio;
+    uint32_t cap = 0;
+    regen_priorities(S, prio, cap);
+    if (prio.empty()) return XT_OK;
+    R.h_prio[0] = (uint32_t)prio.size();
+    R.h_prio[1] = cap;
+    std::memcpy(R.h_prio + 2, prio.data(), prio.size() * 4);
+    XT_TRY(hipMemcpyAsync(&V.ctl->n_prio, R.h_prio, 8, hipMemcpyHostToDevice, d->stream));
+    XT_TRY(hipMemcpyAsync(V.ctl->prio, R.h_prio + 2, prio.size() * 4, hipMemcpyHostToDevice, d->stream));
+    XT_TRY(hipEventRecord(S.p0, d->stream));
+    const uint32_t pairs = std::max<uint32_t>(1, S.cfg.rounds_per_step / 2);
+    if (S.cfg.graph == 2) {
+        if (R.dirty || std::memcmp(&R.captured, &d->tuning, sizeof(xt_tuning)) != 0) {
+            rc = regen_capture(d, S);
+            if (rc) return rc;
+        }
+        for (uint32_t k = 0; k < pairs; ++k) XT_TRY(hipGraphLaunch(R.exec, d->stream));
+    } else {
+        for (uint32_t k = 0; k < pairs; ++k) {
+            rc = regen_round(d, S, 0);
+            if (!rc) rc = regen_round(d, S, 1);
+            if (rc) return rc;
+        }
+    }
+    XT_TRY(hipEventRecord(S.p1, d->stream));
+    XT_TRY(hipMemcpyAsync(R.h_done, V.done, 8ull * XT_REGEN_MAX_JOBS, hipMemcpyDeviceToHost, d->stream));
+    XT_TRY(hipMemcpyAsync(R.h_cnt, V.cnt, sizeof(Counters) * XT_REGEN_MAX_LANES, hipMemcpyDeviceToHost, d->stream));
+    XT_TRY(hipStreamSynchronize(d->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, S.p0, S.p1);
+    S.info.ms_device += ms;
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+        if (!R.lanes[l].used) continue;
+        S.info.queries += R.h_cnt[l].queries - R.queries_seen[l];
+        S.info.retests += R.h_cnt[l].retests - R.retests_seen[l];
+        R.queries_seen[l] = R.h_cnt[l].queries;
+        R.retests_seen[l] = R.h_cnt[l].retests;
+    }
+    bool any_done = false;
+    for (uint64_t t : S.active) {
+        SJob &J = S.jobs[t];
+        if (!J.regen || J.entry < 0) continue;
+        J.passes += 1;  // steps the job has taken part in
+        const uint64_t total = (uint64_t)J.jd.total * J.jd.NT;
+        const uint64_t done = R.h_done[J.entry];
+        S.info.samples += (done - J.paths_done) * J.npix / J.jd.NT;
+        J.paths_done = done;
+        if (done < total) continue;
+        hipLaunchKernelGGL(k_regen_resolve, dim3(grid_for(J.npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, J.jd,
+                           (const long long *)J.acc, J.out);
+        XT_LAUNCHED();
+        rc = take_host_block(d, S, J.npix * sizeof(float4), J.host);
+        if (rc) return rc;
+        XT_TRY(hipMemcpyAsync(J.host.p, J.out, J.npix * sizeof(float4), hipMemcpyDeviceToHost, d->stream));
+        XT_TRY(hipEventRecord(J.done_ev, d->stream));
+        J.issued = J.jd.total;
+        R.free_entries.push_back(J.entry);
+        J.entry = -1;
+        finished.push_back(t);
+        any_done = true;
+    }
+    // completion events must have happened before they are timed
+    if (any_done) XT_TRY(hipStreamSynchronize(d->stream));
+    S.info.passes += 1;
+    return XT_OK;
+}
+
+// Lanes whose jobs have all finished give their buffers back.
+void regen_release_idle_lanes(xt_dev *d, ServeState &S) {
+    RegenState &R = S.R;
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+        RegenLaneH &L = R.lanes[l];
+        if (!L.used) continue;
+        bool busy = false;
+        for (uint64_t t : S.active) {
+            const SJob &J = S.jobs[t];
+            busy = busy || (J.regen && J.state == ACTIVE && J.job.scene_id == L.scene && J.jd.integrator == L.integrator &&
+                            J.job.accel == L.accel);
+        }
+        if (busy) continue;
+        regen_free(d, L.blk, L.bytes);
+        L = RegenLaneH();
+        R.dirty = true;
+    }
+}
+
 void release_block(ServeState &S, SJob &J) {
     if (J.blk.p) S.free_dev.push_back(J.blk);
     J.blk = DevBlock();
     J.accf = J.out = nullptr;
     J.acc64 = nullptr;
+    J.acc = nullptr;
 }
 
 }  // namespace
@@ -455,6 +893,7 @@ void xt_serve_shutdown(xt_dev *d) {
     if (!S) return;
     hipSetDevice(d->device);
     hipStreamSynchronize(d->stream);
+    regen_shutdown(d, *S);
     for (auto &kv : S->jobs) {
         if (kv.second.blk.p) {
             hipFree(kv.second.blk.p);
@@ -492,7 +931,7 @@ extern "C" int xt_submit(xt_dev *d, const xt_job *job, uint64_t *ticket) {
     const size_t npix = (size_t)jd.W * jd.H;
     // memory that job buffers can never use: scenes and wavefront buffers
     const uint64_t fixed = d->used - std::min(d->used, S.held);
-    if (fixed + job_bytes(npix) > d->budget) {
+    if (fixed + (regen_eligible(S, jd) ? regen_job_bytes(npix) : job_bytes(npix)) > d->budget) {
         d->err = "job buffers exceed the HBM budget";
         return XT_E_BUDGET;
     }
@@ -520,7 +959,7 @@ extern "C" int xt_poll(xt_dev *d, uint64_t ticket, int *done, float *rgba_out, x
     SJob &J = it->second;
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
-        stats->samples = (uint64_t)J.issued * J.npix;
+        stats->samples = J.regen && J.state != FINISHED ? J.paths_done * J.npix / J.jd.NT : (uint64_t)J.issued * J.npix;
         stats->passes = J.passes;
         stats->queue_ms = J.state == WAITING ? now_ms() - J.t_submit : J.t_admit - J.t_submit;
     }
@@ -546,12 +985,20 @@ extern "C" int xt_serve_step(xt_dev *d, int *active_jobs) {
     XT_TRY(hipSetDevice(d->device));
     int rc = admit(d, S);
     if (rc) return rc;
+    std::vector<uint64_t> finished;
+    bool any_regen = false;
+    for (uint64_t t : S.active) any_regen = any_regen || S.jobs[t].regen;
+    if (any_regen) {
+        rc = regen_step(d, S, finished);
+        if (rc) return rc;
+    }
     std::vector<Segment> segs;
     plan_pass(d, S, segs);
     if (!segs.empty()) {
-        std::vector<uint64_t> finished;
         rc = run_pass(d, S, segs, finished);
         if (rc) return rc;
+    }
+    if (!finished.empty()) {
         for (uint64_t t : finished) {
             SJob &J = S.jobs[t];
             float ms = 0.f;
@@ -563,6 +1010,7 @@ extern "C" int xt_serve_step(xt_dev *d, int *active_jobs) {
             S.info.jobs_finished += 1;
         }
         if (!S.active.empty()) S.rr %= S.active.size();
+        regen_release_idle_lanes(d, S);
         // room freed by finished jobs admits waiting ones for the next pass
         rc = admit(d, S);
         if (rc) return rc;
@@ -588,10 +1036,45 @@ extern "C" int xt_drain(xt_dev *d) {
 }
 
 extern "C" int xt_serve_set_policy(xt_dev *d, int policy) {
-    if (!d || (policy != XT_SERVE_FAIR && policy != XT_SERVE_SRPT)) return XT_E_INVALID;
+    if (!d || policy < XT_SERVE_FAIR || policy > XT_SERVE_HYBRID) return XT_E_INVALID;
     int rc = serve_init(d);
     if (rc) return rc;
     d->serve->policy = policy;
+    return XT_OK;
+}
+
+extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
+    if (!d || !c) return XT_E_INVALID;
+    if (c->regen > 2 || c->graph > 2 || (c->rounds_per_step & 1u) ||
+        c->rounds_per_step > 1024 || (c->paths && (c->paths < 4096 || c->paths > (1u << 28)))) {
+        d->err = "bad serve configuration";
+        return XT_E_INVALID;
+    }
+    int rc = serve_init(d);
+    if (rc) return rc;
+    ServeState &S = *d->serve;
+    bool regen_busy = false;
+    for (uint64_t t : S.active) regen_busy = regen_busy || S.jobs[t].regen;
+    if ((c->paths && c->paths != S.R.P && S.R.ready) || (c->regen && c->regen != S.cfg.regen)) {
+        if (regen_busy || !S.waiting.empty()) {
+            d->err = "the regeneration engine holds jobs";
+            return XT_E_STATE;
+        }
+        regen_shutdown(d, S);
+    }
+    if (c->regen) S.cfg.regen = c->regen;
+    if (c->graph) S.cfg.graph = c->graph;
+    if (c->rounds_per_step) S.cfg.rounds_per_step = c->rounds_per_step;
+    if (c->paths) S.cfg.paths = c->paths;
+    return XT_OK;
+}
+
+extern "C" int xt_serve_get_config(xt_dev *d, xt_serve_config *c) {
+    if (!d || !c) return XT_E_INVALID;
+    int rc = serve_init(d);
+    if (rc) return rc;
+    *c = d->serve->cfg;
+    if (!c->paths) c->paths = d->serve->R.ready ? d->serve->R.P : d->tuning.wave_paths;
     return XT_OK;
 }
 
