@@ -300,8 +300,9 @@ int xt_render_device(xt_dev *d, const xt_job *job, float *d_rgba_out, xt_job_sta
 /* Serving: continuous batching of many jobs on one device, in place of the
  * reference's one blocking IIntegrator::render() per context.
  * xt_submit queues the job and returns a ticket; it is admitted (FIFO) once
- * its buffers (80 B per pixel in whole passes, 40 on the regeneration engine
- * below) fit the HBM budget, and refused with
+ * its buffers (80 B per pixel in whole passes; on the regeneration engine
+ * below 16 B per pixel and sample plus 16, or 40 per pixel) fit the HBM
+ * budget, and refused with
  * XT_E_BUDGET when they never could. xt_serve_step runs the configured
  * query rounds of the regeneration engine and one wavefront pass shared by
  * the other admitted jobs, and reports the jobs not yet finished. xt_poll
@@ -346,18 +347,22 @@ int xt_serve_set_policy(xt_dev *d, int policy);
  * which every query round refills the slots of finished paths with camera
  * rays of waiting samples (in priority order), so jobs join and leave between
  * rounds; each round is one fixed launch sequence, replayed from a hipGraph.
- * Their radiance is summed per pixel in 64-bit fixed point (2^-32), which makes
- * the image independent of the order in which paths end: a served image
- * differs from the same job's xt_render image by the rounding of the two sums
- * (at most a few units in the last place of the float result). Every other
- * job runs in whole passes (bit-identical to xt_render). In
- * xt_serve_configure a 0 field keeps its value. */
+ * Each finished path's radiance is kept per (pixel, sample) and summed per
+ * pixel in sample order when the job ends, so a served image is bit-identical
+ * to xt_render's. A job whose sample buffer would exceed sample_buffer_mb sums
+ * in 64-bit fixed point (2^-32) instead, independent of the order in which
+ * paths end; its image differs from xt_render's by the rounding of the two
+ * sums (a few units in the last place of the float result). Every other job
+ * runs in whole passes (bit-identical to xt_render). In xt_serve_configure a
+ * 0 field keeps its value. */
 typedef struct xt_serve_config {
     uint32_t regen;           /* 2: the regeneration engine as above; 1: every job in whole passes (2) */
     uint32_t graph;           /* 2: rounds replayed from a hipGraph; 1: launched kernel by kernel (2) */
     uint32_t rounds_per_step; /* query rounds per xt_serve_step on the regeneration engine, even (4) */
     uint32_t paths;           /* path slots of the regeneration engine, 4096..2^28 (wave_paths); set while
                                  it holds no job */
+    uint32_t sample_buffer_mb;/* a job keeps its samples' radiance (16 B each) up to this size in MiB and is
+                                 bit-identical to xt_render; a larger one sums in fixed point (4096) */
 } xt_serve_config;
 int xt_serve_configure(xt_dev *d, const xt_serve_config *c);
 int xt_serve_get_config(xt_dev *d, xt_serve_config *c);

@@ -1,9 +1,11 @@
 """Continuous-batching server.
 
-Jobs served in whole passes (the deterministic integrators, and every job
-with the regeneration engine off) equal one-job renders bit for bit. ao and
-pathtracer jobs on the path-regeneration engine sum each pixel's samples in
-64-bit fixed point, so their images do not depend on the schedule (graph or
+Every served job equals its one-job render bit for bit: jobs in whole passes
+(the deterministic integrators, and every job with the regeneration engine
+off) accumulate like xt_render, and ao / pathtracer jobs on the
+path-regeneration engine keep each sample's radiance and sum it per pixel in
+sample order when they finish. Jobs too large for a sample buffer sum in
+64-bit fixed point: their images do not depend on the schedule (graph or
 launch by launch, policy, mix, path slots) and differ from the one-job
 render's compensated FP64 sum only by the rounding of the two sums. Jobs join
 between rounds; admission follows the HBM budget."""
@@ -56,8 +58,8 @@ def _regen(job):
     return job.integrator in (xa.AO, xa.PATHTRACER) and job.accum == xa.ACCUM_DEFAULT
 
 
-def _same(img, ref, regen):
-    if not regen:
+def _same(img, ref, fixed=False):
+    if not fixed:
         return np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     return np.allclose(img, ref, rtol=REGEN_RTOL, atol=REGEN_ATOL)
 
@@ -92,7 +94,7 @@ def test_served_equals_alone(small_passes, policy, regen):
     assert dev.serve_info()["jobs_active"] == 0
     for (t, (img, st)), j in zip(got.items(), jobs):
         ref, _ = dev.render(j)
-        assert _same(img, ref, regen and _regen(j)), f"job {t} ({j.integrator}) differs from its one-job render"
+        assert _same(img, ref), f"job {t} ({j.integrator}) differs from its one-job render"
         assert st.latency_ms >= st.queue_ms >= 0.0
         assert st.samples == j.samples * j.width * j.height
 
@@ -118,6 +120,33 @@ def test_regen_images_independent_of_schedule(small_passes):
         t = dev.submit(j)
         alone = _drain_all(dev, {t: (j.width, j.height)})[t][0]
         assert np.array_equal(alone.view(np.uint32), runs[0][k].view(np.uint32)), f"job {k} alone"
+
+
+@pytest.mark.gpu
+def test_regen_fixed_point_jobs(small_passes):
+    """With a 1 MiB sample-buffer limit the larger jobs sum in fixed point:
+    within the rounding of the two sums of their one-job renders, and the same
+    bits under every schedule."""
+    dev = small_passes
+    dev.serve_configure(sample_buffer_mb=1)
+    try:
+        sc = _scenes(dev)
+        jobs = [j for j in _mix_jobs(sc) if _regen(j)]
+        fixed = [j.width * j.height * j.samples * 16 > (1 << 20) for j in jobs]
+        assert any(fixed) and not all(fixed)
+        runs = []
+        for policy, graph in (("srpt", True), ("fcfs", False)):
+            dev.set_serve_policy(policy)
+            dev.serve_configure(graph=graph)
+            tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
+            runs.append([img for img, _ in _drain_all(dev, tickets).values()])
+        for a, b in zip(*runs):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        for img, j, fx in zip(runs[0], jobs, fixed):
+            ref, _ = dev.render(j)
+            assert _same(img, ref, fx)
+    finally:
+        dev.serve_configure(sample_buffer_mb=4096, graph=True)
 
 
 @pytest.mark.gpu
@@ -182,9 +211,9 @@ def test_small_job_overtakes_large(small_passes):
     done_big, img_big, _ = dev.poll(tb, 256, 256)
     assert done_big
     ref, _ = dev.render(big)
-    assert _same(img_big, ref, True)
+    assert _same(img_big, ref)
     ref_small, _ = dev.render(small)
-    assert _same(img, ref_small, True)
+    assert _same(img, ref_small)
 
 
 @pytest.mark.gpu

@@ -79,7 +79,7 @@ class Tuning(ctypes.Structure):
 
 
 class ServeConfig(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_uint32) for n in ("regen", "graph", "rounds_per_step", "paths")]
+    _fields_ = [(n, ctypes.c_uint32) for n in ("regen", "graph", "rounds_per_step", "paths", "sample_buffer_mb")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -482,12 +482,12 @@ class Device:
             policy = SERVE_POLICIES[policy]
         self._check(lib().xt_serve_set_policy(self._d, int(policy)))
 
-    def serve_configure(self, regen=0, graph=0, rounds_per_step=0, paths=0):
-        """regen / graph: True or False (None keeps the value); rounds_per_step,
-        paths: 0 keeps the value. Returns the configuration in force."""
+    def serve_configure(self, regen=None, graph=None, rounds_per_step=0, paths=0, sample_buffer_mb=0):
+        """regen / graph: True or False (None keeps the value); the counts: 0
+        keeps the value. Returns the configuration in force."""
         def flag(v):
-            return 0 if v in (0, None) and v is not False else (2 if v else 1)
-        c = ServeConfig(flag(regen), flag(graph), int(rounds_per_step), int(paths))
+            return 0 if v is None else (2 if v else 1)
+        c = ServeConfig(flag(regen), flag(graph), int(rounds_per_step), int(paths), int(sample_buffer_mb))
         self._check(lib().xt_serve_configure(self._d, ctypes.byref(c)))
         return self.serve_config()
 

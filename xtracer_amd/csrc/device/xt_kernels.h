@@ -816,14 +816,92 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_channels(const SceneView S,
 // SAMPLED: the material colours and the environment come from k_channels'
 // output `chan` (slot-indexed) instead of the material and environment
 // records.
-// With `retire` (the server's path regeneration), every path that ends here
-// is appended to that list.
+// One shading step of a pathtracer path at its hit h (ray a, b): the closed
+// form of pathtracer/integrator.cc:21-47 with the path state s0 = (T, key),
+// s1 = (F, query word). A miss ends the path with L = T env, an emissive hit
+// with L = T F Le, a dielectric with 0, a continuing material at query
+// rdepth with 0; otherwise the path continues (cont) with the next ray
+// (no, nd, nprim, nself) and s0 / s1 updated. chan: the hit's three channel
+// colours (SAMPLED only).
+template <bool SAMPLED>
+__device__ __forceinline__ void pt_step(const SceneView &S, const float4 h, const float4 a, const float4 b, float4 &s0,
+                                        float4 &s1, const float4 *chan, f3 &L, bool &cont, f3 &no, f3 &nd,
+                                        uint32_t &nprim, float &nself) {
+    const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+    const f3 T = mk(s0.x, s0.y, s0.z), F = mk(s1.x, s1.y, s1.z);
+    const uint32_t key = __float_as_uint(s0.w), qword = __float_as_uint(s1.w);
+    const uint32_t q = qword & 0xffffu, rdepth = qword >> 16;
+    const uint32_t prim = __float_as_uint(h.y);
+    cont = false;
+    L = mk(0.f, 0.f, 0.f);
+    if (prim == XT_NONE) {
+        f3 e;
+        if (SAMPLED) {
+            const float4 c = chan[0];
+            e = mk(c.x, c.y, c.z);
+        } else {
+            e = env_constant(S.env, d);
+        }
+        L = cmul(T, e);
+        return;
+    }
+    f3 p, N, g;
+    uint32_t obj;
+    surface(S, prim, o, d, h.x, h.z, h.w, p, N, g, obj);
+    const uint32_t mi = S.obj_material[obj];
+    const XtMaterial m = S.materials[mi == XT_NONE ? 0 : mi];
+    // channel colours: the material's constants, or its samplers at the hit's
+    // texture coordinates (material.cc:65-70)
+    f3 kd = mk(m.kd[0], m.kd[1], m.kd[2]), ks = mk(m.ks[0], m.ks[1], m.ks[2]);
+    f3 ke = mk(m.ke[0], m.ke[1], m.ke[2]);
+    if (SAMPLED) {
+        const float4 c0 = chan[0], c1 = chan[1], c2 = chan[2];
+        kd = mk(c0.x, c0.y, c0.z);
+        ks = mk(c1.x, c1.y, c1.z);
+        ke = mk(c2.x, c2.y, c2.z);
+    }
+    if (m.type == XT_MAT_EMISSIVE) {
+        L = cmul(cmul(T, F), ke);
+    } else if (m.type == XT_MAT_DIELECTRIC) {
+        // ends with zero intensity
+    } else if (q < rdepth) {
+        Draws r{xt_stream(key, q), 0};
+        f3 f, dir;
+        if (m.type == XT_MAT_LAMBERT) {
+            dir = norm3(smp_diffuse(N, r));
+            f = cscale(kd, dot3(N, dir));
+        } else {
+            const float k = r.next();
+            if (k > m.reflectance) {
+                f = kd;
+                dir = smp_diffuse(N, r);
+            } else {
+                f = ks;
+                dir = smp_lobe(N, scl(d, -1.f), m.exponent, r);
+                if (m.type == XT_MAT_BLINNPHONG) dir = norm3(dir);
+            }
+            f = cscale(f, 1.f);
+        }
+        const f3 T2 = cmul(cmul(T, F), f);
+        s0 = make_float4(T2.x, T2.y, T2.z, s0.w);
+        s1 = make_float4(f.x, f.y, f.z, __uint_as_float(qword + 1));
+        cont = true;
+        no = p;
+        nd = dir;
+        nprim = prim;
+        nself = self_hit_t(g, N, dir);
+    }
+    // q == rdepth with a continuing material: eval(0) == 0
+}
+
+// Pathtracer shading over a queue with path state by path id: a path that
+// ends writes its radiance to rad[path]; continuing paths are appended to
+// the next queue (one atomic per block).
 template <bool SAMPLED>
 static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S, const Queue in,
                                                        const uint32_t *in_count, const float4 *hit, Queue out,
                                                        uint32_t *out_count, float4 *ps0, float4 *ps1, float4 *rad,
-                                                       const float4 *chan, uint32_t *retire = nullptr,
-                                                       uint32_t *retire_n = nullptr) {
+                                                       const float4 *chan) {
     __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -835,82 +913,19 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_pt(const SceneView S,
         float nself = 0.f;
         if (i < n) {
             path = in.path[i];
-            const float4 h = hit[i];
-            const float4 a = in.r0[i], b = in.r1[i];
-            const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-            const float4 s0 = ps0[path], s1 = ps1[path];
-            const f3 T = mk(s0.x, s0.y, s0.z), F = mk(s1.x, s1.y, s1.z);
-            const uint32_t key = __float_as_uint(s0.w), qword = __float_as_uint(s1.w);
-            const uint32_t q = qword & 0xffffu, rdepth = qword >> 16;
-            const uint32_t prim = __float_as_uint(h.y);
-            if (prim == XT_NONE) {
-                f3 e;
-                if (SAMPLED) {
-                    const float4 c = chan[3 * (size_t)i];
-                    e = mk(c.x, c.y, c.z);
-                } else {
-                    e = env_constant(S.env, d);
-                }
-                const f3 L = cmul(T, e);
-                rad[path] = make_float4(L.x, L.y, L.z, 0.f);
+            float4 s0 = ps0[path], s1 = ps1[path];
+            f3 L;
+            pt_step<SAMPLED>(S, hit[i], in.r0[i], in.r1[i], s0, s1, SAMPLED ? chan + 3 * (size_t)i : nullptr, L,
+                             cont, no, nd, nprim, nself);
+            if (cont) {
+                ps0[path] = s0;
+                ps1[path] = s1;
             } else {
-                f3 p, N, g;
-                uint32_t obj;
-                surface(S, prim, o, d, h.x, h.z, h.w, p, N, g, obj);
-                const uint32_t mi = S.obj_material[obj];
-                const XtMaterial m = S.materials[mi == XT_NONE ? 0 : mi];
-                // channel colours: the material's constants, or its samplers at
-                // the hit's texture coordinates (material.cc:65-70)
-                f3 kd = mk(m.kd[0], m.kd[1], m.kd[2]), ks = mk(m.ks[0], m.ks[1], m.ks[2]);
-                f3 ke = mk(m.ke[0], m.ke[1], m.ke[2]);
-                if (SAMPLED) {
-                    const float4 c0 = chan[3 * (size_t)i], c1 = chan[3 * (size_t)i + 1], c2 = chan[3 * (size_t)i + 2];
-                    kd = mk(c0.x, c0.y, c0.z);
-                    ks = mk(c1.x, c1.y, c1.z);
-                    ke = mk(c2.x, c2.y, c2.z);
-                }
-                if (m.type == XT_MAT_EMISSIVE) {
-                    const f3 L = cmul(cmul(T, F), ke);
-                    rad[path] = make_float4(L.x, L.y, L.z, 0.f);
-                } else if (m.type == XT_MAT_DIELECTRIC) {
-                    rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
-                } else if (q < rdepth) {
-                    Draws r{xt_stream(key, q), 0};
-                    f3 f, dir;
-                    if (m.type == XT_MAT_LAMBERT) {
-                        dir = norm3(smp_diffuse(N, r));
-                        f = cscale(kd, dot3(N, dir));
-                    } else {
-                        const float k = r.next();
-                        if (k > m.reflectance) {
-                            f = kd;
-                            dir = smp_diffuse(N, r);
-                        } else {
-                            f = ks;
-                            dir = smp_lobe(N, scl(d, -1.f), m.exponent, r);
-                            if (m.type == XT_MAT_BLINNPHONG) dir = norm3(dir);
-                        }
-                        f = cscale(f, 1.f);
-                    }
-                    const f3 T2 = cmul(cmul(T, F), f);
-                    ps0[path] = make_float4(T2.x, T2.y, T2.z, s0.w);
-                    ps1[path] = make_float4(f.x, f.y, f.z, __uint_as_float(qword + 1));
-                    cont = true;
-                    no = p;
-                    nd = dir;
-                    nprim = prim;
-                    nself = self_hit_t(g, N, dir);
-                }
-                // q == rdepth with a continuing material: eval(0) == 0
+                rad[path] = make_float4(L.x, L.y, L.z, 0.f);
             }
         }
         const uint32_t slot = block_append<XT_BLOCK>(cont, out_count, lds);
         if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
-        if (retire) {
-            const bool ended = i < n && !cont;
-            const uint32_t rs = block_append<XT_BLOCK>(ended, retire_n, lds);
-            if (ended) retire[rs] = path;
-        }
     }
 }
 
@@ -1178,6 +1193,19 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_shade_stencil(const Queue i
 
 // --------------------------------------------------------------- accumulation
 
+#pragma clang fp contract(off)
+// A sample's value times its weight in FP64, and its addition to a
+// Neumaier-compensated sum (sum, compensation), without contraction so that
+// every kernel that accumulates samples rounds them alike.
+__device__ __forceinline__ double weighted(float v, double w) { return (double)v * w; }
+__device__ __forceinline__ void neumaier_add(double2 &a, double x) {
+    const double t = a.x + x;
+    if (fabs(a.x) >= fabs(x)) a.y += (a.x - t) + x;
+    else a.y += (x - t) + a.x;
+    a.x = t;
+}
+#pragma clang fp contract(on)
+
 // Per pixel of tile positions [q0, q0 + qn), samples of this pass in sample
 // order (radiance of sample pass_base + sl at rad[sl * qn + q - q0]).
 // REFERENCE reproduces the float read-modify-write of each integrator
@@ -1236,19 +1264,14 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_accumulate(const JobDev j, 
                     a.w = 1.f;
                 }
             } else {
-                const double w = j.weight;
-                double xs[3] = {(double)v.x * w, (double)v.y * w, (double)v.z * w};
+                double xs[3] = {weighted(v.x, j.weight), weighted(v.y, j.weight), weighted(v.z, j.weight)};
                 if (j.integrator == XT_INTEGRATOR_STENCIL || j.integrator == XT_INTEGRATOR_WIREFRAME) xs[1] = xs[2] = xs[0];
                 if (j.integrator == XT_INTEGRATOR_EMISSION && v.w == 0.f) xs[0] = xs[1] = xs[2] = 0.0;
                 if (j.integrator == XT_INTEGRATOR_DEPTH && v.y == 0.f) xs[0] = xs[1] = xs[2] = 0.0;
                 if (j.integrator == XT_INTEGRATOR_DEPTH) xs[1] = xs[2] = xs[0];
-                double2 *acc[3] = {&s0, &s1, &s2};
-                for (int c = 0; c < 3; ++c) {
-                    const double sum = acc[c]->x, t = sum + xs[c];
-                    if (fabs(sum) >= fabs(xs[c])) acc[c]->y += (sum - t) + xs[c];
-                    else acc[c]->y += (xs[c] - t) + sum;
-                    acc[c]->x = t;
-                }
+                neumaier_add(s0, xs[0]);
+                neumaier_add(s1, xs[1]);
+                neumaier_add(s2, xs[2]);
             }
         }
         accf[pix] = a;

@@ -1,72 +1,91 @@
 // Kernels of the server's path-regeneration engine (ao and pathtracer jobs).
 //
-// A fixed pool of P path slots is shared by every active job. Each query
-// round is one fixed launch sequence, captured once as a hipGraph:
+// A pool of P path slots is shared by every active job: the queues of all
+// lanes together hold at most P live paths. Each query round is one fixed
+// launch sequence, captured once as a hipGraph:
 //
-//   k_regen_plan   one wave: hands the free slots to jobs in priority order
-//                  (a segment per job: its next slot indices, its lane and
-//                  where its rays go in that lane's queue)
+//   k_regen_plan   one wave: hands the free room (P minus the live paths) to
+//                  jobs in priority order (a segment per job: its next slot
+//                  indices, its lane and where its rays go in that lane's
+//                  queue)
 //   k_regen_gen    camera rays of the new segments, appended to each lane's
 //                  queue behind the paths that are still running
-//   per lane       k_trace / k_recheck / k_retest / shading, as for one job;
-//                  shading appends every path that ends to the retire list
-//   k_regen_retire each ended path's radiance added to its job's pixel in
-//                  64-bit fixed point (exact, so the order in which paths end
-//                  does not matter), its job's done count raised and its slot
-//                  returned to the free ring
+//   per lane       k_trace / k_recheck / k_retest as for one job, then the
+//                  shading step; a path that continues is appended to the
+//                  lane's next queue, one that ends to the retire list
+//   k_regen_retire each ended path's radiance stored in its job's sample
+//                  buffer at (pixel, sample), or, for a job too large for
+//                  one, added to its pixel in 64-bit fixed point (exact, so
+//                  the order in which paths end does not matter); the job's
+//                  done count raised
 //
-// A lane is one (scene, integrator, accel) with its own queues and counters,
-// as in a one-job render. Jobs are admitted, prioritised and finished by the
-// host between graph launches; the device never waits on the host inside a
-// round.
+// A finished job's sample buffer is summed per pixel in sample order with the
+// compensated FP64 sum of a one-job render (k_regen_resolve_ordered), so its
+// image is bit for bit the xt_render image.
+//
+// A path's state (throughput, last factor, random key, query count, job and
+// pixel) travels with its ray in the queue, so every kernel reads and writes
+// it at consecutive queue positions. A lane is one (scene, integrator, accel)
+// with its own queues and counters, as in a one-job render. Jobs are
+// admitted, prioritised and finished by the host between graph launches; the
+// device never waits on the host inside a round.
 #pragma once
 #include "xt_kernels.h"
 
 #define XT_REGEN_MAX_JOBS 1024
 #define XT_REGEN_MAX_LANES 8
 #define XT_REGEN_FIX 4294967296.0  // 2^32: fixed-point scale of the radiance sums
+#define XT_REGEN_SHARDS 64         // done counters per job, summed at the end of a step
 
 // One job of the device table (written by the host when the job is admitted).
 struct RegenJob {
     JobDev jd;
     unsigned long long total;  // path slots of the job: NT tile positions x samples per pixel
     unsigned long long next;   // first slot not yet handed out (k_regen_plan)
-    long long *acc;            // 3 fixed-point sums per pixel
+    float4 *sbuf;              // radiance per (pixel, sample): pixel * samples + sample; or null
+    long long *acc;            // without sbuf: 3 fixed-point sums per pixel
     uint32_t lane;
     uint32_t pad;
 };
 
 // A job's share of one round: `count` slots from slot index `slot0`, new rays
 // [first, first + count) of the round, written at lane queue position qpos.
+// Slot index k of a job is sample k % total of tile position k / total (the
+// samples of one pixel follow each other); slot0 is sample s0 of position q0.
 struct RegenSeg {
     unsigned long long slot0;
-    uint32_t job, lane, first, count, qpos, pad;
+    uint32_t job, lane, first, count, qpos, q0, s0, pad;
 };
 
 struct RegenCtl {
-    unsigned long long fl_head, fl_tail;  // free ring: slots [fl_head, fl_tail) mod P are free
-    unsigned long long gen_head;          // fl_head when this round's plan ran
     uint32_t P;
     uint32_t n_prio;       // jobs in prio[], highest priority first
     uint32_t cap_per_job;  // 0: none; else the most slots a job takes per round
     uint32_t n_seg, n_new;
     uint32_t retire_n;
-    uint32_t lane_in_use[XT_REGEN_MAX_LANES];
     uint32_t prio[XT_REGEN_MAX_JOBS];
+};
+
+// Path state next to a lane queue's rays: s0 = (T rgb, random key),
+// s1 = (F rgb, query word), info = (job-table entry, pixel or XT_NONE for a
+// slot past the image edge, sample, 0).
+struct RegenState {
+    float4 *s0, *s1;
+    uint4 *info;
 };
 
 // Everything one round reads and writes besides the lanes' queues.
 struct RegenView {
     RegenJob *jobs;
-    unsigned long long *done;  // paths finished, per job-table entry
+    unsigned long long *done;  // paths finished, per job-table entry (k_regen_done_sum)
+    unsigned long long *done_shard;  // the same in XT_REGEN_SHARDS parts: [shard * XT_REGEN_MAX_JOBS + entry]
     RegenCtl *ctl;
     RegenSeg *segs;
-    float4 *ps0, *ps1, *rad;
-    uint2 *pinfo;     // per slot: job-table entry, pixel (XT_NONE: a slot past the image edge)
-    uint32_t *ring;   // free ring, P entries
-    uint32_t *retire; // ended paths of the round
-    Counters *cnt;    // one counter block per lane
-    const Queue *lq;  // lane queues: lq[qi * XT_REGEN_MAX_LANES + lane]
+    float4 *ret;           // ended paths: radiance rgb, pixel bits
+    uint2 *ret_js;         // ended paths: job-table entry, sample
+    Counters *cnt;         // one counter block per lane
+    const Queue *lq;       // lane queues: lq[qi * XT_REGEN_MAX_LANES + lane]
+    const RegenState *ls;  // their path state, same indexing
 };
 
 __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, uint32_t &total) {
@@ -80,18 +99,19 @@ __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, uint32_t &total) {
     return x - v;
 }
 
-// One wave. Free slots go to the jobs of ctl->prio in order, each taking what
-// it still needs (at most cap_per_job); each job with a share becomes a
-// segment, and each segment's rays are placed behind the lane's current queue
-// (queue qi).
+// One wave. The room left by the live paths goes to the jobs of ctl->prio in
+// order, each taking what it still needs (at most cap_per_job); each job with
+// a share becomes a segment, and each segment's rays are placed behind the
+// lane's current queue (queue qi).
 static __global__ __launch_bounds__(64) void k_regen_plan(RegenView V, uint32_t qi) {
     RegenCtl *C = V.ctl;
     const uint32_t lane = lane_id();
-    const unsigned long long head = C->fl_head, tail = C->fl_tail;
-    uint32_t room = (uint32_t)(tail - head);
+    uint32_t lane_count = lane < XT_REGEN_MAX_LANES ? V.cnt[lane].n[qi] : 0u;
+    uint32_t live;
+    wave_excl_sum(lane_count, live);
+    uint32_t room = live < C->P ? C->P - live : 0u;
     const uint32_t cap = C->cap_per_job ? C->cap_per_job : 0xffffffffu;
     uint32_t given = 0, nseg = 0;
-    uint32_t lane_count = lane < XT_REGEN_MAX_LANES ? V.cnt[lane].n[qi] : 0u;
     for (uint32_t c = 0; c < C->n_prio && room > 0; c += 64) {
         const uint32_t k = c + lane;
         uint32_t job = XT_NONE, want = 0;
@@ -123,12 +143,15 @@ static __global__ __launch_bounds__(64) void k_regen_plan(RegenView V, uint32_t 
         }
         if (give) {
             RegenSeg &g = V.segs[nseg + below];
+            const uint32_t T = V.jobs[job].jd.total;
             g.slot0 = nx;
             g.job = job;
             g.lane = jl;
             g.first = given + gpre;
             g.count = give;
             g.qpos = qpos;
+            g.q0 = (uint32_t)(nx / T);
+            g.s0 = (uint32_t)(nx - (unsigned long long)g.q0 * T);
             V.jobs[job].next = nx + give;
         }
         nseg += (uint32_t)__popcll(m);
@@ -137,44 +160,43 @@ static __global__ __launch_bounds__(64) void k_regen_plan(RegenView V, uint32_t 
     }
     if (lane < XT_REGEN_MAX_LANES) V.cnt[lane].n[qi] = lane_count;
     if (lane == 0) {
-        C->gen_head = head;
-        C->fl_head = head + given;
         C->n_seg = nseg;
         C->n_new = given;
     }
 }
 
 // New camera rays: ray k of the round belongs to the segment whose range holds
-// k; slot index s of a job is sample s / NT at tile position s % NT, like a
-// one-job pass, and takes path slot ring[(gen_head + k) mod P].
+// k (the segments' first entries are searched in LDS).
 static __global__ __launch_bounds__(XT_BLOCK) void k_regen_gen(RegenView V, uint32_t qi) {
+    __shared__ uint32_t sfirst[XT_REGEN_MAX_JOBS];
     const RegenCtl *C = V.ctl;
-    const uint32_t n = C->n_new, nseg = C->n_seg, P = C->P;
-    const unsigned long long head = C->gen_head;
+    const uint32_t n = C->n_new, nseg = C->n_seg;
+    if ((uint32_t)blockIdx.x * blockDim.x >= n) return;
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) sfirst[k] = V.segs[k].first;
+    __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         uint32_t lo = 0, hi = nseg;  // last segment with first <= k
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (V.segs[mid].first <= k) lo = mid;
+            if (sfirst[mid] <= k) lo = mid;
             else hi = mid;
         }
         const RegenSeg g = V.segs[lo];
-        const uint32_t off = k - g.first;
         const JobDev &j = V.jobs[g.job].jd;
-        const unsigned long long sidx = g.slot0 + off;
-        const uint32_t s = (uint32_t)(sidx / j.NT), q = (uint32_t)(sidx - (unsigned long long)s * j.NT);
-        const uint32_t path = V.ring[(uint32_t)((head + k) % P)];
+        const uint32_t u = g.s0 + (k - g.first);
+        const uint32_t q = g.q0 + u / j.total, s = u % j.total;
         uint32_t x = 0, y = 0;
-        const bool live = slot_pixel(j, q, x, y) && s < j.total;
+        const bool live = slot_pixel(j, q, x, y);
         const Queue out = V.lq[qi * XT_REGEN_MAX_LANES + g.lane];
-        const uint32_t at = g.qpos + off;
-        V.rad[path] = make_float4(0.f, 0.f, 0.f, 0.f);
-        V.pinfo[path] = make_uint2(g.job, live ? y * j.W + x : XT_NONE);
-        out.path[at] = path;
+        const RegenState st = V.ls[qi * XT_REGEN_MAX_LANES + g.lane];
+        const uint32_t at = g.qpos + (k - g.first);
+        st.info[at] = make_uint4(g.job, live ? y * j.W + x : XT_NONE, s, 0u);
+        st.s1[at] = make_float4(1.f, 1.f, 1.f, __uint_as_float(1u | (j.rdepth << 16)));
         if (!live) {
             out.r0[at] = make_float4(0.f, 0.f, 0.f, __uint_as_float(XT_DEAD));
             out.r1[at] = make_float4(0.f, 0.f, 1.f, 0.f);
+            st.s0[at] = make_float4(1.f, 1.f, 1.f, 0.f);
             continue;
         }
         const uint32_t key = xt_path_key(j.seed, j.job_key, y * j.W + x, s);
@@ -182,8 +204,7 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_regen_gen(RegenView V, uint
         aa_coords(j, x, y, s / j.spp, cx, cy);
         double o[3], d[3];
         camera_ray(j, cx, cy, key, o, d);
-        V.ps0[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
-        V.ps1[path] = make_float4(1.f, 1.f, 1.f, __uint_as_float(1u | (j.rdepth << 16)));
+        st.s0[at] = make_float4(1.f, 1.f, 1.f, __uint_as_float(key));
         out.r0[at] = make_float4((float)o[0], (float)o[1], (float)o[2], __uint_as_float(XT_NONE));
         out.r1[at] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
     }
@@ -193,27 +214,82 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_regen_gen(RegenView V, uint
 // is replayed from a graph).
 static __global__ void k_regen_round_start(RegenCtl *C) { C->retire_n = 0; }
 
-// AO for lanes whose queue mixes first and second queries (the query index is
-// in the path state): src/xtcore/integrator/ao/integrator.cc:12-53 as in
-// k_shade_ao1 / k_shade_ao2. Ended paths go to the retire list.
-static __global__ __launch_bounds__(XT_BLOCK) void k_regen_shade_ao(const SceneView S, const Queue in,
+// Appends the block's continuing paths (ray and state) to `out` and its ended
+// paths to the retire list; every thread of the block calls it.
+__device__ __forceinline__ void regen_emit(const RegenView &V, uint32_t *lds, bool have, bool cont, Queue &out,
+                                           const RegenState &ost, uint32_t *out_count, f3 no, f3 nd, uint32_t nprim,
+                                           float nself, float4 s0, float4 s1, uint4 info, f3 L) {
+    const uint32_t slot = block_append<XT_BLOCK>(cont, out_count, lds);
+    if (cont) {
+        out.r0[slot] = make_float4(no.x, no.y, no.z, __uint_as_float(nprim));
+        out.r1[slot] = make_float4(nd.x, nd.y, nd.z, nself);
+        ost.s0[slot] = s0;
+        ost.s1[slot] = s1;
+        ost.info[slot] = info;
+    }
+    const bool ended = have && !cont;
+    const uint32_t rs = block_append<XT_BLOCK>(ended, &V.ctl->retire_n, lds);
+    if (ended) {
+        V.ret[rs] = make_float4(L.x, L.y, L.z, __uint_as_float(info.y));
+        V.ret_js[rs] = make_uint2(info.x, info.z);
+    }
+}
+
+// Pathtracer shading of a lane (pt_step with the queue-resident state).
+template <bool SAMPLED>
+static __global__ __launch_bounds__(XT_BLOCK) void k_regen_shade_pt(const SceneView S, const RegenView V,
+                                                             const Queue in, const RegenState ist,
                                                              const uint32_t *in_count, const float4 *hit, Queue out,
-                                                             uint32_t *out_count, const float4 *ps0, float4 *ps1,
-                                                             float4 *rad, uint32_t *retire, uint32_t *retire_n) {
+                                                             const RegenState ost, uint32_t *out_count,
+                                                             const float4 *chan) {
     __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
     const uint32_t n = *in_count;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false, ended = false;
-        f3 no = mk(0, 0, 0), nd = mk(0, 0, 1);
-        uint32_t nprim = XT_NONE, path = 0;
+        const bool have = i < n;
+        bool cont = false;
+        f3 no = mk(0, 0, 0), nd = mk(0, 0, 1), L = mk(0, 0, 0);
+        uint32_t nprim = XT_NONE;
         float nself = 0.f;
-        if (i < n) {
-            path = in.path[i];
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        uint4 info = make_uint4(XT_NONE, XT_NONE, 0u, 0u);
+        if (have) {
+            s0 = ist.s0[i];
+            s1 = ist.s1[i];
+            info = ist.info[i];
+            pt_step<SAMPLED>(S, hit[i], in.r0[i], in.r1[i], s0, s1, SAMPLED ? chan + 3 * (size_t)i : nullptr, L, cont,
+                             no, nd, nprim, nself);
+        }
+        regen_emit(V, lds, have, cont, out, ost, out_count, no, nd, nprim, nself, s0, s1, info, L);
+    }
+}
+
+// AO for lanes whose queue mixes first and second queries (the query index is
+// in the path state): src/xtcore/integrator/ao/integrator.cc:12-53 as in
+// k_shade_ao1 / k_shade_ao2.
+static __global__ __launch_bounds__(XT_BLOCK) void k_regen_shade_ao(const SceneView S, const RegenView V,
+                                                             const Queue in, const RegenState ist,
+                                                             const uint32_t *in_count, const float4 *hit, Queue out,
+                                                             const RegenState ost, uint32_t *out_count) {
+    __shared__ uint32_t lds[XT_BLOCK / 64 + 1];
+    const uint32_t n = *in_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool have = i < n;
+        bool cont = false;
+        f3 no = mk(0, 0, 0), nd = mk(0, 0, 1), L = mk(0, 0, 0);
+        uint32_t nprim = XT_NONE;
+        float nself = 0.f;
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        uint4 info = make_uint4(XT_NONE, XT_NONE, 0u, 0u);
+        if (have) {
+            s0 = ist.s0[i];
+            s1 = ist.s1[i];
+            info = ist.info[i];
             const float4 h = hit[i];
             const uint32_t prim = __float_as_uint(h.y);
-            const float4 s1 = ps1[path];
             const uint32_t qword = __float_as_uint(s1.w);
             const float4 b = in.r1[i];
             if ((qword & 0xffffu) >= 2u) {  // the AO ray
@@ -222,20 +298,18 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_regen_shade_ao(const SceneV
                     const float dist = h.x * sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
                     w = fminf(dist, 100.0f) / 100.0f;
                 }
-                rad[path] = make_float4(w, w, w, 0.f);
-                ended = true;
+                L = mk(w, w, w);
             } else if (prim == XT_NONE) {
-                rad[path] = make_float4(1.f, 1.f, 1.f, 0.f);
-                ended = true;
+                L = mk(1.f, 1.f, 1.f);
             } else {
                 const float4 a = in.r0[i];
                 const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
                 f3 p, N, g;
                 uint32_t obj;
                 surface(S, prim, o, d, h.x, h.z, h.w, p, N, g, obj);
-                Draws r{xt_stream(__float_as_uint(ps0[path].w), 1), 0};
+                Draws r{xt_stream(__float_as_uint(s0.w), 1), 0};
                 const f3 dir = smp_hemisphere(N, N, r);
-                ps1[path] = make_float4(s1.x, s1.y, s1.z, __uint_as_float(qword + 1));
+                s1.w = __uint_as_float(qword + 1);
                 cont = true;
                 no = p;
                 nd = dir;
@@ -243,24 +317,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_regen_shade_ao(const SceneV
                 nself = self_hit_t(g, N, dir);
             }
         }
-        const uint32_t slot = block_append<XT_BLOCK>(cont, out_count, lds);
-        if (cont) push_ray(out, slot, no, nd, nprim, nself, path);
-        const uint32_t rs = block_append<XT_BLOCK>(ended, retire_n, lds);
-        if (ended) retire[rs] = path;
+        regen_emit(V, lds, have, cont, out, ost, out_count, no, nd, nprim, nself, s0, s1, info, L);
     }
-}
-
-// 64-bit wave-aggregated append on a monotonic counter.
-__device__ __forceinline__ unsigned long long wave_append64(bool want, unsigned long long *counter) {
-    const unsigned long long m = __ballot(want);
-    const uint32_t n = (uint32_t)__popcll(m);
-    const uint32_t lane = lane_id();
-    const uint32_t leader = m ? (uint32_t)__ffsll((long long)m) - 1 : 0;
-    unsigned long long base = 0;
-    if (lane == leader && n) base = atomicAdd(counter, (unsigned long long)n);
-    base = __shfl(base, (int)leader);
-    const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
-    return base + (unsigned long long)__popcll(below);
 }
 
 __device__ __forceinline__ long long to_fixed(float v) {
@@ -268,37 +326,112 @@ __device__ __forceinline__ long long to_fixed(float v) {
     return (long long)__double2ll_rn(x * XT_REGEN_FIX);
 }
 
-// Ended paths: radiance into the job's pixel, done count, slot back to the
-// free ring. Lanes of a wave that share the first lane's job count with one
-// atomic.
+// Ended paths: radiance into the job's sample buffer (or its pixel's
+// fixed-point sums) and the job's done count. For fixed-point jobs,
+// consecutive entries of a wave for the same (job, pixel) (the samples of a
+// pixel sit next to each other in the queues) are summed first, so each run
+// costs one set of atomics. Lanes of a wave that share the first lane's job
+// count with one atomic.
 static __global__ __launch_bounds__(XT_BLOCK) void k_regen_retire(RegenView V) {
-    RegenCtl *C = V.ctl;
-    const uint32_t n = C->retire_n, P = C->P;
+    const uint32_t n = V.ctl->retire_n;
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = lane_id();
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool have = i < n;
-        uint32_t path = 0, job = XT_NONE;
+        uint32_t job = XT_NONE, pixel = XT_NONE;
+        bool fixed = false;
+        long long f[3] = {0, 0, 0};
         if (have) {
-            path = V.retire[i];
-            const uint2 pi = V.pinfo[path];
-            job = pi.x;
-            if (pi.y != XT_NONE) {
-                const float4 L = V.rad[path];
-                long long *a = V.jobs[job].acc + 3 * (size_t)pi.y;
-                atomicAdd((unsigned long long *)a, (unsigned long long)to_fixed(L.x));
-                atomicAdd((unsigned long long *)(a + 1), (unsigned long long)to_fixed(L.y));
-                atomicAdd((unsigned long long *)(a + 2), (unsigned long long)to_fixed(L.z));
+            const float4 r = V.ret[i];
+            const uint2 js = V.ret_js[i];
+            job = js.x;
+            pixel = __float_as_uint(r.w);
+            if (pixel != XT_NONE) {
+                const RegenJob &J = V.jobs[job];
+                float4 *sb = J.sbuf;
+                if (sb) {
+                    sb[(size_t)pixel * J.jd.total + js.y] = make_float4(r.x, r.y, r.z, 0.f);
+                } else {
+                    fixed = true;
+                    f[0] = to_fixed(r.x);
+                    f[1] = to_fixed(r.y);
+                    f[2] = to_fixed(r.z);
+                }
             }
         }
-        const unsigned long long act = __ballot(have);
-        const uint32_t first = act ? (uint32_t)__ffsll((long long)act) - 1 : 0;
-        const uint32_t j0 = __shfl(job, (int)first);
-        const unsigned long long same = __ballot(have && job == j0);
-        if (lane_id() == first && same) atomicAdd(&V.done[j0], (unsigned long long)__popcll(same));
-        if (have && job != j0) atomicAdd(&V.done[job], 1ull);
-        const unsigned long long at = wave_append64(have, &C->fl_tail);
-        if (have) V.ring[(uint32_t)(at % P)] = path;
+        // segmented inclusive sum over runs of equal (job, pixel)
+        const unsigned long long key = fixed ? ((unsigned long long)job << 32 | pixel) : ~0ull - lane;
+        const unsigned long long prev = __shfl_up(key, 1), next = __shfl_down(key, 1);
+        bool head = lane == 0 || prev != key;
+        const bool tail = lane == 63 || next != key;
+        if (__ballot(fixed)) {
+            for (int o = 1; o < 64; o <<= 1) {
+                long long y[3];
+                for (int c = 0; c < 3; ++c) y[c] = __shfl_up(f[c], o);
+                const bool hy = __shfl_up(head, o);
+                if (lane >= (uint32_t)o && !head) {
+                    for (int c = 0; c < 3; ++c) f[c] += y[c];
+                    head = hy;
+                }
+            }
+        }
+        if (fixed && tail) {
+            long long *a = V.jobs[job].acc + 3 * (size_t)pixel;
+            for (int c = 0; c < 3; ++c)
+                if (f[c]) atomicAdd((unsigned long long *)(a + c), (unsigned long long)f[c]);
+        }
+        // done counts: one atomic per run of equal jobs in the wave, on one of
+        // XT_REGEN_SHARDS counters of the job so that the waves of the GPU do
+        // not queue on one address
+        const uint32_t jkey = have ? job : XT_NONE;
+        const uint32_t jprev = __shfl_up(jkey, 1), jnext = __shfl_down(jkey, 1);
+        const bool jhead = lane == 0 || jprev != jkey, jtail = lane == 63 || jnext != jkey;
+        const unsigned long long heads = __ballot(jhead);
+        // run length = lane - (last head at or below this lane) + 1
+        const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+        const uint32_t start = 63 - (uint32_t)__clzll((long long)upto);
+        if (have && jtail) {
+            const uint32_t shard = (blockIdx.x * (XT_BLOCK / 64) + (threadIdx.x >> 6)) % XT_REGEN_SHARDS;
+            atomicAdd(&V.done_shard[shard * XT_REGEN_MAX_JOBS + job], (unsigned long long)(lane - start + 1));
+        }
+    }
+}
+
+// done[e] = the sum of entry e's shards.
+static __global__ void k_regen_done_sum(RegenView V) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= XT_REGEN_MAX_JOBS) return;
+    unsigned long long t = 0;
+    for (uint32_t sh = 0; sh < XT_REGEN_SHARDS; ++sh) t += V.done_shard[sh * XT_REGEN_MAX_JOBS + e];
+    V.done[e] = t;
+}
+
+// Zeroes the done shards of job-table entry e (a new job).
+static __global__ void k_regen_job_reset(RegenView V, uint32_t e) {
+    const uint32_t sh = threadIdx.x;
+    if (sh < XT_REGEN_SHARDS) V.done_shard[sh * XT_REGEN_MAX_JOBS + e] = 0;
+    if (sh == 0) V.done[e] = 0;
+}
+
+// Resolve of a finished job from its sample buffer: per pixel, the samples
+// in order with k_accumulate's compensated FP64 sum and k_resolve's rounding,
+// so the image is the one-job render's bit for bit.
+static __global__ __launch_bounds__(XT_BLOCK) void k_regen_resolve_ordered(const JobDev j, const float4 *sbuf,
+                                                                    float4 *out) {
+    const uint32_t n = j.W * j.H;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        double2 acc[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+        const float4 *sp = sbuf + (size_t)p * j.total;
+        for (uint32_t s = 0; s < j.total; ++s) {
+            const float4 v = sp[s];
+            neumaier_add(acc[0], weighted(v.x, j.weight));
+            neumaier_add(acc[1], weighted(v.y, j.weight));
+            neumaier_add(acc[2], weighted(v.z, j.weight));
+        }
+        out[p] = make_float4(fmaxf((float)(acc[0].x + acc[0].y), 0.f), fmaxf((float)(acc[1].x + acc[1].y), 0.f),
+                             fmaxf((float)(acc[2].x + acc[2].y), 0.f), 1.f);
     }
 }
 
@@ -314,15 +447,8 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_regen_resolve(const JobDev 
     }
 }
 
-// Free ring with every slot free: ring[k] = k, head 0, tail P.
-static __global__ void k_regen_init(RegenCtl *C, uint32_t *ring, uint32_t P) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += stride) ring[k] = k;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        C->fl_head = 0;
-        C->fl_tail = P;
-        C->P = P;
-        C->retire_n = 0;
-        C->n_seg = C->n_new = 0;
-    }
+static __global__ void k_regen_init(RegenCtl *C, uint32_t P) {
+    C->P = P;
+    C->retire_n = 0;
+    C->n_seg = C->n_new = 0;
 }

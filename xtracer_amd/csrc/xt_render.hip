@@ -180,11 +180,10 @@ int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t
                 hipLaunchKernelGGL(k_channels, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, chan);
                 XT_LAUNCHED();
                 hipLaunchKernelGGL(k_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out,
-                                   n_out, w.ps0, w.ps1, w.rad, chan, (uint32_t *)nullptr, (uint32_t *)nullptr);
+                                   n_out, w.ps0, w.ps1, w.rad, chan);
             } else {
                 hipLaunchKernelGGL(k_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, n_in, hit, out,
-                                   n_out, w.ps0, w.ps1, w.rad, (const float4 *)nullptr, (uint32_t *)nullptr,
-                                   (uint32_t *)nullptr);
+                                   n_out, w.ps0, w.ps1, w.rad, (const float4 *)nullptr);
             }
     }
     XT_LAUNCHED();

@@ -64,10 +64,19 @@ constexpr uint64_t kJobBytesPerPixel = sizeof(float4) + 3 * sizeof(double2) + si
 
 uint64_t job_bytes(size_t npix) { return ((uint64_t)npix * kJobBytesPerPixel + 255) & ~255ull; }
 
-// regeneration engine: 3 fixed-point sums and the resolved image per pixel
-constexpr uint64_t kRegenJobBytesPerPixel = 3 * sizeof(long long) + sizeof(float4);
+// regeneration engine: per pixel, the resolved image and either a radiance
+// record per sample or 3 fixed-point sums (jobs whose sample buffer would
+// exceed the configured size)
+uint64_t regen_sample_bytes(size_t npix, const JobDev &jd) { return (uint64_t)npix * jd.total * sizeof(float4); }
 
-uint64_t regen_job_bytes(size_t npix) { return ((uint64_t)npix * kRegenJobBytesPerPixel + 255) & ~255ull; }
+bool regen_ordered(const xt_serve_config &cfg, size_t npix, const JobDev &jd) {
+    return regen_sample_bytes(npix, jd) <= ((uint64_t)cfg.sample_buffer_mb << 20);
+}
+
+uint64_t regen_job_bytes(const xt_serve_config &cfg, size_t npix, const JobDev &jd) {
+    const uint64_t per = regen_ordered(cfg, npix, jd) ? regen_sample_bytes(npix, jd) : (uint64_t)npix * 3 * sizeof(long long);
+    return (per + (uint64_t)npix * sizeof(float4) + 511) & ~255ull;
+}
 
 struct DevBlock {
     void *p = nullptr;
@@ -89,6 +98,7 @@ struct SJob {
     int entry = -1;        // its device job-table entry once started there
     uint64_t paths_done = 0;
     long long *acc = nullptr;
+    float4 *sbuf = nullptr;  // regeneration engine: radiance per (pixel, sample), or fixed-point sums in acc
     uint32_t issued = 0;  // whole samples handed to passes
     uint32_t qc = 0;      // tile positions of sample `issued` handed out so far
     uint32_t passes = 0;
@@ -122,14 +132,16 @@ struct RegenLaneH {
     uint4 *recheck = nullptr;
 };
 
-struct RegenState {
+struct RegenHost {
     bool ready = false;
     uint32_t P = 0;
     void *blk = nullptr;
     uint64_t bytes = 0;
     RegenView V{};
-    Queue *lq = nullptr;  // device copy of lq_host
+    Queue *lq = nullptr;        // device copy of lq_host
+    RegenState *ls = nullptr;   // device copy of ls_host
     Queue lq_host[2 * XT_REGEN_MAX_LANES] = {};
+    RegenState ls_host[2 * XT_REGEN_MAX_LANES] = {};
     RegenLaneH lanes[XT_REGEN_MAX_LANES];
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -157,8 +169,8 @@ struct ServeState {
     double epoch_ms = 0;
     xt_serve_info info{};
     int policy = XT_SERVE_SRPT;
-    xt_serve_config cfg{2, 2, 4, 0};
-    RegenState R;
+    xt_serve_config cfg{2, 2, 4, 0, 4096};
+    RegenHost R;
 };
 
 namespace {
@@ -241,19 +253,26 @@ int admit(xt_dev *d, ServeState &S) {
         SJob &J = S.jobs[S.waiting.front()];
         bool ok = false;
         J.regen = regen_eligible(S, J.jd);
-        int rc = take_dev_block(d, S, J.regen ? regen_job_bytes(J.npix) : job_bytes(J.npix), J.blk, ok);
+        int rc = take_dev_block(d, S, J.regen ? regen_job_bytes(S.cfg, J.npix, J.jd) : job_bytes(J.npix), J.blk, ok);
         if (rc) return rc;
         if (!ok) break;
         char *base = (char *)J.blk.p;
         if (J.regen) {
-            // 3 fixed-point sums, then the resolved image
-            J.acc = (long long *)base;
-            J.out = (float4 *)(base + J.npix * 3 * sizeof(long long));
+            // the resolved image, then the sample buffer or 3 fixed-point sums
+            J.out = (float4 *)base;
+            char *rest = (char *)(((uintptr_t)(base + J.npix * sizeof(float4)) + 255) & ~(uintptr_t)255);
             J.accf = nullptr;
             J.acc64 = nullptr;
             J.entry = -1;
-            hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J.npix * 6, 256, 8192)), dim3(256), 0, d->stream,
-                               (uint32_t *)J.blk.p, 0u, J.npix * 6);
+            if (regen_ordered(S.cfg, J.npix, J.jd)) {
+                J.sbuf = (float4 *)rest;
+                J.acc = nullptr;
+            } else {
+                J.sbuf = nullptr;
+                J.acc = (long long *)rest;
+                hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J.npix * 6, 256, 8192)), dim3(256), 0, d->stream,
+                                   (uint32_t *)J.acc, 0u, J.npix * 6);
+            }
         } else {
             J.accf = (float4 *)base;
             J.acc64 = (double2 *)(base + J.npix * sizeof(float4));
@@ -520,16 +539,17 @@ void regen_free(xt_dev *d, void *&p, uint64_t &bytes) {
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Shared buffers of the engine: path state, free ring, retire list, job
-// table, control block, segments, lane counters and lane queue table.
+// Shared buffers of the engine: retire list, job table, done counts,
+// control block, segments, lane counters and the lane queue tables.
 int regen_init(xt_dev *d, ServeState &S) {
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     if (R.ready) return XT_OK;
     const uint32_t P = S.cfg.paths ? S.cfg.paths : d->tuning.wave_paths;
-    const size_t sz[] = {16ull * P, 16ull * P, 16ull * P, 8ull * P, 4ull * P, 4ull * P,
-                         sizeof(RegenJob) * XT_REGEN_MAX_JOBS, 8ull * XT_REGEN_MAX_JOBS, sizeof(RegenCtl),
-                         sizeof(RegenSeg) * XT_REGEN_MAX_JOBS, sizeof(Counters) * XT_REGEN_MAX_LANES,
-                         sizeof(Queue) * 2 * XT_REGEN_MAX_LANES};
+    const size_t sz[] = {16ull * P, 8ull * P, sizeof(RegenJob) * XT_REGEN_MAX_JOBS, 8ull * XT_REGEN_MAX_JOBS,
+                         sizeof(RegenCtl), sizeof(RegenSeg) * XT_REGEN_MAX_JOBS,
+                         sizeof(Counters) * XT_REGEN_MAX_LANES, sizeof(Queue) * 2 * XT_REGEN_MAX_LANES,
+                         sizeof(RegenState) * 2 * XT_REGEN_MAX_LANES,
+                         8ull * XT_REGEN_MAX_JOBS * XT_REGEN_SHARDS};
     size_t off[sizeof(sz) / sizeof(sz[0])], total = 0;
     for (size_t k = 0; k < sizeof(sz) / sizeof(sz[0]); ++k) {
         off[k] = total;
@@ -540,32 +560,35 @@ int regen_init(xt_dev *d, ServeState &S) {
     R.bytes = total;
     char *b = (char *)R.blk;
     RegenView &V = R.V;
-    V.ps0 = (float4 *)(b + off[0]);
-    V.ps1 = (float4 *)(b + off[1]);
-    V.rad = (float4 *)(b + off[2]);
-    V.pinfo = (uint2 *)(b + off[3]);
-    V.ring = (uint32_t *)(b + off[4]);
-    V.retire = (uint32_t *)(b + off[5]);
-    V.jobs = (RegenJob *)(b + off[6]);
-    V.done = (unsigned long long *)(b + off[7]);
-    V.ctl = (RegenCtl *)(b + off[8]);
-    V.segs = (RegenSeg *)(b + off[9]);
-    V.cnt = (Counters *)(b + off[10]);
-    R.lq = (Queue *)(b + off[11]);
+    V.ret = (float4 *)(b + off[0]);
+    V.ret_js = (uint2 *)(b + off[1]);
+    V.jobs = (RegenJob *)(b + off[2]);
+    V.done = (unsigned long long *)(b + off[3]);
+    V.ctl = (RegenCtl *)(b + off[4]);
+    V.segs = (RegenSeg *)(b + off[5]);
+    V.cnt = (Counters *)(b + off[6]);
+    R.lq = (Queue *)(b + off[7]);
+    R.ls = (RegenState *)(b + off[8]);
+    V.done_shard = (unsigned long long *)(b + off[9]);
     V.lq = R.lq;
+    V.ls = R.ls;
     XT_TRY(hipHostMalloc((void **)&R.h_job, sizeof(RegenJob) * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_done, 8ull * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_prio, 4ull * (XT_REGEN_MAX_JOBS + 2), hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_cnt, sizeof(Counters) * XT_REGEN_MAX_LANES, hipHostMallocDefault));
     std::memset(R.lq_host, 0, sizeof R.lq_host);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(align256(sz[8]) / 4, 256, 64)), dim3(256), 0, d->stream,
-                       (uint32_t *)V.ctl, 0u, align256(sz[8]) / 4);
+    std::memset(R.ls_host, 0, sizeof R.ls_host);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(align256(sz[4]) / 4, 256, 64)), dim3(256), 0, d->stream,
+                       (uint32_t *)V.ctl, 0u, align256(sz[4]) / 4);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2ull * XT_REGEN_MAX_JOBS, 256, 64)), dim3(256), 0, d->stream,
                        (uint32_t *)V.done, 0u, 2ull * XT_REGEN_MAX_JOBS);
-    hipLaunchKernelGGL(k_regen_init, dim3(grid_for(P, 256, 4096)), dim3(256), 0, d->stream, V.ctl, V.ring, P);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2ull * XT_REGEN_MAX_JOBS * XT_REGEN_SHARDS, 256, 512)), dim3(256), 0,
+                       d->stream, (uint32_t *)V.done_shard, 0u, 2ull * XT_REGEN_MAX_JOBS * XT_REGEN_SHARDS);
+    hipLaunchKernelGGL(k_regen_init, dim3(1), dim3(1), 0, d->stream, V.ctl, P);
     hipLaunchKernelGGL(k_reset_lanes, dim3(XT_REGEN_MAX_LANES), dim3(64), 0, d->stream, V.cnt, 0x800000ffu);
     XT_LAUNCHED();
     XT_TRY(hipMemcpyAsync(R.lq, R.lq_host, sizeof R.lq_host, hipMemcpyHostToDevice, d->stream));
+    XT_TRY(hipMemcpyAsync(R.ls, R.ls_host, sizeof R.ls_host, hipMemcpyHostToDevice, d->stream));
     XT_TRY(hipStreamSynchronize(d->stream));
     R.P = P;
     R.free_entries.clear();
@@ -577,7 +600,7 @@ int regen_init(xt_dev *d, ServeState &S) {
     return XT_OK;
 }
 
-void regen_drop_graph(RegenState &R) {
+void regen_drop_graph(RegenHost &R) {
     if (R.exec) (void)hipGraphExecDestroy(R.exec);
     if (R.graph) (void)hipGraphDestroy(R.graph);
     R.exec = nullptr;
@@ -586,7 +609,7 @@ void regen_drop_graph(RegenState &R) {
 }
 
 void regen_shutdown(xt_dev *d, ServeState &S) {
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     regen_drop_graph(R);
     for (RegenLaneH &L : R.lanes) regen_free(d, L.blk, L.bytes);
     regen_free(d, R.blk, R.bytes);
@@ -594,13 +617,13 @@ void regen_shutdown(xt_dev *d, ServeState &S) {
     if (R.h_done) (void)hipHostFree(R.h_done);
     if (R.h_prio) (void)hipHostFree(R.h_prio);
     if (R.h_cnt) (void)hipHostFree(R.h_cnt);
-    R = RegenState();
+    R = RegenHost();
 }
 
 // The lane of (scene, integrator, accel), created on first use (its queues,
 // hit and re-test buffers sized for every path slot). -1: all lanes busy.
 int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     lane = -1;
     int idle = -1;
     for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
@@ -616,18 +639,23 @@ int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
     const DevScene &ds = d->scenes.at(J.job.scene_id);
     const bool chan = xt_needs_chan(J.jd.integrator, ds.view);
     const uint64_t P = R.P;
-    const size_t qsz = align256(16 * P), psz = align256(4 * P);
-    const size_t total = 4 * qsz + 2 * psz + qsz /*hit*/ + psz /*retest*/ + qsz /*recheck*/ + (chan ? 3 * qsz : 0);
+    const size_t qsz = align256(16 * P), psz = align256(4 * P), isz = qsz;
+    // per parity: r0, r1, path, s0, s1, info; then hit, retest, recheck, channels
+    const size_t total = 2 * (4 * qsz + psz + isz) + qsz + psz + qsz + (chan ? 3 * qsz : 0);
     int rc = regen_malloc(d, &L.blk, total);
     if (rc) return rc;
     L.bytes = total;
     char *b = (char *)L.blk;
     for (int qi = 0; qi < 2; ++qi) {
         Queue &q = R.lq_host[qi * XT_REGEN_MAX_LANES + idle];
+        RegenState &st = R.ls_host[qi * XT_REGEN_MAX_LANES + idle];
         q.r0 = (float4 *)b;
         q.r1 = (float4 *)(b + qsz);
-        q.path = (uint32_t *)(b + 2 * qsz);
-        b += 2 * qsz + psz;
+        st.s0 = (float4 *)(b + 2 * qsz);
+        st.s1 = (float4 *)(b + 3 * qsz);
+        q.path = (uint32_t *)(b + 4 * qsz);
+        st.info = (uint4 *)(b + 4 * qsz + psz);
+        b += 4 * qsz + psz + isz;
     }
     L.hit = (float4 *)b;
     b += qsz;
@@ -643,6 +671,7 @@ int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
     hipLaunchKernelGGL(k_reset_lanes, dim3(1), dim3(64), 0, d->stream, R.V.cnt + idle, 0x800000ffu);
     XT_LAUNCHED();
     XT_TRY(hipMemcpyAsync(R.lq, R.lq_host, sizeof R.lq_host, hipMemcpyHostToDevice, d->stream));
+    XT_TRY(hipMemcpyAsync(R.ls, R.ls_host, sizeof R.ls_host, hipMemcpyHostToDevice, d->stream));
     R.queries_seen[idle] = R.retests_seen[idle] = 0;
     R.dirty = true;
     lane = idle;
@@ -652,7 +681,7 @@ int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
 // One query round over queue parity qi: plan, new camera rays, trace and
 // shade per lane, retire.
 int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     RegenView &V = R.V;
     const hipStream_t st = d->stream;
     const unsigned gP = grid_for(R.P, XT_BLOCK, 8192);
@@ -679,19 +708,18 @@ int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
         int rc = xt_launch_trace_bufs(d, ds, (int)L.accel, B, nullptr, nullptr, false, st);
         if (rc) return rc;
         const unsigned g = grid_for(R.P, XT_BLOCK, 8192);
+        const RegenState ist = R.ls_host[qi * XT_REGEN_MAX_LANES + l], ost = R.ls_host[(qi ^ 1) * XT_REGEN_MAX_LANES + l];
         if (L.integrator == XT_INTEGRATOR_AO) {
-            hipLaunchKernelGGL(k_regen_shade_ao, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, out,
-                               &cnt->n[qi ^ 1], V.ps0, V.ps1, V.rad, V.retire, &V.ctl->retire_n);
+            hipLaunchKernelGGL(k_regen_shade_ao, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, V, in, ist, &cnt->n[qi], L.hit,
+                               out, ost, &cnt->n[qi ^ 1]);
         } else if (L.chan) {
             hipLaunchKernelGGL(k_channels, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, L.chan);
             XT_LAUNCHED();
-            hipLaunchKernelGGL(k_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, out,
-                               &cnt->n[qi ^ 1], V.ps0, V.ps1, V.rad, (const float4 *)L.chan, V.retire,
-                               &V.ctl->retire_n);
+            hipLaunchKernelGGL(k_regen_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, V, in, ist, &cnt->n[qi],
+                               L.hit, out, ost, &cnt->n[qi ^ 1], (const float4 *)L.chan);
         } else {
-            hipLaunchKernelGGL(k_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit,
-                               out, &cnt->n[qi ^ 1], V.ps0, V.ps1, V.rad, (const float4 *)nullptr, V.retire,
-                               &V.ctl->retire_n);
+            hipLaunchKernelGGL(k_regen_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, V, in, ist,
+                               &cnt->n[qi], L.hit, out, ost, &cnt->n[qi ^ 1], (const float4 *)nullptr);
         }
         XT_LAUNCHED();
     }
@@ -703,7 +731,7 @@ int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
 // Captures rounds over parity 0 then 1 (queue parity is back at 0 after a
 // replay) as one hipGraph.
 int regen_capture(xt_dev *d, ServeState &S) {
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     regen_drop_graph(R);
     XT_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
     int rc = regen_round(d, S, 0);
@@ -761,7 +789,7 @@ void regen_priorities(ServeState &S, std::vector<uint32_t> &prio, uint32_t &cap)
 int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
     int rc = regen_init(d, S);
     if (rc) return rc;
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     RegenView &V = R.V;
     // start admitted jobs that have no table entry yet
     for (uint64_t t : S.active) {
@@ -779,10 +807,12 @@ int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
         h.total = (unsigned long long)J.jd.total * J.jd.NT;
         h.next = 0;
         h.acc = J.acc;
+        h.sbuf = J.sbuf;
         h.lane = (uint32_t)lane;
         R.h_done[J.entry] = 0;
         XT_TRY(hipMemcpyAsync(V.jobs + J.entry, &h, sizeof h, hipMemcpyHostToDevice, d->stream));
-        XT_TRY(hipMemcpyAsync(V.done + J.entry, R.h_done + J.entry, 8, hipMemcpyHostToDevice, d->stream));
+        hipLaunchKernelGGL(k_regen_job_reset, dim3(1), dim3(64), 0, d->stream, V, (uint32_t)J.entry);
+        XT_LAUNCHED();
         J.paths_done = 0;
     }
     std::vector<uint32_t> prio;
@@ -809,6 +839,8 @@ int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
             if (rc) return rc;
         }
     }
+    hipLaunchKernelGGL(k_regen_done_sum, dim3(XT_REGEN_MAX_JOBS / 256), dim3(256), 0, d->stream, V);
+    XT_LAUNCHED();
     XT_TRY(hipEventRecord(S.p1, d->stream));
     XT_TRY(hipMemcpyAsync(R.h_done, V.done, 8ull * XT_REGEN_MAX_JOBS, hipMemcpyDeviceToHost, d->stream));
     XT_TRY(hipMemcpyAsync(R.h_cnt, V.cnt, sizeof(Counters) * XT_REGEN_MAX_LANES, hipMemcpyDeviceToHost, d->stream));
@@ -833,8 +865,12 @@ int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
         S.info.samples += (done - J.paths_done) * J.npix / J.jd.NT;
         J.paths_done = done;
         if (done < total) continue;
-        hipLaunchKernelGGL(k_regen_resolve, dim3(grid_for(J.npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, J.jd,
-                           (const long long *)J.acc, J.out);
+        if (J.sbuf)
+            hipLaunchKernelGGL(k_regen_resolve_ordered, dim3(grid_for(J.npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0,
+                               d->stream, J.jd, (const float4 *)J.sbuf, J.out);
+        else
+            hipLaunchKernelGGL(k_regen_resolve, dim3(grid_for(J.npix, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream,
+                               J.jd, (const long long *)J.acc, J.out);
         XT_LAUNCHED();
         rc = take_host_block(d, S, J.npix * sizeof(float4), J.host);
         if (rc) return rc;
@@ -854,7 +890,7 @@ int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
 
 // Lanes whose jobs have all finished give their buffers back.
 void regen_release_idle_lanes(xt_dev *d, ServeState &S) {
-    RegenState &R = S.R;
+    RegenHost &R = S.R;
     for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
         RegenLaneH &L = R.lanes[l];
         if (!L.used) continue;
@@ -877,6 +913,7 @@ void release_block(ServeState &S, SJob &J) {
     J.accf = J.out = nullptr;
     J.acc64 = nullptr;
     J.acc = nullptr;
+    J.sbuf = nullptr;
 }
 
 }  // namespace
@@ -931,7 +968,7 @@ extern "C" int xt_submit(xt_dev *d, const xt_job *job, uint64_t *ticket) {
     const size_t npix = (size_t)jd.W * jd.H;
     // memory that job buffers can never use: scenes and wavefront buffers
     const uint64_t fixed = d->used - std::min(d->used, S.held);
-    if (fixed + (regen_eligible(S, jd) ? regen_job_bytes(npix) : job_bytes(npix)) > d->budget) {
+    if (fixed + (regen_eligible(S, jd) ? regen_job_bytes(S.cfg, npix, jd) : job_bytes(npix)) > d->budget) {
         d->err = "job buffers exceed the HBM budget";
         return XT_E_BUDGET;
     }
@@ -1046,7 +1083,8 @@ extern "C" int xt_serve_set_policy(xt_dev *d, int policy) {
 extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     if (!d || !c) return XT_E_INVALID;
     if (c->regen > 2 || c->graph > 2 || (c->rounds_per_step & 1u) ||
-        c->rounds_per_step > 1024 || (c->paths && (c->paths < 4096 || c->paths > (1u << 28)))) {
+        c->rounds_per_step > 1024 || (c->paths && (c->paths < 4096 || c->paths > (1u << 28))) ||
+        c->sample_buffer_mb > (1u << 20)) {
         d->err = "bad serve configuration";
         return XT_E_INVALID;
     }
@@ -1066,6 +1104,7 @@ extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     if (c->graph) S.cfg.graph = c->graph;
     if (c->rounds_per_step) S.cfg.rounds_per_step = c->rounds_per_step;
     if (c->paths) S.cfg.paths = c->paths;
+    if (c->sample_buffer_mb) S.cfg.sample_buffer_mb = c->sample_buffer_mb;
     return XT_OK;
 }
 
