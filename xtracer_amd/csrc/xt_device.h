@@ -112,3 +112,4 @@ int xt_launch_trace_bufs(xt_dev *d, const DevScene &ds, int accel, const TraceBu
                          uint32_t *flags, bool time_it, hipStream_t st);
 void xt_serve_shutdown(xt_dev *d);
 bool xt_serve_uses_scene(const xt_dev *d, uint32_t scene_id);
+void xt_serve_forget_scene(xt_dev *d, uint32_t scene_id);

@@ -377,6 +377,7 @@ int xt_scene_release(xt_dev *d, uint32_t scene_id) {
     }
     hipSetDevice(d->device);
     hipStreamSynchronize(d->stream);
+    xt_serve_forget_scene(d, scene_id);  // idle serving lanes set up for it
     for (void *p : it->second.allocs) hipFree(p);
     d->used -= std::min<uint64_t>(d->used, it->second.bytes);
     d->scenes.erase(it);

@@ -620,6 +620,25 @@ void regen_shutdown(xt_dev *d, ServeState &S) {
     R = RegenHost();
 }
 
+// True when an unfinished started job runs on lane l.
+bool regen_lane_busy(const ServeState &S, const RegenLaneH &L) {
+    for (uint64_t t : S.active) {
+        const SJob &J = S.jobs.at(t);
+        if (J.regen && J.entry >= 0 && J.job.scene_id == L.scene && J.jd.integrator == L.integrator &&
+            J.job.accel == L.accel)
+            return true;
+    }
+    return false;
+}
+
+// Gives lane l's buffers back (its queues are empty: every path of its jobs
+// has ended).
+void regen_drop_lane(xt_dev *d, RegenHost &R, int l) {
+    regen_free(d, R.lanes[l].blk, R.lanes[l].bytes);
+    R.lanes[l] = RegenLaneH();
+    R.dirty = true;
+}
+
 // The lane of (scene, integrator, accel), created on first use (its queues,
 // hit and re-test buffers sized for every path slot). -1: all lanes busy.
 int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
@@ -634,8 +653,16 @@ int regen_lane(xt_dev *d, ServeState &S, const SJob &J, int &lane) {
         }
         if (!L.used && idle < 0) idle = l;
     }
-    if (idle < 0) return XT_OK;
+    if (idle < 0) {
+        // every lane is set up: take one whose jobs have all finished (lanes
+        // stay set up between jobs, so a scene's next job starts at once)
+        for (int l = 0; l < XT_REGEN_MAX_LANES && idle < 0; ++l)
+            if (!regen_lane_busy(S, R.lanes[l])) idle = l;
+        if (idle < 0) return XT_OK;
+        regen_drop_lane(d, R, idle);
+    }
     RegenLaneH &L = R.lanes[idle];
+
     const DevScene &ds = d->scenes.at(J.job.scene_id);
     const bool chan = xt_needs_chan(J.jd.integrator, ds.view);
     const uint64_t P = R.P;
@@ -888,24 +915,6 @@ int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
     return XT_OK;
 }
 
-// Lanes whose jobs have all finished give their buffers back.
-void regen_release_idle_lanes(xt_dev *d, ServeState &S) {
-    RegenHost &R = S.R;
-    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
-        RegenLaneH &L = R.lanes[l];
-        if (!L.used) continue;
-        bool busy = false;
-        for (uint64_t t : S.active) {
-            const SJob &J = S.jobs[t];
-            busy = busy || (J.regen && J.state == ACTIVE && J.job.scene_id == L.scene && J.jd.integrator == L.integrator &&
-                            J.job.accel == L.accel);
-        }
-        if (busy) continue;
-        regen_free(d, L.blk, L.bytes);
-        L = RegenLaneH();
-        R.dirty = true;
-    }
-}
 
 void release_block(ServeState &S, SJob &J) {
     if (J.blk.p) S.free_dev.push_back(J.blk);
@@ -917,6 +926,13 @@ void release_block(ServeState &S, SJob &J) {
 }
 
 }  // namespace
+
+void xt_serve_forget_scene(xt_dev *d, uint32_t scene_id) {
+    if (!d->serve) return;
+    RegenHost &R = d->serve->R;
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l)
+        if (R.lanes[l].used && R.lanes[l].scene == scene_id) regen_drop_lane(d, R, l);
+}
 
 bool xt_serve_uses_scene(const xt_dev *d, uint32_t scene_id) {
     if (!d->serve) return false;
@@ -1047,7 +1063,6 @@ extern "C" int xt_serve_step(xt_dev *d, int *active_jobs) {
             S.info.jobs_finished += 1;
         }
         if (!S.active.empty()) S.rr %= S.active.size();
-        regen_release_idle_lanes(d, S);
         // room freed by finished jobs admits waiting ones for the next pass
         rc = admit(d, S);
         if (rc) return rc;
