@@ -211,11 +211,11 @@ typedef struct xt_tuning {
     uint32_t wave_paths;      /* paths in flight per wavefront pass, 4096..2^28 (16M) */
     uint32_t trace_blocks_per_cu; /* persistent trace grid: blocks per CU, <= 64 (resident fill) */
     uint32_t trace_segments;  /* 1: one work cursor over a trace queue; 8: the queue cut into 8 contiguous
-                                 segments, one per XCD, each drained by its own XCD's waves first (8) */
+                                 segments, one per XCD, each drained by its own XCD's waves first (1) */
     uint32_t ray_order;       /* camera rays of a pass: 1 sample by sample over the image; 2 all samples of
-                                 an 8x8 tile together (2) */
+                                 an 8x8 tile together; 3 all samples of a pixel together (1) */
     uint32_t kd_layout;       /* kd-tree node order of scenes uploaded afterwards: 1 depth-first; 2 child
-                                 pairs grouped into 128-byte lines three levels deep (2) */
+                                 pairs grouped into 128-byte lines three levels deep (1) */
     uint32_t kd_leaf;         /* kd leaf primitives in the FP32 walk: 1 a copy of each referenced record in
                                  leaf order; 2 read from the soup through the reference list (1) */
 } xt_tuning;

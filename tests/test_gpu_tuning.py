@@ -24,7 +24,7 @@ def test_tuning_roundtrip_and_rejects(tuned):
     t = tuned.set_tuning(**DEFAULT)
     assert t == dict(DEFAULT, trace_blocks_per_cu=0)
     for bad in ({"kd_stack": 10}, {"trace_block": 64}, {"wave_paths": 100}, {"trace_blocks_per_cu": 65},
-                {"trace_segments": 4}, {"ray_order": 3}, {"kd_layout": 3}, {"kd_leaf": 3}):
+                {"trace_segments": 4}, {"ray_order": 4}, {"kd_layout": 3}, {"kd_leaf": 3}):
         with pytest.raises(xa.XtError):
             tuned.set_tuning(**bad)
     assert tuned.tuning() == t
@@ -66,16 +66,16 @@ def test_render_bit_exact_small_shapes(tuned, integrator):
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
 
 
-LAYOUTS = [{"trace_segments": 8}, {"ray_order": 2}, {"kd_layout": 2}, {"kd_leaf": 2},
+LAYOUTS = [{"trace_segments": 8}, {"ray_order": 2}, {"ray_order": 3}, {"kd_layout": 2}, {"kd_leaf": 2},
            {"trace_segments": 8, "ray_order": 2, "kd_layout": 2, "kd_leaf": 2}]
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("knobs", LAYOUTS)
 def test_layout_knobs_invariant(tuned, knobs):
-    """Per-XCD queue segments, tile-major camera rays, the 128-byte-line node
-    order and leaves read through the reference list give the default's hits
-    and the oracle's bit-exact depth image."""
+    """Per-XCD queue segments, tile-major and pixel-major camera rays, the
+    128-byte-line node order and leaves read through the reference list give
+    the default's hits and the oracle's bit-exact depth image."""
     s = xa.Scene(scene_path("icosphere", subdiv=5))
     tuned.set_tuning(**DEFAULT)
     sid = tuned.upload(s)

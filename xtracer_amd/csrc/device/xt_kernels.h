@@ -344,19 +344,23 @@ __device__ __forceinline__ void aa_coords(const JobDev &j, uint32_t px, uint32_t
 static __global__ __launch_bounds__(XT_BLOCK) void k_camera(const JobDev j, uint32_t pass_base, uint32_t spp_pass,
                                                      uint32_t q0, uint32_t qn, uint32_t path_base, Queue out,
                                                      float4 *ps0, float4 *ps1, float4 *rad, double *ray64,
-                                                     uint32_t tile_major) {
+                                                     uint32_t order) {
     const uint32_t n = spp_pass * qn;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t qword = 1u | (j.rdepth << 16);
-    // tile_major (with whole tiles): queue entry i holds sample sl of slot w
-    // of tile t, i = (t * spp_pass + sl) * 64 + w, so that a stretch of the
-    // queue covers one part of the image; otherwise the pass's samples follow
-    // each other over the whole image. The path id (and so the radiance slot
-    // and the accumulation order) is sl * qn + q - q0 either way.
-    const bool tiles = tile_major && (qn & 63u) == 0;
+    // order 2 (with whole tiles): queue entry i holds sample sl of slot w of
+    // tile t, i = (t * spp_pass + sl) * 64 + w, so that a stretch of the
+    // queue covers one part of the image; order 3: the pass's samples of one
+    // pixel follow each other, i = qi * spp_pass + sl; otherwise the pass's
+    // samples follow each other over the whole image. The path id (and so the
+    // radiance slot and the accumulation order) is sl * qn + q - q0 either way.
+    const bool tiles = order == 2 && (qn & 63u) == 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint32_t sl, qi;
-        if (tiles) {
+        if (order == 3) {
+            qi = i / spp_pass;
+            sl = i - qi * spp_pass;
+        } else if (tiles) {
             const uint32_t t = i / (64u * spp_pass), r = i - t * 64u * spp_pass;
             sl = r >> 6;
             qi = t * 64u + (r & 63u);

@@ -220,8 +220,8 @@ int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t) {
         n.trace_segments = t->trace_segments;
     }
     if (t->ray_order) {
-        if (t->ray_order > 2) {
-            d->err = "ray_order must be 1 or 2";
+        if (t->ray_order > 3) {
+            d->err = "ray_order must be 1, 2 or 3";
             return XT_E_INVALID;
         }
         n.ray_order = t->ray_order;

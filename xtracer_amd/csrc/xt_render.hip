@@ -319,7 +319,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         Queue q0{w.q[0].r0, w.q[0].r1, w.q[0].path};
         hipLaunchKernelGGL(k_camera, dim3(grid_for(cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
                            spp_pass, 0u, j.NT, 0u, q0, w.ps0, w.ps1, w.rad, need64 ? w.ray64 : nullptr,
-                           (uint32_t)(d->tuning.ray_order == 2));
+                           d->tuning.ray_order);
         XT_LAUNCHED();
         rc = xt_run_rounds(d, &G, 1, time_trace);
         if (rc) return rc;

@@ -466,7 +466,7 @@ int run_pass(xt_dev *d, ServeState &S, const std::vector<Segment> &segs, std::ve
             Queue q0{w.q[0].r0 + at, w.q[0].r1 + at, w.q[0].path + at};
             hipLaunchKernelGGL(k_camera, dim3(grid_for(n, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, J.jd,
                                J.issued, segs[k].samples, segs[k].q0, segs[k].qn, (uint32_t)at, q0, w.ps0, w.ps1,
-                               w.rad, need64 ? w.ray64 : nullptr, (uint32_t)(d->tuning.ray_order == 2));
+                               w.rad, need64 ? w.ray64 : nullptr, d->tuning.ray_order);
             XT_LAUNCHED();
         }
     }
