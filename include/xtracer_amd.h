@@ -250,6 +250,19 @@ int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *rays, xt_hi
 int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const void *ray0, const void *ray1, void *hit,
                     uint32_t *flags, uint64_t n, float *kernel_ms);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). xt_trace_device with its scratch (the FP64 re-check and
+ * re-test lists and the counters, 20 B per ray) in a fixed device array of
+ * XT_TRACE_STATIC_MAX_RAYS rays that each GPU holds until the process ends;
+ * it calls no allocator. n above that limit: XT_E_INVALID. The scene must be
+ * uploaded. Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (other handles, other threads) race. Synchronous on the
+ * device's stream. */
+#define XT_TRACE_STATIC_MAX_RAYS (1u << 20)
+int xt_trace_device_static(xt_dev *d, uint32_t scene_id, int accel, const void *ray0, const void *ray1, void *hit,
+                           uint32_t *flags, uint64_t n);
+
 typedef struct xt_job {
     uint32_t scene_id;
     uint32_t integrator;   /* XT_INTEGRATOR_* */

@@ -35,7 +35,7 @@ SYMBOLS = [
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
     "xt_image_write", "xt_image_read", "xt_render_raygraph", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
-    "xt_trace_device", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
+    "xt_trace_device", "xt_trace_device_static", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
     "xt_serve_get_info", "xt_serve_set_policy", "xt_serve_configure", "xt_serve_get_config",
 ]
 
@@ -172,6 +172,7 @@ def lib():
         "xt_scene_release": (I, [P, U32]),
         "xt_trace": (I, [P, U32, I, ctypes.POINTER(Rays), ctypes.POINTER(Hits), U64]),
         "xt_trace_device": (I, [P, U32, I, P, P, P, P, U64, ctypes.POINTER(ctypes.c_float)]),
+        "xt_trace_device_static": (I, [P, U32, I, P, P, P, P, U64]),
         "xt_render": (I, [P, ctypes.POINTER(Job), P, ctypes.POINTER(JobStats)]),
         "xt_render_device": (I, [P, ctypes.POINTER(Job), P, ctypes.POINTER(JobStats)]),
         "xt_submit": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(U64)]),

@@ -73,6 +73,7 @@ struct xt_dev {
     std::vector<int> timed_pairs;
     ServeState *serve = nullptr;
     xt_tuning tuning{12, 256, 16u << 20, 0, 1, 1, 1, 1};
+    void *scratch = nullptr;  // this device's address of xt_trace_device_static's scratch array
     std::string err;
 };
 

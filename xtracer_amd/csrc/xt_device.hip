@@ -128,6 +128,18 @@ int xt_dev_ensure_accum(xt_dev *d, size_t npix) {
 
 extern "C" {
 
+// Scratch of xt_trace_device_static: counters, then the re-check and re-test
+// lists of XT_TRACE_STATIC_MAX_RAYS rays. In host code the array's name is not
+// a device address: xt_dev_create looks its address up per device.
+struct StaticTraceScratch {
+    Counters cnt;
+    uint32_t pad[64 - sizeof(Counters) / 4];
+    uint4 recheck[XT_TRACE_STATIC_MAX_RAYS];
+    uint32_t retest[XT_TRACE_STATIC_MAX_RAYS];
+};
+static_assert(sizeof(Counters) % 4 == 0 && sizeof(Counters) < 256, "scratch header");
+__device__ __align__(256) StaticTraceScratch scratch;
+
 int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes) {
     if (!out) return XT_E_INVALID;
     *out = nullptr;
@@ -150,6 +162,9 @@ int xt_dev_create(xt_dev **out, int hip_device, uint64_t hbm_budget_bytes) {
         // default budget: 90% of the device memory
         if (!d->budget) d->budget = (uint64_t)(0.9 * (double)prop.totalGlobalMem);
     }
+    // this device's copy of the static scratch (with the device current)
+    void *sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess) d->scratch = sp;
     *out = d;
     return XT_OK;
 }
@@ -517,6 +532,43 @@ extern "C" int xt_trace_device(xt_dev *d, uint32_t scene_id, int accel, const vo
         hipEventElapsedTime(&ms, d->tev[d->timed_pairs[0]], d->tev[d->timed_pairs[0] + 1]);
         *kernel_ms = ms;
     }
+    return XT_OK;
+}
+
+// xt_trace_device with the scratch in the per-device static array: the caller's
+// rays and hits in place, counters and re-test lists in `scratch`. No
+// allocation: the scene is resident, the counters are cleared by a kernel.
+extern "C" int xt_trace_device_static(xt_dev *d, uint32_t scene_id, int accel, const void *ray0, const void *ray1,
+                                      void *hit, uint32_t *flags, uint64_t n) {
+    if (!d || !ray0 || !ray1 || !hit) return XT_E_INVALID;
+    if (n > XT_TRACE_STATIC_MAX_RAYS) {
+        d->err = "more rays than the static trace scratch holds (XT_TRACE_STATIC_MAX_RAYS)";
+        return XT_E_INVALID;
+    }
+    auto it = d->scenes.find(scene_id);
+    if (it == d->scenes.end()) return XT_E_NOTFOUND;
+    if (accel != XT_ACCEL_KDTREE && accel != XT_ACCEL_OCTREE) return XT_E_INVALID;
+    if (!d->scratch) {
+        d->err = "the static trace scratch has no device address";
+        return XT_E_DEVICE;
+    }
+    XT_TRY(hipSetDevice(d->device));
+    StaticTraceScratch *sc = (StaticTraceScratch *)d->scratch;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, &sc->cnt, 0x800000ffu);
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, &sc->cnt, 0u, (uint32_t)n);
+    if (flags) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, flags, 0u, (size_t)n);
+    XT_LAUNCHED();
+    TraceBufs B;
+    B.q = Queue{(float4 *)ray0, (float4 *)ray1, nullptr};
+    B.count = &sc->cnt.n[0];
+    B.hit = (float4 *)hit;
+    B.retest = sc->retest;
+    B.recheck = sc->recheck;
+    B.cnt = &sc->cnt;
+    B.cap = (size_t)n;
+    int rc = xt_launch_trace_bufs(d, it->second, accel, B, nullptr, flags, false, d->stream);
+    if (rc) return rc;
+    XT_TRY(hipStreamSynchronize(d->stream));
     return XT_OK;
 }
 
