@@ -581,25 +581,25 @@ extern "C" int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *
     XT_TRY(hipSetDevice(d->device));
     int rc = xt_dev_ensure_wave(d, n, false);
     if (rc) return rc;
-    std::vector<void *> tmp;
-    float *soa[6];
     const float *src[6] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz};
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 6; ++k)
         if (!src[k]) return XT_E_INVALID;
-        rc = dev_alloc(d, &soa[k], n, &tmp, nullptr);
-        if (rc) break;
-        XT_TRY(hipMemcpyAsync(soa[k], src[k], n * sizeof(float), hipMemcpyHostToDevice, d->stream));
-    }
-    auto release = [&]() {
-        hipStreamSynchronize(d->stream);
-        for (void *p : tmp) {
-            hipFree(p);
+    // temporary SoA buffers, given back on every return below
+    struct Temp {
+        xt_dev *d;
+        uint64_t bytes;
+        std::vector<void *> p;
+        ~Temp() {
+            (void)hipStreamSynchronize(d->stream);
+            for (void *q : p) (void)hipFree(q);
+            d->used -= std::min<uint64_t>(d->used, (uint64_t)p.size() * bytes);
         }
-        d->used -= std::min<uint64_t>(d->used, (uint64_t)tmp.size() * n * sizeof(float));
-    };
-    if (rc) {
-        release();
-        return rc;
+    } tmp{d, n * sizeof(float), {}};
+    float *soa[6];
+    for (int k = 0; k < 6; ++k) {
+        rc = dev_alloc(d, &soa[k], n, &tmp.p, nullptr);
+        if (rc) return rc;
+        XT_TRY(hipMemcpyAsync(soa[k], src[k], n * sizeof(float), hipMemcpyHostToDevice, d->stream));
     }
     WaveBuffers &w = d->wave;
     Queue q{w.q[0].r0, w.q[0].r1, w.q[0].path};
@@ -611,10 +611,7 @@ extern "C" int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *
     d->trace_launches = 0;
     d->timed_pairs.clear();
     rc = xt_launch_trace(d, it->second, accel, 0, WaveLane{0, (size_t)n, w.cnt}, nullptr, w.flags, false, d->stream);
-    if (rc) {
-        release();
-        return rc;
-    }
+    if (rc) return rc;
     hipLaunchKernelGGL(k_unpack_hits, dim3(grid_for(n, 256, 4096)), dim3(256), 0, d->stream, w.hit, soa[0],
                        (uint32_t *)soa[1], soa[2], soa[3], (uint32_t)n);
     XT_LAUNCHED();
@@ -623,6 +620,5 @@ extern "C" int xt_trace(xt_dev *d, uint32_t scene_id, int accel, const xt_rays *
         if (dst[k]) XT_TRY(hipMemcpyAsync(dst[k], soa[k], n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
     if (hits->flags) XT_TRY(hipMemcpyAsync(hits->flags, w.flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
     XT_TRY(hipStreamSynchronize(d->stream));
-    release();
     return XT_OK;
 }
