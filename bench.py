@@ -45,11 +45,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["icosphere", "strands"], default="icosphere",
+                    help="icosphere: displaced icosphere, 1920x1080 16 spp depth 8 (the headline); strands: 50,000 "
+                         "random-walk strands (2.95 M thin triangles), 1024x1024 32 spp depth 6")
     ap.add_argument("--subdiv", type=int, default=8)
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--spp", type=int, default=16)
-    ap.add_argument("--rdepth", type=int, default=8)
+    ap.add_argument("--width", type=int, default=0, help="0: the workload's")
+    ap.add_argument("--height", type=int, default=0, help="0: the workload's")
+    ap.add_argument("--spp", type=int, default=0, help="0: the workload's")
+    ap.add_argument("--rdepth", type=int, default=0, help="0: the workload's")
     ap.add_argument("--accel", choices=["kdtree", "octree"], default="kdtree")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per wavefront pass (0: auto)")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -72,7 +75,13 @@ def parse():
                     help="auto: before touching the GPU, run two 2-frame copies of this workload under rocprofv3 "
                          "counter collection (FETCH_SIZE; WRITE_SIZE + TCC_HIT/TCC_MISS) and report the trace "
                          "kernel's measured traffic per query and L2 hit rate")
-    return ap.parse_args()
+    args = ap.parse_args()
+    dflt = {"icosphere": (1920, 1080, 16, 8), "strands": (1024, 1024, 32, 6)}[args.workload]
+    args.width = args.width or dflt[0]
+    args.height = args.height or dflt[1]
+    args.spp = args.spp or dflt[2]
+    args.rdepth = args.rdepth or dflt[3]
+    return args
 
 
 def _counter_rows(out_dir):
@@ -97,6 +106,7 @@ def traffic_probe(args):
         return {"error": "rocprofv3 not found"}
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "1", "--no-cpu-baseline",
              "--serve-jobs", "0", "--traffic", "off", "--copy-probe", str(COPY_PROBE_MIB),
+             "--workload", args.workload, "--strands", str(args.strands),
              "--subdiv", str(args.subdiv), "--width", str(args.width), "--height", str(args.height),
              "--spp", str(args.spp), "--rdepth", str(args.rdepth), "--accel", args.accel]
     if args.tuning:
@@ -194,7 +204,7 @@ def cpu_baseline(scene_path, args, threads):
     from xtracer_amd import scenes
     s = xa.Scene(scene_path, build=True)
     orc = O.OracleScene(s)
-    cam = scenes.camera_dict(**scenes.ICOSPHERE_CAMERA)
+    cam = scenes.camera_dict(**(scenes.ICOSPHERE_CAMERA if args.workload == "icosphere" else scenes.STRANDS_CAMERA))
     stride = max(1, tiles // (4 * threads))
     t0 = time.perf_counter()
     _, nq = orc.render(3, W, H, cam, samples=1, rdepth=args.rdepth, threads=threads, tile_stride=stride)
@@ -387,7 +397,10 @@ def main():
     from xtracer_amd import scenes
 
     t0 = time.perf_counter()
-    path = scenes.icosphere_scene(seed=0, subdiv=args.subdiv)
+    if args.workload == "icosphere":
+        path = scenes.icosphere_scene(seed=0, subdiv=args.subdiv)
+    else:
+        path = scenes.strands_scene(strands=args.strands)
     gen_s = time.perf_counter() - t0
     log(f"scene generated in {gen_s:.1f}s: {path}")
     t0 = time.perf_counter()
@@ -409,10 +422,12 @@ def main():
     if args.tuning:
         dev.set_tuning(**json.loads(args.tuning))
     tuning = dev.tuning()
+    tune_table = None
     if args.autotune and not args.tuning:
         from xtracer_amd.autotune import autotune
         t0 = time.perf_counter()
-        tuning, _ = autotune(dev, job(999), out.data_ptr(), wave_paths=(4 << 20, 16 << 20), log=log)
+        tuning, table = autotune(dev, job(999), out.data_ptr(), wave_paths=(4 << 20, 16 << 20), log=log)
+        tune_table = [dict(t, ms=round(ms, 3)) for t, ms in table]
         log(f"autotune picked {tuning} in {time.perf_counter() - t0:.1f}s")
     for k in range(args.warmup):
         t0 = time.perf_counter()
@@ -448,7 +463,8 @@ def main():
     serve = serving(dev, xa, scenes, args, sid, scene, value) if args.serve_jobs > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(path, args, threads)
     line = {
-        "metric": "Mrays/s at 1080p 16spp (pathtracer, plain one-job-per-launch)",
+        "metric": ("Mrays/s at 1080p 16spp (pathtracer, plain one-job-per-launch)" if args.workload == "icosphere"
+                   else f"Mrays/s at {args.width}x{args.height} {args.spp}spp (strands pathtracer, plain)"),
         "value": round(value, 3),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -459,9 +475,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded displaced icosphere scene, generated in-process)",
-        "config": {"workload": f"pathtracer icosphere {info.n_prims} prims {args.width}x{args.height} "
-                               f"{args.spp}spp rdepth {args.rdepth} {args.accel} thin-lens",
+        "data": f"synthetic (seeded {args.workload} scene, generated in-process)",
+        "config": {"workload": (f"pathtracer icosphere {info.n_prims} prims {args.width}x{args.height} "
+                                f"{args.spp}spp rdepth {args.rdepth} {args.accel} thin-lens"
+                                if args.workload == "icosphere" else
+                                f"pathtracer strands {info.n_prims} prims {args.width}x{args.height} "
+                                f"{args.spp}spp rdepth {args.rdepth} {args.accel} blinn-phong"),
                    "width": args.width, "height": args.height, "spp": args.spp, "rdepth": args.rdepth,
                    "prims": info.n_prims, "accel": args.accel},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -482,6 +501,8 @@ def main():
                   "kd_nodes": info.n_kd_nodes, "kd_refs": info.n_kd_refs, "kd_depth": info.kd_depth,
                   "passes": stats[0].passes if stats else 0, "tuning": tuning},
     }
+    if tune_table:
+        line["autotune"] = tune_table
     if serve:
         line["serving"] = serve
     if cpu:

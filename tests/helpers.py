@@ -1,4 +1,6 @@
 """Shared helpers for the parity tests."""
+import ctypes
+
 import numpy as np
 
 import xtracer_amd as xa
@@ -415,3 +417,35 @@ def edge_band(arr, prim, point, tol=EDGE_BAND):
         bc = bary(point[ok], q[:, 0:3], q[:, 3:6], q[:, 6:9])
         out[np.flatnonzero(ok)] = tri & (np.abs(bc).min(1) < tol)
     return out
+
+
+class HipBuffers:
+    """Device buffers through the HIP runtime the library itself links (the
+    test allocates; the entry point under test does not)."""
+
+    def __init__(self):
+        xa.lib()  # loads libxtracer_amd.so and with it the runtime
+        h = ctypes.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        self.h = h
+        self.held = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        assert self.h.hipMalloc(ctypes.byref(p), max(a.nbytes, 4)) == 0
+        assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        self.held.append(p)
+        return p
+
+    def get(self, p, like):
+        out = np.empty_like(like)
+        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.held:
+            self.h.hipFree(p)
+        self.held = []

@@ -94,3 +94,31 @@ def test_layout_knobs_invariant(tuned, knobs):
     ref, nq = OracleScene(s).render(xa.DEPTH, W, H, cam, samples=spp, seed=4)
     assert st.queries == nq
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_autotune_stages(tuned):
+    """The staged search tries every value of every stage, leaves the device on
+    the fastest shape it timed, and that shape renders the same bits."""
+    from xtracer_amd.autotune import autotune
+    from tests.helpers import HipBuffers
+    s = xa.Scene(scene_path("icosphere", subdiv=4))
+    sid = tuned.upload(s)
+    W, H = 64, 48
+    hip = HipBuffers()
+    out = hip.put(np.zeros((H, W, 4), np.float32))
+    try:
+        job = xa.make_job(sid, xa.PATHTRACER, W, H, s.camera(), samples=4, rdepth=4, seed=3)
+        ref, _ = tuned.render(job)
+        stages = [{"kd_stack": (8, 16)}, {"ray_order": (1, 2, 3)}, {"trace_segments": (1, 8)}]
+        best, table = autotune(tuned, job, out.value, samples=2, repeats=1, wave_paths=(1 << 16,), stages=stages)
+        assert len(table) == 2 + 3 + 2 + 2
+        assert {t["kd_stack"] for t, _ in table[:2]} == {8, 16}
+        assert {t["ray_order"] for t, _ in table[2:5]} == {1, 2, 3}
+        assert tuned.tuning() == best
+        assert best["kd_stack"] in (8, 16) and best["ray_order"] in (1, 2, 3) and best["trace_segments"] in (1, 8)
+        img, _ = tuned.render(job)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    finally:
+        hip.free()
+        tuned.release(sid)

@@ -9,41 +9,9 @@ import numpy as np
 import pytest
 
 import xtracer_amd as xa
-from tests.helpers import mixed_rays, scene_path
+from tests.helpers import HipBuffers, mixed_rays, scene_path
 
 MAX_RAYS = 1 << 20  # XT_TRACE_STATIC_MAX_RAYS
-
-
-class _Hip:
-    """Device buffers through the HIP runtime the library itself links (the
-    test allocates; the entry point under test does not)."""
-
-    def __init__(self):
-        xa.lib()  # loads libxtracer_amd.so and with it the runtime
-        h = ctypes.CDLL("libamdhip64.so.7")
-        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-        h.hipFree.argtypes = [ctypes.c_void_p]
-        self.h = h
-        self.held = []
-
-    def put(self, a):
-        a = np.ascontiguousarray(a)
-        p = ctypes.c_void_p()
-        assert self.h.hipMalloc(ctypes.byref(p), max(a.nbytes, 4)) == 0
-        assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
-        self.held.append(p)
-        return p
-
-    def get(self, p, like):
-        out = np.empty_like(like)
-        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
-        return out
-
-    def free(self):
-        for p in self.held:
-            self.h.hipFree(p)
-        self.held = []
 
 
 def _rays(o, d):
@@ -59,7 +27,7 @@ def _rays(o, d):
 @pytest.mark.parametrize("accel", [xa.ACCEL_KDTREE, xa.ACCEL_OCTREE])
 def test_static_trace_equals_trace_device(accel):
     L = xa.lib()
-    hip = _Hip()
+    hip = HipBuffers()
     dev = xa.Device(0)
     try:
         s = xa.Scene(scene_path("icosphere", subdiv=5))

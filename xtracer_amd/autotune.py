@@ -1,20 +1,30 @@
 """Launch-shape autotuner.
 
-Searches the device's tuning space (kd short-stack depth, trace block size,
-persistent grid size, wavefront size) by timing short renders of the caller's
+Searches the device's tuning space by timing short renders of the caller's
 own job, keeps the fastest shape on the device and returns the table. The
 knobs change speed only: every shape gives the same hits and images (the
-kd-tree traversal restarts from the root when its short stack overflows).
+kd-tree traversal restarts from the root when its short stack overflows, and
+ray order only permutes the queues).
+
+The search is staged: the trace kernel's shape (kd short-stack depth, block
+size, persistent grid) over its full product first, then the ray-sort
+granularity (camera rays sample by sample over the image, by 8x8 tile, or by
+pixel), then whether the trace queue is cut into per-XCD segments, then
+whether kd leaves are read as per-reference copies or through the reference
+list, each stage keeping the best of the one before; the wavefront size
+(paths in flight per pass) last, on the full job.
 """
 import itertools
 
 from ._native import TUNING_AUTO
 
-DEFAULT_SPACE = {
+KERNEL_SPACE = {
     "kd_stack": (8, 12, 16, 24),
     "trace_block": (128, 256),
     "trace_blocks_per_cu": (TUNING_AUTO, 2, 8),
 }
+STAGES = [KERNEL_SPACE, {"ray_order": (1, 2, 3)}, {"trace_segments": (1, 8)}, {"kd_leaf": (1, 2)}]
+DEFAULT_SPACE = KERNEL_SPACE  # the first stage alone (older callers)
 
 
 def _time_job(dev, job, out_ptr, repeats):
@@ -25,41 +35,57 @@ def _time_job(dev, job, out_ptr, repeats):
     return best
 
 
-def autotune(dev, job, out_ptr, space=None, samples=2, repeats=2, wave_paths=(), log=None):
-    """Times `job` (cut to `samples` samples per pixel) under every shape of
-    `space` on `dev`, then leaves `dev` set to the fastest. `wave_paths` adds
-    wavefront sizes to try once the best kernel shape is known. Returns
-    (best tuning dict, [(tuning dict, ms), ...])."""
-    space = dict(DEFAULT_SPACE if space is None else space)
+def _shape(t):
+    """A tuning dict as xt_dev_set_tuning takes it (0: keep, so the resident
+    grid is asked for by name)."""
+    out = {k: v for k, v in t.items() if k != "wave_paths"}
+    if not out.get("trace_blocks_per_cu"):
+        out["trace_blocks_per_cu"] = TUNING_AUTO
+    return out
+
+
+def autotune(dev, job, out_ptr, space=None, samples=2, repeats=2, wave_paths=(), log=None, stages=None):
+    """Times `job` (cut to `samples` samples per pixel) under the shapes of
+    each stage (`stages`, default STAGES; `space` alone: one stage) on `dev`,
+    then leaves `dev` set to the fastest. `wave_paths` adds wavefront sizes to
+    try once the best shape is known. Returns (best tuning dict,
+    [(tuning dict, ms), ...])."""
+    if stages is None:
+        stages = [dict(space)] if space is not None else STAGES
     probe = type(job).from_buffer_copy(job)
     probe.samples = min(job.samples, samples)
     probe.samples_per_pass = 0
-    keys = list(space)
     table = []
     start = dev.tuning()
     _time_job(dev, probe, out_ptr, 1)  # first launch of each kernel loads its code object
-    for combo in itertools.product(*(space[k] for k in keys)):
-        knobs = dict(zip(keys, combo))
-        dev.set_tuning(**knobs)
-        _time_job(dev, probe, out_ptr, 1)
-        ms = _time_job(dev, probe, out_ptr, repeats)
-        table.append((dev.tuning(), ms))
-        if log:
-            log(f"autotune {knobs}: {ms:.2f} ms")
-    best, _ = min(table, key=lambda e: e[1])
-    shape = {k: (v if v else TUNING_AUTO) for k, v in best.items() if k != "wave_paths"}
+    best_shape, best_ms = _shape(start), float("inf")
+    for stage in stages:
+        keys = list(stage)
+        stage_best = None
+        for combo in itertools.product(*(stage[k] for k in keys)):
+            knobs = dict(best_shape, **dict(zip(keys, combo)))
+            dev.set_tuning(**knobs)
+            _time_job(dev, probe, out_ptr, 1)
+            ms = _time_job(dev, probe, out_ptr, repeats)
+            table.append((dev.tuning(), ms))
+            if log:
+                log(f"autotune {dict(zip(keys, combo))}: {ms:.2f} ms")
+            if stage_best is None or ms < stage_best[1]:
+                stage_best = (_shape(dev.tuning()), ms)
+        best_shape, best_ms = stage_best
+    wp_best = start["wave_paths"]
     if wave_paths:
         # the wavefront size matters only across passes: time the full job
         full = type(job).from_buffer_copy(job)
         full.samples_per_pass = 0
-        best_ms = float("inf")
+        wp_ms = float("inf")
         for wp in sorted(set(wave_paths) | {start["wave_paths"]}):
-            dev.set_tuning(**shape, wave_paths=wp)
+            dev.set_tuning(**best_shape, wave_paths=wp)
             ms = _time_job(dev, full, out_ptr, repeats)
             table.append((dev.tuning(), ms))
             if log:
                 log(f"autotune wave_paths={wp}: {ms:.2f} ms")
-            if ms < best_ms:
-                best, best_ms = dev.tuning(), ms
-    dev.set_tuning(**shape, wave_paths=best["wave_paths"] if wave_paths else start["wave_paths"])
+            if ms < wp_ms:
+                wp_best, wp_ms = wp, ms
+    dev.set_tuning(**best_shape, wave_paths=wp_best)
     return dev.tuning(), table
