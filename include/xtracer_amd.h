@@ -294,6 +294,26 @@ typedef struct xt_job_stats {
     uint64_t retest_fail;  /* FP64 re-test traversals that ran out of steps (closest hit so far kept) */
 } xt_job_stats;
 
+/* IMaterial::shade (src/xtcore/material/lambert.cc:9-28, phong.cc:11-33,
+ * blinnphong.cc:9-32, emissive.cc:8-18, dielectric.cc:9-21): the light one
+ * point emitter sends from a hit through the hit object's material, as the
+ * reference's materials define it (none of its integrators calls it).
+ * intensity: 3 floats per query, read and written as shade() updates its
+ * `intensity` argument (Lambert and Phong add to it; Blinn-Phong, emissive and
+ * dielectric assign it); result (optional): shade()'s return value, 1 or 0,
+ * or -1 for an object index out of range or an object without a material
+ * (intensity left as it was). Host buffers. */
+typedef struct xt_shade_query {
+    double point[3];     /* hit_record.point */
+    double normal[3];    /* hit_record.normal, as the hit record holds it */
+    double texcoord[3];  /* hit_record.texcoord */
+    double camera[3];    /* camera->position */
+    double emitter[3];   /* emitter->position */
+    float intensity[3];  /* emitter->intensity */
+    uint32_t object;     /* object index in scan order: its material shades */
+} xt_shade_query;
+int xt_shade(xt_dev *d, uint32_t scene_id, const xt_shade_query *q, uint64_t n, float *intensity, int32_t *result);
+
 /* Plain one-job-per-launch render. rgba_out: width*height*4 floats, host. */
 int xt_render(xt_dev *d, const xt_job *job, float *rgba_out, xt_job_stats *stats);
 

@@ -116,6 +116,7 @@ def lib():
         L.or_path_key.restype = ctypes.c_uint32
         L.or_path_key.argtypes = [ctypes.c_uint32] * 4
         L.or_mesh_octree.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), P]
+        L.or_shade.argtypes = [P, ctypes.c_uint64, P, P, P, P]
         L.or_sample.restype = ctypes.c_uint64
         L.or_sample.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, P]
         _lib = L
@@ -212,6 +213,17 @@ class OracleScene:
             return img, nq, mean, m2
         return img, nq
 
+    def shade(self, q, emitter, intensity):
+        """IMaterial::shade per query. q: (n, 16) float64 rows of hit point,
+        normal, texcoord, camera position, emitter position, object index;
+        emitter, intensity: (n, 3) float32. Returns (intensity after shade,
+        shade()'s results: 1 / 0, -1 for an object without a material)."""
+        q = np.ascontiguousarray(q, np.float64)
+        io = np.array(intensity, np.float32, copy=True, order="C")
+        ret = np.zeros(len(q), np.int32)
+        lib().or_shade(self._h, len(q), _p(q), _p(np.ascontiguousarray(emitter, np.float32)), _p(io), _p(ret))
+        return io, ret
+
     def mesh_preorder(self, obj):
         n = ctypes.c_uint64()
         cnt = int(self.mesh_count[obj])
@@ -266,6 +278,7 @@ def ref():
         L.xr_aa_random.argtypes = [ctypes.c_uint64, P, ctypes.c_uint, ctypes.c_uint, P, P, P]
         L.xr_load_image.argtypes = [ctypes.c_char_p, P, P, P]
         L.xr_write_raygraph.argtypes = [P, ctypes.c_char_p]
+        L.xr_shade.argtypes = [P, ctypes.c_uint64, P, P, P, P]
         _ref = L
     return _ref
 
@@ -353,6 +366,14 @@ class RefScene:
             raise RuntimeError("xr_render failed")
         return img
 
+
+    def shade(self, q, emitter, intensity):
+        """The reference materials' own shade() (OracleScene.shade's layout)."""
+        q = np.ascontiguousarray(q, np.float64)
+        io = np.array(intensity, np.float32, copy=True, order="C")
+        ret = np.zeros(len(q), np.int32)
+        ref().xr_shade(self._h, len(q), _p(q), _p(np.ascontiguousarray(emitter, np.float32)), _p(io), _p(ret))
+        return io, ret
 
     def write_raygraph(self, path):
         """PLY of the last render's ray graph, written by the reference."""

@@ -24,6 +24,8 @@
 #include <xtcore/util/raygraph.h>
 #include <xtcore/aa.h>
 #include <xtcore/camera/perspective.h>
+#include <xtcore/emitter.h>
+#include <xtcore/material.h>
 #include <nmath/prng.h>
 #include <xtcore/integrator/wireframe/integrator.h>
 #include <nimg/img.h>
@@ -359,6 +361,40 @@ int xr_aa_random(uint64_t n, const float *pixels, unsigned level, unsigned seed,
 
 // Writes rgba (W*H*4 floats) through the reference's own image writers
 // (lib/nimg/img.cc): 0 png, 1 bmp, 2 tga, 3 hdr, 4 exr.
+// IMaterial::shade of object i's material for n queries: q holds 16 doubles
+// per query (hit point, normal, texcoord, camera position, emitter position,
+// object index), ei the emitter intensity and io the intensity argument (3
+// floats each, io updated in place); ret gets shade()'s return value, or -1
+// for an object without a material.
+int xr_shade(void *h, uint64_t n, const double *q, const float *ei, float *io, int *ret) {
+    Scene &s = ((render::context_t *)h)->scene;
+    camera::Perspective cam;
+    for (uint64_t k = 0; k < n; ++k) {
+        const double *a = q + 16 * k;
+        const uint32_t oi = (uint32_t)a[15];
+        ret[k] = -1;
+        if (oi >= s.m_objects.size()) continue;
+        auto it = s.m_objects.begin();
+        std::advance(it, oi);
+        const asset::IMaterial *m = s.get_material(it->first);
+        if (!m) continue;
+        hit_record_t hr;
+        hr.point = nmath::Vector3f(a[0], a[1], a[2]);
+        hr.normal = nmath::Vector3f(a[3], a[4], a[5]);
+        hr.texcoord = nmath::Vector3f(a[6], a[7], a[8]);
+        cam.position = nmath::Vector3f(a[9], a[10], a[11]);
+        asset::emitter_t em;
+        em.position = nmath::Vector3f(a[12], a[13], a[14]);
+        em.intensity = nimg::ColorRGBf(ei[3 * k], ei[3 * k + 1], ei[3 * k + 2]);
+        nimg::ColorRGBf in(io[3 * k], io[3 * k + 1], io[3 * k + 2]);
+        ret[k] = m->shade(in, &cam, &em, hr) ? 1 : 0;
+        io[3 * k] = in.r();
+        io[3 * k + 1] = in.g();
+        io[3 * k + 2] = in.b();
+    }
+    return 0;
+}
+
 int xr_save_image(int fmt, const char *path, const float *rgba, int w, int h) {
     nimg::Pixmap pm;
     if (pm.init((size_t)w, (size_t)h)) return 1;

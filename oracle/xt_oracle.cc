@@ -1036,6 +1036,64 @@ uint32_t or_path_key(uint32_t seed, uint32_t job_key, uint32_t pixel, uint32_t s
     return path_key(seed, job_key, pixel, sample);
 }
 
+// IMaterial::shade (material/lambert.cc:9-28, phong.cc:11-33,
+// blinnphong.cc:9-32, emissive.cc:8-18, dielectric.cc:9-21): the light of one
+// point emitter through object oi's material. q: 16 doubles per query (hit
+// point, normal, texcoord, camera position, emitter position, object index);
+// ei: emitter intensity, io: the intensity argument, in and out (3 floats
+// each, ColorRGBf: clamped at 0); ret: shade()'s result, or -1 for an object
+// without a material.
+int or_shade(void *h, uint64_t n, const double *q, const float *ei, float *io, int32_t *ret) {
+    const Scene &s = *(const Scene *)h;
+    for (uint64_t k = 0; k < n; ++k) {
+        const double *a = q + 16 * k;
+        const uint32_t oi = (uint32_t)a[15];
+        ret[k] = -1;
+        if (oi >= s.objs.size() || s.objs[oi].mat >= s.mats.size()) continue;
+        const Mat &m = s.mats[s.objs[oi].mat];
+        const V P = mk(a[0], a[1], a[2]), N = mk(a[3], a[4], a[5]), TC = mk(a[6], a[7], a[8]);
+        const V cam = mk(a[9], a[10], a[11]), L = mk(a[12], a[13], a[14]);
+        auto chan = [&](int c, const float *konst) {
+            return m.smp[c] != 0xffffffffu ? sample(s, s.smps[m.smp[c]], TC) : C3{konst[0], konst[1], konst[2]};
+        };
+        const C3 E{cl(ei[3 * k]), cl(ei[3 * k + 1]), cl(ei[3 * k + 2])};
+        C3 I{cl(io[3 * k]), cl(io[3 * k + 1]), cl(io[3 * k + 2])};
+        int r = 1;
+        auto mx = [](double x, double y) { return x > y ? x : y; };  // nmath::max (mutil.inl:41)
+        if (m.type == 0) {  // Lambert: I += E * d * kd when d > 0
+            const V ld = nrm(L - P);
+            const double d = dot(ld, N);
+            if (d > 0) I = cadd(I, cmul(cscale(E, (float)d), chan(0, m.kd)));
+        } else if (m.type == 1) {  // Phong: I += E * (d kd + ks max(0, r.v)^e)
+            const V ld = nrm(L - P);
+            const double d = mx(0, dot(ld, N));
+            const V v = nrm(cam - P);
+            const V li = nrm(ld), nn = nrm(N);
+            const V rf = nrm((2 * dot(li, nn)) * nn - li);  // Vector3f::reflected (vector.inl:455-460)
+            const double rmv = mx(0, dot(rf, v));
+            const float pw = (float)powl((long double)rmv, (long double)m.expo);
+            I = cadd(I, cmul(E, cadd(cscale(chan(0, m.kd), (float)d), cscale(chan(1, m.ks), pw))));
+        } else if (m.type == 2) {  // Blinn-Phong: I = E * (d kd + ks max(h.n, 0)^e)
+            const V ld = nrm(L - P);
+            const double d = mx(dot(ld, N), 0);
+            const V hv = nrm(ld + nrm(cam - P));
+            const double rmv = mx(dot(hv, N), 0);
+            const float pw = (float)powl((long double)rmv, (long double)m.expo);
+            I = cmul(E, cadd(cscale(chan(0, m.kd), (float)d), cscale(chan(1, m.ks), pw)));
+        } else if (m.type == 3) {  // emissive: I = ke, returns false
+            I = chan(2, m.ke);
+            r = 0;
+        } else {  // dielectric: I = 0
+            I = C3{};
+        }
+        io[3 * k] = I.r;
+        io[3 * k + 1] = I.g;
+        io[3 * k + 2] = I.b;
+        ret[k] = r;
+    }
+    return 0;
+}
+
 uint64_t or_render(void *h, const or_job *jb, float *rgba, double *mean, double *m2) {
     Scene *s = (Scene *)h;
     const Cam cam = cam_of(jb);

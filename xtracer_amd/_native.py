@@ -35,7 +35,7 @@ SYMBOLS = [
     "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
     "xt_image_write", "xt_image_read", "xt_render_raygraph", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
-    "xt_trace_device", "xt_trace_device_static", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
+    "xt_trace_device", "xt_trace_device_static", "xt_shade", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
     "xt_serve_get_info", "xt_serve_set_policy", "xt_serve_configure", "xt_serve_get_config",
 ]
 
@@ -126,6 +126,8 @@ PRIM_DTYPE = np.dtype([("p0", "<f4", 3), ("tf", "<u4"), ("p1", "<f4", 3), ("obje
                        ("aux", "<f4")])
 PLANE_DTYPE = np.dtype([("normal", "<f4", 3), ("object", "<u4"), ("point", "<f4", 3), ("pad", "<u4"),
                         ("normal64", "<f8", 3), ("point64", "<f8", 3)])
+SHADE_QUERY_DTYPE = np.dtype([("point", "<f8", 3), ("normal", "<f8", 3), ("texcoord", "<f8", 3), ("camera", "<f8", 3),
+                              ("emitter", "<f8", 3), ("intensity", "<f4", 3), ("object", "<u4")])
 MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("kd", "<f4", 3), ("ks", "<f4", 3), ("ke", "<f4", 3),
                            ("reflectance", "<f4"), ("exponent", "<f4")])
 
@@ -173,6 +175,7 @@ def lib():
         "xt_trace": (I, [P, U32, I, ctypes.POINTER(Rays), ctypes.POINTER(Hits), U64]),
         "xt_trace_device": (I, [P, U32, I, P, P, P, P, U64, ctypes.POINTER(ctypes.c_float)]),
         "xt_trace_device_static": (I, [P, U32, I, P, P, P, P, U64]),
+        "xt_shade": (I, [P, U32, P, U64, P, P]),
         "xt_render": (I, [P, ctypes.POINTER(Job), P, ctypes.POINTER(JobStats)]),
         "xt_render_device": (I, [P, ctypes.POINTER(Job), P, ctypes.POINTER(JobStats)]),
         "xt_submit": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(U64)]),
@@ -460,6 +463,22 @@ class Device:
                                           ctypes.c_void_p(ray1_ptr), ctypes.c_void_p(hit_ptr),
                                           ctypes.c_void_p(flags_ptr) if flags_ptr else None, n, ctypes.byref(ms)))
         return ms.value
+
+    def shade(self, scene_id, q, emitter, intensity):
+        """IMaterial::shade on the device (see xt_shade). q: (n, 16) float64
+        rows of hit point, normal, texcoord, camera position, emitter
+        position, object index; emitter, intensity: (n, 3) float32. Returns
+        (intensity after shade, results)."""
+        q = np.asarray(q, np.float64)
+        rec = np.zeros(len(q), SHADE_QUERY_DTYPE)
+        rec["point"], rec["normal"], rec["texcoord"] = q[:, 0:3], q[:, 3:6], q[:, 6:9]
+        rec["camera"], rec["emitter"] = q[:, 9:12], q[:, 12:15]
+        rec["intensity"] = np.asarray(emitter, np.float32)
+        rec["object"] = q[:, 15].astype(np.uint32)
+        io = np.array(intensity, np.float32, copy=True, order="C")
+        ret = np.zeros(len(q), np.int32)
+        self._check(lib().xt_shade(self._d, scene_id, _ptr(rec), len(q), _ptr(io), _ptr(ret)))
+        return io, ret
 
     def submit(self, job):
         t = ctypes.c_uint64()

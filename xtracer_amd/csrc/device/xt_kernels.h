@@ -817,9 +817,6 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_channels(const SceneView S,
     }
 }
 
-// SAMPLED: the material colours and the environment come from k_channels'
-// output `chan` (slot-indexed) instead of the material and environment
-// records.
 // One shading step of a pathtracer path at its hit h (ray a, b): the closed
 // form of pathtracer/integrator.cc:21-47 with the path state s0 = (T, key),
 // s1 = (F, query word). A miss ends the path with L = T env, an emissive hit
@@ -1308,6 +1305,79 @@ static __global__ __launch_bounds__(XT_BLOCK) void k_resolve(const JobDev j, con
         }
     }
 }
+
+#pragma clang fp contract(off)
+// ColorRGBf arithmetic (lib/nimg/color.inl): float components, every result
+// clamped at 0 by the setters.
+__device__ __forceinline__ float cclamp(float v) { return v < 0.f ? 0.f : v; }
+__device__ __forceinline__ f3 cmulc(f3 a, f3 b) { return mk(cclamp(a.x * b.x), cclamp(a.y * b.y), cclamp(a.z * b.z)); }
+__device__ __forceinline__ f3 cmuls(f3 a, float s) { return mk(cclamp(a.x * s), cclamp(a.y * s), cclamp(a.z * s)); }
+__device__ __forceinline__ f3 caddc(f3 a, f3 b) { return mk(cclamp(a.x + b.x), cclamp(a.y + b.y), cclamp(a.z + b.z)); }
+// nmath::max (lib/nmath/mutil.inl:41): x > y ? x : y
+__device__ __forceinline__ double nmax(double x, double y) { return x > y ? x : y; }
+
+// IMaterial::shade of one query (material/lambert.cc:9-28, phong.cc:11-33,
+// blinnphong.cc:9-32, emissive.cc:8-18, dielectric.cc:9-21): FP64 vectors,
+// float colours, the exponent's power in FP64 (the reference's long double
+// power rounded to float). Channel colours are the material's constants or
+// its samplers at the texture coordinate.
+static __global__ __launch_bounds__(XT_BLOCK) void k_shade_direct(const SceneView S, const xt_shade_query *qs,
+                                                           uint32_t n, float *inout, int32_t *ret) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const xt_shade_query q = qs[i];
+        const uint32_t mi = q.object < S.n_objects ? S.obj_material[q.object] : XT_NONE;
+        if (mi == XT_NONE) {
+            ret[i] = -1;
+            continue;
+        }
+        const XtMaterial m = S.materials[mi];
+        const uint32_t *ms = S.mat_smp + 3 * mi;
+        f3 ch[3] = {mk(m.kd[0], m.kd[1], m.kd[2]), mk(m.ks[0], m.ks[1], m.ks[2]), mk(m.ke[0], m.ke[1], m.ke[2])};
+        for (int c = 0; c < 3; ++c)
+            if (ms[c] != XT_NONE) ch[c] = sampler_eval(S, ms[c], q.texcoord[0], q.texcoord[1], q.texcoord[2]);
+        // both colours are ColorRGBf in the reference: clamped at 0 already
+        f3 I = mk(cclamp(inout[3 * i]), cclamp(inout[3 * i + 1]), cclamp(inout[3 * i + 2]));
+        const f3 E = mk(cclamp(q.intensity[0]), cclamp(q.intensity[1]), cclamp(q.intensity[2]));
+        const d3 P = {q.point[0], q.point[1], q.point[2]}, N = {q.normal[0], q.normal[1], q.normal[2]};
+        const d3 C = {q.camera[0], q.camera[1], q.camera[2]}, L = {q.emitter[0], q.emitter[1], q.emitter[2]};
+        int r = 1;
+        if (m.type == XT_MAT_LAMBERT) {
+            const d3 ld = dnorm(dsub(L, P));
+            const double dd = ddot(ld, N);
+            if (dd > 0) I = caddc(I, cmulc(cmuls(E, (float)dd), ch[0]));
+        } else if (m.type == XT_MAT_PHONG || m.type == XT_MAT_BLINNPHONG) {
+            const d3 ld = dnorm(dsub(L, P));
+            const d3 ray = dnorm(dsub(C, P));
+            double dd, rmv;
+            if (m.type == XT_MAT_PHONG) {
+                dd = nmax(0, ddot(ld, N));
+                // light_dir.reflected(normal) (vector.inl:455-460), normalised
+                const d3 ii = dnorm(ld), nn = dnorm(N);
+                const d3 rf = dnorm(dsub(dscl(nn, 2 * ddot(ii, nn)), ii));
+                rmv = nmax(0, ddot(rf, ray));
+            } else {
+                dd = nmax(ddot(ld, N), 0);
+                const d3 h = dnorm(dadd(ld, ray));
+                rmv = nmax(ddot(h, N), 0);
+            }
+            const float pw = (float)pow(rmv, (double)m.exponent);
+            const f3 term = caddc(cmuls(ch[0], (float)dd), cmuls(ch[1], pw));
+            if (m.type == XT_MAT_PHONG) I = caddc(I, cmulc(E, term));
+            else I = cmulc(E, term);
+        } else if (m.type == XT_MAT_EMISSIVE) {
+            I = ch[2];
+            r = 0;
+        } else {  // dielectric
+            I = mk(0.f, 0.f, 0.f);
+        }
+        inout[3 * i] = I.x;
+        inout[3 * i + 1] = I.y;
+        inout[3 * i + 2] = I.z;
+        ret[i] = r;
+    }
+}
+#pragma clang fp contract(on)
 
 // Generic-query helpers (xt_trace): pack SoA rays into the queue layout and
 // unpack hits.

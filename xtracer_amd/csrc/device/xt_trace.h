@@ -30,7 +30,7 @@ struct SceneView {
     const uint32_t *mat_smp;  // 3 per material: sampler of diffuse / specular / emissive, or XT_NONE (colour)
     XtEnv env;
     float kb_lo[3], kb_hi[3];
-    uint32_t n_prims, n_planes, n_oct;
+    uint32_t n_prims, n_planes, n_oct, n_objects;
     uint32_t sampled;  // 1 when a material channel or the environment has a sampler
     uint32_t kd_indexed;  // 1: the FP32 walk reads leaf primitives from `prims` through `kd_refs`
 };

@@ -369,6 +369,7 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     v.n_prims = (uint32_t)f.prims.size();
     v.n_planes = (uint32_t)f.planes.size();
     v.n_oct = (uint32_t)f.oct_nodes.size();
+    v.n_objects = (uint32_t)f.objects.size();
     v.sampled = f.samplers.empty() ? 0u : 1u;
     if (f.kd.nodes.empty()) {  // empty soup: a root leaf with no refs
         XtKdNode leaf;
@@ -568,6 +569,42 @@ extern "C" int xt_trace_device_static(xt_dev *d, uint32_t scene_id, int accel, c
     B.cap = (size_t)n;
     int rc = xt_launch_trace_bufs(d, it->second, accel, B, nullptr, flags, false, d->stream);
     if (rc) return rc;
+    XT_TRY(hipStreamSynchronize(d->stream));
+    return XT_OK;
+}
+
+extern "C" int xt_shade(xt_dev *d, uint32_t scene_id, const xt_shade_query *q, uint64_t n, float *intensity,
+                        int32_t *result) {
+    if (!d || !q || !intensity) return XT_E_INVALID;
+    if (n == 0) return XT_OK;
+    if (n >= (1ull << 31)) return XT_E_INVALID;
+    auto it = d->scenes.find(scene_id);
+    if (it == d->scenes.end()) return XT_E_NOTFOUND;
+    XT_TRY(hipSetDevice(d->device));
+    struct Temp {
+        xt_dev *d;
+        std::vector<void *> p;
+        uint64_t bytes = 0;
+        ~Temp() {
+            (void)hipStreamSynchronize(d->stream);
+            for (void *x : p) (void)hipFree(x);
+            d->used -= std::min<uint64_t>(d->used, bytes);
+        }
+    } tmp{d, {}};
+    xt_shade_query *dq = nullptr;
+    float *di = nullptr;
+    int32_t *dr = nullptr;
+    int rc = dev_alloc(d, &dq, n, &tmp.p, &tmp.bytes);
+    if (!rc) rc = dev_alloc(d, &di, 3 * n, &tmp.p, &tmp.bytes);
+    if (!rc) rc = dev_alloc(d, &dr, n, &tmp.p, &tmp.bytes);
+    if (rc) return rc;
+    XT_TRY(hipMemcpyAsync(dq, q, n * sizeof(xt_shade_query), hipMemcpyHostToDevice, d->stream));
+    XT_TRY(hipMemcpyAsync(di, intensity, 3 * n * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    hipLaunchKernelGGL(k_shade_direct, dim3(grid_for(n, XT_BLOCK, 4096)), dim3(XT_BLOCK), 0, d->stream,
+                       it->second.view, (const xt_shade_query *)dq, (uint32_t)n, di, dr);
+    XT_LAUNCHED();
+    XT_TRY(hipMemcpyAsync(intensity, di, 3 * n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    if (result) XT_TRY(hipMemcpyAsync(result, dr, n * sizeof(int32_t), hipMemcpyDeviceToHost, d->stream));
     XT_TRY(hipStreamSynchronize(d->stream));
     return XT_OK;
 }
