@@ -67,6 +67,7 @@ def parse():
     ap.add_argument("--serve-checked", type=int, default=4,
                     help="served requests per arrival rate whose image is compared with the oracle's")
     ap.add_argument("--serve-rates", default="0.5,0.8,0.95", help="arrival rates as fractions of capacity")
+    ap.add_argument("--serve-log", default="", help="file to append one JSON line per served or plain request to")
     ap.add_argument("--strands", type=int, default=50000, help="strands of the hair scene in the serving mix")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed steps, copy MIB MiB device-to-device once (a kernel of known traffic "
@@ -281,12 +282,13 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
             st = dev.render_device(j, out.data_ptr())
             q += st.queries
             lat.append((time.perf_counter() - t0 - (arrivals[k] if arrivals is not None else 0.0)) * 1e3)
-        return time.perf_counter() - t0, q, lat, {}
+        return time.perf_counter() - t0, q, lat, {}, lat
 
     def served(jobs, arrivals=None, keep=()):
         info0 = dev.serve_info()
         t0 = time.perf_counter()
         pending, lat, k, imgs = {}, [], 0, {}
+        by_job = [0.0] * len(jobs)
         while k < len(jobs) or pending:
             now = time.perf_counter() - t0
             while k < len(jobs) and (arrivals is None or arrivals[k] <= now):
@@ -303,14 +305,27 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
                 done, img, st = dev.poll(t, w, h)
                 if done:
                     lat.append(late + st.latency_ms)
+                    by_job[i] = late + st.latency_ms
                     if i in keep:
                         imgs[i] = img
                     del pending[t]
         wall = time.perf_counter() - t0
         info1 = dev.serve_info()
-        return wall, info1["queries"] - info0["queries"], lat, imgs
+        return wall, info1["queries"] - info0["queries"], lat, imgs, by_job
 
-    def summary(wall, q, lat, _imgs):
+    def reqlog(part, load, policy, jobs, arrivals, by_job):
+        """One JSON line per request (scene, size, samples, arrival, latency)
+        into --serve-log."""
+        if not args.serve_log:
+            return
+        with open(args.serve_log, "a") as f:
+            for k, ((e, _, j), ms) in enumerate(zip(jobs, by_job)):
+                f.write(json.dumps({"part": part, "load": load, "policy": policy, "k": k, "scene": e[0],
+                                    "res": j.width, "spp": j.samples,
+                                    "arrival_s": None if arrivals is None else round(float(arrivals[k]), 5),
+                                    "latency_ms": round(ms, 2)}) + "\n")
+
+    def summary(wall, q, lat, _imgs, _by_job=None):
         return {"wall_s": round(wall, 3), "Mrays_s": round(q / wall / 1e6, 1),
                 "p50_ms": round(_pct(lat, 50), 1), "p99_ms": round(_pct(lat, 99), 1)}
 
@@ -340,12 +355,14 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
     jobs = draw(args.serve_jobs, 10000, False)
     samples = sum(j.width * j.height * j.samples for _, _, j in jobs)
     served(jobs[:4])  # warm the server's buffers and graph
-    pw, pq, plat, _ = plain(jobs)
+    pw, pq, plat, _, pby = plain(jobs)
+    reqlog("batch", None, "plain", jobs, None, pby)
     res["batch"] = {"jobs": len(jobs), "samples": samples, "plain": summary(pw, pq, plat, None)}
     walls = {}
     for pol in policies:
         dev.set_serve_policy(pol)
-        sw, sq, slat, _ = served(jobs)
+        sw, sq, slat, _, sby = served(jobs)
+        reqlog("batch", None, pol, jobs, None, sby)
         walls[pol] = sw
         res["batch"]["served_" + pol] = summary(sw, sq, slat, None)
         log(f"serving batch ({pol}): plain {pw:.2f}s served {sw:.2f}s")
@@ -366,10 +383,13 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
         imgs = {}
         for pol in ("srpt", "hybrid"):
             dev.set_serve_policy(pol)
-            wall, q, lat, got = served(jobs, arrivals, keep)
+            wall, q, lat, got, by = served(jobs, arrivals, keep)
+            reqlog("poisson", rate, pol, jobs, arrivals, by)
             entry["served_" + pol] = summary(wall, q, lat, None)
             imgs = imgs or got
-        entry["plain"] = summary(*plain(jobs, arrivals))
+        pr = plain(jobs, arrivals)
+        reqlog("poisson", rate, "plain", jobs, arrivals, pr[4])
+        entry["plain"] = summary(*pr)
         t0 = time.perf_counter()
         entry["request_error"] = request_errors(jobs, imgs)
         entry["request_error_s"] = round(time.perf_counter() - t0, 1)
