@@ -255,6 +255,15 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
         log(f"serving scene {kind}: {sc.info().n_prims} prims")
     pool.insert(1, ("icosphere", ico_scene, ico_sid, xa.PATHTRACER, 8, scenes.ICOSPHERE_CAMERA))
     setup_s = time.perf_counter() - t0
+    # kd-trees of the resident scenes rebuilt on the device (the host build's
+    # tree node for node), against the host build's time for the same tree
+    kd_build = []
+    for kind, sc, sid, _, _, _ in pool:
+        st = dev.build_kd(sid)
+        i = sc.info()
+        kd_build.append({"scene": kind, "prims": i.n_prims, "nodes": st["n_nodes"], "refs": st["n_refs"],
+                         "device_ms": round(st["ms"], 2), "host_s": round(i.build_seconds_kd, 3)})
+        log(f"kd-tree of {kind} built on the device in {st['ms']:.1f} ms (host build {i.build_seconds_kd:.2f} s)")
 
     def draw(n, key0, orbit):
         jobs = []
@@ -351,7 +360,8 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
 
     policies = ("fcfs", "srpt", "hybrid")
     res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1), "engine": dev.serve_config(),
-           "policies": list(policies)}
+           "policies": list(policies), "kd_build": kd_build,
+           "kd_build_host_threads": os.cpu_count()}
     jobs = draw(args.serve_jobs, 10000, False)
     samples = sum(j.width * j.height * j.samples for _, _, j in jobs)
     served(jobs[:4])  # warm the server's buffers and graph

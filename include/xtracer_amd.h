@@ -74,10 +74,14 @@ typedef struct xt_build_params {
     int kd_leaf_prims;    /* 0: 4 */
     float kd_cost_intersect; /* 0: 12 */
     int threads;          /* 0: all cores */
+    int kd_on_device;     /* 1: xt_scene_build leaves the kd-tree out and xt_scene_upload builds it on the
+                             device (xt_scene_build_kd_device with these parameters) */
 } xt_build_params;
 
 /* Flattens the scene (objects in scan order, mesh triangles in octree
- * pre-order) and builds the octree (reference rules) and the SAH kd-tree. */
+ * pre-order) and builds the octree (reference rules) and the SAH kd-tree
+ * (src/xtcore/bvh.cc:13-110 is the reference's empty stub of a tree over
+ * the scene; its README.md:47 lists a kd-tree). */
 int xt_scene_build(xt_scene *s, const xt_build_params *p);
 void xt_scene_free(xt_scene *s);
 const char *xt_scene_error(const xt_scene *s);
@@ -137,6 +141,26 @@ int xt_scene_get_samplers(const xt_scene *s, uint32_t counts[3], uint32_t *tex_i
 /* The kd-tree as uploaded: n_kd_nodes * 2 words (XtKdNode), n_kd_refs
  * primitive references and the root box (lo xyz, hi xyz). */
 int xt_scene_get_kdtree(const xt_scene *s, uint32_t *nodes, uint32_t *refs, float bounds[6]);
+
+/* Device kd-tree build of a resident scene (no counterpart in the
+ * reference, which has no kd-tree: bvh.cc:13-110 is a stub). The host build's
+ * binned SAH with triangle clipping, run level by level on the device: the
+ * tree is the one xt_scene_build makes on one thread (params->threads = 1),
+ * node for node and in the same depth-first order. It replaces the scene's
+ * kd-tree (node order as the depth-first build, whatever xt_tuning.kd_layout
+ * says). XT_E_STATE while served jobs use the scene. params may be NULL
+ * (defaults); threads and kd_on_device are not read. */
+typedef struct xt_kd_build_stats {
+    double ms;                /* device time of the build (HIP events), 0 for a host-built tree */
+    uint32_t n_nodes, n_refs; /* of the tree the scene holds */
+    uint32_t depth, leaves, levels;
+    uint32_t pad;
+} xt_kd_build_stats;
+int xt_scene_build_kd_device(xt_dev *d, uint32_t scene_id, const xt_build_params *params, xt_kd_build_stats *stats);
+/* The resident kd-tree: n_nodes * 2 words, n_refs references, the root box
+ * (lo xyz, hi xyz), and the stats (sizes first: NULL arrays copy nothing). */
+int xt_scene_get_kd_device(xt_dev *d, uint32_t scene_id, uint32_t *nodes, uint32_t *refs, float bounds[6],
+                           xt_kd_build_stats *stats);
 
 /* Per-mesh reference octree statistics (node count, deepest level). */
 int xt_scene_mesh_octree(const xt_scene *s, uint32_t k, uint64_t *geometry, uint64_t *nodes, uint64_t *max_level,

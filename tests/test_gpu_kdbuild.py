@@ -1,0 +1,178 @@
+"""Device kd-tree build (xt_scene_build_kd_device, and xt_scene_upload of a
+scene built with kd_on_device): the tree equals the host build's on one
+thread node for node (splits, leaf references in order, depth-first
+numbering, root box), with default and other build parameters, on the
+benchmark scenes, flat and degenerate geometry, a one-leaf and an empty
+soup; traces and renders on it equal those on the host-built tree; a scene
+in use by served jobs is not rebuilt."""
+import os
+
+import numpy as np
+import pytest
+
+import xtracer_amd as xa
+from xtracer_amd import scenes
+from tests import scn
+from tests.helpers import camera_rays, mixed_rays, scene_path
+
+
+def flat_scene(tmp_path):
+    """A flat 24 x 24 grid (zero extent in y), a stack of coplanar copies of
+    one triangle, a degenerate (zero-area) triangle and two spheres."""
+    d = str(tmp_path)
+    n = 25
+    xs = np.linspace(-2.0, 2.0, n)
+    X, Z = np.meshgrid(xs, xs)
+    verts = np.stack([X.ravel(), np.zeros(X.size), Z.ravel()], 1)
+    i = np.arange(n - 1)
+    I, J = np.meshgrid(i, i)
+    a = (J * n + I).ravel()
+    faces = np.concatenate([np.stack([a, a + n, a + 1], 1), np.stack([a + 1, a + n, a + n + 1], 1)])
+    scenes.write_obj(os.path.join(d, "grid.obj"), verts, faces)
+    geometry = {"grid": dict(type="mesh", source="grid.obj"),
+                "s0": dict(type="sphere", position=(0.5, 0.5, 0.5), radius=0.25),
+                "s1": dict(type="sphere", position=(-0.5, 0.3, 0.2), radius=0.0),
+                "deg": dict(type="triangle", v=[(0.0, 1.0, 0.0), (1.0, 1.0, 0.0), (2.0, 1.0, 0.0)])}
+    objects = {"grid": ("grid", "m"), "s0": ("s0", "m"), "s1": ("s1", "m"), "deg": ("deg", "m")}
+    for k in range(6):
+        geometry[f"t{k}"] = dict(type="triangle", v=[(-1.0, 0.5, -1.0), (1.0, 0.5, -1.0), (0.0, 0.5, 1.0)])
+        objects[f"t{k}"] = (f"t{k}", "m")
+    return scn.write(os.path.join(d, "flat.scn"), geometry, objects, {"m": dict(type="lambert", kd=(0.5, 0.5, 0.5))},
+                     camera=dict(position=(0.0, 3.0, -4.0), target=(0.0, 0.0, 0.0), fov=50.0))
+
+
+def one_leaf_scene(tmp_path):
+    return scn.write(os.path.join(str(tmp_path), "one.scn"),
+                     {"t": dict(type="triangle", v=[(-1.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)]),
+                      "p": dict(type="plane", normal=(0, 1, 0), distance=-1.0)},
+                     {"t": ("t", "m"), "p": ("p", "m")}, {"m": dict(type="lambert", kd=(0.5, 0.5, 0.5))})
+
+
+def empty_soup_scene(tmp_path):
+    return scn.write(os.path.join(str(tmp_path), "empty.scn"), {"p": dict(type="plane", normal=(0, 1, 0), distance=0.0)},
+                     {"p": ("p", "m")}, {"m": dict(type="lambert", kd=(0.5, 0.5, 0.5))})
+
+
+def _same_tree(device, sid, host, label):
+    hn, hr, hb = host.kdtree()
+    dn, dr, db, st = device.kdtree(sid)
+    assert st["n_nodes"] == len(hn) and st["n_refs"] == len(hr), f"{label}: sizes {st} vs {len(hn)} / {len(hr)}"
+    assert np.array_equal(dn, hn), f"{label}: first differing node {np.argwhere((dn != hn).any(axis=1))[:3].ravel()}"
+    assert np.array_equal(dr, hr), f"{label}: first differing reference {np.flatnonzero(dr != hr)[:3]}"
+    assert np.array_equal(db.view(np.uint32), hb.view(np.uint32)), f"{label}: bounds {db} vs {hb}"
+    assert st["depth"] == host.info().kd_depth
+    return st
+
+
+SCENES = [("spheres", {}), ("icosphere", {"subdiv": 5}), ("strands", {"strands": 600}), ("terrain", {"n": 61})]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,kw", SCENES)
+def test_device_tree_equals_host_tree(device, kind, kw):
+    path = scene_path(kind, **kw)
+    host = xa.Scene(path, threads=1)
+    sid = device.upload(xa.Scene(path))
+    st = device.build_kd(sid)
+    assert st["ms"] > 0.0 and st["levels"] >= st["depth"] + 1
+    _same_tree(device, sid, host, kind)
+    device.release(sid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("params", [dict(kd_leaf_prims=2), dict(kd_leaf_prims=9, kd_cost_intersect=3.0),
+                                    dict(kd_max_depth=9), dict(kd_cost_intersect=40.0)])
+def test_device_tree_equals_host_tree_other_parameters(device, params):
+    path = scene_path("icosphere", subdiv=4)
+    host = xa.Scene(path, threads=1, **params)
+    sid = device.upload(xa.Scene(path))
+    device.build_kd(sid, **params)
+    _same_tree(device, sid, host, str(params))
+    device.release(sid)
+
+
+@pytest.mark.gpu
+def test_flat_and_degenerate_geometry(device, tmp_path):
+    path = flat_scene(tmp_path)
+    host = xa.Scene(path, threads=1)
+    sid = device.upload(xa.Scene(path))
+    device.build_kd(sid)
+    _same_tree(device, sid, host, "flat")
+    device.release(sid)
+
+
+@pytest.mark.gpu
+def test_one_leaf_and_empty_soup(device, tmp_path):
+    for path, n_prims in ((one_leaf_scene(tmp_path), 1), (empty_soup_scene(tmp_path), 0)):
+        host = xa.Scene(path, threads=1)
+        assert host.info().n_prims == n_prims
+        sid = device.upload(host)
+        st = device.build_kd(sid)
+        assert st["n_nodes"] == 1 and st["leaves"] == 1 and st["depth"] == 0
+        _same_tree(device, sid, host, os.path.basename(path))
+        o, d = mixed_rays(None, 4096, seed=5)
+        t, prim, _, _, _ = device.trace(sid, o, d, xa.ACCEL_KDTREE)
+        rid = device.upload(host)
+        ref = device.trace(rid, o, d, xa.ACCEL_KDTREE)
+        assert np.array_equal(prim, ref[1]) and np.array_equal(t.view(np.uint32), ref[0].view(np.uint32))
+        device.release(sid)
+        device.release(rid)
+
+
+@pytest.mark.gpu
+def test_kd_on_device_upload_renders_as_host_built(device):
+    """A scene built with kd_on_device has no host kd-tree; its upload builds
+    it on the device, and renders equal those of the host-built scene."""
+    path = scene_path("icosphere", subdiv=5)
+    lazy = xa.Scene(path, kd_on_device=True)
+    assert lazy.info().n_kd_nodes == 0
+    eager = xa.Scene(path)
+    a, b = device.upload(lazy), device.upload(eager)
+    _same_tree(device, a, xa.Scene(path, threads=1), "upload")
+    o, d = camera_rays(eager.camera(), 96, 64)
+    o2, d2 = mixed_rays(None, 1 << 14, seed=9)
+    o = np.concatenate([o.astype(np.float32), o2])
+    d = np.concatenate([d.astype(np.float32), d2])
+    ta = device.trace(a, o, d, xa.ACCEL_KDTREE)
+    tb = device.trace(b, o, d, xa.ACCEL_KDTREE)
+    for x, y in zip(ta[:4], tb[:4]):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    for integ in (xa.DEPTH, xa.PATHTRACER):
+        ia, sa = device.render(xa.make_job(a, integ, 64, 48, eager.camera(), samples=4, rdepth=4, seed=3))
+        ib, sb = device.render(xa.make_job(b, integ, 64, 48, eager.camera(), samples=4, rdepth=4, seed=3))
+        assert np.array_equal(ia.view(np.uint32), ib.view(np.uint32)) and sa.queries == sb.queries
+    device.release(a)
+    device.release(b)
+
+
+@pytest.mark.gpu
+def test_benchmark_scene_full_size(device):
+    """The 1.31 M-triangle benchmark scene: the device tree is the host's
+    one-thread tree, and the benchmark frame renders to the same bits."""
+    path = scenes.icosphere_scene(seed=0, subdiv=8)
+    host = xa.Scene(path, threads=1)
+    sid = device.upload(xa.Scene(path, kd_on_device=True))
+    st = _same_tree(device, sid, host, "icosphere 1.31M")
+    assert st["ms"] > 0.0
+    ref = device.upload(host)
+    job = lambda s: xa.make_job(s, xa.PATHTRACER, 480, 270, host.camera(), samples=2, rdepth=8, seed=11)
+    ia, sa = device.render(job(sid))
+    ib, sb = device.render(job(ref))
+    assert np.array_equal(ia.view(np.uint32), ib.view(np.uint32)) and sa.queries == sb.queries
+    device.release(sid)
+    device.release(ref)
+
+
+@pytest.mark.gpu
+def test_rebuild_refused_while_served(device):
+    s = xa.Scene(scene_path("spheres"))
+    sid = device.upload(s)
+    t = device.submit(xa.make_job(sid, xa.AO, 32, 32, s.camera(), samples=2))
+    with pytest.raises(xa.XtError) as e:
+        device.build_kd(sid)
+    assert "served" in str(e.value)
+    device.drain()
+    done, _, _ = device.poll(t, 32, 32)
+    assert done
+    device.build_kd(sid)  # idle again: rebuilt
+    device.release(sid)

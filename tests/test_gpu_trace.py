@@ -12,11 +12,15 @@ SCENES = [("spheres", {}), ("icosphere", {"subdiv": 4}), ("strands", {"strands":
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("accel", [xa.ACCEL_KDTREE, xa.ACCEL_OCTREE])
+@pytest.mark.parametrize("accel,kd_build", [(xa.ACCEL_KDTREE, "host"), (xa.ACCEL_OCTREE, "host"),
+                                            (xa.ACCEL_KDTREE, "device")])
 @pytest.mark.parametrize("kind,kw", SCENES)
-def test_hit_parity(device, kind, kw, accel):
+def test_hit_parity(device, kind, kw, accel, kd_build):
+    """kd_build "device": the kd-tree rebuilt on the device."""
     s = xa.Scene(scene_path(kind, **kw))
     sid = device.upload(s)
+    if kd_build == "device":
+        device.build_kd(sid)
     arr = s.arrays()
     orc = OracleScene(s)
     o1, d1 = camera_rays(s.camera(), 160, 120)

@@ -32,7 +32,7 @@ CAMERA_THIN_LENS, CAMERA_ODS, CAMERA_ERP, CAMERA_CUBEMAP = 0, 1, 2, 3
 SYMBOLS = [
     "xt_scene_load", "xt_scene_build", "xt_scene_free", "xt_scene_error", "xt_scene_get_info",
     "xt_scene_get_object", "xt_scene_get_arrays", "xt_scene_get_object_params", "xt_scene_get_vnormals", "xt_scene_get_texcoords", "xt_scene_get_samplers",
-    "xt_scene_get_kdtree", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
+    "xt_scene_get_kdtree", "xt_scene_build_kd_device", "xt_scene_get_kd_device", "xt_scene_mesh_octree", "xt_scene_camera", "xt_camera_make",
     "xt_image_write", "xt_image_read", "xt_render_raygraph", "xt_dev_create", "xt_dev_destroy", "xt_last_error", "xt_dev_set_tuning", "xt_dev_get_tuning",
     "xt_scene_upload", "xt_scene_release", "xt_trace",
     "xt_trace_device", "xt_trace_device_static", "xt_shade", "xt_render", "xt_render_device", "xt_submit", "xt_poll", "xt_serve_step", "xt_drain",
@@ -48,7 +48,15 @@ class XtError(RuntimeError):
 
 class BuildParams(ctypes.Structure):
     _fields_ = [("kd_max_depth", ctypes.c_int), ("kd_leaf_prims", ctypes.c_int),
-                ("kd_cost_intersect", ctypes.c_float), ("threads", ctypes.c_int)]
+                ("kd_cost_intersect", ctypes.c_float), ("threads", ctypes.c_int), ("kd_on_device", ctypes.c_int)]
+
+
+class KdBuildStats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double)] + [(n, ctypes.c_uint32) for n in (
+        "n_nodes", "n_refs", "depth", "leaves", "levels", "pad")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
 
 
 class SceneInfo(ctypes.Structure):
@@ -157,6 +165,8 @@ def lib():
         "xt_scene_get_texcoords": (I, [P, P, P]),
         "xt_scene_get_samplers": (I, [P, P, P, P, P, P, P]),
         "xt_scene_get_kdtree": (I, [P, P, P, P]),
+        "xt_scene_build_kd_device": (I, [P, U32, ctypes.POINTER(BuildParams), ctypes.POINTER(KdBuildStats)]),
+        "xt_scene_get_kd_device": (I, [P, U32, P, P, P, ctypes.POINTER(KdBuildStats)]),
         "xt_scene_mesh_octree": (I, [P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64),
                                      ctypes.POINTER(U64)]),
         "xt_scene_camera": (I, [P, ctypes.c_char_p, ctypes.POINTER(Camera)]),
@@ -216,8 +226,10 @@ class Scene:
         if build:
             self.build(**build_params)
 
-    def build(self, kd_max_depth=0, kd_leaf_prims=0, kd_cost_intersect=0.0, threads=0):
-        p = BuildParams(kd_max_depth, kd_leaf_prims, kd_cost_intersect, threads)
+    def build(self, kd_max_depth=0, kd_leaf_prims=0, kd_cost_intersect=0.0, threads=0, kd_on_device=False):
+        """kd_on_device: leave the kd-tree to the device, which builds it when
+        the scene is uploaded."""
+        p = BuildParams(kd_max_depth, kd_leaf_prims, kd_cost_intersect, threads, int(bool(kd_on_device)))
         rc = lib().xt_scene_build(self._h, ctypes.byref(p))
         if rc != XT_OK:
             raise XtError(rc, lib().xt_scene_error(self._h).decode())
@@ -422,6 +434,25 @@ class Device:
 
     def release(self, scene_id):
         self._check(lib().xt_scene_release(self._d, scene_id))
+
+    def build_kd(self, scene_id, kd_max_depth=0, kd_leaf_prims=0, kd_cost_intersect=0.0):
+        """Rebuilds the resident scene's kd-tree on the device (the host
+        build's tree, node for node); returns the build's stats."""
+        p = BuildParams(kd_max_depth, kd_leaf_prims, kd_cost_intersect, 0, 0)
+        st = KdBuildStats()
+        self._check(lib().xt_scene_build_kd_device(self._d, scene_id, ctypes.byref(p), ctypes.byref(st)))
+        return st.as_dict()
+
+    def kdtree(self, scene_id):
+        """The resident kd-tree: nodes (n, 2) uint32, refs, bounds (6,), stats."""
+        st = KdBuildStats()
+        self._check(lib().xt_scene_get_kd_device(self._d, scene_id, None, None, None, ctypes.byref(st)))
+        nodes = np.zeros((max(st.n_nodes, 1), 2), np.uint32)
+        refs = np.zeros(max(st.n_refs, 1), np.uint32)
+        bounds = np.zeros(6, np.float32)
+        self._check(lib().xt_scene_get_kd_device(self._d, scene_id, _ptr(nodes), _ptr(refs), _ptr(bounds),
+                                                 ctypes.byref(st)))
+        return nodes[:st.n_nodes], refs[:st.n_refs], bounds, st.as_dict()
 
     def trace(self, scene_id, origins, directions, accel=ACCEL_KDTREE):
         o = np.ascontiguousarray(origins, np.float32)

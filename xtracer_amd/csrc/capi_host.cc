@@ -44,6 +44,7 @@ int xt_scene_build(xt_scene *s, const xt_build_params *p) {
         if (p->kd_leaf_prims > 0) kp.leaf_prims = p->kd_leaf_prims;
         if (p->kd_cost_intersect > 0.f) kp.cost_intersect = p->kd_cost_intersect;
         if (p->threads > 0) kp.threads = p->threads;
+        kp.on_device = p->kd_on_device == 1;
     }
     try {
         std::string err;

@@ -306,7 +306,10 @@ int flatten_scene(const Scene &sc, FlatScene &out, const KdParams &kdp, std::str
     std::vector<uint8_t> is_tri(np);
     for (size_t i = 0; i < np; ++i) is_tri[i] = XT_PRIM_TYPE(out.prims[i].tf) == XT_PRIM_TRIANGLE;
     static_assert(sizeof(XtPrim64) == 9 * sizeof(double), "XtPrim64 is v0 v1 v2 as 9 doubles");
-    kdtree_build(boxes32, np ? (const double *)out.prims64.data() : nullptr, is_tri.data(), kdp, out.kd);
+    out.kd_deferred = kdp.on_device;
+    out.kd_params = kdp;
+    if (!kdp.on_device)
+        kdtree_build(boxes32, np ? (const double *)out.prims64.data() : nullptr, is_tri.data(), kdp, out.kd);
     auto t3 = std::chrono::steady_clock::now();
     out.build_seconds_octree = std::chrono::duration<double>(t1 - t0).count() + std::chrono::duration<double>(t2 - t1).count();
     out.build_seconds_kd = std::chrono::duration<double>(t3 - t2).count();

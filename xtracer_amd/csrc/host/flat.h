@@ -55,6 +55,8 @@ struct FlatScene {
     std::vector<MeshOctreeInfo> mesh_octrees;
 
     double build_seconds_octree = 0, build_seconds_kd = 0;
+    bool kd_deferred = false;  // the kd-tree is built on the device when the scene is uploaded
+    KdParams kd_params;        // its parameters
 };
 
 // Builds the flat form. `kd` parameters control the kd-tree; the octree uses

@@ -26,6 +26,7 @@ struct KdParams {
     float empty_bonus = 0.2f;  // cost reduction when one side is empty
     int bins = 64;
     int threads = 0;           // 0: hardware concurrency
+    bool on_device = false;    // leave the tree to the device build at upload
 };
 
 struct KdTree {

@@ -16,6 +16,9 @@ struct DevScene {
     SceneView view;
     std::vector<void *> allocs;
     uint64_t bytes = 0;
+    uint64_t kd_bytes = 0;            // of the kd nodes, references and leaf copies (in `allocs`)
+    uint32_t n_kd_nodes = 0, n_kd_refs = 0;
+    xt_kd_build_stats kd_stats{};     // of the last device build (zero for a host-built tree)
     bool pt_supported = true;
     bool materials_supported = true;
     bool missing_material = false;
@@ -111,6 +114,9 @@ struct TraceBufs {
 };
 int xt_launch_trace_bufs(xt_dev *d, const DevScene &ds, int accel, const TraceBufs &B, const double *ray64,
                          uint32_t *flags, bool time_it, hipStream_t st);
+// Builds the kd-tree of a resident scene on the device (xt_kdbuild.hip) and
+// replaces the scene's kd arrays with it.
+int xt_kd_build_into(xt_dev *d, DevScene &ds, const xt_build_params *p);
 void xt_serve_shutdown(xt_dev *d);
 bool xt_serve_uses_scene(const xt_dev *d, uint32_t scene_id);
 void xt_serve_forget_scene(xt_dev *d, uint32_t scene_id);

@@ -293,8 +293,9 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     }
     // kd leaves' primitives copied in leaf order, the object word replaced
     // by the soup index, so the FP32 traversal reads a leaf's records
-    // without going through the reference list
-    std::vector<XtPrim> leaf(f.kd.refs.size());
+    // without going through the reference list (a tree left to the device
+    // build is made after the other arrays are resident)
+    std::vector<XtPrim> leaf(f.kd_deferred ? 0 : f.kd.refs.size());
     for (size_t r = 0; r < f.kd.refs.size(); ++r) {
         leaf[r] = f.prims[f.kd.refs[r]];
         leaf[r].object = f.kd.refs[r];
@@ -319,9 +320,16 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     int rc = upload(d, &prims, f.prims, t, b);
     rc = rc ? rc : upload(d, &p64, f.prims64, t, b);
     rc = rc ? rc : upload(d, &planes, f.planes, t, b);
-    rc = rc ? rc : upload(d, &kd, d->tuning.kd_layout == 2 ? xt::kd_line_layout(f.kd.nodes) : f.kd.nodes, t, b);
-    rc = rc ? rc : upload(d, &kdr, f.kd.refs, t, b);
-    rc = rc ? rc : upload(d, &kdl, leaf, t, b);
+    if (!f.kd_deferred) {
+        const std::vector<XtKdNode> kdn = d->tuning.kd_layout == 2 ? xt::kd_line_layout(f.kd.nodes) : f.kd.nodes;
+        rc = rc ? rc : upload(d, &kd, kdn, t, b);
+        rc = rc ? rc : upload(d, &kdr, f.kd.refs, t, b);
+        rc = rc ? rc : upload(d, &kdl, leaf, t, b);
+        const size_t nr = std::max<size_t>(1, f.kd.refs.size());
+        ds.kd_bytes = sizeof(XtKdNode) * std::max<size_t>(1, kdn.size()) + 4 * nr + sizeof(XtPrim) * nr;
+        ds.n_kd_nodes = (uint32_t)std::max<size_t>(1, kdn.size());
+        ds.n_kd_refs = (uint32_t)f.kd.refs.size();
+    }
     rc = rc ? rc : upload(d, &oct, f.oct_nodes, t, b);
     rc = rc ? rc : upload(d, &octr, f.oct_refs, t, b);
     rc = rc ? rc : upload(d, &om, obj_mat, t, b);
@@ -371,11 +379,21 @@ int xt_scene_upload(xt_dev *d, const xt_scene *s, uint32_t *scene_id) {
     v.n_oct = (uint32_t)f.oct_nodes.size();
     v.n_objects = (uint32_t)f.objects.size();
     v.sampled = f.samplers.empty() ? 0u : 1u;
-    if (f.kd.nodes.empty()) {  // empty soup: a root leaf with no refs
+    if (!f.kd_deferred && f.kd.nodes.empty()) {  // empty soup: a root leaf with no refs
         XtKdNode leaf;
         leaf.word = 3u;
         leaf.count = 0;
         XT_TRY(hipMemcpy(kd, &leaf, sizeof leaf, hipMemcpyHostToDevice));
+    }
+    if (f.kd_deferred) {
+        xt_build_params bp{f.kd_params.max_depth, f.kd_params.leaf_prims, f.kd_params.cost_intersect, 0, 0};
+        rc = xt_kd_build_into(d, ds, &bp);
+        if (rc) {
+            (void)hipStreamSynchronize(d->stream);
+            for (void *p : ds.allocs) hipFree(p);
+            d->used -= std::min<uint64_t>(d->used, ds.bytes);
+            return rc;
+        }
     }
     const uint32_t id = d->next_scene++;
     d->scenes[id] = std::move(ds);
