@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--serve-checked", type=int, default=4,
                     help="served requests per arrival rate whose image is compared with the oracle's")
     ap.add_argument("--serve-rates", default="0.5,0.8,0.95", help="arrival rates as fractions of capacity")
+    ap.add_argument("--serve-config", default="",
+                    help="serving engine settings as JSON (xt_serve_config: regen, graph, lane_streams as booleans; "
+                         "rounds_per_step, paths, sample_buffer_mb)")
     ap.add_argument("--serve-log", default="", help="file to append one JSON line per served or plain request to")
     ap.add_argument("--strands", type=int, default=50000, help="strands of the hair scene in the serving mix")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -359,6 +362,8 @@ def serving(dev, xa, scenes, args, ico_sid, ico_scene, frame_mrays):
         return set(small[:n])
 
     policies = ("fcfs", "srpt", "hybrid")
+    if args.serve_config:
+        dev.serve_configure(**json.loads(args.serve_config))
     res = {"scenes": [p[0] for p in pool], "setup_s": round(setup_s, 1), "engine": dev.serve_config(),
            "policies": list(policies), "kd_build": kd_build,
            "kd_build_host_threads": os.cpu_count()}

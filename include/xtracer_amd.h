@@ -420,6 +420,8 @@ typedef struct xt_serve_config {
                                  it holds no job */
     uint32_t sample_buffer_mb;/* a job keeps its samples' radiance (16 B each) up to this size in MiB and is
                                  bit-identical to xt_render; a larger one sums in fixed point (4096) */
+    uint32_t lane_streams;    /* 2: the lanes of a round (trace, re-check, re-test, shading) run concurrently,
+                                 on up to 4 streams (branches of the graph); 1: one after another (2) */
 } xt_serve_config;
 int xt_serve_configure(xt_dev *d, const xt_serve_config *c);
 int xt_serve_get_config(xt_dev *d, xt_serve_config *c);

@@ -101,18 +101,23 @@ def test_served_equals_alone(small_passes, policy, regen):
 
 @pytest.mark.gpu
 def test_regen_images_independent_of_schedule(small_passes):
-    """Graph replay or kernel-by-kernel launches, every policy, other path-slot
-    counts and rounds per step, alone or in the mix: bit-identical images."""
+    """Graph replay or kernel-by-kernel launches, a round's lanes concurrent or
+    one after another, every policy, other path-slot counts and rounds per
+    step, alone or in the mix: bit-identical images."""
     dev = small_passes
     sc = _scenes(dev)
     jobs = [j for j in _mix_jobs(sc) if _regen(j)]
     runs = []
-    for policy, graph, rounds in (("srpt", True, 4), ("fcfs", False, 2), ("hybrid", True, 8), ("fair", False, 6)):
-        dev.set_serve_policy(policy)
-        dev.serve_configure(graph=graph, rounds_per_step=rounds)
-        tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
-        got = _drain_all(dev, tickets)
-        runs.append([img for img, _ in got.values()])
+    try:
+        for policy, graph, rounds, par in (("srpt", True, 4, True), ("fcfs", False, 2, True), ("hybrid", True, 8, False),
+                                           ("fair", False, 6, False)):
+            dev.set_serve_policy(policy)
+            dev.serve_configure(graph=graph, rounds_per_step=rounds, lane_streams=par)
+            tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
+            got = _drain_all(dev, tickets)
+            runs.append([img for img, _ in got.values()])
+    finally:
+        dev.serve_configure(lane_streams=True)
     for r in runs[1:]:
         for a, b in zip(runs[0], r):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
@@ -135,9 +140,9 @@ def test_regen_fixed_point_jobs(small_passes):
         fixed = [j.width * j.height * j.samples * 16 > (1 << 20) for j in jobs]
         assert any(fixed) and not all(fixed)
         runs = []
-        for policy, graph in (("srpt", True), ("fcfs", False)):
+        for policy, graph, par in (("srpt", True, True), ("fcfs", False, False)):
             dev.set_serve_policy(policy)
-            dev.serve_configure(graph=graph)
+            dev.serve_configure(graph=graph, lane_streams=par)
             tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
             runs.append([img for img, _ in _drain_all(dev, tickets).values()])
         for a, b in zip(*runs):
@@ -146,7 +151,7 @@ def test_regen_fixed_point_jobs(small_passes):
             ref, _ = dev.render(j)
             assert _same(img, ref, fx)
     finally:
-        dev.serve_configure(sample_buffer_mb=4096, graph=True)
+        dev.serve_configure(sample_buffer_mb=4096, graph=True, lane_streams=True)
 
 
 @pytest.mark.gpu

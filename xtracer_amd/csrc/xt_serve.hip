@@ -143,6 +143,8 @@ struct RegenHost {
     Queue lq_host[2 * XT_REGEN_MAX_LANES] = {};
     RegenState ls_host[2 * XT_REGEN_MAX_LANES] = {};
     RegenLaneH lanes[XT_REGEN_MAX_LANES];
+    // fork / join events of a round's concurrent lanes, per queue parity
+    hipEvent_t fork_ev[2] = {}, join_ev[2][XT_LANE_STREAMS] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
@@ -169,7 +171,7 @@ struct ServeState {
     double epoch_ms = 0;
     xt_serve_info info{};
     int policy = XT_SERVE_SRPT;
-    xt_serve_config cfg{2, 2, 4, 0, 4096};
+    xt_serve_config cfg{2, 2, 4, 0, 4096, 2};
     RegenHost R;
 };
 
@@ -572,6 +574,15 @@ int regen_init(xt_dev *d, ServeState &S) {
     V.done_shard = (unsigned long long *)(b + off[9]);
     V.lq = R.lq;
     V.ls = R.ls;
+    for (int k = 0; k < XT_LANE_STREAMS; ++k)
+        if (!d->lane_stream[k]) {
+            XT_TRY(hipStreamCreateWithFlags(&d->lane_stream[k], hipStreamNonBlocking));
+            XT_TRY(hipEventCreateWithFlags(&d->lane_ev[k], hipEventDisableTiming));
+        }
+    for (int q = 0; q < 2; ++q) {
+        XT_TRY(hipEventCreateWithFlags(&R.fork_ev[q], hipEventDisableTiming));
+        for (int k = 0; k < XT_LANE_STREAMS; ++k) XT_TRY(hipEventCreateWithFlags(&R.join_ev[q][k], hipEventDisableTiming));
+    }
     XT_TRY(hipHostMalloc((void **)&R.h_job, sizeof(RegenJob) * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_done, 8ull * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_prio, 4ull * (XT_REGEN_MAX_JOBS + 2), hipHostMallocDefault));
@@ -617,6 +628,11 @@ void regen_shutdown(xt_dev *d, ServeState &S) {
     if (R.h_done) (void)hipHostFree(R.h_done);
     if (R.h_prio) (void)hipHostFree(R.h_prio);
     if (R.h_cnt) (void)hipHostFree(R.h_cnt);
+    for (int q = 0; q < 2; ++q) {
+        if (R.fork_ev[q]) (void)hipEventDestroy(R.fork_ev[q]);
+        for (int k = 0; k < XT_LANE_STREAMS; ++k)
+            if (R.join_ev[q][k]) (void)hipEventDestroy(R.join_ev[q][k]);
+    }
     R = RegenHost();
 }
 
@@ -716,12 +732,26 @@ int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
     hipLaunchKernelGGL(k_regen_gen, dim3(gP), dim3(XT_BLOCK), 0, st, V, qi);
     hipLaunchKernelGGL(k_regen_round_start, dim3(1), dim3(1), 0, st, V.ctl);
     XT_LAUNCHED();
-    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+    // lanes share nothing but the retire list, the job table and the done
+    // shards, which they update with atomics whose results do not depend on
+    // their order: with several lanes they run on their own streams, forked
+    // from and joined back into the round's stream
+    int used[XT_REGEN_MAX_LANES], nu = 0;
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l)
+        if (R.lanes[l].used) used[nu++] = l;
+    const bool par = S.cfg.lane_streams == 2 && nu > 1;
+    const int ns = par ? std::min(nu, XT_LANE_STREAMS) : 0;
+    if (par) {
+        XT_TRY(hipEventRecord(R.fork_ev[qi], st));
+        for (int k = 0; k < ns; ++k) XT_TRY(hipStreamWaitEvent(d->lane_stream[k], R.fork_ev[qi], 0));
+    }
+    for (int u = 0; u < nu; ++u) {
+        const int l = used[u];
         RegenLaneH &L = R.lanes[l];
-        if (!L.used) continue;
+        const hipStream_t ls = par ? d->lane_stream[u % ns] : st;
         const DevScene &ds = d->scenes.at(L.scene);
         Counters *cnt = V.cnt + l;
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, cnt,
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ls, cnt,
                            (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
         const Queue in = R.lq_host[qi * XT_REGEN_MAX_LANES + l], out = R.lq_host[(qi ^ 1) * XT_REGEN_MAX_LANES + l];
         TraceBufs B;
@@ -732,24 +762,29 @@ int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
         B.recheck = L.recheck;
         B.cnt = cnt;
         B.cap = R.P;
-        int rc = xt_launch_trace_bufs(d, ds, (int)L.accel, B, nullptr, nullptr, false, st);
+        int rc = xt_launch_trace_bufs(d, ds, (int)L.accel, B, nullptr, nullptr, false, ls);
         if (rc) return rc;
         const unsigned g = grid_for(R.P, XT_BLOCK, 8192);
         const RegenState ist = R.ls_host[qi * XT_REGEN_MAX_LANES + l], ost = R.ls_host[(qi ^ 1) * XT_REGEN_MAX_LANES + l];
         if (L.integrator == XT_INTEGRATOR_AO) {
-            hipLaunchKernelGGL(k_regen_shade_ao, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, V, in, ist, &cnt->n[qi], L.hit,
+            hipLaunchKernelGGL(k_regen_shade_ao, dim3(g), dim3(XT_BLOCK), 0, ls, ds.view, V, in, ist, &cnt->n[qi], L.hit,
                                out, ost, &cnt->n[qi ^ 1]);
         } else if (L.chan) {
-            hipLaunchKernelGGL(k_channels, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, in, &cnt->n[qi], L.hit, L.chan);
+            hipLaunchKernelGGL(k_channels, dim3(g), dim3(XT_BLOCK), 0, ls, ds.view, in, &cnt->n[qi], L.hit, L.chan);
             XT_LAUNCHED();
-            hipLaunchKernelGGL(k_regen_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, V, in, ist, &cnt->n[qi],
+            hipLaunchKernelGGL(k_regen_shade_pt<true>, dim3(g), dim3(XT_BLOCK), 0, ls, ds.view, V, in, ist, &cnt->n[qi],
                                L.hit, out, ost, &cnt->n[qi ^ 1], (const float4 *)L.chan);
         } else {
-            hipLaunchKernelGGL(k_regen_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, st, ds.view, V, in, ist,
+            hipLaunchKernelGGL(k_regen_shade_pt<false>, dim3(g), dim3(XT_BLOCK), 0, ls, ds.view, V, in, ist,
                                &cnt->n[qi], L.hit, out, ost, &cnt->n[qi ^ 1], (const float4 *)nullptr);
         }
         XT_LAUNCHED();
     }
+    if (par)
+        for (int k = 0; k < ns; ++k) {
+            XT_TRY(hipEventRecord(R.join_ev[qi][k], d->lane_stream[k]));
+            XT_TRY(hipStreamWaitEvent(st, R.join_ev[qi][k], 0));
+        }
     hipLaunchKernelGGL(k_regen_retire, dim3(gP), dim3(XT_BLOCK), 0, st, V);
     XT_LAUNCHED();
     return XT_OK;
@@ -1099,7 +1134,7 @@ extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     if (!d || !c) return XT_E_INVALID;
     if (c->regen > 2 || c->graph > 2 || (c->rounds_per_step & 1u) ||
         c->rounds_per_step > 1024 || (c->paths && (c->paths < 4096 || c->paths > (1u << 28))) ||
-        c->sample_buffer_mb > (1u << 20)) {
+        c->sample_buffer_mb > (1u << 20) || c->lane_streams > 2) {
         d->err = "bad serve configuration";
         return XT_E_INVALID;
     }
@@ -1120,6 +1155,10 @@ extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     if (c->rounds_per_step) S.cfg.rounds_per_step = c->rounds_per_step;
     if (c->paths) S.cfg.paths = c->paths;
     if (c->sample_buffer_mb) S.cfg.sample_buffer_mb = c->sample_buffer_mb;
+    if (c->lane_streams && c->lane_streams != S.cfg.lane_streams) {
+        S.cfg.lane_streams = c->lane_streams;
+        S.R.dirty = true;  // the captured rounds follow the lane streams
+    }
     return XT_OK;
 }
 
