@@ -6,6 +6,9 @@ synthetic scenes (regenerated identically anywhere by xtracer_amd.scenes):
   ref_trace_<scene>.npz   FP32 rays -> Scene::intersection t, object index, point
   ref_render_<scene>.npz  stencil and depth framebuffers (pinhole, GRID aa=1)
   ref_mc_<scene>.npz      high-spp ao / pathtracer framebuffers (glibc rand())
+  ref_mc4096_<scene>.npz  4096-spp ao (spheres) / pathtracer (terrain, and the
+                          displaced icosphere with its thin-lens camera, Phong
+                          floor and area lights at rdepth 8) framebuffers
   ref_images.npz          a framebuffer and the bytes of the reference's
                           png / bmp / tga / hdr / exr files of it (nimg::io::save)
   ref_raygraph_spheres.npz  a depth render (33x33, GRID aa 2) and the bytes of
@@ -13,7 +16,7 @@ synthetic scenes (regenerated identically anywhere by xtracer_amd.scenes):
 Every scene is loaded in a fresh process: the reference's string pool is
 process-global, and its handle order decides the scan order.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [mc4096]   (mc4096: those fixtures only)
 """
 import json
 import os
@@ -115,7 +118,40 @@ print("ok raygraph")
 """
 
 
+MC4096 = r"""
+import sys
+import numpy as np
+sys.path.insert(0, {root!r})
+from oracle import oracle as O
+from xtracer_amd import scenes
+path = scenes.SCENES[{kind!r}][0](**{kw})
+r = O.RefScene(path)
+img = r.render({integ}, {w}, {h}, samples=4096, rdepth={rdepth}, threads={threads}, seed=4096)
+np.savez_compressed({out!r} + "/ref_mc4096_{name}.npz", image=img, integrator={integ}, width={w}, height={h},
+                    spp=4096, rdepth={rdepth})
+print("ok mc4096 {name}")
+"""
+
+# name: (scene kind, its parameters, integrator (2 ao, 3 pathtracer), width, height, rdepth)
+MC4096_SCENES = {
+    "spheres": ("spheres", {}, 2, 24, 16, 6),
+    "terrain": ("terrain", {"n": 41}, 3, 24, 16, 6),
+    "ico3": ("icosphere", {"subdiv": 3}, 3, 32, 18, 8),
+}
+
+
+def make_mc4096():
+    threads = os.cpu_count() or 1
+    for name, (kind, kw, integ, w, h, rdepth) in MC4096_SCENES.items():
+        code = MC4096.format(root=ROOT, kind=kind, kw=repr(kw), integ=integ, w=w, h=h, rdepth=rdepth,
+                             threads=threads, out=HERE, name=name)
+        print(subprocess.check_output([sys.executable, "-c", code], timeout=7200).decode().strip())
+
+
 def main():
+    if sys.argv[1:] == ["mc4096"]:
+        make_mc4096()
+        return
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         code = RAYGRAPH.format(root=ROOT, tmp=os.path.join(d, "rg.ply"), out=HERE)

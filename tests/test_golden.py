@@ -109,3 +109,62 @@ def test_device_monte_carlo_vs_reference_fixture(device, name):
     # the spread comes from 8 images: 99% two-sided quantile of Student's t, 7 dof
     inside = np.abs(mean - g["image"][..., :3]) <= 3.499 * se + 1e-4
     assert inside.mean() >= 0.95, f"{inside.mean():.3f} inside the 99% bound"
+
+
+# The 4096-spp reference images (ref_mc4096_*.npz): ao on the spheres,
+# pathtracer on the terrain and on the displaced icosphere with the thin-lens
+# camera, Phong floor and area lights at rdepth 8.
+MC4096 = ["spheres", "terrain", "ico3"]
+
+
+def _sample_var(var, mean, n):
+    """Per-sample variance, where the n samples of a channel were all equal
+    that of an event of probability 3 / n (rule of three: not seen in n
+    samples with 95% confidence) changing the value by its size."""
+    return np.where(var > 0, var, np.maximum(mean ** 2, 1e-6) * 3.0 / n)
+
+
+def _mc_bound(mean, var, n, n_ref, z=2.576):
+    """Per-channel 99% bound of (estimate over n samples) - (reference over
+    n_ref samples), both with per-sample variance var."""
+    v = _sample_var(var, mean, n)
+    return z * np.sqrt(v / n + v / n_ref) + 1e-4
+
+
+@pytest.mark.parametrize("name", MC4096)
+def test_oracle_monte_carlo_vs_4096spp_reference(name):
+    """The oracle at 512 spp (its own per-sample variance) against the
+    reference's 4096-spp image: at least 97% of channels inside the 99%
+    bound, and no channel beyond 3 times it."""
+    g = _load("ref_mc4096", name)
+    s, cam = _scene(name)
+    w, h, n = int(g["width"]), int(g["height"]), 512
+    _, _, mean, m2 = OracleScene(s).render(int(g["integrator"]), w, h, cam, samples=n, rdepth=int(g["rdepth"]),
+                                          seed=77, stats=True)
+    var = m2 / (n - 1)
+    bound = _mc_bound(mean, var, n, int(g["spp"]))
+    dev = np.abs(mean - g["image"][..., :3])
+    assert (dev <= bound).mean() >= 0.97, f"{(dev <= bound).mean():.4f} inside the 99% bound"
+    assert (dev <= 3 * bound).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", MC4096)
+def test_device_monte_carlo_vs_4096spp_reference(device, name):
+    """8 device renders of 128 spp against the reference's 4096-spp image: at
+    least 98% of channels inside the 99% bound (Student's t on the 8 images'
+    spread, 7 degrees of freedom, plus the reference's own variance scaled
+    from it)."""
+    g = _load("ref_mc4096", name)
+    s, _ = _scene(name)
+    sid = device.upload(s)
+    spp = 128
+    imgs = np.stack([device.render(xa.make_job(sid, int(g["integrator"]), int(g["width"]), int(g["height"]), s.camera(),
+                                               samples=spp, rdepth=int(g["rdepth"]), seed=900 + k))[0][..., :3]
+                     for k in range(8)])
+    mean, sd = imgs.mean(0), imgs.std(0, ddof=1)
+    # per-sample variance from the 8 images' spread; Student's t, 7 dof
+    bound = _mc_bound(mean, sd ** 2 * spp, 8 * spp, int(g["spp"]), z=3.499)
+    inside = np.abs(mean - g["image"][..., :3]) <= bound
+    assert inside.mean() >= 0.98, f"{inside.mean():.4f} inside the 99% bound"
+    device.release(sid)
