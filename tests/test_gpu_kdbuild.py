@@ -108,7 +108,7 @@ def test_one_leaf_and_empty_soup(device, tmp_path):
         assert host.info().n_prims == n_prims
         sid = device.upload(host)
         st = device.build_kd(sid)
-        assert st["n_nodes"] == 1 and st["leaves"] == 1 and st["depth"] == 0
+        assert st["n_nodes"] == 2 and st["leaves"] == 1 and st["depth"] == 0  # the root and the padding leaf
         _same_tree(device, sid, host, os.path.basename(path))
         o, d = mixed_rays(None, 4096, seed=5)
         t, prim, _, _, _ = device.trace(sid, o, d, xa.ACCEL_KDTREE)

@@ -203,6 +203,10 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
         uint32_t axis = nd.word & 3u;
         while (axis != 3u) {
             const uint32_t c = nd.word >> 2;
+            // both children in one aligned 16-byte load (pairs start on even
+            // nodes), issued as soon as the node is known, so that its
+            // latency overlaps the plane test below
+            const uint4 pair = *(const uint4 *)(S.kd + c);
             const float split = nd.split;
             const float oo = axis == 0 ? oa[0] : (axis == 1 ? oa[1] : oa[2]);
             const float dd = axis == 0 ? da[0] : (axis == 1 ? da[1] : da[2]);
@@ -223,7 +227,9 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             head = both ? next : head;
             tmax = both ? fminf(tmax, tp * (1.f + eps)) : tmax;
             node = only_second ? second : first;
-            nd = S.kd[node];
+            const bool lower = node == c;
+            nd.word = lower ? pair.x : pair.z;
+            nd.count = lower ? pair.y : pair.w;
             axis = nd.word & 3u;
         }
         const uint32_t first_ref = nd.word >> 2, n = nd.count;

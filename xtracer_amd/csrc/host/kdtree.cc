@@ -234,7 +234,13 @@ void kdtree_build(const std::vector<Box32> &pb, const double *tri_verts, const u
     std::vector<Task> tasks;
     Builder top{tri_verts, is_tri, p, spawn, &tasks};
     Local G;
-    G.nodes.resize(1);
+    // node 1 is an empty leaf that nothing references, so that child pairs
+    // start on even nodes: a pair is one aligned 16-byte record, which the
+    // traversal loads whole (spawned subtrees keep the parity: each adds an
+    // even number of nodes)
+    G.nodes.resize(2);
+    G.nodes[1].word = 3u;
+    G.nodes[1].count = 0;
     top.build(G, 0, root, refs, 0);
 
     std::vector<Local> locals(tasks.size());

@@ -30,7 +30,8 @@ struct KdParams {
 };
 
 struct KdTree {
-    std::vector<XtKdNode> nodes;  // nodes[0] is the root
+    std::vector<XtKdNode> nodes;  // nodes[0] is the root, nodes[1] an unreferenced empty leaf; child pairs
+                                  // start on even nodes (16-byte aligned)
     std::vector<uint32_t> refs;   // leaf primitive references (soup indices)
     Box32 bounds;                 // root box
     int depth = 0;                // deepest leaf level

@@ -188,13 +188,11 @@ int build(xt_dev *d, const SceneView &v, const xt_build_params *bp, KdOut &o) {
             out_free(d, o);
             return rc;
         }
-        XtKdNode leaf;
-        leaf.word = 3u;
-        leaf.count = 0;
-        XT_TRY(hipMemcpyAsync(o.nodes, &leaf, sizeof leaf, hipMemcpyHostToDevice, d->stream));
+        static const XtKdNode leaves[2] = {{3u, {0.f}}, {3u, {0.f}}};  // the root and the padding leaf
+        XT_TRY(hipMemcpyAsync(o.nodes, leaves, sizeof leaves, hipMemcpyHostToDevice, d->stream));
         XT_TRY(hipEventRecord(e1, d->stream));
         XT_TRY(hipStreamSynchronize(d->stream));
-        o.stats.n_nodes = 1;
+        o.stats.n_nodes = 2;
         o.stats.leaves = 1;
         o.stats.levels = 1;
         float ms = 0.f;
@@ -347,11 +345,14 @@ int build(xt_dev *d, const SceneView &v, const xt_build_params *bp, KdOut &o) {
                            (const XtKdNode *)as<XtKdNode>(gnodes), it->first, it->second, total, as<uint32_t>(desc),
                            as<uint32_t>(refc), as<KdbCtl>(ctl));
     XT_LAUNCHED();
-    static const uint32_t root_init[3] = {0u, 1u, 0u};  // pos, pair position, reference position of the root
+    // the root at 0, an unreferenced empty leaf at 1 (child pairs on even
+    // nodes, as the host build lays them out), the root's pair from 2
+    static const uint32_t root_init[3] = {0u, 2u, 0u};  // pos, pair position, reference position of the root
+    static const XtKdNode pad = {3u, {0.f}};
     XT_TRY(hipMemcpyAsync(pos.p, &root_init[0], 4, hipMemcpyHostToDevice, d->stream));
     XT_TRY(hipMemcpyAsync(pairpos.p, &root_init[1], 4, hipMemcpyHostToDevice, d->stream));
     XT_TRY(hipMemcpyAsync(refpos.p, &root_init[2], 4, hipMemcpyHostToDevice, d->stream));
-    o.nodes_bytes = (sizeof(XtKdNode) * (uint64_t)total + 255) & ~255ull;
+    o.nodes_bytes = (sizeof(XtKdNode) * ((uint64_t)total + 1) + 255) & ~255ull;
     o.refs_bytes = (4ull * pool_total + 255) & ~255ull;
     o.leaf_bytes = (48ull * pool_total + 255) & ~255ull;
     rc = out_alloc(d, (void **)&o.nodes, o.nodes_bytes);
@@ -362,12 +363,13 @@ int build(xt_dev *d, const SceneView &v, const xt_build_params *bp, KdOut &o) {
         out_free(d, o);
         return rc;
     }
+    XT_TRY(hipMemcpyAsync(o.nodes + 1, &pad, sizeof pad, hipMemcpyHostToDevice, d->stream));
     for (const auto &lv : levels)
         hipLaunchKernelGGL(k_kdb_place, dim3(grid_for(lv.second, 256, 8192)), dim3(256), 0, d->stream,
                            (const XtKdNode *)as<XtKdNode>(gnodes), lv.first, lv.second,
                            (const uint32_t *)as<uint32_t>(desc), (const uint32_t *)as<uint32_t>(refc),
                            as<uint32_t>(pos), as<uint32_t>(pairpos), as<uint32_t>(refpos),
-                           (const uint32_t *)as<uint32_t>(pool), pool_total, total, o.nodes, total, o.refs,
+                           (const uint32_t *)as<uint32_t>(pool), pool_total, total, o.nodes, total + 1, o.refs,
                            pool_total, as<KdbCtl>(ctl));
     if (pool_total)
         hipLaunchKernelGGL(k_kdb_leaf_copies, dim3(grid_for(pool_total, 256, 16384)), dim3(256), 0, d->stream,
@@ -385,7 +387,7 @@ int build(xt_dev *d, const SceneView &v, const xt_build_params *bp, KdOut &o) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     o.stats.ms = ms;
-    o.stats.n_nodes = total;
+    o.stats.n_nodes = total + 1;
     o.stats.n_refs = pool_total;
     o.stats.depth = depth;
     o.stats.leaves = leaves;

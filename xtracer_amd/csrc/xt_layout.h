@@ -96,7 +96,8 @@ struct XtTexture {
 
 // kd-tree node, 8 bytes.
 //   inner: word = (first_child << 2) | axis, split = plane position;
-//          children are stored as a pair at first_child, first_child + 1.
+//          children are stored as a pair at first_child, first_child + 1,
+//          first_child even (the pair is one aligned 16-byte record).
 //   leaf:  word = (first_ref << 2) | 3, count = number of prim refs.
 struct XtKdNode {
     uint32_t word;
