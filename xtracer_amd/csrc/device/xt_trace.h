@@ -193,13 +193,12 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
     float t_root0 = 0.f, t_root1 = tst.limit();
     if (!slab(o, inv, S.kb_lo, S.kb_hi, t_root0, t_root1, eps)) return true;
     float tmin = t_root0, tmax = t_root1;
-    uint32_t node = 0;
     constexpr int SLOTS = STACK + 1;
-    int head = -1, count = 0;
+    int head = 0, count = 0;  // the ring starts empty at any slot; head stays in [0, SLOTS)
     bool dropped = false;
     const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z}, ia[3] = {inv.x, inv.y, inv.z};
+    XtKdNode nd = S.kd[0];
     for (int steps = 0; steps < XT_MAX_STEPS; ++steps) {
-        XtKdNode nd = S.kd[node];
         uint32_t axis = nd.word & 3u;
         while (axis != 3u) {
             const uint32_t c = nd.word >> 2;
@@ -226,13 +225,16 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             count = both && count < STACK ? count + 1 : count;
             head = both ? next : head;
             tmax = both ? fminf(tmax, tp * (1.f + eps)) : tmax;
-            node = only_second ? second : first;
+            const uint32_t node = only_second ? second : first;
             const bool lower = node == c;
             nd.word = lower ? pair.x : pair.z;
             nd.count = lower ? pair.y : pair.w;
             axis = nd.word & 3u;
         }
         const uint32_t first_ref = nd.word >> 2, n = nd.count;
+        // the node a pop resumes with (with an empty stack, the root that a
+        // restart resumes with), loaded while the leaf is tested
+        const XtKdNode pre = S.kd[count > 0 ? stk[head * stride].x : 0u];
         tst.test_leaf_range(first_ref, n);
         const float lim = tst.limit();
         if (lim <= tmax) return true;
@@ -241,7 +243,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
             // entries were lost: restart from the root beyond the processed range
             tmin = tmax;
             tmax = fminf(t_root1, lim);
-            node = 0;
+            nd = pre;
             dropped = false;
             if (tmin > tmax) return true;
             continue;
@@ -249,7 +251,7 @@ __device__ __forceinline__ bool walk_kd(const SceneView &S, T &tst, const f3 o, 
         const uint2 e = stk[head * stride];
         head = head == 0 ? SLOTS - 1 : head - 1;
         --count;
-        node = e.x;
+        nd = pre;
         tmin = tmax;
         tmax = fminf(__uint_as_float(e.y), lim);
         // the popped range starts beyond the closest hit, and so do all
