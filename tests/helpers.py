@@ -449,3 +449,24 @@ class HipBuffers:
         for p in self.held:
             self.h.hipFree(p)
         self.held = []
+
+
+def same_stream_check(img, ref, mean, m2, spp, min_close, z=4.0, max_far=1e-3):
+    """Device image against the oracle's on the same random streams: at least
+    `min_close` of the pixels within 1e-3 relative, and at most `max_far` of
+    the channels beyond z standard errors of the difference of two
+    independent spp-sample estimates (a path that took another FP32 decision
+    is another sample; with a few samples a pixel whose samples happened to
+    agree can see such a sample far out). The per-sample variance comes from
+    the oracle's own samples (mean, m2); where they were all equal, from an
+    event of probability 3 / spp changing the value by its size (not seen in
+    spp samples with 95% confidence). Returns the fraction of close pixels."""
+    close = np.isclose(img[..., :3], ref[..., :3], rtol=1e-3, atol=1e-4).all(axis=2)
+    var = m2 / max(1, spp - 1)
+    var = np.where(var > 0, var, np.maximum(mean ** 2, 1e-6) * 3.0 / spp)
+    bound = z * np.sqrt(2.0 * var / spp) + 1e-4
+    dev = np.abs(img[..., :3].astype(np.float64) - ref[..., :3])
+    far = dev > bound
+    assert close.mean() >= min_close, f"only {close.mean():.4f} of pixels agree"
+    assert far.mean() <= max_far, f"{far.sum()} channels beyond {z} standard errors, e.g. {np.argwhere(far)[:3].tolist()}"
+    return close.mean()

@@ -15,7 +15,7 @@ import pytest
 import xtracer_amd as xa
 from oracle.oracle import OracleScene
 from xtracer_amd import scenes
-from tests.helpers import scene_path
+from tests.helpers import same_stream_check, scene_path
 
 # served (fixed-point sum) against xt_render (Neumaier FP64 sum): both are
 # exact to far below float resolution, so they agree to a few float ulps
@@ -187,9 +187,8 @@ def test_regen_same_stream_as_oracle(small_passes):
     job = xa.make_job(sid, xa.PATHTRACER, W, H, s.camera(), samples=spp, rdepth=6, seed=11)
     t = dev.submit(job)
     img = _drain_all(dev, {t: (W, H)})[t][0]
-    ref, _ = OracleScene(s).render(xa.PATHTRACER, W, H, cam, samples=spp, rdepth=6, seed=11)
-    close = np.isclose(img[..., :3], ref[..., :3], rtol=1e-3, atol=1e-4).all(axis=2)
-    assert close.mean() >= 0.98, f"only {close.mean():.4f} of pixels agree"
+    ref, _, mean, m2 = OracleScene(s).render(xa.PATHTRACER, W, H, cam, samples=spp, rdepth=6, seed=11, stats=True)
+    same_stream_check(img, ref, mean, m2, spp, 0.995)
 
 
 @pytest.mark.gpu

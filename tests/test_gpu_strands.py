@@ -12,7 +12,7 @@ import pytest
 import xtracer_amd as xa
 from oracle.oracle import OracleScene
 from xtracer_amd import scenes
-from tests.helpers import soup_to_object_local
+from tests.helpers import same_stream_check, soup_to_object_local
 
 CAM = scenes.camera_dict(**scenes.STRANDS_CAMERA)
 
@@ -25,9 +25,8 @@ def test_strands_pathtracer_same_stream(device):
     job = xa.make_job(sid, xa.PATHTRACER, W, H, s.camera(), samples=spp, rdepth=6, seed=21,
                       accum=xa.ACCUM_REFERENCE)
     img, st = device.render(job)
-    ref, nq = OracleScene(s).render(xa.PATHTRACER, W, H, CAM, samples=spp, rdepth=6, seed=21)
-    close = np.isclose(img[..., :3], ref[..., :3], rtol=1e-3, atol=1e-4).all(axis=2)
-    assert close.mean() >= 0.98, f"only {close.mean():.4f} of pixels agree"
+    ref, nq, mean, m2 = OracleScene(s).render(xa.PATHTRACER, W, H, CAM, samples=spp, rdepth=6, seed=21, stats=True)
+    same_stream_check(img, ref, mean, m2, spp, 0.995)
     assert abs(st.queries - nq) <= 0.01 * nq
     assert ref[..., :3].max() > 0.0  # the light and the Blinn-Phong lobes are seen
     device.release(sid)
