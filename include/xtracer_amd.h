@@ -422,6 +422,13 @@ typedef struct xt_serve_config {
                                  bit-identical to xt_render; a larger one sums in fixed point (4096) */
     uint32_t lane_streams;    /* 2: the lanes of a round (trace, re-check, re-test, shading) run concurrently,
                                  on up to 4 streams (branches of the graph); 1: one after another (2) */
+    uint32_t loop;            /* 2: device-fed: the host writes admitted jobs to a request ring in host memory;
+                                 every round the device takes new requests and orders the jobs by the policy,
+                                 and each replay of two rounds resolves the jobs it finishes and posts them
+                                 to a completion ring; xt_serve_step keeps two replays queued and waits on the
+                                 device only when nothing else moved. 1: host-stepped: each step sets the
+                                 priorities, replays the rounds and synchronises (2); set while the engine
+                                 holds no job */
 } xt_serve_config;
 int xt_serve_configure(xt_dev *d, const xt_serve_config *c);
 int xt_serve_get_config(xt_dev *d, xt_serve_config *c);

@@ -33,7 +33,7 @@ def small_passes(device):
     device.drain()
     device.set_tuning(wave_paths=16 << 20)
     device.set_serve_policy("srpt")
-    device.serve_configure(regen=True, graph=True, rounds_per_step=4, paths=16 << 20)
+    device.serve_configure(regen=True, graph=True, rounds_per_step=4, paths=16 << 20, loop=True)
 
 
 def _scenes(device):
@@ -82,11 +82,12 @@ def _mix_jobs(sc):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("policy", ["srpt", "fair", "fcfs", "hybrid"])
-@pytest.mark.parametrize("regen", [True, False])
-def test_served_equals_alone(small_passes, policy, regen):
+@pytest.mark.parametrize("regen,loop", [(True, True), (True, False), (False, True)])
+def test_served_equals_alone(small_passes, policy, regen, loop):
+    """loop: device-fed (True) or host-stepped (False) regeneration engine."""
     dev = small_passes
     dev.set_serve_policy(policy)
-    dev.serve_configure(regen=regen)
+    dev.serve_configure(regen=regen, loop=loop)
     sc = _scenes(dev)
     jobs = _mix_jobs(sc)
     tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
@@ -285,3 +286,68 @@ def test_scene_in_use(device):
     device.drain()
     assert device.poll(t, 32, 32)[0]
     device.release(sid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", ["srpt", "fair", "fcfs", "hybrid"])
+@pytest.mark.parametrize("graph", [False])
+def test_device_fed_equals_alone(small_passes, policy, graph):
+    """Device-fed serving (jobs reach the device through a request ring in
+    host memory, the device orders them by the policy, resolves finished
+    ones and posts them to a completion ring): every job of the mix equals
+    its one-job render bit for bit, the deterministic integrators running in
+    whole passes beside them."""
+    dev = small_passes
+    dev.set_serve_policy(policy)
+    dev.serve_configure(loop=True, graph=graph)
+    try:
+        assert dev.serve_config()["loop"] == 2
+        sc = _scenes(dev)
+        jobs = _mix_jobs(sc)
+        tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
+        got = _drain_all(dev, tickets)
+        assert dev.serve_info()["jobs_active"] == 0
+        for (t, (img, st)), j in zip(got.items(), jobs):
+            ref, _ = dev.render(j)
+            assert _same(img, ref), f"job {t} ({j.integrator}) differs from its one-job render"
+            assert st.latency_ms >= st.queue_ms >= 0.0
+            assert st.samples == j.samples * j.width * j.height
+    finally:
+        dev.serve_configure(loop=True, graph=True)
+
+
+@pytest.mark.gpu
+def test_device_fed_arrivals_and_fixed_point(small_passes):
+    """Jobs submitted one per step while earlier ones are on the device, the
+    larger ones over a 1 MiB sample-buffer limit (fixed-point sums), three
+    rounds of the mix so that table entries are reused: every image is the
+    host-stepped engine's bit for bit."""
+    dev = small_passes
+    dev.serve_configure(sample_buffer_mb=1, loop=False)
+    try:
+        sc = _scenes(dev)
+        base = [j for j in _mix_jobs(sc) if _regen(j)]
+        jobs = [type(j).from_buffer_copy(j) for _ in range(3) for j in base]
+        for k, j in enumerate(jobs):
+            j.job_key = 100 + k
+        tickets = {dev.submit(j): (j.width, j.height) for j in jobs}
+        ref = [img for img, _ in _drain_all(dev, tickets).values()]
+        dev.serve_configure(loop=True)
+        pending, got, k, steps = {}, {}, 0, 0
+        while k < len(jobs) or pending:
+            if k < len(jobs):
+                pending[dev.submit(jobs[k])] = k
+                k += 1
+            dev.serve_step()
+            steps += 1
+            for t in list(pending):
+                j = jobs[pending[t]]
+                done, img, _ = dev.poll(t, j.width, j.height)
+                if done:
+                    got[pending.pop(t)] = img
+            assert steps < 20000
+        for i, r in enumerate(ref):
+            assert np.array_equal(got[i].view(np.uint32), r.view(np.uint32)), f"job {i}"
+        assert dev.serve_info()["jobs_active"] == 0
+    finally:
+        dev.serve_configure(loop=True, sample_buffer_mb=4096)

@@ -88,7 +88,7 @@ class Tuning(ctypes.Structure):
 
 class ServeConfig(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("regen", "graph", "rounds_per_step", "paths", "sample_buffer_mb",
-                                                "lane_streams")]
+                                                "lane_streams", "loop")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -535,13 +535,14 @@ class Device:
         self._check(lib().xt_serve_set_policy(self._d, int(policy)))
 
     def serve_configure(self, regen=None, graph=None, rounds_per_step=0, paths=0, sample_buffer_mb=0,
-                        lane_streams=None):
-        """regen / graph / lane_streams: True or False (None keeps the value);
-        the counts: 0 keeps the value. Returns the configuration in force."""
+                        lane_streams=None, loop=None):
+        """regen / graph / lane_streams / loop (device-fed serving): True or
+        False (None keeps the value); the counts: 0 keeps the value. Returns
+        the configuration in force."""
         def flag(v):
             return 0 if v is None else (2 if v else 1)
         c = ServeConfig(flag(regen), flag(graph), int(rounds_per_step), int(paths), int(sample_buffer_mb),
-                        flag(lane_streams))
+                        flag(lane_streams), flag(loop))
         self._check(lib().xt_serve_configure(self._d, ctypes.byref(c)))
         return self.serve_config()
 

@@ -45,7 +45,9 @@ struct RegenJob {
     float4 *sbuf;              // radiance per (pixel, sample): pixel * samples + sample; or null
     long long *acc;            // without sbuf: 3 fixed-point sums per pixel
     uint32_t lane;
-    uint32_t pad;
+    uint32_t state;            // device-fed serving: 0 free, 1 active, 2 finished (being resolved)
+    float4 *out;               // device-fed serving: the job's resolved image
+    unsigned long long seq;    // device-fed serving: admission order
 };
 
 // A job's share of one round: `count` slots from slot index `slot0`, new rays
@@ -64,6 +66,33 @@ struct RegenCtl {
     uint32_t n_seg, n_new;
     uint32_t retire_n;
     uint32_t prio[XT_REGEN_MAX_JOBS];
+    // device-fed serving
+    uint32_t policy;    // XT_SERVE_*: how k_regen_intake orders the jobs
+    uint32_t req_tail;  // requests taken from the ring
+    uint32_t fin_head;  // completions posted to the ring
+    uint32_t n_fin;     // jobs finished in this replay
+    uint32_t fin[XT_REGEN_MAX_JOBS];
+};
+
+// Device-fed serving: the host admits a job (buffers, table entry, lane) and
+// writes it to the request ring; the device takes new requests at the start
+// of every round, orders the jobs itself, resolves finished jobs in the
+// replay that finishes them and posts them to the completion ring. Both
+// rings sit in fine-grained host memory; the heads are published with
+// system-scope release stores and read with acquire loads. The host never
+// waits on the device to start, order or finish a job.
+#define XT_REGEN_RING XT_REGEN_MAX_JOBS
+struct RegenReq {
+    RegenJob job;
+    uint32_t entry, pad[3];
+};
+struct RegenRing {
+    uint32_t req_head;  // requests written (host)
+    uint32_t pad0[31];
+    uint32_t fin_head;  // completions written (device)
+    uint32_t pad1[31];
+    RegenReq req[XT_REGEN_RING];
+    uint32_t fin[XT_REGEN_RING];
 };
 
 // Path state next to a lane queue's rays: s0 = (T rgb, random key),
@@ -417,34 +446,152 @@ static __global__ void k_regen_job_reset(RegenView V, uint32_t e) {
 // Resolve of a finished job from its sample buffer: per pixel, the samples
 // in order with k_accumulate's compensated FP64 sum and k_resolve's rounding,
 // so the image is the one-job render's bit for bit.
+__device__ __forceinline__ float4 regen_resolve_ordered_px(const JobDev &j, const float4 *sbuf, uint32_t p) {
+    double2 acc[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    const float4 *sp = sbuf + (size_t)p * j.total;
+    for (uint32_t s = 0; s < j.total; ++s) {
+        const float4 v = sp[s];
+        neumaier_add(acc[0], weighted(v.x, j.weight));
+        neumaier_add(acc[1], weighted(v.y, j.weight));
+        neumaier_add(acc[2], weighted(v.z, j.weight));
+    }
+    return make_float4(fmaxf((float)(acc[0].x + acc[0].y), 0.f), fmaxf((float)(acc[1].x + acc[1].y), 0.f),
+                       fmaxf((float)(acc[2].x + acc[2].y), 0.f), 1.f);
+}
+
+__device__ __forceinline__ float4 regen_resolve_fixed_px(const JobDev &j, const long long *acc, uint32_t p) {
+    float c[3];
+    for (int k = 0; k < 3; ++k) c[k] = fmaxf((float)((double)acc[3 * (size_t)p + k] / XT_REGEN_FIX * j.weight), 0.f);
+    return make_float4(c[0], c[1], c[2], 1.f);
+}
+
 static __global__ __launch_bounds__(XT_BLOCK) void k_regen_resolve_ordered(const JobDev j, const float4 *sbuf,
                                                                     float4 *out) {
     const uint32_t n = j.W * j.H;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-        double2 acc[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-        const float4 *sp = sbuf + (size_t)p * j.total;
-        for (uint32_t s = 0; s < j.total; ++s) {
-            const float4 v = sp[s];
-            neumaier_add(acc[0], weighted(v.x, j.weight));
-            neumaier_add(acc[1], weighted(v.y, j.weight));
-            neumaier_add(acc[2], weighted(v.z, j.weight));
-        }
-        out[p] = make_float4(fmaxf((float)(acc[0].x + acc[0].y), 0.f), fmaxf((float)(acc[1].x + acc[1].y), 0.f),
-                             fmaxf((float)(acc[2].x + acc[2].y), 0.f), 1.f);
-    }
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) out[p] = regen_resolve_ordered_px(j, sbuf, p);
 }
 
 // Resolve of a finished job: fixed-point sums times the per-sample weight.
 static __global__ __launch_bounds__(XT_BLOCK) void k_regen_resolve(const JobDev j, const long long *acc, float4 *out) {
     const uint32_t n = j.W * j.H;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-        float c[3];
-        for (int k = 0; k < 3; ++k)
-            c[k] = fmaxf((float)((double)acc[3 * (size_t)p + k] / XT_REGEN_FIX * j.weight), 0.f);
-        out[p] = make_float4(c[0], c[1], c[2], 1.f);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) out[p] = regen_resolve_fixed_px(j, acc, p);
+}
+
+// Device-fed serving, start of a round (one block of 256): new requests from
+// the ring into the job table (their done counts zeroed), then every active
+// job with slots left ordered by the policy into ctl->prio for k_regen_plan.
+// SRPT: fewest slots left; FCFS and FAIR: admission order (FAIR also caps a
+// job's slots per round at P / active jobs); HYBRID: jobs with at most P / 2
+// slots left by that count, then the rest in admission order.
+static __global__ __launch_bounds__(256) void k_regen_intake(RegenView V, const RegenRing *ring) {
+    __shared__ unsigned long long key[XT_REGEN_MAX_JOBS];
+    __shared__ uint32_t s_head, s_valid, s_active;
+    RegenCtl *C = V.ctl;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        s_head = __hip_atomic_load(&ring->req_head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        s_valid = 0;
+        s_active = 0;
     }
+    __syncthreads();
+    const uint32_t head = s_head, tail = C->req_tail;
+    // at most a ring of requests can be pending (each holds a table entry)
+    const uint32_t pending = head - tail <= XT_REGEN_RING ? head - tail : 0u;
+    constexpr uint32_t WORDS = sizeof(RegenJob) / 4;
+    for (uint32_t k = 0; k < pending; ++k) {
+        const RegenReq *q = &ring->req[(tail + k) % XT_REGEN_RING];
+        const uint32_t e = __hip_atomic_load(&q->entry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (e >= XT_REGEN_MAX_JOBS) continue;  // not a table entry: skipped
+        const uint32_t *src = (const uint32_t *)&q->job;
+        uint32_t *dst = (uint32_t *)&V.jobs[e];
+        for (uint32_t w = t; w < WORDS; w += 256) dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (t < XT_REGEN_SHARDS) V.done_shard[t * XT_REGEN_MAX_JOBS + e] = 0;
+        if (t == 0) V.done[e] = 0;
+    }
+    __syncthreads();
+    if (t == 0) C->req_tail = tail + pending;
+    const uint32_t policy = C->policy;
+    for (uint32_t e = t; e < XT_REGEN_MAX_JOBS; e += 256) {
+        const RegenJob &J = V.jobs[e];
+        unsigned long long k = ~0ull;
+        if (J.state == 1u) atomicAdd(&s_active, 1u);
+        if (J.state == 1u && J.next < J.total) {
+            const unsigned long long left = J.total - J.next, seq = J.seq & 0x3fffull;
+            unsigned long long primary = J.seq & 0xffffffffffull;  // FCFS, FAIR
+            if (policy == XT_SERVE_SRPT) primary = left;
+            else if (policy == XT_SERVE_HYBRID) primary = left <= C->P / 2 ? left : (1ull << 39) + (J.seq & 0x7fffffffffull);
+            k = (primary << 24) | (seq << 10) | e;
+            atomicAdd(&s_valid, 1u);
+        }
+        key[e] = k;
+    }
+    __syncthreads();
+    // bitonic sort of the 1024 keys, ascending
+    for (uint32_t size = 2; size <= XT_REGEN_MAX_JOBS; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = t; i < XT_REGEN_MAX_JOBS; i += 256) {
+                const uint32_t j = i ^ stride;
+                if (j > i) {
+                    const unsigned long long a = key[i], b = key[j];
+                    const bool up = (i & size) == 0;
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t nv = s_valid;
+    for (uint32_t k = t; k < nv; k += 256) C->prio[k] = (uint32_t)(key[k] & 0x3ffull);
+    if (t == 0) {
+        C->n_prio = nv;
+        const uint32_t na = s_active ? s_active : 1u;
+        C->cap_per_job = policy == XT_SERVE_FAIR ? (C->P / na > 64u ? C->P / na : 64u) : 0u;
+    }
+}
+
+// Device-fed serving, after the replay's rounds and k_regen_done_sum: the
+// active jobs whose paths have all ended, listed for resolving.
+static __global__ __launch_bounds__(256) void k_regen_finish(RegenView V) {
+    RegenCtl *C = V.ctl;
+    for (uint32_t e = threadIdx.x; e < XT_REGEN_MAX_JOBS; e += 256) {
+        RegenJob &J = V.jobs[e];
+        if (J.state == 1u && V.done[e] >= J.total) {
+            J.state = 2u;
+            C->fin[atomicAdd(&C->n_fin, 1u)] = e;
+        }
+    }
+}
+
+// Resolves every job that k_regen_finish listed into its image.
+static __global__ __launch_bounds__(XT_BLOCK) void k_regen_resolve_fin(RegenView V) {
+    const RegenCtl *C = V.ctl;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t f = 0; f < C->n_fin; ++f) {
+        const RegenJob &J = V.jobs[C->fin[f]];
+        const uint32_t n = J.jd.W * J.jd.H;
+        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+            J.out[p] = J.sbuf ? regen_resolve_ordered_px(J.jd, J.sbuf, p) : regen_resolve_fixed_px(J.jd, J.acc, p);
+    }
+}
+
+// Posts the resolved jobs to the completion ring and frees their entries.
+static __global__ void k_regen_post(RegenView V, RegenRing *ring) {
+    RegenCtl *C = V.ctl;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t n = C->n_fin, h = C->fin_head;
+    for (uint32_t f = 0; f < n; ++f) {
+        const uint32_t e = C->fin[f];
+        __hip_atomic_store(&ring->fin[(h + f) % XT_REGEN_RING], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        V.jobs[e].state = 0u;
+    }
+    __hip_atomic_store(&ring->fin_head, h + n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    C->fin_head = h + n;
+    C->n_fin = 0;
 }
 
 static __global__ void k_regen_init(RegenCtl *C, uint32_t P) {

@@ -108,6 +108,8 @@ struct SJob {
     float4 *accf = nullptr, *out = nullptr;
     double2 *acc64 = nullptr;
     hipEvent_t done_ev = nullptr;
+    bool copying = false;  // device-fed serving: resolved, its image on the way to host memory
+    bool zeroing = false;  // its fixed-point sums are being cleared (done_ev marks the end)
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
@@ -155,7 +157,22 @@ struct RegenHost {
     Counters *h_cnt = nullptr;             // pinned, per lane
     std::vector<int> free_entries;
     unsigned long long queries_seen[XT_REGEN_MAX_LANES] = {}, retests_seen[XT_REGEN_MAX_LANES] = {};
+    // device-fed serving (xt_serve_config.loop = 2)
+    RegenRing *ring = nullptr;      // fine-grained host memory
+    RegenRing *ring_dev = nullptr;  // its device address
+    uint32_t req_head = 0, fin_tail = 0;
+    unsigned long long seq = 0;
+    uint64_t entry_ticket[XT_REGEN_MAX_JOBS] = {};
+    std::deque<std::pair<hipEvent_t, int>> inflight;  // replays on the stream: end event, counter snapshot slot
+    std::vector<hipEvent_t> free_ev;
+    Counters *h_snap = nullptr;     // pinned: lane counters after a replay, XT_REGEN_INFLIGHT slots
+    uint32_t *h_policy = nullptr;   // pinned
+    uint32_t policy_sent = 0xffffffffu;
+    uint64_t launches = 0;
+    std::vector<uint64_t> resolving;  // tickets whose image is being copied to host memory
 };
+
+#define XT_REGEN_INFLIGHT 2  // device-fed serving: replays kept queued on the stream
 
 struct ServeState {
     std::map<uint64_t, SJob> jobs;
@@ -171,7 +188,7 @@ struct ServeState {
     double epoch_ms = 0;
     xt_serve_info info{};
     int policy = XT_SERVE_SRPT;
-    xt_serve_config cfg{2, 2, 4, 0, 4096, 2};
+    xt_serve_config cfg{2, 2, 4, 0, 4096, 2, 2};
     RegenHost R;
 };
 
@@ -288,6 +305,12 @@ int admit(xt_dev *d, ServeState &S) {
             S.free_events.pop_back();
         } else {
             XT_TRY(hipEventCreate(&J.done_ev));
+        }
+        if (J.regen && J.acc) {
+            // device-fed serving hands the job to the device only once its
+            // sums are zero (a replay already queued could take it earlier)
+            XT_TRY(hipEventRecord(J.done_ev, d->stream));
+            J.zeroing = true;
         }
         J.state = ACTIVE;
         J.t_admit = now_ms();
@@ -583,6 +606,14 @@ int regen_init(xt_dev *d, ServeState &S) {
         XT_TRY(hipEventCreateWithFlags(&R.fork_ev[q], hipEventDisableTiming));
         for (int k = 0; k < XT_LANE_STREAMS; ++k) XT_TRY(hipEventCreateWithFlags(&R.join_ev[q][k], hipEventDisableTiming));
     }
+    XT_TRY(hipHostMalloc((void **)&R.ring, sizeof(RegenRing), hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(R.ring, 0, sizeof(RegenRing));
+    XT_TRY(hipHostGetDevicePointer((void **)&R.ring_dev, R.ring, 0));
+    XT_TRY(hipHostMalloc((void **)&R.h_snap, sizeof(Counters) * XT_REGEN_MAX_LANES * XT_REGEN_INFLIGHT,
+                         hipHostMallocDefault));
+    XT_TRY(hipHostMalloc((void **)&R.h_policy, 4, hipHostMallocDefault));
+    R.req_head = R.fin_tail = 0;
+    R.policy_sent = 0xffffffffu;
     XT_TRY(hipHostMalloc((void **)&R.h_job, sizeof(RegenJob) * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_done, 8ull * XT_REGEN_MAX_JOBS, hipHostMallocDefault));
     XT_TRY(hipHostMalloc((void **)&R.h_prio, 4ull * (XT_REGEN_MAX_JOBS + 2), hipHostMallocDefault));
@@ -621,6 +652,7 @@ void regen_drop_graph(RegenHost &R) {
 
 void regen_shutdown(xt_dev *d, ServeState &S) {
     RegenHost &R = S.R;
+    if (R.ready) (void)hipStreamSynchronize(d->stream);  // replays still queued use the buffers
     regen_drop_graph(R);
     for (RegenLaneH &L : R.lanes) regen_free(d, L.blk, L.bytes);
     regen_free(d, R.blk, R.bytes);
@@ -628,6 +660,11 @@ void regen_shutdown(xt_dev *d, ServeState &S) {
     if (R.h_done) (void)hipHostFree(R.h_done);
     if (R.h_prio) (void)hipHostFree(R.h_prio);
     if (R.h_cnt) (void)hipHostFree(R.h_cnt);
+    if (R.ring) (void)hipHostFree(R.ring);
+    if (R.h_snap) (void)hipHostFree(R.h_snap);
+    if (R.h_policy) (void)hipHostFree(R.h_policy);
+    for (auto &f : R.inflight) (void)hipEventDestroy(f.first);
+    for (hipEvent_t e : R.free_ev) (void)hipEventDestroy(e);
     for (int q = 0; q < 2; ++q) {
         if (R.fork_ev[q]) (void)hipEventDestroy(R.fork_ev[q]);
         for (int k = 0; k < XT_LANE_STREAMS; ++k)
@@ -728,6 +765,8 @@ int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
     RegenView &V = R.V;
     const hipStream_t st = d->stream;
     const unsigned gP = grid_for(R.P, XT_BLOCK, 8192);
+    if (S.cfg.loop == 2)
+        hipLaunchKernelGGL(k_regen_intake, dim3(1), dim3(256), 0, st, V, (const RegenRing *)R.ring_dev);
     hipLaunchKernelGGL(k_regen_plan, dim3(1), dim3(64), 0, st, V, qi);
     hipLaunchKernelGGL(k_regen_gen, dim3(gP), dim3(XT_BLOCK), 0, st, V, qi);
     hipLaunchKernelGGL(k_regen_round_start, dim3(1), dim3(1), 0, st, V.ctl);
@@ -790,6 +829,18 @@ int regen_round(xt_dev *d, ServeState &S, uint32_t qi) {
     return XT_OK;
 }
 
+// Device-fed serving, after a replay's two rounds: finish, resolve and post
+// the jobs whose paths have all ended.
+int regen_tail(xt_dev *d, ServeState &S) {
+    RegenHost &R = S.R;
+    hipLaunchKernelGGL(k_regen_done_sum, dim3(XT_REGEN_MAX_JOBS / 256), dim3(256), 0, d->stream, R.V);
+    hipLaunchKernelGGL(k_regen_finish, dim3(1), dim3(256), 0, d->stream, R.V);
+    hipLaunchKernelGGL(k_regen_resolve_fin, dim3(1024), dim3(XT_BLOCK), 0, d->stream, R.V);
+    hipLaunchKernelGGL(k_regen_post, dim3(1), dim3(64), 0, d->stream, R.V, R.ring_dev);
+    XT_LAUNCHED();
+    return XT_OK;
+}
+
 // Captures rounds over parity 0 then 1 (queue parity is back at 0 after a
 // replay) as one hipGraph.
 int regen_capture(xt_dev *d, ServeState &S) {
@@ -798,6 +849,7 @@ int regen_capture(xt_dev *d, ServeState &S) {
     XT_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
     int rc = regen_round(d, S, 0);
     if (!rc) rc = regen_round(d, S, 1);
+    if (!rc && S.cfg.loop == 2) rc = regen_tail(d, S);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(d->stream, &g);
     if (rc) {
@@ -951,6 +1003,207 @@ int regen_step(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
 }
 
 
+// The lane of J's (scene, integrator, accel) if it is set up, else -1.
+int regen_find_lane(const RegenHost &R, const SJob &J) {
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+        const RegenLaneH &L = R.lanes[l];
+        if (L.used && L.scene == J.job.scene_id && L.integrator == J.jd.integrator && L.accel == J.job.accel) return l;
+    }
+    return -1;
+}
+
+// Device-fed serving: takes the lane counters a finished replay left in its
+// snapshot slot.
+void regen_take_snapshot(ServeState &S, int slot) {
+    RegenHost &R = S.R;
+    const Counters *c = R.h_snap + (size_t)slot * XT_REGEN_MAX_LANES;
+    for (int l = 0; l < XT_REGEN_MAX_LANES; ++l) {
+        if (!R.lanes[l].used || c[l].queries < R.queries_seen[l]) continue;
+        S.info.queries += c[l].queries - R.queries_seen[l];
+        S.info.retests += c[l].retests - R.retests_seen[l];
+        R.queries_seen[l] = c[l].queries;
+        R.retests_seen[l] = c[l].retests;
+    }
+}
+
+// Device-fed serving: retires the oldest queued replay (waiting for it when
+// `wait`); false when it has not ended.
+int regen_retire_replay(xt_dev *d, ServeState &S, bool wait, bool &done) {
+    RegenHost &R = S.R;
+    done = false;
+    if (R.inflight.empty()) return XT_OK;
+    const auto f = R.inflight.front();
+    if (wait) {
+        XT_TRY(hipEventSynchronize(f.first));
+    } else {
+        const hipError_t e = hipEventQuery(f.first);
+        if (e == hipErrorNotReady) return XT_OK;
+        if (e != hipSuccess) XT_TRY(e);
+    }
+    regen_take_snapshot(S, f.second);
+    for (uint64_t t : S.active) {
+        SJob &J = S.jobs[t];
+        if (J.regen && J.entry >= 0) J.passes += 1;  // replays the job has taken part in
+    }
+    R.free_ev.push_back(f.first);
+    R.inflight.pop_front();
+    S.info.passes += 1;
+    done = true;
+    return XT_OK;
+}
+
+int regen_drain_replays(xt_dev *d, ServeState &S) {
+    bool done = true;
+    while (!S.R.inflight.empty()) {
+        const int rc = regen_retire_replay(d, S, true, done);
+        if (rc) return rc;
+    }
+    return XT_OK;
+}
+
+// One step of device-fed serving. The host writes newly admitted jobs to the
+// request ring, copies out the images the device posted to the completion
+// ring, and keeps XT_REGEN_INFLIGHT replays queued; it waits only for the
+// oldest replay when nothing else moved (so a drain loop does not spin), or
+// for an image copy when nothing else runs.
+int regen_step_device(xt_dev *d, ServeState &S, std::vector<uint64_t> &finished) {
+    int rc = regen_init(d, S);
+    if (rc) return rc;
+    RegenHost &R = S.R;
+    RegenView &V = R.V;
+    if (R.policy_sent != (uint32_t)S.policy) {
+        *R.h_policy = (uint32_t)S.policy;
+        XT_TRY(hipMemcpyAsync(&V.ctl->policy, R.h_policy, 4, hipMemcpyHostToDevice, d->stream));
+        R.policy_sent = (uint32_t)S.policy;
+    }
+    // requests: admitted jobs without a table entry
+    bool wrote = false;
+    for (uint64_t t : S.active) {
+        SJob &J = S.jobs[t];
+        if (!J.regen || J.entry >= 0 || J.copying || R.free_entries.empty()) continue;
+        if (J.zeroing) {
+            const hipError_t e = hipEventQuery(J.done_ev);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) XT_TRY(e);
+            J.zeroing = false;
+        }
+        int lane = regen_find_lane(R, J);
+        if (lane < 0) {
+            // a new lane changes the captured rounds: let the queued replays end first
+            rc = regen_drain_replays(d, S);
+            if (!rc) rc = regen_lane(d, S, J, lane);
+            if (rc) return rc;
+            if (lane < 0) continue;  // every lane serves another key: wait
+        }
+        J.entry = R.free_entries.back();
+        R.free_entries.pop_back();
+        R.entry_ticket[J.entry] = t;
+        RegenReq &q = R.ring->req[R.req_head % XT_REGEN_RING];
+        std::memset(&q, 0, sizeof q);
+        q.job.jd = J.jd;
+        q.job.total = (unsigned long long)J.jd.total * J.jd.NT;
+        q.job.next = 0;
+        q.job.sbuf = J.sbuf;
+        q.job.acc = J.acc;
+        q.job.lane = (uint32_t)lane;
+        q.job.state = 1u;
+        q.job.out = J.out;
+        q.job.seq = R.seq++;
+        q.entry = (uint32_t)J.entry;
+        ++R.req_head;
+        J.paths_done = 0;
+        wrote = true;
+    }
+    if (wrote) __atomic_store_n(&R.ring->req_head, R.req_head, __ATOMIC_RELEASE);
+    // completions: the image goes to host memory behind the queued replays
+    const uint32_t fh = __atomic_load_n(&R.ring->fin_head, __ATOMIC_ACQUIRE);
+    if (fh - R.fin_tail > XT_REGEN_RING) {
+        d->err = "device-fed serving: completion ring overrun";
+        return XT_E_STATE;
+    }
+    for (; R.fin_tail != fh; ++R.fin_tail) {
+        const uint32_t e = R.ring->fin[R.fin_tail % XT_REGEN_RING];
+        if (e >= XT_REGEN_MAX_JOBS) continue;
+        const uint64_t t = R.entry_ticket[e];
+        SJob &J = S.jobs.at(t);
+        rc = take_host_block(d, S, J.npix * sizeof(float4), J.host);
+        if (rc) return rc;
+        XT_TRY(hipMemcpyAsync(J.host.p, J.out, J.npix * sizeof(float4), hipMemcpyDeviceToHost, d->stream));
+        XT_TRY(hipEventRecord(J.done_ev, d->stream));
+        J.copying = true;
+        J.entry = -1;
+        J.issued = J.jd.total;
+        J.paths_done = (uint64_t)J.jd.total * J.jd.NT;
+        S.info.samples += (uint64_t)J.jd.total * J.npix;
+        R.free_entries.push_back((int)e);
+        R.resolving.push_back(t);
+    }
+    // keep the device fed while any job is on it
+    bool on_device = false;
+    for (uint64_t t : S.active) on_device = on_device || (S.jobs[t].regen && S.jobs[t].entry >= 0);
+    while (on_device && R.inflight.size() < XT_REGEN_INFLIGHT) {
+        if (S.cfg.graph == 2 && (R.dirty || std::memcmp(&R.captured, &d->tuning, sizeof(xt_tuning)) != 0)) {
+            rc = regen_drain_replays(d, S);
+            if (!rc) rc = regen_capture(d, S);
+            if (rc) return rc;
+        }
+        if (S.cfg.graph == 2) {
+            XT_TRY(hipGraphLaunch(R.exec, d->stream));
+        } else {
+            rc = regen_round(d, S, 0);
+            if (!rc) rc = regen_round(d, S, 1);
+            if (!rc) rc = regen_tail(d, S);
+            if (rc) return rc;
+        }
+        const int slot = (int)(R.launches++ % XT_REGEN_INFLIGHT);
+        XT_TRY(hipMemcpyAsync(R.h_snap + (size_t)slot * XT_REGEN_MAX_LANES, V.cnt,
+                              sizeof(Counters) * XT_REGEN_MAX_LANES, hipMemcpyDeviceToHost, d->stream));
+        hipEvent_t ev = nullptr;
+        if (!R.free_ev.empty()) {
+            ev = R.free_ev.back();
+            R.free_ev.pop_back();
+        } else {
+            XT_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
+        XT_TRY(hipEventRecord(ev, d->stream));
+        R.inflight.push_back({ev, slot});
+    }
+    bool done = true;
+    while (done && !R.inflight.empty()) {
+        rc = regen_retire_replay(d, S, false, done);
+        if (rc) return rc;
+    }
+    auto collect = [&]() -> int {
+        for (size_t k = 0; k < R.resolving.size();) {
+            const uint64_t t = R.resolving[k];
+            SJob &J = S.jobs.at(t);
+            const hipError_t e = hipEventQuery(J.done_ev);
+            if (e == hipErrorNotReady) {
+                ++k;
+                continue;
+            }
+            if (e != hipSuccess) XT_TRY(e);
+            J.copying = false;
+            finished.push_back(t);
+            R.resolving[k] = R.resolving.back();
+            R.resolving.pop_back();
+        }
+        return XT_OK;
+    };
+    rc = collect();
+    if (rc) return rc;
+    if (finished.empty()) {
+        if (R.inflight.size() >= XT_REGEN_INFLIGHT) {
+            rc = regen_retire_replay(d, S, true, done);  // the device has the next replay queued
+        } else if (!R.resolving.empty()) {
+            XT_TRY(hipEventSynchronize(S.jobs.at(R.resolving.front()).done_ev));
+            rc = collect();
+        }
+        if (rc) return rc;
+    }
+    return XT_OK;
+}
+
 void release_block(ServeState &S, SJob &J) {
     if (J.blk.p) S.free_dev.push_back(J.blk);
     J.blk = DevBlock();
@@ -1077,7 +1330,7 @@ extern "C" int xt_serve_step(xt_dev *d, int *active_jobs) {
     bool any_regen = false;
     for (uint64_t t : S.active) any_regen = any_regen || S.jobs[t].regen;
     if (any_regen) {
-        rc = regen_step(d, S, finished);
+        rc = S.cfg.loop == 2 ? regen_step_device(d, S, finished) : regen_step(d, S, finished);
         if (rc) return rc;
     }
     std::vector<Segment> segs;
@@ -1134,7 +1387,7 @@ extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     if (!d || !c) return XT_E_INVALID;
     if (c->regen > 2 || c->graph > 2 || (c->rounds_per_step & 1u) ||
         c->rounds_per_step > 1024 || (c->paths && (c->paths < 4096 || c->paths > (1u << 28))) ||
-        c->sample_buffer_mb > (1u << 20) || c->lane_streams > 2) {
+        c->sample_buffer_mb > (1u << 20) || c->lane_streams > 2 || c->loop > 2) {
         d->err = "bad serve configuration";
         return XT_E_INVALID;
     }
@@ -1143,7 +1396,8 @@ extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     ServeState &S = *d->serve;
     bool regen_busy = false;
     for (uint64_t t : S.active) regen_busy = regen_busy || S.jobs[t].regen;
-    if ((c->paths && c->paths != S.R.P && S.R.ready) || (c->regen && c->regen != S.cfg.regen)) {
+    if ((c->paths && c->paths != S.R.P && S.R.ready) || (c->regen && c->regen != S.cfg.regen) ||
+        (c->loop && c->loop != S.cfg.loop)) {
         if (regen_busy || !S.waiting.empty()) {
             d->err = "the regeneration engine holds jobs";
             return XT_E_STATE;
@@ -1155,6 +1409,7 @@ extern "C" int xt_serve_configure(xt_dev *d, const xt_serve_config *c) {
     if (c->rounds_per_step) S.cfg.rounds_per_step = c->rounds_per_step;
     if (c->paths) S.cfg.paths = c->paths;
     if (c->sample_buffer_mb) S.cfg.sample_buffer_mb = c->sample_buffer_mb;
+    if (c->loop) S.cfg.loop = c->loop;
     if (c->lane_streams && c->lane_streams != S.cfg.lane_streams) {
         S.cfg.lane_streams = c->lane_streams;
         S.R.dirty = true;  // the captured rounds follow the lane streams
