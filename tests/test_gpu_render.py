@@ -251,3 +251,23 @@ def test_raygraph_equals_reference_file(device, tmp_path):
     text = (tmp_path / "st.ply").read_text()
     assert "element vertex 0\n" in text and "element edge 0\n" in text and text.endswith("end_header\n")
     device.release(sid)
+
+
+@pytest.mark.gpu
+def test_ao_and_depth_on_the_octree_at_256(device):
+    """The CPU-runnable configuration: ground plane and 64 Lambert spheres,
+    pinhole camera, 256x256, the reference's octree; ao with 16 hemisphere
+    samples per pixel on the oracle's random streams, depth bit-exact."""
+    s, sid, cam = _setup(device, "spheres")
+    W = H = 256
+    job = xa.make_job(sid, xa.AO, W, H, s.camera(), samples=16, seed=5, accel=xa.ACCEL_OCTREE,
+                      accum=xa.ACCUM_REFERENCE)
+    img, st = device.render(job)
+    ref, nq, mean, m2 = OracleScene(s).render(xa.AO, W, H, cam, samples=16, seed=5, stats=True)
+    same_stream_check(img, ref, mean, m2, 16, 0.995)
+    assert abs(st.queries - nq) <= 0.005 * nq
+    job = xa.make_job(sid, xa.DEPTH, W, H, s.camera(), samples=1, accel=xa.ACCEL_OCTREE)
+    img, st = device.render(job)
+    ref, nq = OracleScene(s).render(xa.DEPTH, W, H, cam, samples=1)
+    assert st.queries == nq and np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    device.release(sid)
