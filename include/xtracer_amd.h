@@ -242,6 +242,10 @@ typedef struct xt_tuning {
                                  pairs grouped into 128-byte lines three levels deep (1) */
     uint32_t kd_leaf;         /* kd leaf primitives in the FP32 walk: 1 a copy of each referenced record in
                                  leaf order; 2 read from the soup through the reference list (1) */
+    uint32_t frame_lanes;     /* one-job renders: 1 a pass's paths in one group on the device stream; 2..4
+                                 its pixels cut into that many groups whose rounds run on streams of their
+                                 own, so that one group's shading and launch tails overlap another's trace;
+                                 images are the same bits either way (2) */
 } xt_tuning;
 int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t);
 int xt_dev_get_tuning(const xt_dev *d, xt_tuning *t);
@@ -316,6 +320,8 @@ typedef struct xt_job_stats {
     double queue_ms;       /* serving: admission - submission */
     uint64_t trace_fail;   /* FP32 traversals that ran out of steps or stack (re-tested in FP64) */
     uint64_t retest_fail;  /* FP64 re-test traversals that ran out of steps (closest hit so far kept) */
+    double ms_trace_busy;  /* time in which at least one trace kernel ran: the union of the timed launches'
+                              intervals (equal to ms_trace when the launches do not overlap) */
 } xt_job_stats;
 
 /* IMaterial::shade (src/xtcore/material/lambert.cc:9-28, phong.cc:11-33,

@@ -10,7 +10,7 @@ from xtracer_amd import scenes
 from tests.helpers import mixed_rays, scene_path
 
 DEFAULT = {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO,
-           "trace_segments": 1, "ray_order": 1, "kd_layout": 1, "kd_leaf": 1}
+           "trace_segments": 1, "ray_order": 1, "kd_layout": 1, "kd_leaf": 1, "frame_lanes": 2}
 
 
 @pytest.fixture
@@ -24,7 +24,8 @@ def test_tuning_roundtrip_and_rejects(tuned):
     t = tuned.set_tuning(**DEFAULT)
     assert t == dict(DEFAULT, trace_blocks_per_cu=0)
     for bad in ({"kd_stack": 10}, {"trace_block": 64}, {"wave_paths": 100}, {"trace_blocks_per_cu": 65},
-                {"trace_segments": 4}, {"ray_order": 4}, {"kd_layout": 3}, {"kd_leaf": 3}):
+                {"trace_segments": 4}, {"ray_order": 4}, {"kd_layout": 3}, {"kd_leaf": 3},
+                {"frame_lanes": 5}):
         with pytest.raises(xa.XtError):
             tuned.set_tuning(**bad)
     assert tuned.tuning() == t
@@ -67,7 +68,8 @@ def test_render_bit_exact_small_shapes(tuned, integrator):
 
 
 LAYOUTS = [{"trace_segments": 8}, {"ray_order": 2}, {"ray_order": 3}, {"kd_layout": 2}, {"kd_leaf": 2},
-           {"trace_segments": 8, "ray_order": 2, "kd_layout": 2, "kd_leaf": 2}]
+           {"frame_lanes": 1}, {"frame_lanes": 4},
+           {"trace_segments": 8, "ray_order": 2, "kd_layout": 2, "kd_leaf": 2, "frame_lanes": 2}]
 
 
 @pytest.mark.gpu
@@ -93,6 +95,30 @@ def test_layout_knobs_invariant(tuned, knobs):
     tuned.release(sid2)
     ref, nq = OracleScene(s).render(xa.DEPTH, W, H, cam, samples=spp, seed=4)
     assert st.queries == nq
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", [xa.PATHTRACER, xa.AO, xa.NORMAL, xa.DEPTH])
+@pytest.mark.parametrize("ray_order", [1, 2, 3])
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_frame_lanes_same_bits(tuned, integrator, ray_order, lanes):
+    """A one-job render whose passes run as 2 or 4 pixel groups on streams of
+    their own gives the one-group image bit for bit and the same query count,
+    over several passes, with an odd number of 8x8 tiles (77: groups of 38 and
+    39, or 19, 19, 19 and 20 tiles), and with paths that all end long before
+    the depth limit of 100 (the early stop)."""
+    s = xa.Scene(scene_path("spheres"))
+    sid = tuned.upload(s)
+    W, H, spp = 88, 56, 6
+    job = xa.make_job(sid, integrator, W, H, s.camera(), samples=spp, rdepth=100, seed=21)
+    tuned.set_tuning(**dict(DEFAULT, wave_paths=3 * 77 * 64, ray_order=ray_order, frame_lanes=1))
+    ref, st1 = tuned.render(job)
+    tuned.set_tuning(frame_lanes=lanes)
+    img, st2 = tuned.render(job)
+    tuned.release(sid)
+    assert st1.passes == st2.passes == 2
+    assert st2.queries == st1.queries
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
 
 

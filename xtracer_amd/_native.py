@@ -80,7 +80,8 @@ class Camera(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("kd_stack", "trace_block", "wave_paths", "trace_blocks_per_cu",
-                                                "trace_segments", "ray_order", "kd_layout", "kd_leaf")]
+                                                "trace_segments", "ray_order", "kd_layout", "kd_leaf",
+                                                "frame_lanes")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -114,7 +115,7 @@ class JobStats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_uint64), ("retests", ctypes.c_uint64), ("samples", ctypes.c_uint64),
                 ("ms_total", ctypes.c_double), ("ms_trace", ctypes.c_double), ("trace_launches", ctypes.c_uint32),
                 ("passes", ctypes.c_uint32), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
-                ("trace_fail", ctypes.c_uint64), ("retest_fail", ctypes.c_uint64)]
+                ("trace_fail", ctypes.c_uint64), ("retest_fail", ctypes.c_uint64), ("ms_trace_busy", ctypes.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -422,8 +423,8 @@ class Device:
 
     def set_tuning(self, **knobs):
         """Sets launch-shape knobs (kd_stack, trace_block, wave_paths,
-        trace_blocks_per_cu, trace_segments, ray_order, kd_layout, kd_leaf); unnamed
-        knobs keep their value."""
+        trace_blocks_per_cu, trace_segments, ray_order, kd_layout, kd_leaf,
+        frame_lanes); unnamed knobs keep their value."""
         t = Tuning(**{k: int(v) for k, v in knobs.items()})
         self._check(lib().xt_dev_set_tuning(self._d, ctypes.byref(t)))
         return self.tuning()

@@ -11,8 +11,9 @@ size, persistent grid) over its full product first, then the ray-sort
 granularity (camera rays sample by sample over the image, by 8x8 tile, or by
 pixel), then whether the trace queue is cut into per-XCD segments, then
 whether kd leaves are read as per-reference copies or through the reference
-list, each stage keeping the best of the one before; the wavefront size
-(paths in flight per pass) last, on the full job.
+list, then how many pixel groups a pass runs as on streams of their own,
+each stage keeping the best of the one before; the wavefront size (paths in
+flight per pass) last, on the full job.
 """
 import itertools
 
@@ -23,7 +24,8 @@ KERNEL_SPACE = {
     "trace_block": (128, 256),
     "trace_blocks_per_cu": (TUNING_AUTO, 2, 8),
 }
-STAGES = [KERNEL_SPACE, {"ray_order": (1, 2, 3)}, {"trace_segments": (1, 8)}, {"kd_leaf": (1, 2)}]
+STAGES = [KERNEL_SPACE, {"ray_order": (1, 2, 3)}, {"trace_segments": (1, 8)}, {"kd_leaf": (1, 2)},
+          {"frame_lanes": (1, 2)}]
 DEFAULT_SPACE = KERNEL_SPACE  # the first stage alone (older callers)
 
 

@@ -70,12 +70,14 @@ struct xt_dev {
     WaveBuffers wave;
     AccumBuffers accum;
     Counters *host_counters = nullptr;  // pinned, XT_MAX_LANES blocks
+    uint32_t *host_snap = nullptr;      // pinned, 2 x XT_MAX_LANES queue counts of the early-stop check
+    hipEvent_t snap_ev[2][XT_MAX_LANES] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t tev[XT_TRACE_EVENTS];
     int trace_launches = 0;
     std::vector<int> timed_pairs;
     ServeState *serve = nullptr;
-    xt_tuning tuning{12, 256, 16u << 20, 0, 1, 1, 1, 1};
+    xt_tuning tuning{12, 256, 16u << 20, 0, 1, 1, 1, 1, 2};
     void *scratch = nullptr;  // this device's address of xt_trace_device_static's scratch array
     std::string err;
 };

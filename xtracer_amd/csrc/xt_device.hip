@@ -179,6 +179,10 @@ void xt_dev_destroy(xt_dev *d) {
     xt_dev_free_wave(d);
     xt_dev_free_accum(d);
     if (d->host_counters) hipHostFree(d->host_counters);
+    if (d->host_snap) hipHostFree(d->host_snap);
+    for (auto &row : d->snap_ev)
+        for (hipEvent_t e : row)
+            if (e) hipEventDestroy(e);
     for (int k = 0; k < XT_LANE_STREAMS; ++k) {
         if (d->lane_stream[k]) hipStreamDestroy(d->lane_stream[k]);
         if (d->lane_ev[k]) hipEventDestroy(d->lane_ev[k]);
@@ -254,6 +258,13 @@ int xt_dev_set_tuning(xt_dev *d, const xt_tuning *t) {
             return XT_E_INVALID;
         }
         n.kd_layout = t->kd_layout;
+    }
+    if (t->frame_lanes) {
+        if (t->frame_lanes > XT_LANE_STREAMS) {
+            d->err = "frame_lanes must be 1, 2, 3 or 4";
+            return XT_E_INVALID;
+        }
+        n.frame_lanes = t->frame_lanes;
     }
     d->tuning = n;
     return XT_OK;

@@ -199,11 +199,18 @@ int xt_ensure_host_counters(xt_dev *d) {
 // Query rounds of one pass over every group: trace + shade per round, queue
 // 0 holding the camera rays. With several groups, each group's kernels run
 // on a stream of its own so that one group's launch tails overlap another's
-// work; the device stream forks to them and joins them again. Every second
-// round past the second, the host reads the queue counts and drops groups
-// whose paths have all ended.
+// work; the device stream forks to them and joins them again at the end.
+// Every second round past the second, each live group's next queue count is
+// copied to pinned memory on the group's stream. The host then reads the
+// copies of the check before, two rounds back, so that the device still has
+// two rounds queued while the host waits and the groups never wait for each
+// other, and issues no more rounds to groups whose paths have all ended.
 int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
     WaveBuffers &w = d->wave;
+    if (n_groups < 1 || n_groups > XT_MAX_LANES) {
+        d->err = "pass groups out of range";
+        return XT_E_INVALID;
+    }
     uint32_t max_rounds = 0;
     for (int k = 0; k < n_groups; ++k) {
         groups[k].live = true;
@@ -217,23 +224,18 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
         }
     }
     if (n_st && !d->fork_ev) XT_TRY(hipEventCreateWithFlags(&d->fork_ev, hipEventDisableTiming));
+    if (!d->host_snap)
+        XT_TRY(hipHostMalloc((void **)&d->host_snap, 2 * XT_MAX_LANES * sizeof(uint32_t), hipHostMallocDefault));
+    for (int s = 0; s < 2; ++s)
+        for (int k = 0; k < n_groups; ++k)
+            if (!d->snap_ev[s][k]) XT_TRY(hipEventCreateWithFlags(&d->snap_ev[s][k], hipEventDisableTiming));
     auto stream_of = [&](int k) { return n_st ? d->lane_stream[k % n_st] : d->stream; };
-    auto fork = [&]() -> int {
-        if (!n_st) return XT_OK;
+    if (n_st) {
         XT_TRY(hipEventRecord(d->fork_ev, d->stream));
         for (int k = 0; k < n_st; ++k) XT_TRY(hipStreamWaitEvent(d->lane_stream[k], d->fork_ev, 0));
-        return XT_OK;
-    };
-    auto join = [&]() -> int {
-        for (int k = 0; k < n_st; ++k) {
-            XT_TRY(hipEventRecord(d->lane_ev[k], d->lane_stream[k]));
-            XT_TRY(hipStreamWaitEvent(d->stream, d->lane_ev[k], 0));
-        }
-        return XT_OK;
-    };
-    int rc = fork();
-    if (rc) return rc;
-    int qi = 0;
+    }
+    int qi = 0, slot = 0;
+    bool pending = false;
     for (uint32_t r = 0; r < max_rounds; ++r) {
         for (int k = 0; k < n_groups; ++k) {
             PassGroup &G = groups[k];
@@ -242,32 +244,41 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, G.lane.cnt,
                                (1u << (qi ^ 1)) | (1u << CNT_RETEST) | (1u << CNT_CURSOR) | (1u << CNT_RECHECK));
             const bool fp64_rays = xt_needs_ray64(G.integrator) && r == 0;
-            rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
-                                 time_trace && !n_st, st);
+            int rc = xt_launch_trace(d, *G.ds, (int)G.accel, qi, G.lane, fp64_rays ? w.ray64 : nullptr, nullptr,
+                                     time_trace, st);
             if (rc) return rc;
             rc = xt_launch_shade(d, *G.ds, G.integrator, r, qi, G.lane, st);
             if (rc) return rc;
         }
         qi ^= 1;
-        // stop early once every path of the pass has ended
         if (max_rounds > 2 && r + 1 < max_rounds && (r % 2 == 1)) {
-            rc = join();
-            if (rc) return rc;
-            XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, XT_MAX_LANES * sizeof(Counters), hipMemcpyDeviceToHost,
-                                  d->stream));
-            XT_TRY(hipStreamSynchronize(d->stream));
-            bool any = false;
+            if (pending) {  // the previous check: paths left after round r - 2
+                bool any = false;
+                for (int k = 0; k < n_groups; ++k) {
+                    PassGroup &G = groups[k];
+                    if (!G.live) continue;
+                    XT_TRY(hipEventSynchronize(d->snap_ev[slot ^ 1][k]));
+                    if (r + 1 >= G.rounds || d->host_snap[(slot ^ 1) * XT_MAX_LANES + k] == 0) G.live = false;
+                    any = any || G.live;
+                }
+                if (!any) break;
+            }
             for (int k = 0; k < n_groups; ++k) {
                 PassGroup &G = groups[k];
-                if (G.live && (r + 1 >= G.rounds || d->host_counters[G.lane.cnt - w.cnt].n[qi] == 0)) G.live = false;
-                any = any || G.live;
+                if (!G.live) continue;
+                XT_TRY(hipMemcpyAsync(d->host_snap + slot * XT_MAX_LANES + k, &G.lane.cnt->n[qi], sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, stream_of(k)));
+                XT_TRY(hipEventRecord(d->snap_ev[slot][k], stream_of(k)));
             }
-            if (!any) return XT_OK;
-            rc = fork();
-            if (rc) return rc;
+            slot ^= 1;
+            pending = true;
         }
     }
-    return join();
+    for (int k = 0; k < n_st; ++k) {
+        XT_TRY(hipEventRecord(d->lane_ev[k], d->lane_stream[k]));
+        XT_TRY(hipStreamWaitEvent(d->stream, d->lane_ev[k], 0));
+    }
+    return XT_OK;
 }
 
 // `after_pass(j, base, spp_pass)`, when given, runs on the host after each
@@ -299,33 +310,50 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
     hipLaunchKernelGGL(k_fill_u32, dim3(gfill), dim3(256), 0, d->stream, (uint32_t *)a.accf, 0u, npix * 4);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(npix * 12, 256, 8192)), dim3(256), 0, d->stream,
                        (uint32_t *)a.acc64, 0u, npix * 12);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x800000ffu);
+    // the pixel slots of each group: the slots cut at 8x8 tiles into
+    // frame_lanes nearly equal parts (every sample of a pixel stays in its
+    // group); one group when a pass hook reads the paths, which it does in
+    // the one-group order (path = sample * NT + slot)
+    const uint32_t tiles = j.NT / 64;
+    const int n_groups = after_pass ? 1 : (int)std::max(1u, std::min(d->tuning.frame_lanes, tiles));
+    uint32_t q_at[XT_LANE_STREAMS + 1];
+    for (int g = 0; g <= n_groups; ++g) q_at[g] = (uint32_t)((uint64_t)tiles * g / n_groups) * 64u;
+    hipLaunchKernelGGL(k_reset_lanes, dim3(n_groups), dim3(64), 0, d->stream, w.cnt, 0x800000ffu);
     XT_LAUNCHED();
     d->trace_launches = 0;
     d->timed_pairs.clear();
     const bool time_trace = stats != nullptr;
     XT_TRY(hipEventRecord(d->ev0, d->stream));
-    PassGroup G;
-    G.ds = ds;
-    G.integrator = j.integrator;
-    G.accel = job->accel;
-    G.rounds = xt_rounds(j);
-    G.lane = WaveLane{0, cap, w.cnt};
+    PassGroup G[XT_LANE_STREAMS];
+    for (int g = 0; g < n_groups; ++g) {
+        G[g].ds = ds;
+        G[g].integrator = j.integrator;
+        G[g].accel = job->accel;
+        G[g].rounds = xt_rounds(j);
+        G[g].lane = WaveLane{(size_t)spp_pass * q_at[g], (size_t)spp_pass * (q_at[g + 1] - q_at[g]), w.cnt + g};
+    }
     uint32_t passes = 0;
     for (uint32_t base = 0; base < j.total; base += spp_pass) {
         ++passes;
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, w.cnt, 0x7u);
-        hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, w.cnt, 0u, (uint32_t)cap);
-        Queue q0{w.q[0].r0, w.q[0].r1, w.q[0].path};
-        hipLaunchKernelGGL(k_camera, dim3(grid_for(cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
-                           spp_pass, 0u, j.NT, 0u, q0, w.ps0, w.ps1, w.rad, need64 ? w.ray64 : nullptr,
-                           d->tuning.ray_order);
-        XT_LAUNCHED();
-        rc = xt_run_rounds(d, &G, 1, time_trace);
+        hipLaunchKernelGGL(k_reset_lanes, dim3(n_groups), dim3(64), 0, d->stream, w.cnt, 0x7u);
+        for (int g = 0; g < n_groups; ++g) {
+            const WaveLane &L = G[g].lane;
+            hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, L.cnt, 0u, (uint32_t)L.cap);
+            Queue q0{w.q[0].r0 + L.off, w.q[0].r1 + L.off, w.q[0].path + L.off};
+            hipLaunchKernelGGL(k_camera, dim3(grid_for(L.cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j,
+                               base, spp_pass, q_at[g], q_at[g + 1] - q_at[g], (uint32_t)L.off, q0, w.ps0, w.ps1,
+                               w.rad, need64 ? w.ray64 : nullptr, d->tuning.ray_order);
+            XT_LAUNCHED();
+        }
+        rc = xt_run_rounds(d, G, n_groups, time_trace);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(j.NT, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j, base,
-                           spp_pass, 0u, j.NT, w.rad, need64 ? w.ray64 : nullptr, a.accf, a.acc64);
-        XT_LAUNCHED();
+        for (int g = 0; g < n_groups; ++g) {
+            const WaveLane &L = G[g].lane;
+            hipLaunchKernelGGL(k_accumulate, dim3(grid_for(q_at[g + 1] - q_at[g], XT_BLOCK, 8192)), dim3(XT_BLOCK), 0,
+                               d->stream, j, base, spp_pass, q_at[g], q_at[g + 1] - q_at[g], w.rad + L.off,
+                               need64 ? w.ray64 + 6 * L.off : nullptr, a.accf, a.acc64);
+            XT_LAUNCHED();
+        }
         if (after_pass) {
             rc = (*after_pass)(j, base, spp_pass);
             if (rc) return rc;
@@ -335,8 +363,14 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
                        a.acc64, (float4 *)d_out);
     XT_LAUNCHED();
     XT_TRY(hipEventRecord(d->ev1, d->stream));
-    XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+    XT_TRY(hipMemcpyAsync(d->host_counters, w.cnt, n_groups * sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
     XT_TRY(hipStreamSynchronize(d->stream));
+    for (int g = 1; g < n_groups; ++g) {
+        d->host_counters->queries += d->host_counters[g].queries;
+        d->host_counters->retests += d->host_counters[g].retests;
+        d->host_counters->trace_fail += d->host_counters[g].trace_fail;
+        d->host_counters->steps_exceeded += d->host_counters[g].steps_exceeded;
+    }
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         float ms = 0.f;
@@ -348,13 +382,27 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         stats->retest_fail = d->host_counters->steps_exceeded;
         stats->samples = (uint64_t)j.total * j.W * j.H;
         stats->passes = passes;
+        // each launch's own time, and the union of the launches' intervals
+        // (launches of different groups run at the same time)
         double tsum = 0;
+        std::vector<std::pair<float, float>> spans;
         for (int k : d->timed_pairs) {
-            float t = 0.f;
+            float t = 0.f, t0 = 0.f, t1 = 0.f;
             hipEventElapsedTime(&t, d->tev[k], d->tev[k + 1]);
+            hipEventElapsedTime(&t0, d->ev0, d->tev[k]);
+            hipEventElapsedTime(&t1, d->ev0, d->tev[k + 1]);
             tsum += t;
+            spans.emplace_back(t0, t1);
+        }
+        std::sort(spans.begin(), spans.end());
+        double busy = 0, at = -1e30;
+        for (const auto &sp : spans) {
+            const double a = std::max<double>(sp.first, at), b = sp.second;
+            if (b > a) busy += b - a;
+            at = std::max<double>(at, b);
         }
         stats->ms_trace = tsum;
+        stats->ms_trace_busy = busy;
         stats->trace_launches = (uint32_t)d->timed_pairs.size();
     }
     return XT_OK;
