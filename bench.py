@@ -62,6 +62,9 @@ def parse():
                     help="search the launch shapes first (default: the library's defaults, which are the shapes "
                          "this search picked for this workload on MI355X)")
     ap.add_argument("--tuning", default="", help="launch shape as JSON (xt_tuning fields); skips the autotuner")
+    ap.add_argument("--one-group-frames", type=int, default=3,
+                    help="frames rendered after the timed steps with each pass as one group (frame_lanes 1), "
+                         "reported as roofline.one_group (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=256, help="jobs of the mixed batch (0: skip serving)")
     ap.add_argument("--serve-requests", type=int, default=512, help="Poisson requests per arrival rate")
     ap.add_argument("--serve-checked", type=int, default=4,
@@ -485,6 +488,23 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
         del src, dst
+    one_group = None
+    if args.one_group_frames > 0 and tuning.get("frame_lanes", 1) != 1:
+        # the same frame with each pass as one group: the trace launches then
+        # run alone, so their summed time is the kernel's own
+        dev.set_tuning(frame_lanes=1)
+        dev.render_device(job(args.warmup + args.steps), out.data_ptr())
+        og = [dev.render_device(job(args.warmup + args.steps + 1 + k), out.data_ptr())
+              for k in range(args.one_group_frames)]
+        dev.set_tuning(frame_lanes=tuning["frame_lanes"])
+        oq, otr = sum(s.queries for s in og), sum(s.ms_trace for s in og)
+        ol = sum(s.trace_launches for s in og)
+        o_ach = oq * TRACE_BYTES_PER_QUERY / (otr * 1e-3) / 1e9 if otr > 0 else 0.0
+        one_group = {"frames": len(og), "device_ms_per_frame": round(sum(s.ms_total for s in og) / len(og), 3),
+                     "trace_ms_per_frame": round(otr / len(og), 3),
+                     "avg_launch_ms": round(otr / max(1, ol), 4), "launches": ol,
+                     "achieved": round(o_ach, 2), "frac": round(o_ach / HBM_PEAK_GBS, 5)}
+        log(f"one group per pass: {one_group}")
     queries = sum(s.queries for s in stats)
     samples = sum(s.samples for s in stats)
     trace_ms = sum(s.ms_trace for s in stats)
@@ -534,7 +554,8 @@ def main():
                      "avg_launch_ms": round(trace_ms / max(1, launches), 4), "launches": launches,
                      "busy_ms_per_frame": round(busy_ms / max(1, args.steps), 3),
                      "concurrent_groups": tuning.get("frame_lanes", 1),
-                     "achieved_formula": "queries * 48 B over the time in which at least one trace launch runs"},
+                     "achieved_formula": "queries * 48 B over the time in which at least one trace launch runs",
+                     "one_group": one_group},
         "roofline_pipeline": {"achieved": round(pipe, 2), "frac": round(pipe / HBM_PEAK_GBS, 5),
                               "bytes": pipe_bytes, "formula": "queries*184 + samples*152 over the timed wall time"},
         "cpu_baseline": cpu,
