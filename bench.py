@@ -112,7 +112,7 @@ def traffic_probe(args):
     if not exe:
         return {"error": "rocprofv3 not found"}
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "1", "--no-cpu-baseline",
-             "--serve-jobs", "0", "--traffic", "off", "--copy-probe", str(COPY_PROBE_MIB),
+             "--serve-jobs", "0", "--traffic", "off", "--one-group-frames", "0", "--copy-probe", str(COPY_PROBE_MIB),
              "--workload", args.workload, "--strands", str(args.strands),
              "--subdiv", str(args.subdiv), "--width", str(args.width), "--height", str(args.height),
              "--spp", str(args.spp), "--rdepth", str(args.rdepth), "--accel", args.accel]
@@ -464,7 +464,7 @@ def main():
     if args.autotune and not args.tuning:
         from xtracer_amd.autotune import autotune
         t0 = time.perf_counter()
-        tuning, table = autotune(dev, job(999), out.data_ptr(), wave_paths=(4 << 20, 16 << 20), log=log)
+        tuning, table = autotune(dev, job(999), out.data_ptr(), wave_paths=(16 << 20, 64 << 20), log=log)
         tune_table = [dict(t, ms=round(ms, 3)) for t, ms in table]
         log(f"autotune picked {tuning} in {time.perf_counter() - t0:.1f}s")
     for k in range(args.warmup):

@@ -232,7 +232,7 @@ const char *xt_last_error(const xt_dev *d);
 typedef struct xt_tuning {
     uint32_t kd_stack;        /* kd short-stack entries per lane: 8, 12, 16 or 24 (12) */
     uint32_t trace_block;     /* lanes per trace block: 128 or 256 (256) */
-    uint32_t wave_paths;      /* paths in flight per wavefront pass, 4096..2^28 (16M) */
+    uint32_t wave_paths;      /* paths in flight per wavefront pass, 4096..2^28 (64M: a 1080p 16 spp frame in one pass) */
     uint32_t trace_blocks_per_cu; /* persistent trace grid: blocks per CU, <= 64 (resident fill) */
     uint32_t trace_segments;  /* 1: one work cursor over a trace queue; 8: the queue cut into 8 contiguous
                                  segments, one per XCD, each drained by its own XCD's waves first (1) */
@@ -422,8 +422,9 @@ typedef struct xt_serve_config {
     uint32_t regen;           /* 2: the regeneration engine as above; 1: every job in whole passes (2) */
     uint32_t graph;           /* 2: rounds replayed from a hipGraph; 1: launched kernel by kernel (2) */
     uint32_t rounds_per_step; /* query rounds per xt_serve_step on the regeneration engine, even (4) */
-    uint32_t paths;           /* path slots of the regeneration engine, 4096..2^28 (wave_paths); set while
-                                 it holds no job */
+    uint32_t paths;           /* path slots of the regeneration engine, 4096..2^28 (16M; a larger pool
+                                 runs a long batch faster and makes new requests wait longer for a
+                                 round to end); set while it holds no job */
     uint32_t sample_buffer_mb;/* a job keeps its samples' radiance (16 B each) up to this size in MiB and is
                                  bit-identical to xt_render; a larger one sums in fixed point (4096) */
     uint32_t lane_streams;    /* 2: the lanes of a round (trace, re-check, re-test, shading) run concurrently,

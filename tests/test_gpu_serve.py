@@ -31,7 +31,7 @@ def small_passes(device):
     device.serve_configure(regen=True, graph=True, rounds_per_step=4, paths=1 << 16)
     yield device
     device.drain()
-    device.set_tuning(wave_paths=16 << 20)
+    device.set_tuning(wave_paths=64 << 20)
     device.set_serve_policy("srpt")
     device.serve_configure(regen=True, graph=True, rounds_per_step=4, paths=16 << 20, loop=True)
 
@@ -173,7 +173,7 @@ def test_regen_other_path_slot_count(device):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     finally:
         device.serve_configure(paths=16 << 20)
-        device.set_tuning(wave_paths=16 << 20)
+        device.set_tuning(wave_paths=64 << 20)
 
 
 @pytest.mark.gpu
@@ -228,7 +228,7 @@ def test_budget_admission(regen):
     dev = xa.Device(0, hbm_budget=512 << 20)
     try:
         dev.set_tuning(wave_paths=1 << 16)
-        dev.serve_configure(regen=regen)
+        dev.serve_configure(regen=regen, paths=1 << 16)
         sid = dev.upload(s)
         with pytest.raises(xa.XtError) as e:
             dev.submit(xa.make_job(sid, xa.STENCIL, 4096, 4096, s.camera()))  # 1.3 GB of job buffers

@@ -27,7 +27,7 @@ def mix(device):
     device.set_tuning(wave_paths=1 << 18)
     device.set_serve_policy("srpt")
     yield out
-    device.set_tuning(wave_paths=16 << 20)
+    device.set_tuning(wave_paths=64 << 20)
     for e in out:
         device.release(e[2])
 

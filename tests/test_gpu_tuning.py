@@ -9,7 +9,7 @@ from oracle.oracle import OracleScene
 from xtracer_amd import scenes
 from tests.helpers import mixed_rays, scene_path
 
-DEFAULT = {"kd_stack": 12, "trace_block": 256, "wave_paths": 16 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO,
+DEFAULT = {"kd_stack": 12, "trace_block": 256, "wave_paths": 64 << 20, "trace_blocks_per_cu": xa.TUNING_AUTO,
            "trace_segments": 1, "ray_order": 1, "kd_layout": 1, "kd_leaf": 1, "frame_lanes": 2}
 
 

@@ -33,6 +33,7 @@
 #include "xt_kernels.h"
 
 #define XT_REGEN_MAX_JOBS 1024
+#define XT_REGEN_PATHS (16u << 20)  // default path slots of the engine
 #define XT_REGEN_MAX_LANES 8
 #define XT_REGEN_FIX 4294967296.0  // 2^32: fixed-point scale of the radiance sums
 #define XT_REGEN_SHARDS 64         // done counters per job, summed at the end of a step

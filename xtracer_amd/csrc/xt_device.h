@@ -77,7 +77,7 @@ struct xt_dev {
     int trace_launches = 0;
     std::vector<int> timed_pairs;
     ServeState *serve = nullptr;
-    xt_tuning tuning{12, 256, 16u << 20, 0, 1, 1, 1, 1, 2};
+    xt_tuning tuning{12, 256, 64u << 20, 0, 1, 1, 1, 1, 2};
     void *scratch = nullptr;  // this device's address of xt_trace_device_static's scratch array
     std::string err;
 };

@@ -569,7 +569,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 int regen_init(xt_dev *d, ServeState &S) {
     RegenHost &R = S.R;
     if (R.ready) return XT_OK;
-    const uint32_t P = S.cfg.paths ? S.cfg.paths : d->tuning.wave_paths;
+    const uint32_t P = S.cfg.paths ? S.cfg.paths : XT_REGEN_PATHS;
     const size_t sz[] = {16ull * P, 8ull * P, sizeof(RegenJob) * XT_REGEN_MAX_JOBS, 8ull * XT_REGEN_MAX_JOBS,
                          sizeof(RegenCtl), sizeof(RegenSeg) * XT_REGEN_MAX_JOBS,
                          sizeof(Counters) * XT_REGEN_MAX_LANES, sizeof(Queue) * 2 * XT_REGEN_MAX_LANES,
@@ -1422,7 +1422,7 @@ extern "C" int xt_serve_get_config(xt_dev *d, xt_serve_config *c) {
     int rc = serve_init(d);
     if (rc) return rc;
     *c = d->serve->cfg;
-    if (!c->paths) c->paths = d->serve->R.ready ? d->serve->R.P : d->tuning.wave_paths;
+    if (!c->paths) c->paths = d->serve->R.ready ? d->serve->R.P : XT_REGEN_PATHS;
     return XT_OK;
 }
 
