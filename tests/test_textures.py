@@ -211,6 +211,8 @@ def _jpeg(path, w, h, seed, **kw):
 
 JPEGS = {
     "444_q50": dict(w=53, h=37, quality=50, subsampling=0),
+    "444_q1": dict(w=40, h=24, quality=1, subsampling=0),
+    "420_q100": dict(w=40, h=24, quality=100, subsampling=2),
     "422_q95": dict(w=53, h=37, quality=95, subsampling=1),
     "420_q75": dict(w=53, h=37, quality=75, subsampling=2),
     "420_odd": dict(w=17, h=9, quality=85, subsampling=2),
@@ -230,8 +232,8 @@ JPEGS = {
 @pytest.mark.parametrize("name", sorted(JPEGS))
 def test_jpeg_matches_reference_loader(name, tmp_path):
     """Baseline and progressive JPEG (4:4:4, 4:2:2, 4:2:0 chroma, odd and
-    one-pixel sizes, optimised Huffman tables, restart intervals; spectral
-    selection and successive approximation) equals the reference's
+    one-pixel sizes, optimised Huffman tables, restart intervals, quality 1
+    and 100; spectral selection and successive approximation) equals the reference's
     stb_image decode bit for bit: integer IDCT, chroma upsampling and
     fixed-point YCbCr -> RGB."""
     kw = dict(JPEGS[name])

@@ -482,83 +482,63 @@ int extend(Bits &b, int k) {
 inline uint8_t clamp8(int x) { return (unsigned)x > 255 ? (x < 0 ? 0 : 255) : (uint8_t)x; }
 inline int f2f(double x) { return (int)(x * 4096 + 0.5); }
 
-// one pass of the islow 1-D IDCT over s0..s7; results in x0..x3, t0..t3
-struct Idct1 {
-    int t0, t1, t2, t3, x0, x1, x2, x3;
-    Idct1(int s0, int s1, int s2, int s3, int s4, int s5, int s6, int s7) {
-        int p1, p2, p3, p4, p5;
-        p2 = s2;
-        p3 = s6;
-        p1 = (p2 + p3) * f2f(0.5411961f);
-        t2 = p1 + p3 * f2f(-1.847759065f);
-        t3 = p1 + p2 * f2f(0.765366865f);
-        p2 = s0;
-        p3 = s4;
-        t0 = (p2 + p3) * 4096;
-        t1 = (p2 - p3) * 4096;
-        x0 = t0 + t3;
-        x3 = t0 - t3;
-        x1 = t1 + t2;
-        x2 = t1 - t2;
-        t0 = s7;
-        t1 = s5;
-        t2 = s3;
-        t3 = s1;
-        p3 = t0 + t2;
-        p4 = t1 + t3;
-        p1 = t0 + t3;
-        p2 = t1 + t2;
-        p5 = (p3 + p4) * f2f(1.175875602f);
-        t0 = t0 * f2f(0.298631336f);
-        t1 = t1 * f2f(2.053119869f);
-        t2 = t2 * f2f(3.072711026f);
-        t3 = t3 * f2f(1.501321110f);
-        p1 = p5 + p1 * f2f(-0.899976223f);
-        p2 = p5 + p2 * f2f(-2.562915447f);
-        p3 = p3 * f2f(-1.961570560f);
-        p4 = p4 * f2f(-0.390180644f);
-        t3 += p1 + p4;
-        t2 += p2 + p3;
-        t1 += p2 + p4;
-        t0 += p1 + p3;
+// The 8-point inverse DCT of the IJG "islow" method: the factorisation of
+// Loeffler, Ligtenberg and Moschytz (1989) with 12-bit fixed-point
+// constants. Its rotations and butterflies multiply out to one integer
+// matrix per pass, and integer products are exact, so applying the matrix
+// gives the factorised sums. The sums are formed in 64 bits and wrapped to
+// 32, as the decoder's 32-bit int arithmetic wraps, before the rounding
+// shift. Input k of a pass is frequency k; output j is sample j.
+struct IdctMatrix {
+    int m[8][8];
+    IdctMatrix() {
+        const int one = 4096, r541 = f2f(0.5411961f), r765 = f2f(0.765366865f), r1848 = f2f(-1.847759065f);
+        const int r1176 = f2f(1.175875602f), r299 = f2f(0.298631336f), r2053 = f2f(2.053119869f),
+                  r3073 = f2f(3.072711026f), r1501 = f2f(1.501321110f), r900 = f2f(-0.899976223f),
+                  r2563 = f2f(-2.562915447f), r1962 = f2f(-1.961570560f), r390 = f2f(-0.390180644f);
+        // even half (frequencies 0, 2, 4, 6), one row per output pair
+        const int even[4][4] = {{one, r541 + r765, one, r541},
+                                {one, r541, -one, r541 + r1848},
+                                {one, -r541, -one, -(r541 + r1848)},
+                                {one, -(r541 + r765), one, -r541}};
+        // odd half (frequencies 1, 3, 5, 7); row i pairs with even row 3 - i
+        const int odd[4][4] = {{r1176 + r900, r1176 + r1962, r1176, r299 + r1176 + r900 + r1962},
+                               {r1176 + r390, r1176 + r2563, r2053 + r1176 + r2563 + r390, r1176},
+                               {r1176, r3073 + r1176 + r2563 + r1962, r1176 + r2563, r1176 + r1962},
+                               {r1501 + r1176 + r900 + r390, r1176, r1176 + r390, r1176 + r900}};
+        for (int j = 0; j < 4; ++j) {
+            for (int k = 0; k < 4; ++k) {
+                m[j][2 * k] = m[7 - j][2 * k] = even[j][k];
+                m[j][2 * k + 1] = odd[3 - j][k];
+                m[7 - j][2 * k + 1] = -odd[3 - j][k];
+            }
+        }
+    }
+    // samples of one pass: in[k * step] -> out[j] = (sum + bias) >> shift
+    void pass(const int *in, int step, int bias, int shift, int *out) const {
+        for (int j = 0; j < 8; ++j) {
+            int64_t acc = bias;
+            for (int k = 0; k < 8; ++k) acc += (int64_t)m[j][k] * in[k * step];
+            out[j] = (int)(int32_t)(uint32_t)(uint64_t)acc >> shift;
+        }
     }
 };
 
+const IdctMatrix idct_matrix;
+
+// Columns first, keeping 2 extra bits (bias 1 << 9, shift 10); then rows,
+// removing those and the 12 fraction bits with rounding and the +128 level
+// shift (bias 1 << 16 plus 128 << 17, shift 17), saturated to 8 bits.
 void idct(uint8_t *out, int stride, const int16_t d[64]) {
-    int val[64];
-    for (int i = 0; i < 8; ++i) {  // columns, 2 extra bits kept
-        const int16_t *c = d + i;
-        int *v = val + i;
-        if (!c[8] && !c[16] && !c[24] && !c[32] && !c[40] && !c[48] && !c[56]) {
-            const int dc = c[0] * 4;
-            for (int k = 0; k < 8; ++k) v[8 * k] = dc;
-            continue;
-        }
-        Idct1 q(c[0], c[8], c[16], c[24], c[32], c[40], c[48], c[56]);
-        q.x0 += 512, q.x1 += 512, q.x2 += 512, q.x3 += 512;
-        v[0] = (q.x0 + q.t3) >> 10;
-        v[56] = (q.x0 - q.t3) >> 10;
-        v[8] = (q.x1 + q.t2) >> 10;
-        v[48] = (q.x1 - q.t2) >> 10;
-        v[16] = (q.x2 + q.t1) >> 10;
-        v[40] = (q.x2 - q.t1) >> 10;
-        v[24] = (q.x3 + q.t0) >> 10;
-        v[32] = (q.x3 - q.t0) >> 10;
+    int col[64], in[8], res[8];
+    for (int c = 0; c < 8; ++c) {
+        for (int k = 0; k < 8; ++k) in[k] = d[8 * k + c];
+        idct_matrix.pass(in, 1, 1 << 9, 10, res);
+        for (int j = 0; j < 8; ++j) col[8 * j + c] = res[j];
     }
-    for (int i = 0; i < 8; ++i) {  // rows: remove 1 << 17, round, add 128
-        const int *v = val + 8 * i;
-        uint8_t *o = out + i * stride;
-        Idct1 q(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
-        const int bias = 65536 + (128 << 17);
-        q.x0 += bias, q.x1 += bias, q.x2 += bias, q.x3 += bias;
-        o[0] = clamp8((q.x0 + q.t3) >> 17);
-        o[7] = clamp8((q.x0 - q.t3) >> 17);
-        o[1] = clamp8((q.x1 + q.t2) >> 17);
-        o[6] = clamp8((q.x1 - q.t2) >> 17);
-        o[2] = clamp8((q.x2 + q.t1) >> 17);
-        o[5] = clamp8((q.x2 - q.t1) >> 17);
-        o[3] = clamp8((q.x3 + q.t0) >> 17);
-        o[4] = clamp8((q.x3 - q.t0) >> 17);
+    for (int r = 0; r < 8; ++r) {
+        idct_matrix.pass(col + 8 * r, 1, (1 << 16) + (128 << 17), 17, res);
+        for (int j = 0; j < 8; ++j) out[r * stride + j] = clamp8(res[j]);
     }
 }
 
