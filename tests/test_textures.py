@@ -213,6 +213,8 @@ JPEGS = {
     "444_q50": dict(w=53, h=37, quality=50, subsampling=0),
     "444_q1": dict(w=40, h=24, quality=1, subsampling=0),
     "420_q100": dict(w=40, h=24, quality=100, subsampling=2),
+    "422_one_pixel": dict(w=1, h=5, quality=85, subsampling=1),
+    "422_narrow": dict(w=3, h=8, quality=70, subsampling=1),
     "422_q95": dict(w=53, h=37, quality=95, subsampling=1),
     "420_q75": dict(w=53, h=37, quality=75, subsampling=2),
     "420_odd": dict(w=17, h=9, quality=85, subsampling=2),

@@ -546,36 +546,27 @@ void idct(uint8_t *out, int stride, const int16_t d[64]) {
 void up_v2(uint8_t *o, const uint8_t *nr, const uint8_t *fr, int w) {
     for (int i = 0; i < w; ++i) o[i] = (uint8_t)((3 * nr[i] + fr[i] + 2) >> 2);
 }
+// 2x horizontal: output x lies between source sample x / 2 and its
+// neighbour on x's side, weighted 3 : 1 toward the nearer; the first and
+// last outputs copy the edge samples. The decoder this matches weights its
+// last even output 3 : 1 toward the neighbour instead.
 void up_h2(uint8_t *o, const uint8_t *in, int w) {
-    if (w == 1) {
-        o[0] = o[1] = in[0];
-        return;
+    for (int x = 0; x < 2 * w; ++x) {
+        const int i = x >> 1, nb = (x & 1) ? i + 1 : i - 1;
+        if (nb < 0 || nb >= w) o[x] = in[i];
+        else if (x == 2 * w - 2) o[x] = (uint8_t)((3 * in[nb] + in[i] + 2) >> 2);
+        else o[x] = (uint8_t)((3 * in[i] + in[nb] + 2) >> 2);
     }
-    o[0] = in[0];
-    o[1] = (uint8_t)((in[0] * 3 + in[1] + 2) >> 2);
-    int i;
-    for (i = 1; i < w - 1; ++i) {
-        const int n = 3 * in[i] + 2;
-        o[2 * i] = (uint8_t)((n + in[i - 1]) >> 2);
-        o[2 * i + 1] = (uint8_t)((n + in[i + 1]) >> 2);
-    }
-    o[2 * i] = (uint8_t)((in[w - 2] * 3 + in[w - 1] + 2) >> 2);
-    o[2 * i + 1] = in[w - 1];
 }
+// 2x both ways: the vertical 3 : 1 sums of the nearer and farther rows,
+// then the horizontal 3 : 1 weighting of up_h2 on those sums in one
+// rounding (/16), the edge outputs from their own sum alone (/4).
 void up_hv2(uint8_t *o, const uint8_t *nr, const uint8_t *fr, int w) {
-    if (w == 1) {
-        o[0] = o[1] = (uint8_t)((3 * nr[0] + fr[0] + 2) >> 2);
-        return;
+    auto col = [&](int i) { return 3 * nr[i] + fr[i]; };
+    for (int x = 0; x < 2 * w; ++x) {
+        const int i = x >> 1, nb = (x & 1) ? i + 1 : i - 1;
+        o[x] = (nb < 0 || nb >= w) ? (uint8_t)((col(i) + 2) >> 2) : (uint8_t)((3 * col(i) + col(nb) + 8) >> 4);
     }
-    int t1 = 3 * nr[0] + fr[0], t0;
-    o[0] = (uint8_t)((t1 + 2) >> 2);
-    for (int i = 1; i < w; ++i) {
-        t0 = t1;
-        t1 = 3 * nr[i] + fr[i];
-        o[2 * i - 1] = (uint8_t)((3 * t0 + t1 + 8) >> 4);
-        o[2 * i] = (uint8_t)((3 * t1 + t0 + 8) >> 4);
-    }
-    o[2 * w - 1] = (uint8_t)((t1 + 2) >> 2);
 }
 
 }  // namespace jpg
