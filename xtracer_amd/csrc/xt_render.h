@@ -1,5 +1,7 @@
 // Shared pieces of the render loop used by one-job renders and the server.
 #pragma once
+#include <functional>
+
 #include "xt_device.h"
 
 int xt_job_check(xt_dev *d, const xt_job *job, JobDev &j, const DevScene **ds_out);
@@ -15,7 +17,11 @@ struct PassGroup {
 };
 int xt_launch_shade(xt_dev *d, const DevScene &ds, uint32_t integrator, uint32_t r, int qi, const WaveLane &lane,
                     hipStream_t st);
-int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace);
+// `before(k, stream)` / `after(k, stream)`, when given, queue group k's work
+// ahead of its first round / after its last, on the group's stream.
+using GroupHook = std::function<int(int, hipStream_t)>;
+int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace, const GroupHook *before = nullptr,
+                  const GroupHook *after = nullptr);
 
 // Integrators whose shading reads the camera ray in FP64 (the ray64 slot):
 // depth's distance, normal, uv and emission's texture coordinates.

@@ -205,7 +205,8 @@ int xt_ensure_host_counters(xt_dev *d) {
 // copies of the check before, two rounds back, so that the device still has
 // two rounds queued while the host waits and the groups never wait for each
 // other, and issues no more rounds to groups whose paths have all ended.
-int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
+int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace, const GroupHook *before,
+                  const GroupHook *after) {
     WaveBuffers &w = d->wave;
     if (n_groups < 1 || n_groups > XT_MAX_LANES) {
         d->err = "pass groups out of range";
@@ -233,6 +234,10 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
     if (n_st) {
         XT_TRY(hipEventRecord(d->fork_ev, d->stream));
         for (int k = 0; k < n_st; ++k) XT_TRY(hipStreamWaitEvent(d->lane_stream[k], d->fork_ev, 0));
+    }
+    for (int k = 0; before && k < n_groups; ++k) {
+        const int rc = (*before)(k, stream_of(k));
+        if (rc) return rc;
     }
     int qi = 0, slot = 0;
     bool pending = false;
@@ -273,6 +278,10 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace) {
             slot ^= 1;
             pending = true;
         }
+    }
+    for (int k = 0; after && k < n_groups; ++k) {
+        const int rc = (*after)(k, stream_of(k));
+        if (rc) return rc;
     }
     for (int k = 0; k < n_st; ++k) {
         XT_TRY(hipEventRecord(d->lane_ev[k], d->lane_stream[k]));
@@ -336,24 +345,29 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
     for (uint32_t base = 0; base < j.total; base += spp_pass) {
         ++passes;
         hipLaunchKernelGGL(k_reset_lanes, dim3(n_groups), dim3(64), 0, d->stream, w.cnt, 0x7u);
-        for (int g = 0; g < n_groups; ++g) {
+        // each group's camera rays and accumulation run on the group's own
+        // stream, so that they overlap the other groups' rounds (the groups
+        // write disjoint pixels)
+        const GroupHook camera = [&](int g, hipStream_t st) -> int {
             const WaveLane &L = G[g].lane;
-            hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, d->stream, L.cnt, 0u, (uint32_t)L.cap);
+            hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, st, L.cnt, 0u, (uint32_t)L.cap);
             Queue q0{w.q[0].r0 + L.off, w.q[0].r1 + L.off, w.q[0].path + L.off};
-            hipLaunchKernelGGL(k_camera, dim3(grid_for(L.cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, d->stream, j,
-                               base, spp_pass, q_at[g], q_at[g + 1] - q_at[g], (uint32_t)L.off, q0, w.ps0, w.ps1,
-                               w.rad, need64 ? w.ray64 : nullptr, d->tuning.ray_order);
+            hipLaunchKernelGGL(k_camera, dim3(grid_for(L.cap, XT_BLOCK, 8192)), dim3(XT_BLOCK), 0, st, j, base,
+                               spp_pass, q_at[g], q_at[g + 1] - q_at[g], (uint32_t)L.off, q0, w.ps0, w.ps1, w.rad,
+                               need64 ? w.ray64 : nullptr, d->tuning.ray_order);
             XT_LAUNCHED();
-        }
-        rc = xt_run_rounds(d, G, n_groups, time_trace);
-        if (rc) return rc;
-        for (int g = 0; g < n_groups; ++g) {
+            return XT_OK;
+        };
+        const GroupHook accumulate = [&](int g, hipStream_t st) -> int {
             const WaveLane &L = G[g].lane;
             hipLaunchKernelGGL(k_accumulate, dim3(grid_for(q_at[g + 1] - q_at[g], XT_BLOCK, 8192)), dim3(XT_BLOCK), 0,
-                               d->stream, j, base, spp_pass, q_at[g], q_at[g + 1] - q_at[g], w.rad + L.off,
+                               st, j, base, spp_pass, q_at[g], q_at[g + 1] - q_at[g], w.rad + L.off,
                                need64 ? w.ray64 + 6 * L.off : nullptr, a.accf, a.acc64);
             XT_LAUNCHED();
-        }
+            return XT_OK;
+        };
+        rc = xt_run_rounds(d, G, n_groups, time_trace, &camera, &accumulate);
+        if (rc) return rc;
         if (after_pass) {
             rc = (*after_pass)(j, base, spp_pass);
             if (rc) return rc;
