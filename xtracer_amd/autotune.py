@@ -1,7 +1,7 @@
 """Launch-shape autotuner.
 
-Searches the device's tuning space by timing short renders of the caller's
-own job, keeps the fastest shape on the device and returns the table. The
+Searches the device's tuning space by timing renders of the caller's own
+job, keeps the fastest shape on the device and returns the table. The
 knobs change speed only: every shape gives the same hits and images (the
 kd-tree traversal restarts from the root when its short stack overflows, and
 ray order only permutes the queues).
@@ -46,16 +46,18 @@ def _shape(t):
     return out
 
 
-def autotune(dev, job, out_ptr, space=None, samples=2, repeats=2, wave_paths=(), log=None, stages=None):
-    """Times `job` (cut to `samples` samples per pixel) under the shapes of
-    each stage (`stages`, default STAGES; `space` alone: one stage) on `dev`,
-    then leaves `dev` set to the fastest. `wave_paths` adds wavefront sizes to
-    try once the best shape is known. Returns (best tuning dict,
-    [(tuning dict, ms), ...])."""
+def autotune(dev, job, out_ptr, space=None, samples=0, repeats=2, wave_paths=(), log=None, stages=None):
+    """Times `job` (cut to `samples` samples per pixel when `samples` > 0;
+    whole by default, since a cut job runs smaller query rounds than the
+    whole one and can favour another shape) under the shapes of each stage
+    (`stages`, default STAGES; `space` alone: one stage) on `dev`, then
+    leaves `dev` set to the fastest. `wave_paths` adds wavefront sizes to try
+    once the best shape is known. Returns (best tuning dict, [(tuning dict,
+    ms), ...])."""
     if stages is None:
         stages = [dict(space)] if space is not None else STAGES
     probe = type(job).from_buffer_copy(job)
-    probe.samples = min(job.samples, samples)
+    probe.samples = min(job.samples, samples) if samples > 0 else job.samples
     probe.samples_per_pass = 0
     table = []
     start = dev.tuning()
