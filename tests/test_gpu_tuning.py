@@ -123,6 +123,27 @@ def test_frame_lanes_same_bits(tuned, integrator, ray_order, lanes):
 
 
 @pytest.mark.gpu
+def test_trace_busy_time(tuned):
+    """The job stats' trace busy time is the union of the timed trace
+    launches' intervals: with one group per pass the launches do not overlap
+    and it equals the sum of their times; with two groups it lies between
+    the longest launch and that sum, and within the render's time."""
+    s = xa.Scene(scene_path("icosphere", subdiv=5))
+    sid = tuned.upload(s)
+    job = xa.make_job(sid, xa.PATHTRACER, 320, 240, s.camera(), samples=8, rdepth=6, seed=2)
+    tuned.set_tuning(**dict(DEFAULT, frame_lanes=1))
+    _, one = tuned.render(job)
+    tuned.set_tuning(frame_lanes=2)
+    _, two = tuned.render(job)
+    tuned.release(sid)
+    assert one.trace_launches > 0 and two.trace_launches > one.trace_launches
+    assert one.ms_trace_busy == pytest.approx(one.ms_trace, rel=1e-3, abs=1e-3)
+    assert 0.0 < two.ms_trace_busy <= two.ms_trace * (1 + 1e-4) + 1e-3
+    assert two.ms_trace_busy <= two.ms_total * (1 + 1e-4) + 1e-3
+    assert two.ms_trace_busy >= two.ms_trace / two.trace_launches
+
+
+@pytest.mark.gpu
 def test_autotune_stages(tuned):
     """The staged search tries every value of every stage, leaves the device on
     the fastest shape it timed, and that shape renders the same bits."""
