@@ -235,6 +235,20 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace, c
         XT_TRY(hipEventRecord(d->fork_ev, d->stream));
         for (int k = 0; k < n_st; ++k) XT_TRY(hipStreamWaitEvent(d->lane_stream[k], d->fork_ev, 0));
     }
+    // the device stream waits for the group streams on every return from
+    // here, errors included, so that later work on it cannot overlap this
+    // call's kernels
+    struct Join {
+        xt_dev *d;
+        int n_st;
+        ~Join() {
+            for (int k = 0; k < n_st; ++k) {
+                if (hipEventRecord(d->lane_ev[k], d->lane_stream[k]) != hipSuccess ||
+                    hipStreamWaitEvent(d->stream, d->lane_ev[k], 0) != hipSuccess)
+                    (void)hipStreamSynchronize(d->lane_stream[k]);
+            }
+        }
+    } join{d, n_st};
     for (int k = 0; before && k < n_groups; ++k) {
         const int rc = (*before)(k, stream_of(k));
         if (rc) return rc;
@@ -282,10 +296,6 @@ int xt_run_rounds(xt_dev *d, PassGroup *groups, int n_groups, bool time_trace, c
     for (int k = 0; after && k < n_groups; ++k) {
         const int rc = (*after)(k, stream_of(k));
         if (rc) return rc;
-    }
-    for (int k = 0; k < n_st; ++k) {
-        XT_TRY(hipEventRecord(d->lane_ev[k], d->lane_stream[k]));
-        XT_TRY(hipStreamWaitEvent(d->stream, d->lane_ev[k], 0));
     }
     return XT_OK;
 }
@@ -388,7 +398,7 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, d->ev0, d->ev1);
+        (void)hipEventElapsedTime(&ms, d->ev0, d->ev1);
         stats->ms_total = ms;
         stats->queries = d->host_counters->queries;
         stats->retests = d->host_counters->retests;
@@ -402,9 +412,9 @@ static int render_core(xt_dev *d, const xt_job *job, float *d_out, xt_job_stats 
         std::vector<std::pair<float, float>> spans;
         for (int k : d->timed_pairs) {
             float t = 0.f, t0 = 0.f, t1 = 0.f;
-            hipEventElapsedTime(&t, d->tev[k], d->tev[k + 1]);
-            hipEventElapsedTime(&t0, d->ev0, d->tev[k]);
-            hipEventElapsedTime(&t1, d->ev0, d->tev[k + 1]);
+            (void)hipEventElapsedTime(&t, d->tev[k], d->tev[k + 1]);
+            (void)hipEventElapsedTime(&t0, d->ev0, d->tev[k]);
+            (void)hipEventElapsedTime(&t1, d->ev0, d->tev[k + 1]);
             tsum += t;
             spans.emplace_back(t0, t1);
         }
